@@ -1,0 +1,216 @@
+/*
+ * rz.h — C ABI of librz, the MI355X (gfx950) frame path of the rasteriser.
+ *
+ * The reference renderer exports exactly one frame-path symbol:
+ *
+ *   void draw_frame(const vector<vec3>& model_vertices, const vector<Triangle>& faces,
+ *                   const vector<vec3>& model_vertnormals, const vector<vec2>& vertuvs,
+ *                   vector<Light>& lights, const vector<Material>& materials,
+ *                   const Args& arguments,
+ *                   CImg<unsigned char>* frame_buffer, CImg<float>* depth_buffer);
+ *   (headers/drawing.h:16-18, drawing.cpp:205-258)
+ *
+ * librz replaces it with plain pointers and sizes.  The record types below keep the
+ * byte layout of the reference's own structs (Triangle, Light), so a caller holding
+ * the reference's std::vectors passes .data() straight through.  Buffers keep the
+ * reference layout: the frame is planar uint8 [3][H][W] (CImg<unsigned char>(W,H,1,3),
+ * element (x,y,c) at x + y*W + c*W*H), the depth is float [H][W] (CImg<float>(W,H,1,1)).
+ *
+ * Every entry point returns an rz_status; no exception crosses the ABI.  A context is
+ * bound to one HIP device and owns one HIP stream; calls on one context must not
+ * overlap (one context per host thread).
+ */
+#ifndef RZ_H
+#define RZ_H
+
+#include <stdint.h>
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define RZ_ABI_VERSION 1
+
+typedef enum rz_status {
+    RZ_OK = 0,
+    RZ_ERR_INVALID_ARG = 1,   /* null pointer, zero/oversized image, bad tuning value   */
+    RZ_ERR_INDEX_OOB = 2,     /* a face index outside its attribute array (reference UB, drawing.cpp:166-173) */
+    RZ_ERR_HIP = 3,           /* a HIP runtime call failed; rz_last_error() has the text */
+    RZ_ERR_OOM = 4,           /* device allocation failed                                 */
+    RZ_ERR_NO_DEVICE = 5      /* no HIP device, or the device index is out of range       */
+} rz_status;
+
+/* Reference: headers/face.h:6-13 (struct Triangle).  Separate index streams per
+ * attribute, OBJ style.  uvs[i] < 0 means "no texture coordinate" (the reference
+ * skips the fetch, drawing.cpp:168-170, and interpolates a zero vec2). normals[i]
+ * and material must be valid indices: the reference reads them unchecked, librz
+ * rejects them with RZ_ERR_INDEX_OOB.  40 bytes, same field order. */
+typedef struct rz_triangle {
+    int32_t vertices[3];
+    int32_t normals[3];
+    int32_t uvs[3];
+    int32_t material;
+} rz_triangle;
+
+/* Reference: headers/light.h:7-14 (struct Light).  direction is the world-space
+ * direction the light travels; trans_dir is written by every draw call with
+ * normalize(view * (direction, 0)) exactly like Light::transform
+ * (geometry.cpp:124-133).  40 bytes, same field order. */
+typedef struct rz_light {
+    float direction[3];
+    float intensity;
+    float colour[3];
+    float trans_dir[3];
+} rz_light;
+
+/* Reference: headers/material.h:11-25 (class Material).  With tex_channels == 0 the
+ * flat diffuse_colour is the albedo (material.cpp:23-25).  Otherwise texels holds the
+ * 8-bit image exactly as the file stores it (tex_height rows of tex_width texels,
+ * tex_channels interleaved bytes each, 3 or 4 channels); librz applies the Material
+ * constructor's CImg::normalize(0,1) (material.h:20-23: min and max taken over all
+ * channels) and samples channels 0..2 with CImg::_linear_atXY (material.cpp:17-22). */
+typedef struct rz_material {
+    float diffuse_colour[3];
+    int32_t tex_width;
+    int32_t tex_height;
+    int32_t tex_channels;
+    const uint8_t* texels;
+} rz_material;
+
+/* Reference: headers/arguments.h:7-21 (struct Args) — the fields draw_frame reads —
+ * plus the projection the reference hard-codes (geometry.cpp:30: fovy 45 degrees,
+ * near 0.1, far 6).  The view matrix stays the reference's fixed
+ * translate(0,0,-3) (drawing.cpp:224).  rz_default_args() fills the reference
+ * defaults (arguments.cpp:20-31). */
+typedef struct rz_args {
+    uint32_t image_width;
+    uint32_t image_height;
+    float aspect_ratio;           /* the reference sets (float)width/(float)height, arguments.cpp:39 */
+    int32_t flat;                 /* 1: one normal per face, normalize(cross(p1-p0, p2-p0)) in model space */
+    int32_t wind_clockwise;       /* --wind-clockwise (drawing.cpp:176-178, 131-132) */
+    float scale;
+    float displacement[3];
+    float tait_bryan_angles[3];   /* (rx, ry, rz), composed Y·X·Z (geometry.cpp:22-25) */
+    float fovy_degrees;
+    float z_near;
+    float z_far;
+} rz_args;
+
+/* A mesh in the reference's own representation (fileloader.cpp:79-121 output).
+ * With raster_space != 0, vertices holds num_vertices raster-space vec4
+ * (x_px, y_px, ndc_z, 1/clip_w) and the position transform is skipped (the
+ * screen-space triangle workloads); normals still pass through the normal matrix. */
+typedef struct rz_mesh_desc {
+    const float* vertices;        /* num_vertices * 3 floats (num_vertices * 4 when raster_space) */
+    const float* normals;         /* num_normals * 3 floats */
+    const float* uvs;             /* num_uvs * 2 floats */
+    const rz_triangle* faces;
+    const rz_material* materials;
+    int32_t num_vertices;
+    int32_t num_normals;
+    int32_t num_uvs;
+    int32_t num_faces;
+    int32_t num_materials;
+    int32_t raster_space;
+} rz_mesh_desc;
+
+/* The four matrices one frame uses, column-major float[16] (GLM layout:
+ * element (col, row) at [4*col + row]). */
+typedef struct rz_matrices {
+    float model[16];              /* transformation_matrix(scale, displacement, angles) */
+    float view[16];               /* transformation_matrix(1, (0,0,-3), 0)              */
+    float modelview[16];          /* view * model                                       */
+    float camera[16];             /* perspective(fovy, aspect, near, far) * modelview   */
+    float normal[16];             /* transpose(inverse(modelview))                      */
+} rz_matrices;
+
+/* Counters of one frame (filled when a stats pointer is passed; costs one sync). */
+typedef struct rz_frame_stats {
+    uint64_t triangles;           /* input faces                                             */
+    uint64_t front_triangles;     /* faces passing the backface test (drawing.cpp:176-178)   */
+    uint64_t bin_pairs;           /* (front face, tile) pairs whose clamped bbox meets the tile */
+    uint64_t overflow_tiles;      /* tiles whose bin overflowed and went the overflow path   */
+    uint64_t fragments;           /* (pixel, front face) pairs passing the inside test (drawing.cpp:111) */
+    uint64_t covered_pixels;      /* pixels holding a winning fragment                       */
+    float ms_vertex;              /* GPU time per pass, HIP events on the context stream;    */
+    float ms_bin;                 /* zero unless rz_set_profiling(ctx, 1)                    */
+    float ms_raster;
+    float ms_overflow;
+    float ms_shade;
+    float ms_frame;
+} rz_frame_stats;
+
+typedef struct rz_ctx rz_ctx;
+typedef struct rz_mesh rz_mesh;
+
+/* rz_draw_frame flags */
+#define RZ_FLAG_CLEARED        1u  /* frame holds 0 and depth holds 1.0 (the caller's clear, renderer.cpp:85-86,107-108): neither is read */
+#define RZ_FLAG_DEVICE_BUFFERS 2u  /* frame/depth/ids are device pointers on the context's device */
+#define RZ_FLAG_ASYNC          4u  /* with DEVICE_BUFFERS: return without waiting for the stream */
+
+/* ---- context ---------------------------------------------------------------- */
+
+/* No reference counterpart (the reference has no device).  Creates a context on HIP
+ * device `device` with its own stream (default priority). */
+rz_status rz_create(int32_t device, rz_ctx** out_ctx);
+void rz_destroy(rz_ctx* ctx);
+const char* rz_status_string(rz_status status);
+const char* rz_last_error(const rz_ctx* ctx);
+int32_t rz_abi_version(void);
+
+/* Tile edge (8, 16 or 32 pixels) and per-tile bin capacity (>= 1 triangles) used by
+ * the next frames.  Defaults: 16 and 512. */
+rz_status rz_set_tuning(rz_ctx* ctx, int32_t tile_size, int32_t bin_capacity);
+rz_status rz_get_tuning(const rz_ctx* ctx, int32_t* tile_size, int32_t* bin_capacity);
+/* 1: bracket every pass with HIP events so rz_frame_stats carries GPU times. */
+rz_status rz_set_profiling(rz_ctx* ctx, int32_t enable);
+/* The context's hipStream_t, for callers that order their own work against it. */
+void* rz_stream(rz_ctx* ctx);
+/* Wait for all work queued on the context's stream. */
+rz_status rz_sync(rz_ctx* ctx);
+
+/* ---- mesh cache ---------------------------------------------------------------- */
+
+/* Validates every index (RZ_ERR_INDEX_OOB) and copies the mesh into device memory
+ * in structure-of-arrays form, textures as RGBA8 plus their normalisation table.
+ * The descriptor's arrays are not referenced after the call returns. */
+rz_status rz_mesh_create(rz_ctx* ctx, const rz_mesh_desc* desc, rz_mesh** out_mesh);
+void rz_mesh_destroy(rz_ctx* ctx, rz_mesh* mesh);
+
+/* ---- frame ------------------------------------------------------------------- */
+
+/* Replaces draw_frame (headers/drawing.h:16-18, drawing.cpp:205-258) for a cached
+ * mesh.  lights[i].trans_dir is written back (drawing.cpp:238 mutates the caller's
+ * lights).  frame_buffer: uint8 [3][H][W]; depth_buffer: float [H][W]; both are read
+ * (unless RZ_FLAG_CLEARED) and written in place like the reference, which does not
+ * clear them.  id_buffer (may be NULL): int32 [H][W], the index of the face whose
+ * fragment the pixel finally holds, or -1; a pixel the frame does not touch keeps
+ * -1.  stats may be NULL. */
+rz_status rz_draw_frame(rz_ctx* ctx, const rz_mesh* mesh,
+                        rz_light* lights, int32_t num_lights,
+                        const rz_args* args,
+                        uint8_t* frame_buffer, float* depth_buffer, int32_t* id_buffer,
+                        uint32_t flags, rz_frame_stats* stats);
+
+/* draw_frame with the reference's by-value semantics: uploads desc, draws one frame,
+ * frees the upload (drawing.h:16-18 signature, host buffers). */
+rz_status rz_draw_frame_desc(rz_ctx* ctx, const rz_mesh_desc* desc,
+                             rz_light* lights, int32_t num_lights,
+                             const rz_args* args,
+                             uint8_t* frame_buffer, float* depth_buffer, uint32_t flags);
+
+/* ---- host helpers -------------------------------------------------------------- */
+
+/* Reference defaults of Args (arguments.cpp:20-31) for a width x height image, plus
+ * the projection of geometry.cpp:30. */
+void rz_default_args(rz_args* args, uint32_t width, uint32_t height);
+/* The matrices rz_draw_frame builds for args (geometry.cpp:22-33, drawing.cpp:222-229,
+ * geometry.cpp:101). */
+rz_status rz_frame_matrices(const rz_args* args, rz_matrices* out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* RZ_H */

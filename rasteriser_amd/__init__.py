@@ -1,0 +1,193 @@
+"""rasteriser_amd — the rasteriser's frame path on MI355X (gfx950).
+
+The product is ``librz.so`` (HIP kernels + the C ABI of ``include/rz.h``); this module
+is a thin ctypes host over that ABI for Python callers, tests and the benchmark.  It
+never computes a frame itself: when ``librz.so`` is missing it raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+from . import abi
+from .abi import (RZ_FLAG_ASYNC, RZ_FLAG_CLEARED, RZ_FLAG_DEVICE_BUFFERS, Args, FrameStats,
+                  Matrices, make_args)
+from .scene import MaterialSpec, Scene, make_faces, make_lights
+
+__all__ = ["Context", "Mesh", "RzError", "Scene", "MaterialSpec", "make_faces", "make_lights",
+           "make_args", "library_path", "load_library", "frame_matrices",
+           "RZ_FLAG_CLEARED", "RZ_FLAG_DEVICE_BUFFERS", "RZ_FLAG_ASYNC"]
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+
+def library_path() -> str:
+    return os.path.join(_HERE, "_lib", "librz.so")
+
+
+def load_library():
+    """Load librz.so (built in-tree by __graft_entry__.build()); raise if absent."""
+    global _LIB
+    if _LIB is None:
+        path = library_path()
+        if not os.path.exists(path):
+            raise RuntimeError(f"librz.so is not built ({path}); run __graft_entry__.build()")
+        _LIB = abi.declare(C.CDLL(path))
+    return _LIB
+
+
+class RzError(RuntimeError):
+    def __init__(self, status, message=""):
+        self.status = status
+        super().__init__(f"rz status {status}: {message}")
+
+
+def _check(status, ctx=None):
+    if status != abi.RZ_OK:
+        lib = load_library()
+        msg = lib.rz_last_error(ctx).decode() if ctx else lib.rz_status_string(status).decode()
+        raise RzError(status, msg)
+
+
+def frame_matrices(args: Args) -> dict:
+    """The five matrices librz builds for ``args`` (float32, column-major (4,4) as [col][row])."""
+    m = Matrices()
+    _check(load_library().rz_frame_matrices(C.byref(args), C.byref(m)))
+    return {k: np.frombuffer(bytes(getattr(m, k)), np.float32).reshape(4, 4).copy()
+            for k in ("model", "view", "modelview", "camera", "normal")}
+
+
+class Mesh:
+    """A device-resident mesh (rz_mesh_create)."""
+
+    def __init__(self, ctx: "Context", scene: Scene):
+        self.ctx = ctx
+        self.num_faces = scene.num_faces
+        self.raster_space = scene.raster_space
+        desc, keep = scene.desc()
+        h = C.c_void_p()
+        _check(ctx.lib.rz_mesh_create(ctx.handle, C.byref(desc), C.byref(h)), ctx.handle)
+        del keep
+        self.handle = h
+
+    def close(self):
+        if self.handle:
+            self.ctx.lib.rz_mesh_destroy(self.ctx.handle, self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Context:
+    """One rz_ctx: a HIP device and its stream."""
+
+    def __init__(self, device: int = 0):
+        self.lib = load_library()
+        h = C.c_void_p()
+        _check(self.lib.rz_create(int(device), C.byref(h)))
+        self.handle = h
+
+    def close(self):
+        if self.handle:
+            self.lib.rz_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def set_tuning(self, tile_size: int, bin_capacity: int):
+        _check(self.lib.rz_set_tuning(self.handle, tile_size, bin_capacity), self.handle)
+
+    def tuning(self):
+        t, c = C.c_int32(), C.c_int32()
+        _check(self.lib.rz_get_tuning(self.handle, C.byref(t), C.byref(c)), self.handle)
+        return t.value, c.value
+
+    def set_profiling(self, on: bool):
+        _check(self.lib.rz_set_profiling(self.handle, int(bool(on))), self.handle)
+
+    def stream(self) -> int:
+        return int(self.lib.rz_stream(self.handle) or 0)
+
+    def sync(self):
+        _check(self.lib.rz_sync(self.handle), self.handle)
+
+    def upload(self, scene: Scene) -> Mesh:
+        return Mesh(self, scene)
+
+    def draw(self, mesh: Mesh, lights: np.ndarray, args: Args, frame=None, depth=None,
+             ids: bool = False, stats: bool = False):
+        """draw_frame into host buffers.  With frame/depth None the buffers start cleared
+        (frame 0, depth 1.0, renderer.cpp:85-86); otherwise they are updated in place.
+        ``lights`` (L,10) float32 gets trans_dir written back.  Returns
+        (frame [3,H,W] u8, depth [H,W] f32, ids [H,W] i32 or None, stats dict or None)."""
+        W, H = args.image_width, args.image_height
+        flags = 0
+        if frame is None and depth is None:
+            flags |= RZ_FLAG_CLEARED
+            frame = np.zeros((3, H, W), np.uint8)
+            depth = np.ones((H, W), np.float32)
+        assert frame.dtype == np.uint8 and frame.shape == (3, H, W) and frame.flags.c_contiguous
+        assert depth.dtype == np.float32 and depth.shape == (H, W) and depth.flags.c_contiguous
+        lights = _lights(lights)
+        idbuf = np.empty((H, W), np.int32) if ids else None
+        st = FrameStats() if stats else None
+        _check(self.lib.rz_draw_frame(self.handle, mesh.handle, lights.ctypes.data, lights.shape[0],
+                                      C.byref(args), frame.ctypes.data, depth.ctypes.data,
+                                      idbuf.ctypes.data if ids else None, flags,
+                                      C.byref(st) if stats else None), self.handle)
+        return frame, depth, idbuf, (st.as_dict() if stats else None)
+
+    def draw_device(self, mesh: Mesh, lights: np.ndarray, args: Args, frame_ptr: int,
+                    depth_ptr: int, ids_ptr: int = 0, flags: int = RZ_FLAG_CLEARED,
+                    stats: bool = False):
+        """draw_frame into device buffers (RZ_FLAG_DEVICE_BUFFERS is added)."""
+        lights = _lights(lights)
+        st = FrameStats() if stats else None
+        _check(self.lib.rz_draw_frame(self.handle, mesh.handle, lights.ctypes.data, lights.shape[0],
+                                      C.byref(args), C.c_void_p(frame_ptr), C.c_void_p(depth_ptr),
+                                      C.c_void_p(ids_ptr) if ids_ptr else None,
+                                      flags | RZ_FLAG_DEVICE_BUFFERS,
+                                      C.byref(st) if stats else None), self.handle)
+        return st.as_dict() if stats else None
+
+    def draw_scene(self, scene: Scene, args: Args, frame=None, depth=None):
+        """rz_draw_frame_desc: upload, draw and free in one call (draw_frame's by-value form)."""
+        W, H = args.image_width, args.image_height
+        flags = 0
+        if frame is None and depth is None:
+            flags |= RZ_FLAG_CLEARED
+            frame = np.zeros((3, H, W), np.uint8)
+            depth = np.ones((H, W), np.float32)
+        lights = _lights(scene.lights)
+        desc, keep = scene.desc()
+        _check(self.lib.rz_draw_frame_desc(self.handle, C.byref(desc), lights.ctypes.data,
+                                           lights.shape[0], C.byref(args), frame.ctypes.data,
+                                           depth.ctypes.data, flags), self.handle)
+        del keep
+        scene.lights[:] = lights
+        return frame, depth
+
+
+def _lights(lights) -> np.ndarray:
+    lights = np.asarray(lights)
+    if lights.dtype != np.float32 or not lights.flags.c_contiguous or lights.ndim != 2 \
+            or lights.shape[1] != 10:
+        raise ValueError("lights must be a C-contiguous (L, 10) float32 array")
+    return lights

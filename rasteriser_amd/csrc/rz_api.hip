@@ -1,0 +1,556 @@
+// rz_api.hip — the C ABI of librz (include/rz.h): contexts, the device mesh cache
+// and the per-frame pass sequence vertex -> bin -> raster -> overflow -> shade.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/rz.h"
+#include "rz_device.h"
+#include "rz_host_math.h"
+
+namespace rz {
+hipError_t launch_vertex(const VertexParams& p, hipStream_t s);
+hipError_t launch_bin(const BinParams& p, hipStream_t s);
+hipError_t launch_raster(const RasterParams& p, int tile_size, hipStream_t s);
+hipError_t launch_overflow(const RasterParams& p, int ts_log2, hipStream_t s);
+hipError_t launch_shade(const ShadeParams& p, hipStream_t s);
+}  // namespace rz
+
+using namespace rz;
+
+namespace {
+
+constexpr uint32_t kMaxImageSide = 32767;   // bbox corners are packed in 16 bits
+
+struct DevBuf {
+    void* ptr = nullptr;
+    size_t bytes = 0;
+};
+
+}  // namespace
+
+struct rz_mesh {
+    int device = 0;
+    int32_t nv = 0, nn = 0, nt = 0, nf = 0, nm = 0;
+    bool raster_space = false;
+    float *px = nullptr, *py = nullptr, *pz = nullptr;
+    float4* rin = nullptr;
+    float *nx = nullptr, *ny = nullptr, *nz = nullptr;      // vertex normals
+    float *fnx = nullptr, *fny = nullptr, *fnz = nullptr;   // face normals (flat)
+    float2* uvs = nullptr;
+    int32_t *fv = nullptr, *fn = nullptr, *fuv = nullptr, *fmat = nullptr;
+    DevMaterial* mats = nullptr;
+    uint32_t* texels = nullptr;
+    float* luts = nullptr;
+    std::vector<void*> allocs;
+};
+
+struct rz_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int tile_size = 16;
+    int bin_cap = 512;
+    bool profiling = false;
+    hipEvent_t ev[6] = {};
+    std::string err;
+    DevBuf raster, cnormal, rec, tile_count, bins, vis, frame, depth, ids, counters;
+    unsigned long long* host_counters = nullptr;   // pinned
+};
+
+namespace {
+
+rz_status fail(rz_ctx* ctx, rz_status st, const char* fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    if (ctx) ctx->err = buf;
+    fprintf(stderr, "librz: %s\n", buf);
+    return st;
+}
+
+#define RZ_HIP(ctx, call)                                                                          \
+    do {                                                                                           \
+        hipError_t e_ = (call);                                                                    \
+        if (e_ != hipSuccess)                                                                      \
+            return fail((ctx), e_ == hipErrorOutOfMemory ? RZ_ERR_OOM : RZ_ERR_HIP, "%s: %s (%s:%d)", \
+                        #call, hipGetErrorString(e_), __FILE__, __LINE__);                         \
+    } while (0)
+
+rz_status ensure(rz_ctx* ctx, DevBuf& b, size_t bytes) {
+    if (bytes == 0) bytes = 256;
+    if (b.bytes >= bytes) return RZ_OK;
+    if (b.ptr) {
+        RZ_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        RZ_HIP(ctx, hipFree(b.ptr));
+        b.ptr = nullptr;
+        b.bytes = 0;
+    }
+    RZ_HIP(ctx, hipMalloc(&b.ptr, bytes));
+    b.bytes = bytes;
+    return RZ_OK;
+}
+
+int log2_exact(int v) {
+    int l = 0;
+    while ((1 << l) < v) ++l;
+    return (1 << l) == v ? l : -1;
+}
+
+template <typename T>
+rz_status upload(rz_ctx* ctx, rz_mesh* m, T** dst, const void* src, size_t bytes) {
+    *dst = nullptr;
+    if (bytes == 0) bytes = 16;
+    void* p = nullptr;
+    RZ_HIP(ctx, hipMalloc(&p, bytes));
+    m->allocs.push_back(p);
+    if (src) RZ_HIP(ctx, hipMemcpy(p, src, bytes, hipMemcpyHostToDevice));
+    *dst = static_cast<T*>(p);
+    return RZ_OK;
+}
+
+rz_status validate_mesh(rz_ctx* ctx, const rz_mesh_desc* d) {
+    if (!d) return fail(ctx, RZ_ERR_INVALID_ARG, "mesh descriptor is null");
+    if (d->num_vertices < 0 || d->num_normals < 0 || d->num_uvs < 0 || d->num_faces < 0 || d->num_materials < 0)
+        return fail(ctx, RZ_ERR_INVALID_ARG, "negative mesh array size");
+    if ((d->num_vertices && !d->vertices) || (d->num_normals && !d->normals) || (d->num_uvs && !d->uvs) ||
+        (d->num_faces && !d->faces) || (d->num_materials && !d->materials))
+        return fail(ctx, RZ_ERR_INVALID_ARG, "mesh array pointer is null");
+    for (int32_t f = 0; f < d->num_faces; ++f) {
+        const rz_triangle& t = d->faces[f];
+        for (int k = 0; k < 3; ++k) {
+            if (t.vertices[k] < 0 || t.vertices[k] >= d->num_vertices)
+                return fail(ctx, RZ_ERR_INDEX_OOB, "face %d: vertex index %d out of [0,%d)", f, t.vertices[k],
+                            d->num_vertices);
+            if (t.normals[k] < 0 || t.normals[k] >= d->num_normals)
+                return fail(ctx, RZ_ERR_INDEX_OOB, "face %d: normal index %d out of [0,%d)", f, t.normals[k],
+                            d->num_normals);
+            if (t.uvs[k] >= d->num_uvs)
+                return fail(ctx, RZ_ERR_INDEX_OOB, "face %d: uv index %d out of [0,%d)", f, t.uvs[k], d->num_uvs);
+        }
+        if (t.material < 0 || t.material >= d->num_materials)
+            return fail(ctx, RZ_ERR_INDEX_OOB, "face %d: material %d out of [0,%d)", f, t.material,
+                        d->num_materials);
+    }
+    for (int32_t i = 0; i < d->num_materials; ++i) {
+        const rz_material& m = d->materials[i];
+        if (m.tex_channels == 0) continue;
+        if ((m.tex_channels != 3 && m.tex_channels != 4) || !m.texels || m.tex_width <= 0 || m.tex_height <= 0)
+            return fail(ctx, RZ_ERR_INVALID_ARG, "material %d: texture needs 3 or 4 channels and a size", i);
+    }
+    return RZ_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int32_t rz_abi_version(void) { return RZ_ABI_VERSION; }
+
+const char* rz_status_string(rz_status s) {
+    switch (s) {
+        case RZ_OK: return "ok";
+        case RZ_ERR_INVALID_ARG: return "invalid argument";
+        case RZ_ERR_INDEX_OOB: return "index out of bounds";
+        case RZ_ERR_HIP: return "HIP runtime error";
+        case RZ_ERR_OOM: return "out of device memory";
+        case RZ_ERR_NO_DEVICE: return "no HIP device";
+    }
+    return "unknown status";
+}
+
+const char* rz_last_error(const rz_ctx* ctx) { return ctx ? ctx->err.c_str() : ""; }
+
+void rz_default_args(rz_args* a, uint32_t width, uint32_t height) {
+    if (!a) return;
+    std::memset(a, 0, sizeof *a);
+    a->image_width = width;
+    a->image_height = height;
+    a->aspect_ratio = height ? (float)width / (float)height : 1.0f;
+    a->scale = 1.0f;
+    a->fovy_degrees = 45.0f;
+    a->z_near = 0.1f;
+    a->z_far = 6.0f;
+}
+
+rz_status rz_frame_matrices(const rz_args* args, rz_matrices* out) {
+    if (!args || !out) return RZ_ERR_INVALID_ARG;
+    build_matrices(*args, *out);
+    return RZ_OK;
+}
+
+rz_status rz_create(int32_t device, rz_ctx** out) {
+    if (!out) return RZ_ERR_INVALID_ARG;
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return fail(nullptr, RZ_ERR_NO_DEVICE, "no HIP device");
+    if (device < 0 || device >= n) return fail(nullptr, RZ_ERR_NO_DEVICE, "device %d out of range [0,%d)", device, n);
+    rz_ctx* ctx = new rz_ctx();
+    ctx->device = device;
+    rz_status st = RZ_OK;
+    do {
+        hipError_t e = hipSetDevice(device);
+        if (e == hipSuccess) e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
+        for (int i = 0; i < 6 && e == hipSuccess; ++i) e = hipEventCreate(&ctx->ev[i]);
+        if (e == hipSuccess)
+            e = hipHostMalloc((void**)&ctx->host_counters, sizeof(unsigned long long) * kNumCounters,
+                              hipHostMallocDefault);
+        if (e != hipSuccess) st = fail(ctx, RZ_ERR_HIP, "context setup: %s", hipGetErrorString(e));
+    } while (0);
+    if (st != RZ_OK) {
+        rz_destroy(ctx);
+        return st;
+    }
+    *out = ctx;
+    return RZ_OK;
+}
+
+void rz_destroy(rz_ctx* ctx) {
+    if (!ctx) return;
+    (void)hipSetDevice(ctx->device);
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    DevBuf* bufs[] = {&ctx->raster, &ctx->cnormal, &ctx->rec, &ctx->tile_count, &ctx->bins,
+                      &ctx->vis,    &ctx->frame,   &ctx->depth, &ctx->ids,     &ctx->counters};
+    for (DevBuf* b : bufs)
+        if (b->ptr) (void)hipFree(b->ptr);
+    for (hipEvent_t e : ctx->ev)
+        if (e) (void)hipEventDestroy(e);
+    if (ctx->host_counters) (void)hipHostFree(ctx->host_counters);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+}
+
+rz_status rz_set_tuning(rz_ctx* ctx, int32_t tile_size, int32_t bin_capacity) {
+    if (!ctx) return RZ_ERR_INVALID_ARG;
+    if (tile_size != 8 && tile_size != 16 && tile_size != 32)
+        return fail(ctx, RZ_ERR_INVALID_ARG, "tile size %d not in {8,16,32}", tile_size);
+    if (bin_capacity < 1 || bin_capacity > (1 << 20))
+        return fail(ctx, RZ_ERR_INVALID_ARG, "bin capacity %d not in [1, 2^20]", bin_capacity);
+    ctx->tile_size = tile_size;
+    ctx->bin_cap = bin_capacity;
+    return RZ_OK;
+}
+
+rz_status rz_get_tuning(const rz_ctx* ctx, int32_t* tile_size, int32_t* bin_capacity) {
+    if (!ctx) return RZ_ERR_INVALID_ARG;
+    if (tile_size) *tile_size = ctx->tile_size;
+    if (bin_capacity) *bin_capacity = ctx->bin_cap;
+    return RZ_OK;
+}
+
+rz_status rz_set_profiling(rz_ctx* ctx, int32_t enable) {
+    if (!ctx) return RZ_ERR_INVALID_ARG;
+    ctx->profiling = enable != 0;
+    return RZ_OK;
+}
+
+void* rz_stream(rz_ctx* ctx) { return ctx ? (void*)ctx->stream : nullptr; }
+
+rz_status rz_sync(rz_ctx* ctx) {
+    if (!ctx) return RZ_ERR_INVALID_ARG;
+    RZ_HIP(ctx, hipSetDevice(ctx->device));
+    RZ_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return RZ_OK;
+}
+
+rz_status rz_mesh_create(rz_ctx* ctx, const rz_mesh_desc* d, rz_mesh** out) {
+    if (!ctx || !out) return RZ_ERR_INVALID_ARG;
+    *out = nullptr;
+    rz_status st = validate_mesh(ctx, d);
+    if (st != RZ_OK) return st;
+    RZ_HIP(ctx, hipSetDevice(ctx->device));
+    rz_mesh* m = new rz_mesh();
+    m->device = ctx->device;
+    m->nv = d->num_vertices;
+    m->nn = d->num_normals;
+    m->nt = d->num_uvs;
+    m->nf = d->num_faces;
+    m->nm = d->num_materials;
+    m->raster_space = d->raster_space != 0;
+    auto cleanup = [&](rz_status s) {
+        rz_mesh_destroy(ctx, m);
+        return s;
+    };
+    const size_t V = (size_t)m->nv, N = (size_t)m->nn, F = (size_t)m->nf;
+    // positions: SoA for the coalesced vertex stage
+    if (m->raster_space) {
+        if ((st = upload(ctx, m, &m->rin, d->vertices, V * 16)) != RZ_OK) return cleanup(st);
+    } else {
+        std::vector<float> sx(V), sy(V), sz(V);
+        for (size_t i = 0; i < V; ++i) {
+            sx[i] = d->vertices[3 * i];
+            sy[i] = d->vertices[3 * i + 1];
+            sz[i] = d->vertices[3 * i + 2];
+        }
+        if ((st = upload(ctx, m, &m->px, sx.data(), V * 4)) != RZ_OK) return cleanup(st);
+        if ((st = upload(ctx, m, &m->py, sy.data(), V * 4)) != RZ_OK) return cleanup(st);
+        if ((st = upload(ctx, m, &m->pz, sz.data(), V * 4)) != RZ_OK) return cleanup(st);
+        // face normals for flat shading, model space
+        std::vector<float> fx(F), fy(F), fz(F);
+        for (size_t f = 0; f < F; ++f) {
+            const rz_triangle& t = d->faces[f];
+            float nrm[3];
+            face_normal(d->vertices + 3 * (size_t)t.vertices[0], d->vertices + 3 * (size_t)t.vertices[1],
+                        d->vertices + 3 * (size_t)t.vertices[2], nrm);
+            fx[f] = nrm[0]; fy[f] = nrm[1]; fz[f] = nrm[2];
+        }
+        if ((st = upload(ctx, m, &m->fnx, fx.data(), F * 4)) != RZ_OK) return cleanup(st);
+        if ((st = upload(ctx, m, &m->fny, fy.data(), F * 4)) != RZ_OK) return cleanup(st);
+        if ((st = upload(ctx, m, &m->fnz, fz.data(), F * 4)) != RZ_OK) return cleanup(st);
+    }
+    {
+        std::vector<float> sx(N), sy(N), sz(N);
+        for (size_t i = 0; i < N; ++i) {
+            sx[i] = d->normals[3 * i];
+            sy[i] = d->normals[3 * i + 1];
+            sz[i] = d->normals[3 * i + 2];
+        }
+        if ((st = upload(ctx, m, &m->nx, sx.data(), N * 4)) != RZ_OK) return cleanup(st);
+        if ((st = upload(ctx, m, &m->ny, sy.data(), N * 4)) != RZ_OK) return cleanup(st);
+        if ((st = upload(ctx, m, &m->nz, sz.data(), N * 4)) != RZ_OK) return cleanup(st);
+    }
+    if ((st = upload(ctx, m, &m->uvs, d->uvs, (size_t)m->nt * 8)) != RZ_OK) return cleanup(st);
+    {
+        std::vector<int32_t> fv(3 * F), fn(3 * F), fuv(3 * F), fmat(F);
+        for (size_t f = 0; f < F; ++f) {
+            const rz_triangle& t = d->faces[f];
+            for (int k = 0; k < 3; ++k) {
+                fv[3 * f + k] = t.vertices[k];
+                fn[3 * f + k] = t.normals[k];
+                fuv[3 * f + k] = t.uvs[k] < 0 ? -1 : t.uvs[k];
+            }
+            fmat[f] = t.material;
+        }
+        if ((st = upload(ctx, m, &m->fv, fv.data(), 12 * F)) != RZ_OK) return cleanup(st);
+        if ((st = upload(ctx, m, &m->fn, fn.data(), 12 * F)) != RZ_OK) return cleanup(st);
+        if ((st = upload(ctx, m, &m->fuv, fuv.data(), 12 * F)) != RZ_OK) return cleanup(st);
+        if ((st = upload(ctx, m, &m->fmat, fmat.data(), 4 * F)) != RZ_OK) return cleanup(st);
+    }
+    {
+        // materials: RGBA8 texel pool plus one 256-entry normalisation table each
+        std::vector<DevMaterial> dm(m->nm);
+        std::vector<uint32_t> pool;
+        std::vector<float> luts((size_t)m->nm * 256, 0.f);
+        for (int32_t i = 0; i < m->nm; ++i) {
+            const rz_material& src = d->materials[i];
+            DevMaterial& o = dm[i];
+            std::memset(&o, 0, sizeof o);
+            o.diffuse[0] = src.diffuse_colour[0];
+            o.diffuse[1] = src.diffuse_colour[1];
+            o.diffuse[2] = src.diffuse_colour[2];
+            o.lut_offset = (uint32_t)(256 * i);
+            if (src.tex_channels == 0) continue;
+            const size_t n = (size_t)src.tex_width * src.tex_height;
+            if (pool.size() + n > 0xFFFFFFFFull)
+                return cleanup(fail(ctx, RZ_ERR_INVALID_ARG, "texture pool above 2^32 texels"));
+            o.tex_w = src.tex_width;
+            o.tex_h = src.tex_height;
+            o.tex_offset = (uint32_t)pool.size();
+            const int C = src.tex_channels;
+            normalisation_table(src.texels, n * C, luts.data() + 256 * (size_t)i);
+            pool.resize(pool.size() + n);
+            uint32_t* dst = pool.data() + o.tex_offset;
+            for (size_t t = 0; t < n; ++t) {
+                const uint8_t* s = src.texels + t * C;
+                dst[t] = (uint32_t)s[0] | ((uint32_t)s[1] << 8) | ((uint32_t)s[2] << 16) |
+                         ((uint32_t)(C == 4 ? s[3] : 0) << 24);
+            }
+        }
+        if ((st = upload(ctx, m, &m->mats, dm.data(), dm.size() * sizeof(DevMaterial))) != RZ_OK) return cleanup(st);
+        if ((st = upload(ctx, m, &m->texels, pool.data(), pool.size() * 4)) != RZ_OK) return cleanup(st);
+        if ((st = upload(ctx, m, &m->luts, luts.data(), luts.size() * 4)) != RZ_OK) return cleanup(st);
+    }
+    *out = m;
+    return RZ_OK;
+}
+
+void rz_mesh_destroy(rz_ctx* ctx, rz_mesh* m) {
+    if (!m) return;
+    if (ctx) {
+        (void)hipSetDevice(ctx->device);
+        (void)hipStreamSynchronize(ctx->stream);
+    }
+    for (void* p : m->allocs) (void)hipFree(p);
+    delete m;
+}
+
+rz_status rz_draw_frame(rz_ctx* ctx, const rz_mesh* mesh, rz_light* lights, int32_t nl, const rz_args* args,
+                        uint8_t* frame_buffer, float* depth_buffer, int32_t* id_buffer, uint32_t flags,
+                        rz_frame_stats* stats) {
+    if (!ctx) return RZ_ERR_INVALID_ARG;
+    if (!mesh || !args || !frame_buffer || !depth_buffer)
+        return fail(ctx, RZ_ERR_INVALID_ARG, "null mesh, args or buffer");
+    if (nl < 0 || nl > kMaxLights || (nl > 0 && !lights))
+        return fail(ctx, RZ_ERR_INVALID_ARG, "light count %d not in [0,%d]", nl, kMaxLights);
+    const uint32_t W = args->image_width, H = args->image_height;
+    if (W == 0 || H == 0 || W > kMaxImageSide || H > kMaxImageSide)
+        return fail(ctx, RZ_ERR_INVALID_ARG, "image %ux%u not in [1,%u]^2", W, H, kMaxImageSide);
+    if (args->flat && mesh->raster_space)
+        return fail(ctx, RZ_ERR_INVALID_ARG, "flat shading needs model-space positions");
+    if (mesh->device != ctx->device) return fail(ctx, RZ_ERR_INVALID_ARG, "mesh lives on another device");
+    const bool cleared = (flags & RZ_FLAG_CLEARED) != 0;
+    const bool on_device = (flags & RZ_FLAG_DEVICE_BUFFERS) != 0;
+    RZ_HIP(ctx, hipSetDevice(ctx->device));
+
+    rz_matrices mats;
+    build_matrices(*args, mats);
+    ShadeParams sp;   // carries the lights; passed by value to k_shade
+    std::memset(&sp, 0, sizeof sp);
+    for (int32_t l = 0; l < nl; ++l) {
+        light_direction(mats.view, lights[l].direction, lights[l].trans_dir);   // drawing.cpp:238
+        for (int c = 0; c < 3; ++c) {
+            sp.light_ic[l][c] = lights[l].intensity * lights[l].colour[c];
+            sp.light_nd[l][c] = -lights[l].trans_dir[c];
+        }
+    }
+
+    const int ts = ctx->tile_size, tsl = log2_exact(ts);
+    const int tiles_x = (int)((W + ts - 1) / ts), tiles_y = (int)((H + ts - 1) / ts);
+    const int ntiles = tiles_x * tiles_y;
+    const size_t np = (size_t)W * H;
+    const int32_t nfn = args->flat ? mesh->nf : mesh->nn;
+    rz_status st;
+    if ((st = ensure(ctx, ctx->raster, (size_t)mesh->nv * 16)) != RZ_OK) return st;
+    if ((st = ensure(ctx, ctx->cnormal, (size_t)nfn * 16)) != RZ_OK) return st;
+    if ((st = ensure(ctx, ctx->rec, (size_t)mesh->nf * sizeof(FaceRecord))) != RZ_OK) return st;
+    if ((st = ensure(ctx, ctx->tile_count, (size_t)ntiles * 4)) != RZ_OK) return st;
+    if ((st = ensure(ctx, ctx->bins, (size_t)ntiles * ctx->bin_cap * 4)) != RZ_OK) return st;
+    if ((st = ensure(ctx, ctx->vis, np * 8)) != RZ_OK) return st;
+    if ((st = ensure(ctx, ctx->counters, kNumCounters * 8)) != RZ_OK) return st;
+
+    uint8_t* frame_d = frame_buffer;
+    float* depth_d = depth_buffer;
+    int32_t* ids_d = id_buffer;
+    if (!on_device) {
+        if ((st = ensure(ctx, ctx->frame, np * 3)) != RZ_OK) return st;
+        if ((st = ensure(ctx, ctx->depth, np * 4)) != RZ_OK) return st;
+        frame_d = (uint8_t*)ctx->frame.ptr;
+        depth_d = (float*)ctx->depth.ptr;
+        if (id_buffer) {
+            if ((st = ensure(ctx, ctx->ids, np * 4)) != RZ_OK) return st;
+            ids_d = (int32_t*)ctx->ids.ptr;
+        }
+        if (!cleared) {
+            RZ_HIP(ctx, hipMemcpyAsync(frame_d, frame_buffer, np * 3, hipMemcpyHostToDevice, ctx->stream));
+            RZ_HIP(ctx, hipMemcpyAsync(depth_d, depth_buffer, np * 4, hipMemcpyHostToDevice, ctx->stream));
+        }
+    }
+    hipStream_t s = ctx->stream;
+    auto* counters = (unsigned long long*)ctx->counters.ptr;
+
+    VertexParams vp;
+    std::memset(&vp, 0, sizeof vp);
+    std::memcpy(vp.camera, mats.camera, 64);
+    std::memcpy(vp.normal, mats.normal, 64);
+    vp.px = mesh->px; vp.py = mesh->py; vp.pz = mesh->pz;
+    vp.raster_in = mesh->raster_space ? mesh->rin : nullptr;
+    vp.nx = args->flat ? mesh->fnx : mesh->nx;
+    vp.ny = args->flat ? mesh->fny : mesh->ny;
+    vp.nz = args->flat ? mesh->fnz : mesh->nz;
+    vp.raster = (float4*)ctx->raster.ptr;
+    vp.cnormal = (float4*)ctx->cnormal.ptr;
+    vp.tile_count = (uint32_t*)ctx->tile_count.ptr;
+    vp.counters = counters;
+    vp.nv = mesh->nv; vp.nn = nfn; vp.ntiles = ntiles;
+    vp.Wf = (float)W; vp.Hf = (float)H;
+
+    BinParams bp;
+    std::memset(&bp, 0, sizeof bp);
+    bp.fv = mesh->fv;
+    bp.raster = vp.raster;
+    bp.rec = (FaceRecord*)ctx->rec.ptr;
+    bp.tile_count = vp.tile_count;
+    bp.bins = (uint32_t*)ctx->bins.ptr;
+    bp.counters = counters;
+    bp.nf = mesh->nf; bp.W = W; bp.H = H;
+    bp.tiles_x = tiles_x; bp.ts_log2 = tsl; bp.cap = (uint32_t)ctx->bin_cap;
+    bp.wind_cw = args->wind_clockwise != 0;
+
+    RasterParams rp;
+    std::memset(&rp, 0, sizeof rp);
+    rp.rec = bp.rec;
+    rp.tile_count = bp.tile_count;
+    rp.bins = bp.bins;
+    rp.depth_in = cleared ? nullptr : depth_d;
+    rp.vis = (unsigned long long*)ctx->vis.ptr;
+    rp.counters = counters;
+    rp.nf = mesh->nf; rp.W = W; rp.H = H;
+    rp.tiles_x = tiles_x; rp.ntiles = ntiles; rp.ts_log2 = tsl; rp.cap = bp.cap;
+
+    sp.vis = rp.vis;
+    sp.rec = bp.rec;
+    sp.fn = mesh->fn; sp.fuv = mesh->fuv; sp.fmat = mesh->fmat;
+    sp.cnormal = vp.cnormal;
+    sp.uvs = mesh->uvs;
+    sp.mats = mesh->mats; sp.texels = mesh->texels; sp.luts = mesh->luts;
+    sp.frame = frame_d; sp.depth = depth_d; sp.ids = ids_d;
+    sp.counters = counters;
+    sp.W = W; sp.H = H;
+    sp.cleared = cleared;
+    sp.wind_cw = args->wind_clockwise != 0;
+    sp.flat = args->flat != 0;
+    sp.nl = nl;
+
+    const bool prof = ctx->profiling;
+    if (prof) RZ_HIP(ctx, hipEventRecord(ctx->ev[0], s));
+    RZ_HIP(ctx, launch_vertex(vp, s));
+    if (prof) RZ_HIP(ctx, hipEventRecord(ctx->ev[1], s));
+    RZ_HIP(ctx, launch_bin(bp, s));
+    if (prof) RZ_HIP(ctx, hipEventRecord(ctx->ev[2], s));
+    RZ_HIP(ctx, launch_raster(rp, ts, s));
+    if (prof) RZ_HIP(ctx, hipEventRecord(ctx->ev[3], s));
+    RZ_HIP(ctx, launch_overflow(rp, tsl, s));
+    if (prof) RZ_HIP(ctx, hipEventRecord(ctx->ev[4], s));
+    RZ_HIP(ctx, launch_shade(sp, s));
+    if (prof) RZ_HIP(ctx, hipEventRecord(ctx->ev[5], s));
+
+    if (!on_device) {
+        RZ_HIP(ctx, hipMemcpyAsync(frame_buffer, frame_d, np * 3, hipMemcpyDeviceToHost, s));
+        RZ_HIP(ctx, hipMemcpyAsync(depth_buffer, depth_d, np * 4, hipMemcpyDeviceToHost, s));
+        if (id_buffer) RZ_HIP(ctx, hipMemcpyAsync(id_buffer, ids_d, np * 4, hipMemcpyDeviceToHost, s));
+    }
+    if (stats) {
+        RZ_HIP(ctx, hipMemcpyAsync(ctx->host_counters, counters, kNumCounters * 8, hipMemcpyDeviceToHost, s));
+    }
+    if (stats || !on_device || !(flags & RZ_FLAG_ASYNC)) RZ_HIP(ctx, hipStreamSynchronize(s));
+    if (stats) {
+        std::memset(stats, 0, sizeof *stats);
+        stats->triangles = (uint64_t)mesh->nf;
+        stats->front_triangles = ctx->host_counters[kCntFront];
+        stats->bin_pairs = ctx->host_counters[kCntPairs];
+        stats->overflow_tiles = ctx->host_counters[kCntOverflowTiles];
+        stats->fragments = ctx->host_counters[kCntFragments];
+        stats->covered_pixels = ctx->host_counters[kCntCovered];
+        if (prof) {
+            float t[5];
+            for (int i = 0; i < 5; ++i) RZ_HIP(ctx, hipEventElapsedTime(&t[i], ctx->ev[i], ctx->ev[i + 1]));
+            stats->ms_vertex = t[0];
+            stats->ms_bin = t[1];
+            stats->ms_raster = t[2];
+            stats->ms_overflow = t[3];
+            stats->ms_shade = t[4];
+            RZ_HIP(ctx, hipEventElapsedTime(&stats->ms_frame, ctx->ev[0], ctx->ev[5]));
+        }
+    }
+    return RZ_OK;
+}
+
+rz_status rz_draw_frame_desc(rz_ctx* ctx, const rz_mesh_desc* desc, rz_light* lights, int32_t nl,
+                             const rz_args* args, uint8_t* frame_buffer, float* depth_buffer, uint32_t flags) {
+    if (!ctx) return RZ_ERR_INVALID_ARG;
+    if (flags & RZ_FLAG_DEVICE_BUFFERS) flags &= ~RZ_FLAG_ASYNC;
+    rz_mesh* m = nullptr;
+    rz_status st = rz_mesh_create(ctx, desc, &m);
+    if (st != RZ_OK) return st;
+    st = rz_draw_frame(ctx, m, lights, nl, args, frame_buffer, depth_buffer, nullptr, flags, nullptr);
+    rz_mesh_destroy(ctx, m);
+    return st;
+}
+
+}  // extern "C"

@@ -1,0 +1,166 @@
+// rz_device.h — data layouts and bit-exact arithmetic shared by librz's gfx950 kernels.
+//
+// Every expression below keeps the reference's operation order (cited per function)
+// and the whole library is built with -ffp-contract=off, IEEE divide/sqrt and IEEE
+// denormals, so each float result is bit-identical to the reference's x86-64 build.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#pragma clang fp contract(off)
+
+namespace rz {
+
+constexpr int kMaxLights = 64;
+constexpr uint64_t kEmptyKey = ~0ull;           // visibility word of a pixel with no fragment
+constexpr uint32_t kNoFace = 0xFFFFFFFFu;
+constexpr int kBigTriangleTiles = 16;           // above this many tiles a face is binned by its whole wave
+
+// Counter slots (uint64) cleared by the vertex pass each frame.
+enum Counter : int {
+    kCntFront = 0,
+    kCntPairs = 1,
+    kCntOverflowTiles = 2,
+    kCntFragments = 3,
+    kCntCovered = 4,
+    kNumCounters = 8
+};
+
+// Per-face setup record written by the bin pass (64 bytes, one per face):
+//   r0 = (x0, y0, z0, area)            raster vertex 0, area = edge(v2, v0, v1)
+//   r1 = (x1, y1, z1, bits(bbox_lo))   bbox_lo = tl.x | tl.y << 16
+//   r2 = (x2, y2, z2, bits(bbox_hi))   bbox_hi = br.x | br.y << 16
+//   r3 = (w0, w1, w2, 0)               1/clip_w per vertex
+// A culled face has bbox_lo = 0xFFFFFFFF, bbox_hi = 0 (empty).
+struct FaceRecord {
+    float4 r0, r1, r2, r3;
+};
+
+// Device copy of one material.
+struct DevMaterial {
+    float diffuse[3];
+    int32_t tex_w;         // 0: no texture
+    int32_t tex_h;
+    uint32_t tex_offset;   // first texel (RGBA8 packed in a uint32) in the mesh texel pool
+    uint32_t lut_offset;   // first entry of the 256-float normalisation table
+    uint32_t pad;
+};
+
+struct VertexParams {
+    float camera[16];
+    float normal[16];
+    const float* px; const float* py; const float* pz;   // model positions (SoA)
+    const float4* raster_in;                             // raster-space input, or null
+    const float* nx; const float* ny; const float* nz;   // normals to transform (SoA)
+    float4* raster;                                      // out: (x_px, y_px, ndc_z, 1/w)
+    float4* cnormal;                                     // out: camera-space normal (xyz)
+    uint32_t* tile_count;
+    unsigned long long* counters;
+    int32_t nv, nn, ntiles;
+    float Wf, Hf;
+};
+
+struct BinParams {
+    const int32_t* fv;          // 3 vertex indices per face
+    const float4* raster;
+    FaceRecord* rec;
+    uint32_t* tile_count;
+    uint32_t* bins;
+    unsigned long long* counters;
+    int32_t nf;
+    uint32_t W, H;
+    int32_t tiles_x;
+    int32_t ts_log2;
+    uint32_t cap;
+    int32_t wind_cw;
+};
+
+struct RasterParams {
+    const FaceRecord* rec;
+    const uint32_t* tile_count;
+    const uint32_t* bins;
+    const float* depth_in;      // null when the depth buffer is known cleared to 1.0
+    unsigned long long* vis;
+    unsigned long long* counters;
+    int32_t nf;
+    uint32_t W, H;
+    int32_t tiles_x, ntiles;
+    int32_t ts_log2;
+    uint32_t cap;
+};
+
+struct ShadeParams {
+    const unsigned long long* vis;
+    const FaceRecord* rec;
+    const int32_t* fn;          // 3 normal indices per face (flat: the face itself)
+    const int32_t* fuv;         // 3 uv indices per face, -1 = none
+    const int32_t* fmat;
+    const float4* cnormal;
+    const float2* uvs;
+    const DevMaterial* mats;
+    const uint32_t* texels;
+    const float* luts;
+    uint8_t* frame;             // planar [3][H][W]
+    float* depth;
+    int32_t* ids;               // may be null
+    unsigned long long* counters;
+    uint32_t W, H;
+    int32_t cleared;
+    int32_t wind_cw;
+    int32_t flat;
+    int32_t nl;
+    float light_ic[kMaxLights][3];   // intensity * colour per light
+    float light_nd[kMaxLights][3];   // -trans_dir per light
+};
+
+// ---------------------------------------------------------------------------------
+// Arithmetic (host+device so host-side unit checks can call the same code).
+
+// GLM 0.9.7 scalar min / max (func_common.inl).
+__host__ __device__ __forceinline__ float gmin(float x, float y) { return x < y ? x : y; }
+__host__ __device__ __forceinline__ float gmax(float x, float y) { return x > y ? x : y; }
+__host__ __device__ __forceinline__ float gclamp(float x, float lo, float hi) { return gmin(gmax(x, lo), hi); }
+
+// (unsigned int)f as GCC compiles it for x86-64 (cvttss2si to 64 bits, low half).
+__host__ __device__ __forceinline__ uint32_t cast_u32(float f) {
+    if (f >= 0.0f && f < 4294967296.0f) return (uint32_t)f;
+    if (!(f > -9223372036854775808.0f && f < 9223372036854775808.0f)) return 0u;
+    return (uint32_t)(uint64_t)(int64_t)f;
+}
+
+// drawing.cpp:36-39: (b.x-a.x)*(p.y-a.y) - (b.y-a.y)*(p.x-a.x)
+__host__ __device__ __forceinline__ float edge(float px, float py, float ax, float ay, float bx, float by) {
+    return (bx - ax) * (py - ay) - (by - ay) * (px - ax);
+}
+
+// geometry.cpp:76-83, left to right.
+__host__ __device__ __forceinline__ float signed_area(float x0, float y0, float x1, float y1, float x2, float y2) {
+    return -0.5f * (x0 * y1 - x1 * y0 + x1 * y2 - x2 * y1 + x2 * y0 - x0 * y2);
+}
+
+// Order-preserving map of a non-NaN float to uint32 with -0 and +0 sharing a key.
+__host__ __device__ __forceinline__ uint32_t depth_order(float z) {
+    uint32_t b = __float_as_uint(z);
+    if ((b & 0x7FFFFFFFu) == 0) b = 0;
+    return (b & 0x80000000u) ? ~b : (b | 0x80000000u);
+}
+
+__host__ __device__ __forceinline__ unsigned long long vis_key(float z, uint32_t face) {
+    return ((unsigned long long)depth_order(z) << 32) | face;
+}
+
+// Clamped inclusive bbox of drawing.cpp:77-93 (GLM clamp, ceil, unsigned cast).
+__host__ __device__ __forceinline__ void bounding_box(float x0, float y0, float x1, float y1, float x2, float y2,
+                                                      uint32_t W, uint32_t H, uint32_t& tlx, uint32_t& tly,
+                                                      uint32_t& brx, uint32_t& bry) {
+    const float bx = (float)(W - 1), by = (float)(H - 1);
+    const float mnx = gmin(gmin(x0, x1), x2), mny = gmin(gmin(y0, y1), y2);
+    const float mxx = gmax(gmax(x0, x1), x2), mxy = gmax(gmax(y0, y1), y2);
+    tlx = cast_u32(gclamp(mnx, 0.f, bx));
+    tly = cast_u32(gclamp(mny, 0.f, by));
+    brx = cast_u32(gclamp(ceilf(mxx), 0.f, bx));
+    bry = cast_u32(gclamp(ceilf(mxy), 0.f, by));
+}
+
+}  // namespace rz
