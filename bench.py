@@ -1,0 +1,208 @@
+#!/usr/bin/env python3
+"""Frame-path benchmark on one MI355X.
+
+Workload: the 81,920-face displaced, textured icosphere (6 subdivisions, smooth
+normals, 1024x1024 RGBA8 texture, three lights) at 1920x1080 with a seeded YXZ
+rotation — one step is one whole frame (vertex, bin, raster/depth, overflow, shade)
+with the mesh resident in HBM and the frame/depth buffers in device memory.
+
+Timed region: `steps` frames captured in one HIP graph, launched once between two
+device synchronisations.  `value` = faces / frame time in Mtri/s.
+
+`roofline` is for the dominant kernel of the frame: the pass with the longest
+average duration, measured with HIP events recorded on the librz stream around
+each pass over `steps` further eager frames; `achieved` is the pass's algorithmic
+bytes per launch (counts from the frame itself) divided by that duration.
+
+`cpu_baseline` is the CPU oracle (a single-threaded restatement of the reference's
+draw_frame) timed on this host for a bounded number of full frames of the same
+workload.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import platform
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "Mtri/s + Gfrag/s at 1080p (frac of HBM peak); 64-light shade ms plain vs MLP"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def algorithmic_bytes(counts, scene, args, cleared=True):
+    """Per-pass algorithmic bytes of one frame (input reads + output writes that the
+    pass must do at minimum)."""
+    W, H = args.image_width, args.image_height
+    V = scene.vertices.shape[0]
+    Vn = scene.normals.shape[0]
+    T = scene.num_faces
+    L = scene.lights.shape[0]
+    tex = sum(int(m.texture.shape[0] * m.texture.shape[1]) for m in scene.materials if m.texture is not None)
+    P16, Tf, Tvis = counts["bin_pairs16"], counts["front_triangles"], counts["visible_faces"]
+    return {
+        "vertex": 28 * V + 24 * Vn,
+        "bin": 12 * T + 16 * V + 4 * P16,
+        "raster": 48 * Tf + 4 * P16 + 8 * W * H + (0 if cleared else 4 * W * H),
+        "overflow": 0,
+        "shade": 8 * W * H + 7 * W * H + 108 * Tvis + 4 * tex + 40 * L,
+    }
+
+
+def cpu_baseline(scene, args, seconds):
+    import oracle
+    oracle.draw_frame(scene, args)   # warm-up
+    n, t0 = 0, time.perf_counter()
+    while True:
+        oracle.draw_frame(scene, args)
+        n += 1
+        el = time.perf_counter() - t0
+        if (el >= seconds and n >= 3) or n >= 10000:
+            break
+    ms = el * 1000.0 / n
+    cpu = platform.processor() or platform.machine()
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    cpu = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {
+        "value": scene.num_faces / (ms / 1000.0) / 1e6,
+        "unit": "Mtri/s",
+        "cores": 1,
+        "kind": "port",
+        "sample": f"{n} full frames (clear + draw_frame) of the same 81,920-face 1080p frame, "
+                  f"{ms:.1f} ms/frame on 1 thread of {cpu} ({os.cpu_count()} logical CPUs)",
+        "ms_per_frame": ms,
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=100)
+    ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--flat", action="store_true", help="flat instead of smooth normals")
+    ap.add_argument("--tile", type=int, default=16)
+    ap.add_argument("--cap", type=int, default=512)
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {a.gpus} is not supported (the frame path is "
+              f"built and measured on a single MI355X)", file=sys.stderr)
+        return 2
+
+    import numpy as np
+
+    import rasteriser_amd as rz
+    from rasteriser_amd import scenes
+
+    scene, args = scenes.icosphere_scene(flat=a.flat)
+    W, H = args.image_width, args.image_height
+    ctx = rz.Context(0)
+    ctx.set_tuning(a.tile, a.cap)
+    mesh = ctx.upload(scene)
+    frame = ctx.empty((3, H, W), np.uint8)
+    depth = ctx.empty((H, W), np.float32)
+    ids = ctx.empty((H, W), np.int32)
+    lights = scene.lights_copy()
+    flags = rz.RZ_FLAG_CLEARED | rz.RZ_FLAG_ASYNC
+
+    # counts of this frame (same counts every step: identical frames)
+    st = ctx.draw_device(mesh, lights, args, frame.ptr, depth.ptr, ids.ptr,
+                         flags, stats=True)
+    idh = ids.download()
+    counts = {"bin_pairs16": st["bin_pairs"] if a.tile == 16 else None,
+              "front_triangles": st["front_triangles"],
+              "visible_faces": int(np.unique(idh[idh >= 0]).size),
+              "fragments": st["fragments"], "covered_pixels": st["covered_pixels"]}
+    if counts["bin_pairs16"] is None:
+        ctx.set_tuning(16, a.cap)
+        counts["bin_pairs16"] = ctx.draw_device(mesh, lights, args, frame.ptr, depth.ptr,
+                                                0, flags, stats=True)["bin_pairs"]
+        ctx.set_tuning(a.tile, a.cap)
+    log(f"frame counts: {counts}")
+
+    # timed region: `steps` frames in one graph
+    g_warm = ctx.graph(mesh, lights, args, frame.ptr, depth.ptr, max(1, a.warmup), flags=flags)
+    g = ctx.graph(mesh, lights, args, frame.ptr, depth.ptr, a.steps, flags=flags)
+    g_warm.launch()
+    ctx.sync()
+    t0 = time.perf_counter()
+    g.launch()
+    ctx.sync()
+    t1 = time.perf_counter()
+    ms_per_step = (t1 - t0) * 1000.0 / a.steps
+    value = scene.num_faces / (ms_per_step / 1000.0) / 1e6
+    log(f"graph: {a.steps} frames in {(t1 - t0) * 1000:.2f} ms -> {ms_per_step * 1000:.2f} us/frame")
+
+    # per-pass durations: HIP events on the librz stream around every pass
+    ctx.set_profiling(True)
+    passes = ("vertex", "bin", "raster", "overflow", "shade")
+    acc = {p: 0.0 for p in passes}
+    nprof = max(1, min(a.steps, 500))
+    for _ in range(nprof):
+        s = ctx.draw_device(mesh, lights, args, frame.ptr, depth.ptr, 0, flags, stats=True)
+        for p in passes:
+            acc[p] += s["ms_" + p]
+    ctx.set_profiling(False)
+    avg_ms = {p: acc[p] / nprof for p in passes}
+    nbytes = algorithmic_bytes(counts, scene, args)
+    per_pass = {p: {"avg_us": avg_ms[p] * 1000.0, "alg_bytes": nbytes[p],
+                    "GBps": (nbytes[p] / (avg_ms[p] * 1e-3) / 1e9) if avg_ms[p] > 0 else 0.0}
+                for p in passes}
+    for p in passes:
+        per_pass[p]["frac"] = per_pass[p]["GBps"] / HBM_PEAK_GBS
+    dom = max(passes, key=lambda p: avg_ms[p])
+    log("per pass:", json.dumps(per_pass))
+    roofline = {"bound": "hbm", "achieved": per_pass[dom]["GBps"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": per_pass[dom]["frac"], "traffic": None, "kernel": f"k_{dom}",
+                "alg_bytes_per_launch": nbytes[dom], "avg_launch_us": per_pass[dom]["avg_us"]}
+    gfrag = counts["fragments"] / (avg_ms["raster"] * 1e-3) / 1e9 if avg_ms["raster"] > 0 else None
+
+    cpu = None
+    if not a.no_cpu:
+        cpu = cpu_baseline(scene, args, a.cpu_seconds)
+        log(f"cpu baseline: {cpu}")
+
+    g.close()
+    g_warm.close()
+    mesh.close()
+    for buf in (frame, depth, ids):
+        buf.close()
+    ctx.close()
+    out = {
+        "metric": METRIC, "value": value, "unit": "Mtri/s", "n_gpus": 1, "steps": a.steps,
+        "warmup": a.warmup, "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "icosphere-s6-displaced-textured-1080p", "faces": scene.num_faces,
+                   "width": W, "height": H, "shading": "flat" if a.flat else "smooth",
+                   "lights": int(scene.lights.shape[0]), "texture": "1024x1024 RGBA8",
+                   "tile": a.tile, "bin_capacity": a.cap, "frames_per_graph": a.steps},
+        "gfrag_per_s": gfrag,
+        "fragments_per_frame": counts["fragments"],
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "gpu_vs_cpu_frame_rate": (cpu["ms_per_frame"] / ms_per_step) if cpu else None,
+        "passes": per_pass,
+        "shade64_plain_ms": None,
+        "shade64_mlp_ms": None,
+    }
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

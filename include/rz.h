@@ -200,6 +200,35 @@ rz_status rz_draw_frame_desc(rz_ctx* ctx, const rz_mesh_desc* desc,
                              const rz_args* args,
                              uint8_t* frame_buffer, float* depth_buffer, uint32_t flags);
 
+/* ---- frame graphs ------------------------------------------------------------- */
+
+typedef struct rz_graph rz_graph;
+
+/* No reference counterpart: the reference's spin loop (renderer.cpp:94-126) calls
+ * draw_frame once per iteration.  Captures `frames` consecutive draw_frame calls
+ * into one HIP graph, frame k rotated by k*ry_step about y (the spin loop's
+ * tait_bryan_angles.y += dt), writing every frame into the same device buffers
+ * (RZ_FLAG_DEVICE_BUFFERS required).  The graph refers to the mesh and to the
+ * context's per-frame buffers: the mesh must outlive it, and a later frame that
+ * grows those buffers (a larger image or mesh) makes rz_graph_launch refuse it. */
+rz_status rz_graph_create(rz_ctx* ctx, const rz_mesh* mesh, rz_light* lights, int32_t num_lights,
+                          const rz_args* args, uint8_t* frame_device, float* depth_device,
+                          int32_t* id_device, uint32_t flags, int32_t frames, float ry_step,
+                          rz_graph** out_graph);
+/* Enqueues all captured frames on the context stream (asynchronous). */
+rz_status rz_graph_launch(rz_graph* graph);
+void rz_graph_destroy(rz_graph* graph);
+
+/* ---- device memory ------------------------------------------------------------ */
+
+/* Plain device allocations on the context's device, for callers that keep frame and
+ * depth buffers in HBM (RZ_FLAG_DEVICE_BUFFERS) without an allocator of their own.
+ * The copies run on the context stream and return when complete. */
+rz_status rz_device_alloc(rz_ctx* ctx, size_t bytes, void** out_ptr);
+rz_status rz_device_free(rz_ctx* ctx, void* ptr);
+rz_status rz_copy_to_device(rz_ctx* ctx, void* dst_device, const void* src_host, size_t bytes);
+rz_status rz_copy_to_host(rz_ctx* ctx, void* dst_host, const void* src_device, size_t bytes);
+
 /* ---- host helpers -------------------------------------------------------------- */
 
 /* Reference defaults of Args (arguments.cpp:20-31) for a width x height image, plus

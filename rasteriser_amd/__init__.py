@@ -85,6 +85,63 @@ class Mesh:
             pass
 
 
+class FrameGraph:
+    """A captured run of frames (rz_graph_create); launch() enqueues them all."""
+
+    def __init__(self, ctx: "Context", handle, frames: int):
+        self.ctx, self.handle, self.frames = ctx, handle, frames
+
+    def launch(self):
+        _check(self.ctx.lib.rz_graph_launch(self.handle), self.ctx.handle)
+
+    def close(self):
+        if self.handle:
+            self.ctx.lib.rz_graph_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class DeviceArray:
+    """A device allocation of the context's device holding an array of `shape`/`dtype`."""
+
+    def __init__(self, ctx: "Context", shape, dtype):
+        self.ctx = ctx
+        self.shape = tuple(shape)
+        self.dtype = np.dtype(dtype)
+        self.nbytes = int(np.prod(self.shape)) * self.dtype.itemsize
+        p = C.c_void_p()
+        _check(ctx.lib.rz_device_alloc(ctx.handle, self.nbytes, C.byref(p)), ctx.handle)
+        self.ptr = int(p.value)
+
+    def upload(self, host: np.ndarray):
+        host = np.ascontiguousarray(host, self.dtype)
+        assert host.nbytes == self.nbytes
+        _check(self.ctx.lib.rz_copy_to_device(self.ctx.handle, C.c_void_p(self.ptr), host.ctypes.data,
+                                              self.nbytes), self.ctx.handle)
+
+    def download(self) -> np.ndarray:
+        out = np.empty(self.shape, self.dtype)
+        _check(self.ctx.lib.rz_copy_to_host(self.ctx.handle, out.ctypes.data, C.c_void_p(self.ptr),
+                                            self.nbytes), self.ctx.handle)
+        return out
+
+    def close(self):
+        if self.ptr:
+            self.ctx.lib.rz_device_free(self.ctx.handle, C.c_void_p(self.ptr))
+            self.ptr = 0
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 class Context:
     """One rz_ctx: a HIP device and its stream."""
 
@@ -128,6 +185,9 @@ class Context:
     def sync(self):
         _check(self.lib.rz_sync(self.handle), self.handle)
 
+    def empty(self, shape, dtype) -> DeviceArray:
+        return DeviceArray(self, shape, dtype)
+
     def upload(self, scene: Scene) -> Mesh:
         return Mesh(self, scene)
 
@@ -167,6 +227,18 @@ class Context:
                                       C.byref(st) if stats else None), self.handle)
         return st.as_dict() if stats else None
 
+    def graph(self, mesh: Mesh, lights: np.ndarray, args: Args, frame_ptr: int, depth_ptr: int,
+              frames: int, ids_ptr: int = 0, flags: int = RZ_FLAG_CLEARED, ry_step: float = 0.0):
+        """rz_graph_create: `frames` consecutive frames captured in one HIP graph."""
+        lights = _lights(lights)
+        h = C.c_void_p()
+        _check(self.lib.rz_graph_create(self.handle, mesh.handle, lights.ctypes.data, lights.shape[0],
+                                        C.byref(args), C.c_void_p(frame_ptr), C.c_void_p(depth_ptr),
+                                        C.c_void_p(ids_ptr) if ids_ptr else None,
+                                        flags | RZ_FLAG_DEVICE_BUFFERS, int(frames), float(ry_step),
+                                        C.byref(h)), self.handle)
+        return FrameGraph(self, h, frames)
+
     def draw_scene(self, scene: Scene, args: Args, frame=None, depth=None):
         """rz_draw_frame_desc: upload, draw and free in one call (draw_frame's by-value form)."""
         W, H = args.image_width, args.image_height
@@ -181,7 +253,6 @@ class Context:
                                            lights.shape[0], C.byref(args), frame.ctypes.data,
                                            depth.ctypes.data, flags), self.handle)
         del keep
-        scene.lights[:] = lights
         return frame, depth
 
 

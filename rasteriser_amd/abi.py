@@ -112,6 +112,14 @@ def declare(lib):
                                     C.POINTER(FrameStats)]),
         "rz_draw_frame_desc": (C.c_int, [P, C.POINTER(MeshDesc), P, C.c_int32, C.POINTER(Args),
                                          P, P, C.c_uint32]),
+        "rz_graph_create": (C.c_int, [P, P, P, C.c_int32, C.POINTER(Args), P, P, P, C.c_uint32,
+                                      C.c_int32, C.c_float, C.POINTER(P)]),
+        "rz_graph_launch": (C.c_int, [P]),
+        "rz_graph_destroy": (None, [P]),
+        "rz_device_alloc": (C.c_int, [P, C.c_size_t, C.POINTER(P)]),
+        "rz_device_free": (C.c_int, [P, P]),
+        "rz_copy_to_device": (C.c_int, [P, P, P, C.c_size_t]),
+        "rz_copy_to_host": (C.c_int, [P, P, P, C.c_size_t]),
         "rz_default_args": (None, [C.POINTER(Args), C.c_uint32, C.c_uint32]),
         "rz_frame_matrices": (C.c_int, [C.POINTER(Args), C.POINTER(Matrices)]),
     }
@@ -125,4 +133,6 @@ def declare(lib):
 EXPORTED = ("rz_abi_version", "rz_status_string", "rz_last_error", "rz_create", "rz_destroy",
             "rz_set_tuning", "rz_get_tuning", "rz_set_profiling", "rz_stream", "rz_sync",
             "rz_mesh_create", "rz_mesh_destroy", "rz_draw_frame", "rz_draw_frame_desc",
+            "rz_graph_create", "rz_graph_launch", "rz_graph_destroy",
+            "rz_device_alloc", "rz_device_free", "rz_copy_to_device", "rz_copy_to_host",
             "rz_default_args", "rz_frame_matrices")

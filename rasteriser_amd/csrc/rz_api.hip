@@ -50,6 +50,13 @@ struct rz_mesh {
     std::vector<void*> allocs;
 };
 
+struct rz_graph {
+    rz_ctx* ctx = nullptr;
+    hipGraphExec_t exec = nullptr;
+    int32_t frames = 0;
+    uint64_t buffer_gen = 0;   // ctx->buffer_gen when captured
+};
+
 struct rz_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -58,8 +65,10 @@ struct rz_ctx {
     bool profiling = false;
     hipEvent_t ev[6] = {};
     std::string err;
-    DevBuf raster, cnormal, rec, tile_count, bins, vis, frame, depth, ids, counters;
+    DevBuf raster, cnormal, rec, tile_count, bins, vis, frame, depth, ids, counters, partials;
+    std::vector<uint32_t> host_partials;
     unsigned long long* host_counters = nullptr;   // pinned
+    uint64_t buffer_gen = 0;                       // bumped whenever a per-frame buffer is reallocated
 };
 
 namespace {
@@ -94,6 +103,7 @@ rz_status ensure(rz_ctx* ctx, DevBuf& b, size_t bytes) {
     }
     RZ_HIP(ctx, hipMalloc(&b.ptr, bytes));
     b.bytes = bytes;
+    ++ctx->buffer_gen;
     return RZ_OK;
 }
 
@@ -144,6 +154,157 @@ rz_status validate_mesh(rz_ctx* ctx, const rz_mesh_desc* d) {
         if ((m.tex_channels != 3 && m.tex_channels != 4) || !m.texels || m.tex_width <= 0 || m.tex_height <= 0)
             return fail(ctx, RZ_ERR_INVALID_ARG, "material %d: texture needs 3 or 4 channels and a size", i);
     }
+    return RZ_OK;
+}
+
+
+// One frame's kernel parameters, built on the host before anything is enqueued.
+struct FrameLaunch {
+    VertexParams vp;
+    BinParams bp;
+    RasterParams rp;
+    ShadeParams sp;
+    int tile_size = 16;
+    int ts_log2 = 4;
+    size_t np = 0;
+    // sections of the per-workgroup statistics buffer (uint32 each)
+    size_t off_front = 0, off_pairs = 0, off_frag = 0, off_ovf = 0, off_cov = 0, n_part = 0;
+    uint32_t* part = nullptr;
+};
+
+// Validates, builds matrices and light directions (written back to `lights`, as
+// drawing.cpp:238 does), sizes the context's per-frame buffers and fills every pass's
+// parameters except the output buffers.
+rz_status prepare_frame(rz_ctx* ctx, const rz_mesh* mesh, rz_light* lights, int32_t nl, const rz_args* args,
+                        uint32_t flags, FrameLaunch& L) {
+    if (!mesh || !args) return fail(ctx, RZ_ERR_INVALID_ARG, "null mesh or args");
+    if (nl < 0 || nl > kMaxLights || (nl > 0 && !lights))
+        return fail(ctx, RZ_ERR_INVALID_ARG, "light count %d not in [0,%d]", nl, kMaxLights);
+    const uint32_t W = args->image_width, H = args->image_height;
+    if (W == 0 || H == 0 || W > kMaxImageSide || H > kMaxImageSide)
+        return fail(ctx, RZ_ERR_INVALID_ARG, "image %ux%u not in [1,%u]^2", W, H, kMaxImageSide);
+    if (args->flat && mesh->raster_space)
+        return fail(ctx, RZ_ERR_INVALID_ARG, "flat shading needs model-space positions");
+    if (mesh->device != ctx->device) return fail(ctx, RZ_ERR_INVALID_ARG, "mesh lives on another device");
+    RZ_HIP(ctx, hipSetDevice(ctx->device));
+    const bool cleared = (flags & RZ_FLAG_CLEARED) != 0;
+
+    rz_matrices mats;
+    build_matrices(*args, mats);
+    std::memset(&L.sp, 0, sizeof L.sp);
+    for (int32_t l = 0; l < nl; ++l) {
+        light_direction(mats.view, lights[l].direction, lights[l].trans_dir);   // drawing.cpp:238
+        for (int c = 0; c < 3; ++c) {
+            L.sp.light_ic[l][c] = lights[l].intensity * lights[l].colour[c];
+            L.sp.light_nd[l][c] = -lights[l].trans_dir[c];
+        }
+    }
+    const int ts = ctx->tile_size, tsl = log2_exact(ts);
+    const int tiles_x = (int)((W + ts - 1) / ts), tiles_y = (int)((H + ts - 1) / ts);
+    const int ntiles = tiles_x * tiles_y;
+    const size_t np = (size_t)W * H;
+    const int32_t nfn = args->flat ? mesh->nf : mesh->nn;
+    rz_status st;
+    if ((st = ensure(ctx, ctx->raster, (size_t)mesh->nv * 16)) != RZ_OK) return st;
+    if ((st = ensure(ctx, ctx->cnormal, (size_t)nfn * 16)) != RZ_OK) return st;
+    if ((st = ensure(ctx, ctx->rec, (size_t)mesh->nf * sizeof(FaceRecord))) != RZ_OK) return st;
+    if ((st = ensure(ctx, ctx->tile_count, (size_t)ntiles * 4)) != RZ_OK) return st;
+    if ((st = ensure(ctx, ctx->bins, (size_t)ntiles * ctx->bin_cap * 4)) != RZ_OK) return st;
+    if ((st = ensure(ctx, ctx->vis, np * 8)) != RZ_OK) return st;
+    if ((st = ensure(ctx, ctx->counters, kNumCounters * 8)) != RZ_OK) return st;
+    L.off_front = 0;
+    L.off_pairs = L.off_front + bin_blocks(mesh->nf);
+    L.off_frag = L.off_pairs + bin_blocks(mesh->nf);
+    L.off_ovf = L.off_frag + (size_t)ntiles;
+    L.off_cov = L.off_ovf + overflow_blocks(mesh->nf);
+    L.n_part = L.off_cov + (size_t)shade_blocks_x(W) * shade_blocks_y(H);
+    if ((st = ensure(ctx, ctx->partials, L.n_part * 4)) != RZ_OK) return st;
+    auto* counters = (unsigned long long*)ctx->counters.ptr;
+    auto* part = (uint32_t*)ctx->partials.ptr;
+    L.part = part;
+    L.tile_size = ts;
+    L.ts_log2 = tsl;
+    L.np = np;
+
+    VertexParams& vp = L.vp;
+    std::memset(&vp, 0, sizeof vp);
+    std::memcpy(vp.camera, mats.camera, 64);
+    std::memcpy(vp.normal, mats.normal, 64);
+    vp.px = mesh->px; vp.py = mesh->py; vp.pz = mesh->pz;
+    vp.raster_in = mesh->raster_space ? mesh->rin : nullptr;
+    vp.nx = args->flat ? mesh->fnx : mesh->nx;
+    vp.ny = args->flat ? mesh->fny : mesh->ny;
+    vp.nz = args->flat ? mesh->fnz : mesh->nz;
+    vp.raster = (float4*)ctx->raster.ptr;
+    vp.cnormal = (float4*)ctx->cnormal.ptr;
+    vp.tile_count = (uint32_t*)ctx->tile_count.ptr;
+    vp.counters = counters;
+    vp.nv = mesh->nv; vp.nn = nfn; vp.ntiles = ntiles;
+    vp.Wf = (float)W; vp.Hf = (float)H;
+
+    BinParams& bp = L.bp;
+    std::memset(&bp, 0, sizeof bp);
+    bp.fv = mesh->fv;
+    bp.raster = vp.raster;
+    bp.rec = (FaceRecord*)ctx->rec.ptr;
+    bp.tile_count = vp.tile_count;
+    bp.bins = (uint32_t*)ctx->bins.ptr;
+    bp.counters = counters;
+    bp.part_front = part + L.off_front;
+    bp.part_pairs = part + L.off_pairs;
+    bp.nf = mesh->nf; bp.W = W; bp.H = H;
+    bp.tiles_x = tiles_x; bp.ts_log2 = tsl; bp.cap = (uint32_t)ctx->bin_cap;
+    bp.wind_cw = args->wind_clockwise != 0;
+
+    RasterParams& rp = L.rp;
+    std::memset(&rp, 0, sizeof rp);
+    rp.rec = bp.rec;
+    rp.tile_count = bp.tile_count;
+    rp.bins = bp.bins;
+    rp.vis = (unsigned long long*)ctx->vis.ptr;
+    rp.counters = counters;
+    rp.part_frag = part + L.off_frag;
+    rp.nf = mesh->nf; rp.W = W; rp.H = H;
+    rp.tiles_x = tiles_x; rp.ntiles = ntiles; rp.ts_log2 = tsl; rp.cap = bp.cap;
+
+    ShadeParams& sp = L.sp;
+    sp.vis = rp.vis;
+    sp.rec = bp.rec;
+    sp.fn = mesh->fn; sp.fuv = mesh->fuv; sp.fmat = mesh->fmat;
+    sp.cnormal = vp.cnormal;
+    sp.uvs = mesh->uvs;
+    sp.mats = mesh->mats; sp.texels = mesh->texels; sp.luts = mesh->luts;
+    sp.part_cov = part + L.off_cov;
+    sp.W = W; sp.H = H;
+    sp.cleared = cleared;
+    sp.wind_cw = args->wind_clockwise != 0;
+    sp.flat = args->flat != 0;
+    sp.nl = nl;
+    sp.nmat = mesh->nm;
+    return RZ_OK;
+}
+
+void bind_outputs(FrameLaunch& L, bool cleared, uint8_t* frame_d, float* depth_d, int32_t* ids_d) {
+    L.rp.depth_in = cleared ? nullptr : depth_d;
+    L.sp.frame = frame_d;
+    L.sp.depth = depth_d;
+    L.sp.ids = ids_d;
+}
+
+rz_status enqueue_frame(rz_ctx* ctx, const FrameLaunch& L, hipStream_t s, bool prof) {
+    if (prof) RZ_HIP(ctx, hipEventRecord(ctx->ev[0], s));
+    RZ_HIP(ctx, launch_vertex(L.vp, s));
+    if (prof) RZ_HIP(ctx, hipEventRecord(ctx->ev[1], s));
+    RZ_HIP(ctx, launch_bin(L.bp, s));
+    if (prof) RZ_HIP(ctx, hipEventRecord(ctx->ev[2], s));
+    RZ_HIP(ctx, launch_raster(L.rp, L.tile_size, s));
+    if (prof) RZ_HIP(ctx, hipEventRecord(ctx->ev[3], s));
+    RasterParams orp = L.rp;   // the overflow pass keeps its own statistics section
+    orp.part_frag = L.part + L.off_ovf;
+    RZ_HIP(ctx, launch_overflow(orp, L.ts_log2, s));
+    if (prof) RZ_HIP(ctx, hipEventRecord(ctx->ev[4], s));
+    RZ_HIP(ctx, launch_shade(L.sp, s));
+    if (prof) RZ_HIP(ctx, hipEventRecord(ctx->ev[5], s));
     return RZ_OK;
 }
 
@@ -215,8 +376,9 @@ void rz_destroy(rz_ctx* ctx) {
     if (!ctx) return;
     (void)hipSetDevice(ctx->device);
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
-    DevBuf* bufs[] = {&ctx->raster, &ctx->cnormal, &ctx->rec, &ctx->tile_count, &ctx->bins,
-                      &ctx->vis,    &ctx->frame,   &ctx->depth, &ctx->ids,     &ctx->counters};
+    DevBuf* bufs[] = {&ctx->raster, &ctx->cnormal, &ctx->rec,   &ctx->tile_count, &ctx->bins,
+                      &ctx->vis,    &ctx->frame,   &ctx->depth, &ctx->ids,        &ctx->counters,
+                      &ctx->partials};
     for (DevBuf* b : bufs)
         if (b->ptr) (void)hipFree(b->ptr);
     for (hipEvent_t e : ctx->ev)
@@ -255,6 +417,39 @@ void* rz_stream(rz_ctx* ctx) { return ctx ? (void*)ctx->stream : nullptr; }
 rz_status rz_sync(rz_ctx* ctx) {
     if (!ctx) return RZ_ERR_INVALID_ARG;
     RZ_HIP(ctx, hipSetDevice(ctx->device));
+    RZ_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return RZ_OK;
+}
+
+rz_status rz_device_alloc(rz_ctx* ctx, size_t bytes, void** out) {
+    if (!ctx || !out) return RZ_ERR_INVALID_ARG;
+    *out = nullptr;
+    RZ_HIP(ctx, hipSetDevice(ctx->device));
+    RZ_HIP(ctx, hipMalloc(out, bytes ? bytes : 1));
+    return RZ_OK;
+}
+
+rz_status rz_device_free(rz_ctx* ctx, void* ptr) {
+    if (!ctx) return RZ_ERR_INVALID_ARG;
+    if (!ptr) return RZ_OK;
+    RZ_HIP(ctx, hipSetDevice(ctx->device));
+    RZ_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    RZ_HIP(ctx, hipFree(ptr));
+    return RZ_OK;
+}
+
+rz_status rz_copy_to_device(rz_ctx* ctx, void* dst, const void* src, size_t bytes) {
+    if (!ctx || (bytes && (!dst || !src))) return RZ_ERR_INVALID_ARG;
+    RZ_HIP(ctx, hipSetDevice(ctx->device));
+    RZ_HIP(ctx, hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream));
+    RZ_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return RZ_OK;
+}
+
+rz_status rz_copy_to_host(rz_ctx* ctx, void* dst, const void* src, size_t bytes) {
+    if (!ctx || (bytes && (!dst || !src))) return RZ_ERR_INVALID_ARG;
+    RZ_HIP(ctx, hipSetDevice(ctx->device));
+    RZ_HIP(ctx, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
     RZ_HIP(ctx, hipStreamSynchronize(ctx->stream));
     return RZ_OK;
 }
@@ -384,49 +579,17 @@ rz_status rz_draw_frame(rz_ctx* ctx, const rz_mesh* mesh, rz_light* lights, int3
                         uint8_t* frame_buffer, float* depth_buffer, int32_t* id_buffer, uint32_t flags,
                         rz_frame_stats* stats) {
     if (!ctx) return RZ_ERR_INVALID_ARG;
-    if (!mesh || !args || !frame_buffer || !depth_buffer)
-        return fail(ctx, RZ_ERR_INVALID_ARG, "null mesh, args or buffer");
-    if (nl < 0 || nl > kMaxLights || (nl > 0 && !lights))
-        return fail(ctx, RZ_ERR_INVALID_ARG, "light count %d not in [0,%d]", nl, kMaxLights);
-    const uint32_t W = args->image_width, H = args->image_height;
-    if (W == 0 || H == 0 || W > kMaxImageSide || H > kMaxImageSide)
-        return fail(ctx, RZ_ERR_INVALID_ARG, "image %ux%u not in [1,%u]^2", W, H, kMaxImageSide);
-    if (args->flat && mesh->raster_space)
-        return fail(ctx, RZ_ERR_INVALID_ARG, "flat shading needs model-space positions");
-    if (mesh->device != ctx->device) return fail(ctx, RZ_ERR_INVALID_ARG, "mesh lives on another device");
     const bool cleared = (flags & RZ_FLAG_CLEARED) != 0;
     const bool on_device = (flags & RZ_FLAG_DEVICE_BUFFERS) != 0;
-    RZ_HIP(ctx, hipSetDevice(ctx->device));
-
-    rz_matrices mats;
-    build_matrices(*args, mats);
-    ShadeParams sp;   // carries the lights; passed by value to k_shade
-    std::memset(&sp, 0, sizeof sp);
-    for (int32_t l = 0; l < nl; ++l) {
-        light_direction(mats.view, lights[l].direction, lights[l].trans_dir);   // drawing.cpp:238
-        for (int c = 0; c < 3; ++c) {
-            sp.light_ic[l][c] = lights[l].intensity * lights[l].colour[c];
-            sp.light_nd[l][c] = -lights[l].trans_dir[c];
-        }
-    }
-
-    const int ts = ctx->tile_size, tsl = log2_exact(ts);
-    const int tiles_x = (int)((W + ts - 1) / ts), tiles_y = (int)((H + ts - 1) / ts);
-    const int ntiles = tiles_x * tiles_y;
-    const size_t np = (size_t)W * H;
-    const int32_t nfn = args->flat ? mesh->nf : mesh->nn;
-    rz_status st;
-    if ((st = ensure(ctx, ctx->raster, (size_t)mesh->nv * 16)) != RZ_OK) return st;
-    if ((st = ensure(ctx, ctx->cnormal, (size_t)nfn * 16)) != RZ_OK) return st;
-    if ((st = ensure(ctx, ctx->rec, (size_t)mesh->nf * sizeof(FaceRecord))) != RZ_OK) return st;
-    if ((st = ensure(ctx, ctx->tile_count, (size_t)ntiles * 4)) != RZ_OK) return st;
-    if ((st = ensure(ctx, ctx->bins, (size_t)ntiles * ctx->bin_cap * 4)) != RZ_OK) return st;
-    if ((st = ensure(ctx, ctx->vis, np * 8)) != RZ_OK) return st;
-    if ((st = ensure(ctx, ctx->counters, kNumCounters * 8)) != RZ_OK) return st;
-
+    FrameLaunch L;
+    rz_status st = prepare_frame(ctx, mesh, lights, nl, args, flags, L);
+    if (st != RZ_OK) return st;
+    if (!frame_buffer || !depth_buffer) return fail(ctx, RZ_ERR_INVALID_ARG, "null frame or depth buffer");
+    const size_t np = L.np;
     uint8_t* frame_d = frame_buffer;
     float* depth_d = depth_buffer;
     int32_t* ids_d = id_buffer;
+    hipStream_t s = ctx->stream;
     if (!on_device) {
         if ((st = ensure(ctx, ctx->frame, np * 3)) != RZ_OK) return st;
         if ((st = ensure(ctx, ctx->depth, np * 4)) != RZ_OK) return st;
@@ -437,97 +600,38 @@ rz_status rz_draw_frame(rz_ctx* ctx, const rz_mesh* mesh, rz_light* lights, int3
             ids_d = (int32_t*)ctx->ids.ptr;
         }
         if (!cleared) {
-            RZ_HIP(ctx, hipMemcpyAsync(frame_d, frame_buffer, np * 3, hipMemcpyHostToDevice, ctx->stream));
-            RZ_HIP(ctx, hipMemcpyAsync(depth_d, depth_buffer, np * 4, hipMemcpyHostToDevice, ctx->stream));
+            RZ_HIP(ctx, hipMemcpyAsync(frame_d, frame_buffer, np * 3, hipMemcpyHostToDevice, s));
+            RZ_HIP(ctx, hipMemcpyAsync(depth_d, depth_buffer, np * 4, hipMemcpyHostToDevice, s));
         }
     }
-    hipStream_t s = ctx->stream;
-    auto* counters = (unsigned long long*)ctx->counters.ptr;
-
-    VertexParams vp;
-    std::memset(&vp, 0, sizeof vp);
-    std::memcpy(vp.camera, mats.camera, 64);
-    std::memcpy(vp.normal, mats.normal, 64);
-    vp.px = mesh->px; vp.py = mesh->py; vp.pz = mesh->pz;
-    vp.raster_in = mesh->raster_space ? mesh->rin : nullptr;
-    vp.nx = args->flat ? mesh->fnx : mesh->nx;
-    vp.ny = args->flat ? mesh->fny : mesh->ny;
-    vp.nz = args->flat ? mesh->fnz : mesh->nz;
-    vp.raster = (float4*)ctx->raster.ptr;
-    vp.cnormal = (float4*)ctx->cnormal.ptr;
-    vp.tile_count = (uint32_t*)ctx->tile_count.ptr;
-    vp.counters = counters;
-    vp.nv = mesh->nv; vp.nn = nfn; vp.ntiles = ntiles;
-    vp.Wf = (float)W; vp.Hf = (float)H;
-
-    BinParams bp;
-    std::memset(&bp, 0, sizeof bp);
-    bp.fv = mesh->fv;
-    bp.raster = vp.raster;
-    bp.rec = (FaceRecord*)ctx->rec.ptr;
-    bp.tile_count = vp.tile_count;
-    bp.bins = (uint32_t*)ctx->bins.ptr;
-    bp.counters = counters;
-    bp.nf = mesh->nf; bp.W = W; bp.H = H;
-    bp.tiles_x = tiles_x; bp.ts_log2 = tsl; bp.cap = (uint32_t)ctx->bin_cap;
-    bp.wind_cw = args->wind_clockwise != 0;
-
-    RasterParams rp;
-    std::memset(&rp, 0, sizeof rp);
-    rp.rec = bp.rec;
-    rp.tile_count = bp.tile_count;
-    rp.bins = bp.bins;
-    rp.depth_in = cleared ? nullptr : depth_d;
-    rp.vis = (unsigned long long*)ctx->vis.ptr;
-    rp.counters = counters;
-    rp.nf = mesh->nf; rp.W = W; rp.H = H;
-    rp.tiles_x = tiles_x; rp.ntiles = ntiles; rp.ts_log2 = tsl; rp.cap = bp.cap;
-
-    sp.vis = rp.vis;
-    sp.rec = bp.rec;
-    sp.fn = mesh->fn; sp.fuv = mesh->fuv; sp.fmat = mesh->fmat;
-    sp.cnormal = vp.cnormal;
-    sp.uvs = mesh->uvs;
-    sp.mats = mesh->mats; sp.texels = mesh->texels; sp.luts = mesh->luts;
-    sp.frame = frame_d; sp.depth = depth_d; sp.ids = ids_d;
-    sp.counters = counters;
-    sp.W = W; sp.H = H;
-    sp.cleared = cleared;
-    sp.wind_cw = args->wind_clockwise != 0;
-    sp.flat = args->flat != 0;
-    sp.nl = nl;
-
-    const bool prof = ctx->profiling;
-    if (prof) RZ_HIP(ctx, hipEventRecord(ctx->ev[0], s));
-    RZ_HIP(ctx, launch_vertex(vp, s));
-    if (prof) RZ_HIP(ctx, hipEventRecord(ctx->ev[1], s));
-    RZ_HIP(ctx, launch_bin(bp, s));
-    if (prof) RZ_HIP(ctx, hipEventRecord(ctx->ev[2], s));
-    RZ_HIP(ctx, launch_raster(rp, ts, s));
-    if (prof) RZ_HIP(ctx, hipEventRecord(ctx->ev[3], s));
-    RZ_HIP(ctx, launch_overflow(rp, tsl, s));
-    if (prof) RZ_HIP(ctx, hipEventRecord(ctx->ev[4], s));
-    RZ_HIP(ctx, launch_shade(sp, s));
-    if (prof) RZ_HIP(ctx, hipEventRecord(ctx->ev[5], s));
-
+    bind_outputs(L, cleared, frame_d, depth_d, ids_d);
+    if ((st = enqueue_frame(ctx, L, s, ctx->profiling)) != RZ_OK) return st;
     if (!on_device) {
         RZ_HIP(ctx, hipMemcpyAsync(frame_buffer, frame_d, np * 3, hipMemcpyDeviceToHost, s));
         RZ_HIP(ctx, hipMemcpyAsync(depth_buffer, depth_d, np * 4, hipMemcpyDeviceToHost, s));
         if (id_buffer) RZ_HIP(ctx, hipMemcpyAsync(id_buffer, ids_d, np * 4, hipMemcpyDeviceToHost, s));
     }
     if (stats) {
-        RZ_HIP(ctx, hipMemcpyAsync(ctx->host_counters, counters, kNumCounters * 8, hipMemcpyDeviceToHost, s));
+        RZ_HIP(ctx, hipMemcpyAsync(ctx->host_counters, L.vp.counters, kNumCounters * 8, hipMemcpyDeviceToHost, s));
+        ctx->host_partials.resize(L.n_part);
+        RZ_HIP(ctx, hipMemcpyAsync(ctx->host_partials.data(), ctx->partials.ptr, L.n_part * 4,
+                                   hipMemcpyDeviceToHost, s));
     }
     if (stats || !on_device || !(flags & RZ_FLAG_ASYNC)) RZ_HIP(ctx, hipStreamSynchronize(s));
     if (stats) {
+        auto sum = [&](size_t a, size_t b) {
+            uint64_t t = 0;
+            for (size_t i = a; i < b; ++i) t += ctx->host_partials[i];
+            return t;
+        };
         std::memset(stats, 0, sizeof *stats);
         stats->triangles = (uint64_t)mesh->nf;
-        stats->front_triangles = ctx->host_counters[kCntFront];
-        stats->bin_pairs = ctx->host_counters[kCntPairs];
+        stats->front_triangles = sum(L.off_front, L.off_pairs);
+        stats->bin_pairs = sum(L.off_pairs, L.off_frag);
         stats->overflow_tiles = ctx->host_counters[kCntOverflowTiles];
-        stats->fragments = ctx->host_counters[kCntFragments];
-        stats->covered_pixels = ctx->host_counters[kCntCovered];
-        if (prof) {
+        stats->fragments = sum(L.off_frag, L.off_cov);
+        stats->covered_pixels = sum(L.off_cov, L.n_part);
+        if (ctx->profiling) {
             float t[5];
             for (int i = 0; i < 5; ++i) RZ_HIP(ctx, hipEventElapsedTime(&t[i], ctx->ev[i], ctx->ev[i + 1]));
             stats->ms_vertex = t[0];
@@ -539,6 +643,68 @@ rz_status rz_draw_frame(rz_ctx* ctx, const rz_mesh* mesh, rz_light* lights, int3
         }
     }
     return RZ_OK;
+}
+
+rz_status rz_graph_create(rz_ctx* ctx, const rz_mesh* mesh, rz_light* lights, int32_t nl, const rz_args* args,
+                          uint8_t* frame_d, float* depth_d, int32_t* ids_d, uint32_t flags, int32_t frames,
+                          float ry_step, rz_graph** out) {
+    if (!ctx || !out) return RZ_ERR_INVALID_ARG;
+    *out = nullptr;
+    if (!(flags & RZ_FLAG_DEVICE_BUFFERS))
+        return fail(ctx, RZ_ERR_INVALID_ARG, "a frame graph needs device buffers (RZ_FLAG_DEVICE_BUFFERS)");
+    if (frames < 1 || frames > 100000) return fail(ctx, RZ_ERR_INVALID_ARG, "frames %d not in [1, 100000]", frames);
+    if (!frame_d || !depth_d) return fail(ctx, RZ_ERR_INVALID_ARG, "null frame or depth buffer");
+    std::vector<FrameLaunch> launches(frames);
+    rz_args a = args ? *args : rz_args{};
+    for (int32_t k = 0; k < frames; ++k) {
+        rz_status st = prepare_frame(ctx, mesh, lights, nl, args ? &a : nullptr, flags, launches[k]);
+        if (st != RZ_OK) return st;
+        bind_outputs(launches[k], (flags & RZ_FLAG_CLEARED) != 0, frame_d, depth_d, ids_d);
+        a.tait_bryan_angles[1] += ry_step;   // the spin loop's rotation about y (renderer.cpp:124)
+    }
+    hipStream_t s = ctx->stream;
+    RZ_HIP(ctx, hipStreamSynchronize(s));
+    RZ_HIP(ctx, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    for (int32_t k = 0; k < frames; ++k) {
+        rz_status st = enqueue_frame(ctx, launches[k], s, false);
+        if (st != RZ_OK) {
+            hipGraph_t g = nullptr;
+            (void)hipStreamEndCapture(s, &g);
+            if (g) (void)hipGraphDestroy(g);
+            return st;
+        }
+    }
+    hipGraph_t g = nullptr;
+    RZ_HIP(ctx, hipStreamEndCapture(s, &g));
+    hipGraphExec_t exec = nullptr;
+    hipError_t e = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess) return fail(ctx, RZ_ERR_HIP, "hipGraphInstantiate: %s", hipGetErrorString(e));
+    rz_graph* gr = new rz_graph();
+    gr->ctx = ctx;
+    gr->exec = exec;
+    gr->frames = frames;
+    gr->buffer_gen = ctx->buffer_gen;
+    *out = gr;
+    return RZ_OK;
+}
+
+rz_status rz_graph_launch(rz_graph* g) {
+    if (!g) return RZ_ERR_INVALID_ARG;
+    rz_ctx* ctx = g->ctx;
+    if (g->buffer_gen != ctx->buffer_gen)
+        return fail(ctx, RZ_ERR_INVALID_ARG, "graph is stale: a later frame reallocated the context's buffers");
+    RZ_HIP(ctx, hipSetDevice(ctx->device));
+    RZ_HIP(ctx, hipGraphLaunch(g->exec, ctx->stream));
+    return RZ_OK;
+}
+
+void rz_graph_destroy(rz_graph* g) {
+    if (!g) return;
+    (void)hipSetDevice(g->ctx->device);
+    (void)hipStreamSynchronize(g->ctx->stream);
+    (void)hipGraphExecDestroy(g->exec);
+    delete g;
 }
 
 rz_status rz_draw_frame_desc(rz_ctx* ctx, const rz_mesh_desc* desc, rz_light* lights, int32_t nl,

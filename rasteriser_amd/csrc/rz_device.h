@@ -17,13 +17,13 @@ constexpr uint64_t kEmptyKey = ~0ull;           // visibility word of a pixel wi
 constexpr uint32_t kNoFace = 0xFFFFFFFFu;
 constexpr int kBigTriangleTiles = 16;           // above this many tiles a face is binned by its whole wave
 
-// Counter slots (uint64) cleared by the vertex pass each frame.
+// Global counter slots (uint64) cleared by the vertex pass each frame.  Only the
+// overflow-tile count is a device-wide atomic (it is rare and doubles as the flag
+// the overflow pass tests); every other statistic is a per-workgroup partial sum
+// written with a plain store and reduced only when a caller asks for statistics
+// (same-address device-scope atomics serialise across the eight XCDs).
 enum Counter : int {
-    kCntFront = 0,
-    kCntPairs = 1,
-    kCntOverflowTiles = 2,
-    kCntFragments = 3,
-    kCntCovered = 4,
+    kCntOverflowTiles = 0,
     kNumCounters = 8
 };
 
@@ -68,6 +68,8 @@ struct BinParams {
     uint32_t* tile_count;
     uint32_t* bins;
     unsigned long long* counters;
+    uint32_t* part_front;       // per workgroup: front faces
+    uint32_t* part_pairs;       // per workgroup: (front face, tile) pairs
     int32_t nf;
     uint32_t W, H;
     int32_t tiles_x;
@@ -83,6 +85,7 @@ struct RasterParams {
     const float* depth_in;      // null when the depth buffer is known cleared to 1.0
     unsigned long long* vis;
     unsigned long long* counters;
+    uint32_t* part_frag;        // per workgroup: fragments passing the inside test
     int32_t nf;
     uint32_t W, H;
     int32_t tiles_x, ntiles;
@@ -104,15 +107,25 @@ struct ShadeParams {
     uint8_t* frame;             // planar [3][H][W]
     float* depth;
     int32_t* ids;               // may be null
-    unsigned long long* counters;
+    uint32_t* part_cov;         // per workgroup: pixels with a winning fragment
     uint32_t W, H;
     int32_t cleared;
     int32_t wind_cw;
     int32_t flat;
     int32_t nl;
+    int32_t nmat;
     float light_ic[kMaxLights][3];   // intensity * colour per light
     float light_nd[kMaxLights][3];   // -trans_dir per light
 };
+
+// Launch shapes (shared by the launchers and the host's statistics reduction).
+inline uint32_t bin_blocks(int32_t nf) { return (uint32_t)((nf + 255) / 256); }
+inline uint32_t overflow_blocks(int32_t nf) {
+    const uint32_t b = bin_blocks(nf);
+    return b > 1024u ? 1024u : b;
+}
+inline uint32_t shade_blocks_x(uint32_t W) { return (W + 63) / 64; }
+inline uint32_t shade_blocks_y(uint32_t H) { return (H + 3) / 4; }
 
 // ---------------------------------------------------------------------------------
 // Arithmetic (host+device so host-side unit checks can call the same code).

@@ -29,12 +29,19 @@ __device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
     return v;
 }
 
-// Blocks are dealt to the 8 XCDs round robin; give each XCD a contiguous run of
-// tiles so neighbouring tiles (which share faces) hit the same L2.
-__device__ __forceinline__ int xcd_tile(int bid, int n) {
-    const int xcd = bid & 7, idx = bid >> 3;
-    const int per = n >> 3, rem = n & 7;
-    return xcd < rem ? xcd * (per + 1) + idx : rem * (per + 1) + (xcd - rem) * per + idx;
+// Sum of v over the workgroup, stored by thread 0 (no atomics).  Every thread of the
+// workgroup must call it.
+__device__ __forceinline__ void store_block_sum(uint32_t v, uint32_t* out, int tid, int nthreads) {
+    __shared__ uint32_t s_part[16];
+    v = wave_sum(v);
+    if ((tid & 63) == 0) s_part[tid >> 6] = v;
+    __syncthreads();
+    if (tid == 0) {
+        uint32_t t = 0;
+        for (int w = 0; w < (nthreads >> 6); ++w) t += s_part[w];
+        *out = t;
+    }
+    __syncthreads();
 }
 
 // ---------------------------------------------------------------------------------
@@ -112,14 +119,66 @@ __global__ __launch_bounds__(256) void k_bin(BinParams p) {
         }
     }
     const uint32_t nt = (tx0 <= tx1 && ty0 <= ty1) ? (tx1 - tx0 + 1) * (ty1 - ty0 + 1) : 0u;
-    const unsigned long long fm = __ballot(front);
-    const uint32_t pairs = wave_sum(nt);
-    if (lane_id() == 0) {
-        if (fm) atomicAdd(&p.counters[kCntFront], (unsigned long long)__popcll(fm));
-        if (pairs) atomicAdd(&p.counters[kCntPairs], (unsigned long long)pairs);
-    }
+    store_block_sum(front ? 1u : 0u, p.part_front + blockIdx.x, threadIdx.x, 256);
+    store_block_sum(nt, p.part_pairs + blockIdx.x, threadIdx.x, 256);
     const bool big = nt > (uint32_t)kBigTriangleTiles;
-    if (!big) {
+    const bool small = nt > 0 && !big;
+
+    // Small faces: aggregate the workgroup's (tile, face) pairs in LDS so that each
+    // distinct tile costs one device-scope atomic per workgroup, all issued at once.
+    // Faces of one workgroup are neighbours in the mesh, so their tiles usually fall in
+    // a small rectangle; when the union of their tile rectangles is larger than
+    // kAggTiles the pairs go to the tile counters one by one.
+    constexpr int kAggTiles = 1024;
+    __shared__ int s_rect[4];                       // union tile rect: x0 y0 x1 y1
+    __shared__ uint32_t s_cnt[kAggTiles];
+    __shared__ uint32_t s_base[kAggTiles];
+    __shared__ uint8_t s_slot[256 * kBigTriangleTiles];
+    const int tid = threadIdx.x;
+    if (tid == 0) {
+        s_rect[0] = s_rect[1] = 0x7FFFFFFF;
+        s_rect[2] = s_rect[3] = -1;
+    }
+    __syncthreads();
+    if (small) {
+        atomicMin(&s_rect[0], (int)tx0); atomicMin(&s_rect[1], (int)ty0);
+        atomicMax(&s_rect[2], (int)tx1); atomicMax(&s_rect[3], (int)ty1);
+    }
+    __syncthreads();
+    const int ux0 = s_rect[0], uy0 = s_rect[1];
+    const int uw = s_rect[2] - ux0 + 1, uh = s_rect[3] - uy0 + 1;
+    const bool agg = s_rect[2] >= 0 && uw * uh <= kAggTiles;   // uniform over the workgroup
+    if (agg) {
+        const int area = uw * uh;
+        for (int i = tid; i < area; i += 256) s_cnt[i] = 0u;
+        __syncthreads();
+        if (small) {
+            int k = 0;
+            for (uint32_t ty = ty0; ty <= ty1; ++ty)
+                for (uint32_t tx = tx0; tx <= tx1; ++tx, ++k)
+                    s_slot[tid * kBigTriangleTiles + k] =
+                        (uint8_t)atomicAdd(&s_cnt[((int)ty - uy0) * uw + ((int)tx - ux0)], 1u);
+        }
+        __syncthreads();
+        for (int i = tid; i < area; i += 256) {
+            const uint32_t c = s_cnt[i];
+            if (!c) continue;
+            const uint32_t tile = (uint32_t)(uy0 + i / uw) * p.tiles_x + (uint32_t)(ux0 + i % uw);
+            const uint32_t base = atomicAdd(&p.tile_count[tile], c);
+            s_base[i] = base;
+            if (base <= p.cap && base + c > p.cap) atomicAdd(&p.counters[kCntOverflowTiles], 1ull);
+        }
+        __syncthreads();
+        if (small) {
+            int k = 0;
+            for (uint32_t ty = ty0; ty <= ty1; ++ty)
+                for (uint32_t tx = tx0; tx <= tx1; ++tx, ++k) {
+                    const int i = ((int)ty - uy0) * uw + ((int)tx - ux0);
+                    const uint32_t slot = s_base[i] + s_slot[tid * kBigTriangleTiles + k];
+                    if (slot < p.cap) p.bins[(size_t)(ty * p.tiles_x + tx) * p.cap + slot] = (uint32_t)f;
+                }
+        }
+    } else if (small) {
         for (uint32_t ty = ty0; ty <= ty1; ++ty)
             for (uint32_t tx = tx0; tx <= tx1; ++tx) bin_push(p, (uint32_t)f, ty * p.tiles_x + tx);
     }
@@ -169,7 +228,9 @@ __global__ __launch_bounds__(BLOCK) void k_raster(RasterParams p) {
 
     const int tid = threadIdx.x;
     const int lane = tid & 63, wave = tid >> 6;
-    const int tile = xcd_tile(blockIdx.x, p.ntiles);
+    // Consecutive tiles go to different XCDs (workgroups are dealt round robin), which
+    // spreads a screen region's heavy tiles over all eight.
+    const int tile = blockIdx.x;
     const int tx = tile % p.tiles_x, ty = tile / p.tiles_x;
     const int ox = tx * TS, oy = ty * TS;
 
@@ -260,13 +321,15 @@ __global__ __launch_bounds__(BLOCK) void k_raster(RasterParams p) {
         const uint32_t x = ox + (l % TS), y = oy + (l / TS);
         if (x < p.W && y < p.H) p.vis[(size_t)y * p.W + x] = s_key[l];
     }
-    frags = wave_sum(frags);
-    if (lane == 0 && frags) atomicAdd(&p.counters[kCntFragments], (unsigned long long)frags);
+    store_block_sum(frags, p.part_frag + blockIdx.x, tid, BLOCK);
 }
 
 // ---------------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_overflow(RasterParams p, int32_t ts_log2) {
-    if (p.counters[kCntOverflowTiles] == 0ull) return;
+    if (p.counters[kCntOverflowTiles] == 0ull) {
+        if (threadIdx.x == 0) p.part_frag[blockIdx.x] = 0u;
+        return;
+    }
     const uint32_t ts = 1u << ts_log2;
     uint32_t frags = 0;
     for (int f = blockIdx.x * 256 + threadIdx.x; f < p.nf; f += gridDim.x * 256) {
@@ -296,14 +359,13 @@ __global__ __launch_bounds__(256) void k_overflow(RasterParams p, int32_t ts_log
             }
         }
     }
-    frags = wave_sum(frags);
-    if (lane_id() == 0 && frags) atomicAdd(&p.counters[kCntFragments], (unsigned long long)frags);
+    store_block_sum(frags, p.part_frag + blockIdx.x, threadIdx.x, 256);
 }
 
 // ---------------------------------------------------------------------------------
 // CImg::_linear_atXY on the normalised texture (CImg.h:13475-13491), channels 0..2.
 __device__ __forceinline__ void sample_texture(const DevMaterial& m, const uint32_t* __restrict__ texels,
-                                               const float* __restrict__ lut, float uvx, float uvy, float out[3]) {
+                                               const float* __restrict__ L, float uvx, float uvy, float out[3]) {
     const float u = uvx * (float)m.tex_w;
     const float v = (1.f - uvy) * (float)m.tex_h;
     const float wm = (float)(m.tex_w - 1), hm = (float)(m.tex_h - 1);
@@ -315,7 +377,6 @@ __device__ __forceinline__ void sample_texture(const DevMaterial& m, const uint3
     const uint32_t* t = texels + m.tex_offset;
     const uint32_t W = (uint32_t)m.tex_w;
     const uint32_t tcc = t[y * W + x], tnc = t[y * W + nx], tcn = t[ny * W + x], tnn = t[ny * W + nx];
-    const float* L = lut + m.lut_offset;
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
         const int sh = 8 * c;
@@ -327,6 +388,17 @@ __device__ __forceinline__ void sample_texture(const DevMaterial& m, const uint3
 
 // 64 x 4 pixel blocks: each wave shades 64 consecutive pixels of one row.
 __global__ __launch_bounds__(256) void k_shade(ShadeParams p) {
+    // normalisation tables of the first kLdsLuts materials, staged in LDS
+    constexpr int kLdsLuts = 4;
+    __shared__ float s_lut[kLdsLuts * 256];
+    {
+        const int t = threadIdx.y * 64 + threadIdx.x;
+        const int nl = p.nmat < kLdsLuts ? p.nmat : kLdsLuts;
+#pragma unroll
+        for (int j = 0; j < kLdsLuts; ++j)
+            if (j < nl) s_lut[j * 256 + t] = p.luts[j * 256 + t];
+        __syncthreads();
+    }
     const size_t np = (size_t)p.W * p.H;
     const uint32_t px = blockIdx.x * 64 + threadIdx.x, py = blockIdx.y * 4 + threadIdx.y;
     const bool valid = px < p.W && py < p.H;
@@ -367,7 +439,8 @@ __global__ __launch_bounds__(256) void k_shade(ShadeParams p) {
         const float rn = 1.f / sqrtf((nx * nx + ny * ny) + nz * nz);
         nx = nx * rn; ny = ny * rn; nz = nz * rn;
         if (p.wind_cw) { nx = -nx; ny = -ny; nz = -nz; }
-        const DevMaterial& m = p.mats[p.fmat[f]];
+        const int mi = p.fmat[f];
+        const DevMaterial& m = p.mats[mi];
         float alb[3];
         if (m.tex_w > 0) {
             float2 t[3];
@@ -378,7 +451,10 @@ __global__ __launch_bounds__(256) void k_shade(ShadeParams p) {
             }
             const float u = d * (i0 * t[0].x + i1 * t[1].x + i2 * t[2].x);
             const float v = d * (i0 * t[0].y + i1 * t[1].y + i2 * t[2].y);
-            sample_texture(m, p.texels, p.luts, u, v, alb);
+            if (mi < kLdsLuts)
+                sample_texture(m, p.texels, s_lut + mi * 256, u, v, alb);
+            else
+                sample_texture(m, p.texels, p.luts + m.lut_offset, u, v, alb);
         } else {
             alb[0] = m.diffuse[0]; alb[1] = m.diffuse[1]; alb[2] = m.diffuse[2];
         }
@@ -398,8 +474,8 @@ __global__ __launch_bounds__(256) void k_shade(ShadeParams p) {
         p.depth[i] = z;
         if (p.ids) p.ids[i] = (int32_t)f;
     }
-    const unsigned long long hm = __ballot(hit);
-    if (threadIdx.x == 0 && hm) atomicAdd(&p.counters[kCntCovered], (unsigned long long)__popcll(hm));
+    store_block_sum(hit ? 1u : 0u, p.part_cov + blockIdx.y * gridDim.x + blockIdx.x,
+                    threadIdx.y * 64 + threadIdx.x, 256);
 }
 
 // ---------------------------------------------------------------------------------
@@ -414,7 +490,7 @@ hipError_t launch_vertex(const VertexParams& p, hipStream_t s) {
 }
 
 hipError_t launch_bin(const BinParams& p, hipStream_t s) {
-    if (p.nf > 0) hipLaunchKernelGGL(k_bin, dim3((p.nf + 255) / 256), dim3(256), 0, s, p);
+    if (p.nf > 0) hipLaunchKernelGGL(k_bin, dim3(bin_blocks(p.nf)), dim3(256), 0, s, p);
     return hipGetLastError();
 }
 
@@ -429,16 +505,12 @@ hipError_t launch_raster(const RasterParams& p, int tile_size, hipStream_t s) {
 }
 
 hipError_t launch_overflow(const RasterParams& p, int ts_log2, hipStream_t s) {
-    if (p.nf > 0) {
-        int blocks = (p.nf + 255) / 256;
-        if (blocks > 1024) blocks = 1024;
-        hipLaunchKernelGGL(k_overflow, dim3(blocks), dim3(256), 0, s, p, ts_log2);
-    }
+    if (p.nf > 0) hipLaunchKernelGGL(k_overflow, dim3(overflow_blocks(p.nf)), dim3(256), 0, s, p, ts_log2);
     return hipGetLastError();
 }
 
 hipError_t launch_shade(const ShadeParams& p, hipStream_t s) {
-    hipLaunchKernelGGL(k_shade, dim3((p.W + 63) / 64, (p.H + 3) / 4), dim3(64, 4), 0, s, p);
+    hipLaunchKernelGGL(k_shade, dim3(shade_blocks_x(p.W), shade_blocks_y(p.H)), dim3(64, 4), 0, s, p);
     return hipGetLastError();
 }
 

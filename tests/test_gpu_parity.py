@@ -63,3 +63,50 @@ def test_repeatable(ctx):
     b = render_gpu(ctx, scene, args)
     assert np.array_equal(a[2], b[2]) and np.array_equal(a[0], b[0])
     assert np.array_equal(a[1].view(np.uint32), b[1].view(np.uint32))
+
+
+def test_frame_graph_spin(ctx):
+    """A captured 3-frame spin graph leaves the last frame, rotated by 2*ry_step."""
+    import oracle
+    from tests.parity import assert_same_frame
+    scene, args = scenes.icosphere_scene(subdivisions=4, width=640, height=480)
+    W, H = 640, 480
+    mesh = ctx.upload(scene)
+    frame = ctx.empty((3, H, W), np.uint8)
+    depth = ctx.empty((H, W), np.float32)
+    ids = ctx.empty((H, W), np.int32)
+    lights = scene.lights_copy()
+    step = 0.25
+    g = ctx.graph(mesh, lights, args, frame.ptr, depth.ptr, 3, ids_ptr=ids.ptr, ry_step=step)
+    for _ in range(2):          # replays must give the same result each time
+        frame.upload(np.full((3, H, W), 7, np.uint8))
+        g.launch()
+        ctx.sync()
+        last = np.float32(args.tait_bryan_angles[1])
+        a2 = type(args).from_buffer_copy(args)
+        a2.tait_bryan_angles[1] = float(np.float32(np.float32(last + np.float32(step)) + np.float32(step)))
+        ref = oracle.draw_frame(scene, a2)
+        gpu = (frame.download(), depth.download(), ids.download(),
+               {"fragments": ref[3]["fragments"], "front_triangles": ref[3]["front_triangles"],
+                "covered_pixels": int((ref[2] >= 0).sum())}, lights)
+        assert_same_frame(gpu, ref, "graph spin")
+    g.close()
+    mesh.close()
+
+
+def test_graph_stale_after_realloc(ctx):
+    """A frame that grows the context buffers makes an older graph refuse to launch."""
+    import rasteriser_amd as rz
+    scene, args = scenes.uv_sphere_scene(width=64, height=64)
+    mesh = ctx.upload(scene)
+    frame = ctx.empty((3, 64, 64), np.uint8)
+    depth = ctx.empty((64, 64), np.float32)
+    g = ctx.graph(mesh, scene.lights_copy(), args, frame.ptr, depth.ptr, 1)
+    g.launch()
+    ctx.sync()
+    big, bargs = scenes.uv_sphere_scene(width=2500, height=2500)
+    ctx.draw(mesh, scene.lights_copy(), bargs)
+    with pytest.raises(rz.RzError):
+        g.launch()
+    g.close()
+    mesh.close()
