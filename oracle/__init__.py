@@ -55,6 +55,8 @@ def lib():
         L.oracle_shade.argtypes = [P, P, P, C.c_int32, P]
         L.oracle_sample.argtypes = [C.POINTER(Material), C.c_float, C.c_float, P]
         L.oracle_normalized_texture.argtypes = [C.POINTER(Material), P]
+        L.oracle_linear_atXY.restype = C.c_float
+        L.oracle_linear_atXY.argtypes = [C.POINTER(Material), C.c_float, C.c_float, C.c_int]
         L.oracle_cast_u32.restype = C.c_uint32
         L.oracle_cast_u32.argtypes = [C.c_float]
         _LIB = L
@@ -184,6 +186,11 @@ def normalized_texture(texture) -> np.ndarray:
     out = np.zeros((Cn, H, W), np.float32)
     lib().oracle_normalized_texture(C.byref(m), out.ctypes.data)
     return out
+
+
+def linear_atXY(texture, fx, fy, c) -> float:
+    m, keep = _material(texture)
+    return float(lib().oracle_linear_atXY(C.byref(m), float(fx), float(fy), int(c)))
 
 
 def cast_u32(f) -> int:

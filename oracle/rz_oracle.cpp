@@ -581,6 +581,13 @@ void oracle_sample(const rz_material* m, float u, float v, float out[3]) {
     out[0] = r.x; out[1] = r.y; out[2] = r.z;
 }
 
+// CImg::_linear_atXY of channel c at raster coordinates (fx, fy) of material m's
+// normalised texture.
+float oracle_linear_atXY(const rz_material* m, float fx, float fy, int c) {
+    Texture t = make_texture(*m);
+    return linear_atXY(t, fx, fy, c);
+}
+
 // The normalised texel values of material m, planar [C][H][W].
 void oracle_normalized_texture(const rz_material* m, float* out) {
     Texture t = make_texture(*m);
