@@ -16,6 +16,12 @@ CSRC = os.path.join(HERE, "csrc")
 OUT_DIR = os.path.join(HERE, "_lib")
 LIB = os.path.join(OUT_DIR, "librz.so")
 SOURCES = ["rz_kernels.hip", "rz_api.hip", "rz_host_math.cpp"]
+HOST_DIR = os.path.join(HERE, "host")
+HOST_SOURCES = ["drawing.cpp", "fileloader.cpp", "image_io.cpp"]
+HOST_LIB = os.path.join(OUT_DIR, "librz_host.so")
+RENDERER = os.path.join(OUT_DIR, "renderer")
+# The host layer is ordinary x86-64 C++ with the same arithmetic contract.
+HOST_FLAGS = ["-std=c++17", "-O2", "-ffp-contract=off", "-fno-fast-math", "-Wall", "-Wextra"]
 ARCH = "gfx950"
 
 HIP_FLAGS = [
@@ -45,7 +51,25 @@ def build(verbose: bool = False) -> str:
     if verbose and err.strip():
         print(err, file=sys.stderr)
     os.replace(tmp, LIB)
+    build_host()
     return LIB
+
+
+def _run(cmd):
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"build failed:\n{' '.join(cmd)}\n{r.stderr}")
+
+
+def build_host() -> None:
+    """librz_host.so (draw_frame, loaders, image I/O) and the renderer command line."""
+    link = ["-L", OUT_DIR, "-Wl,-rpath,$ORIGIN", "-Wl,-rpath-link,/opt/rocm/lib"]
+    _run(["g++"] + HOST_FLAGS + ["-fPIC", "-shared", "-o", HOST_LIB + ".tmp"]
+         + [os.path.join(HOST_DIR, s) for s in HOST_SOURCES] + link + ["-lrz", "-lz"])
+    os.replace(HOST_LIB + ".tmp", HOST_LIB)
+    _run(["g++"] + HOST_FLAGS + ["-o", RENDERER + ".tmp", os.path.join(HOST_DIR, "renderer.cpp")]
+         + link + ["-lrz_host", "-lrz", "-lz"])
+    os.replace(RENDERER + ".tmp", RENDERER)
 
 
 if __name__ == "__main__":
