@@ -219,6 +219,40 @@ rz_status rz_graph_create(rz_ctx* ctx, const rz_mesh* mesh, rz_light* lights, in
 rz_status rz_graph_launch(rz_graph* graph);
 void rz_graph_destroy(rz_graph* graph);
 
+/* ---- batches ----------------------------------------------------------------- */
+
+/* One mesh of a batch: its own image size, transform and lights (args), and its own
+ * device frame/depth/id buffers. */
+typedef struct rz_batch_item {
+    const rz_mesh* mesh;
+    rz_light* lights;           /* trans_dir is written back, as for rz_draw_frame */
+    int32_t num_lights;
+    int32_t reserved;
+    rz_args args;
+    uint8_t* frame;             /* device, uint8 [3][H][W] */
+    float* depth;               /* device, float [H][W] */
+    int32_t* ids;               /* device, int32 [H][W], or NULL */
+} rz_batch_item;
+
+/* No reference counterpart (the reference draws one mesh per call).  Draws every
+ * item as rz_draw_frame would, with each pass of every item in one launch (5
+ * launches per batch).  Requires RZ_FLAG_DEVICE_BUFFERS; RZ_FLAG_CLEARED and
+ * RZ_FLAG_ASYNC apply to all items.  stats (may be NULL) sums over the items. */
+rz_status rz_draw_batch(rz_ctx* ctx, const rz_batch_item* items, int32_t num_items, uint32_t flags,
+                        rz_frame_stats* stats);
+/* `frames` repetitions of the batch captured in one HIP graph that owns its working
+ * buffers and parameter tables (independent of later calls on the context); the
+ * meshes and output buffers must outlive it. */
+rz_status rz_batch_graph_create(rz_ctx* ctx, const rz_batch_item* items, int32_t num_items, uint32_t flags,
+                                int32_t frames, rz_graph** out_graph);
+/* Chooses the tile size (8, 16, 32) and bin capacity for this workload by measuring:
+ * one probe frame per tile size gives the largest per-tile face count, the capacity
+ * is the next power of two that holds it (within a 4 GiB bin budget), and the tile
+ * size with the fastest timed frames wins.  The choice is stored in the context
+ * (as rz_set_tuning would) and returned. */
+rz_status rz_autotune(rz_ctx* ctx, const rz_batch_item* items, int32_t num_items, uint32_t flags,
+                      int32_t* tile_size, int32_t* bin_capacity, float* ms_per_frame);
+
 /* ---- device memory ------------------------------------------------------------ */
 
 /* Plain device allocations on the context's device, for callers that keep frame and

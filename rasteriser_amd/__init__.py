@@ -12,8 +12,8 @@ import os
 import numpy as np
 
 from . import abi
-from .abi import (RZ_FLAG_ASYNC, RZ_FLAG_CLEARED, RZ_FLAG_DEVICE_BUFFERS, Args, FrameStats,
-                  Matrices, make_args)
+from .abi import (RZ_FLAG_ASYNC, RZ_FLAG_CLEARED, RZ_FLAG_DEVICE_BUFFERS, Args, BatchItem,
+                  FrameStats, Matrices, make_args)
 from .scene import MaterialSpec, Scene, make_faces, make_lights
 
 __all__ = ["Context", "Mesh", "RzError", "Scene", "MaterialSpec", "make_faces", "make_lights",
@@ -238,6 +238,47 @@ class Context:
                                         flags | RZ_FLAG_DEVICE_BUFFERS, int(frames), float(ry_step),
                                         C.byref(h)), self.handle)
         return FrameGraph(self, h, frames)
+
+    def _batch_items(self, batch):
+        """batch: list of (Mesh, lights (L,10) float32, Args, frame DeviceArray, depth DeviceArray,
+        ids DeviceArray or None)."""
+        arr = (BatchItem * len(batch))()
+        keep = []
+        for i, (mesh, lights, args, frame, depth, ids) in enumerate(batch):
+            lights = _lights(lights)
+            keep.append(lights)
+            arr[i].mesh = mesh.handle.value
+            arr[i].lights = lights.ctypes.data
+            arr[i].num_lights = lights.shape[0]
+            arr[i].args = args
+            arr[i].frame = frame.ptr
+            arr[i].depth = depth.ptr
+            arr[i].ids = ids.ptr if ids is not None else None
+        return arr, keep
+
+    def draw_batch(self, batch, flags: int = RZ_FLAG_CLEARED, stats: bool = False):
+        """rz_draw_batch over device buffers; returns the summed statistics or None."""
+        arr, keep = self._batch_items(batch)
+        st = FrameStats() if stats else None
+        _check(self.lib.rz_draw_batch(self.handle, C.addressof(arr), len(batch), flags | RZ_FLAG_DEVICE_BUFFERS,
+                                      C.byref(st) if stats else None), self.handle)
+        return st.as_dict() if stats else None
+
+    def batch_graph(self, batch, frames: int, flags: int = RZ_FLAG_CLEARED):
+        arr, keep = self._batch_items(batch)
+        h = C.c_void_p()
+        _check(self.lib.rz_batch_graph_create(self.handle, C.addressof(arr), len(batch),
+                                              flags | RZ_FLAG_DEVICE_BUFFERS, int(frames), C.byref(h)),
+               self.handle)
+        return FrameGraph(self, h, frames)
+
+    def autotune(self, batch, flags: int = RZ_FLAG_CLEARED):
+        """rz_autotune: returns (tile_size, bin_capacity, ms per frame); the context keeps them."""
+        arr, keep = self._batch_items(batch)
+        t, c, ms = C.c_int32(), C.c_int32(), C.c_float()
+        _check(self.lib.rz_autotune(self.handle, C.addressof(arr), len(batch), flags | RZ_FLAG_DEVICE_BUFFERS,
+                                    C.byref(t), C.byref(c), C.byref(ms)), self.handle)
+        return t.value, c.value, ms.value
 
     def draw_scene(self, scene: Scene, args: Args, frame=None, depth=None):
         """rz_draw_frame_desc: upload, draw and free in one call (draw_frame's by-value form)."""

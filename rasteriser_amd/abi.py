@@ -54,6 +54,12 @@ class MeshDesc(C.Structure):
                 ("num_materials", C.c_int32), ("raster_space", C.c_int32)]
 
 
+class BatchItem(C.Structure):
+    _fields_ = [("mesh", C.c_void_p), ("lights", C.c_void_p), ("num_lights", C.c_int32),
+                ("reserved", C.c_int32), ("args", Args), ("frame", C.c_void_p), ("depth", C.c_void_p),
+                ("ids", C.c_void_p)]
+
+
 class Matrices(C.Structure):
     _fields_ = [("model", C.c_float * 16), ("view", C.c_float * 16),
                 ("modelview", C.c_float * 16), ("camera", C.c_float * 16),
@@ -116,6 +122,10 @@ def declare(lib):
                                       C.c_int32, C.c_float, C.POINTER(P)]),
         "rz_graph_launch": (C.c_int, [P]),
         "rz_graph_destroy": (None, [P]),
+        "rz_draw_batch": (C.c_int, [P, P, C.c_int32, C.c_uint32, C.POINTER(FrameStats)]),
+        "rz_batch_graph_create": (C.c_int, [P, P, C.c_int32, C.c_uint32, C.c_int32, C.POINTER(P)]),
+        "rz_autotune": (C.c_int, [P, P, C.c_int32, C.c_uint32, C.POINTER(C.c_int32), C.POINTER(C.c_int32),
+                                  C.POINTER(C.c_float)]),
         "rz_device_alloc": (C.c_int, [P, C.c_size_t, C.POINTER(P)]),
         "rz_device_free": (C.c_int, [P, P]),
         "rz_copy_to_device": (C.c_int, [P, P, P, C.c_size_t]),
@@ -134,5 +144,6 @@ EXPORTED = ("rz_abi_version", "rz_status_string", "rz_last_error", "rz_create", 
             "rz_set_tuning", "rz_get_tuning", "rz_set_profiling", "rz_stream", "rz_sync",
             "rz_mesh_create", "rz_mesh_destroy", "rz_draw_frame", "rz_draw_frame_desc",
             "rz_graph_create", "rz_graph_launch", "rz_graph_destroy",
+            "rz_draw_batch", "rz_batch_graph_create", "rz_autotune",
             "rz_device_alloc", "rz_device_free", "rz_copy_to_device", "rz_copy_to_host",
             "rz_default_args", "rz_frame_matrices")

@@ -17,8 +17,14 @@ namespace rz {
 hipError_t launch_vertex(const VertexParams& p, hipStream_t s);
 hipError_t launch_bin(const BinParams& p, hipStream_t s);
 hipError_t launch_raster(const RasterParams& p, int tile_size, hipStream_t s);
-hipError_t launch_overflow(const RasterParams& p, int ts_log2, hipStream_t s);
+hipError_t launch_overflow(const RasterParams& p, hipStream_t s);
 hipError_t launch_shade(const ShadeParams& p, hipStream_t s);
+hipError_t launch_vertex_batch(const VertexParams* items, const uint2* map, uint32_t blocks, hipStream_t s);
+hipError_t launch_bin_batch(const BinParams* items, const uint2* map, uint32_t blocks, hipStream_t s);
+hipError_t launch_raster_batch(const RasterParams* items, const uint2* map, uint32_t blocks, int tile_size,
+                               hipStream_t s);
+hipError_t launch_overflow_batch(const RasterParams* items, const uint2* map, uint32_t blocks, hipStream_t s);
+hipError_t launch_shade_batch(const ShadeParams* items, const uint2* map, uint32_t blocks, hipStream_t s);
 }  // namespace rz
 
 using namespace rz;
@@ -54,7 +60,9 @@ struct rz_graph {
     rz_ctx* ctx = nullptr;
     hipGraphExec_t exec = nullptr;
     int32_t frames = 0;
-    uint64_t buffer_gen = 0;   // ctx->buffer_gen when captured
+    uint64_t buffer_gen = 0;   // ctx->buffer_gen when captured (graphs over the context's buffers)
+    bool owns_buffers = false; // batch graphs carry their own working buffers and tables
+    DevBuf arena, tables;
 };
 
 struct rz_ctx {
@@ -66,6 +74,7 @@ struct rz_ctx {
     hipEvent_t ev[6] = {};
     std::string err;
     DevBuf raster, cnormal, rec, tile_count, bins, vis, frame, depth, ids, counters, partials;
+    DevBuf batch_arena, batch_tables;
     std::vector<uint32_t> host_partials;
     unsigned long long* host_counters = nullptr;   // pinned
     uint64_t buffer_gen = 0;                       // bumped whenever a per-frame buffer is reallocated
@@ -158,25 +167,61 @@ rz_status validate_mesh(rz_ctx* ctx, const rz_mesh_desc* d) {
 }
 
 
+// Sizes and statistics sections of one frame's working buffers.
+struct FrameLayout {
+    int ts = 16, tsl = 4, tiles_x = 0, tiles_y = 0, ntiles = 0;
+    uint32_t W = 0, H = 0, cap = 0;
+    size_t np = 0;
+    int32_t nfn = 0;
+    // per-workgroup statistics sections (uint32 each)
+    size_t off_front = 0, off_pairs = 0, off_frag = 0, off_ovf = 0, off_cov = 0, n_part = 0;
+    size_t bytes[8] = {};   // raster, cnormal, rec, tile_count, bins, vis, counters, partials
+};
+
+enum { kBufRaster, kBufCnormal, kBufRec, kBufTiles, kBufBins, kBufVis, kBufCounters, kBufPart, kNumBufs };
+
+FrameLayout layout_frame(const rz_mesh* mesh, const rz_args& a, int ts, int cap) {
+    FrameLayout L;
+    L.ts = ts;
+    L.tsl = log2_exact(ts);
+    L.W = a.image_width;
+    L.H = a.image_height;
+    L.cap = (uint32_t)cap;
+    L.tiles_x = (int)((L.W + ts - 1) / ts);
+    L.tiles_y = (int)((L.H + ts - 1) / ts);
+    L.ntiles = L.tiles_x * L.tiles_y;
+    L.np = (size_t)L.W * L.H;
+    L.nfn = a.flat ? mesh->nf : mesh->nn;
+    L.off_front = 0;
+    L.off_pairs = L.off_front + bin_blocks(mesh->nf);
+    L.off_frag = L.off_pairs + bin_blocks(mesh->nf);
+    L.off_ovf = L.off_frag + (size_t)L.ntiles;
+    L.off_cov = L.off_ovf + overflow_blocks(mesh->nf);
+    L.n_part = L.off_cov + (size_t)shade_blocks_x(L.W) * shade_blocks_y(L.H);
+    L.bytes[kBufRaster] = (size_t)mesh->nv * 16;
+    L.bytes[kBufCnormal] = (size_t)L.nfn * 16;
+    L.bytes[kBufRec] = (size_t)mesh->nf * sizeof(FaceRecord);
+    L.bytes[kBufTiles] = (size_t)L.ntiles * 4;
+    L.bytes[kBufBins] = (size_t)L.ntiles * L.cap * 4;
+    L.bytes[kBufVis] = L.np * 8;
+    L.bytes[kBufCounters] = kNumCounters * 8;
+    L.bytes[kBufPart] = L.n_part * 4;
+    return L;
+}
+
 // One frame's kernel parameters, built on the host before anything is enqueued.
 struct FrameLaunch {
     VertexParams vp;
     BinParams bp;
     RasterParams rp;
+    RasterParams orp;   // the overflow pass: rp with its own statistics section
     ShadeParams sp;
-    int tile_size = 16;
-    int ts_log2 = 4;
-    size_t np = 0;
-    // sections of the per-workgroup statistics buffer (uint32 each)
-    size_t off_front = 0, off_pairs = 0, off_frag = 0, off_ovf = 0, off_cov = 0, n_part = 0;
+    FrameLayout lay;
     uint32_t* part = nullptr;
+    unsigned long long* counters = nullptr;
 };
 
-// Validates, builds matrices and light directions (written back to `lights`, as
-// drawing.cpp:238 does), sizes the context's per-frame buffers and fills every pass's
-// parameters except the output buffers.
-rz_status prepare_frame(rz_ctx* ctx, const rz_mesh* mesh, rz_light* lights, int32_t nl, const rz_args* args,
-                        uint32_t flags, FrameLaunch& L) {
+rz_status check_frame(rz_ctx* ctx, const rz_mesh* mesh, const rz_light* lights, int32_t nl, const rz_args* args) {
     if (!mesh || !args) return fail(ctx, RZ_ERR_INVALID_ARG, "null mesh or args");
     if (nl < 0 || nl > kMaxLights || (nl > 0 && !lights))
         return fail(ctx, RZ_ERR_INVALID_ARG, "light count %d not in [0,%d]", nl, kMaxLights);
@@ -186,11 +231,17 @@ rz_status prepare_frame(rz_ctx* ctx, const rz_mesh* mesh, rz_light* lights, int3
     if (args->flat && mesh->raster_space)
         return fail(ctx, RZ_ERR_INVALID_ARG, "flat shading needs model-space positions");
     if (mesh->device != ctx->device) return fail(ctx, RZ_ERR_INVALID_ARG, "mesh lives on another device");
-    RZ_HIP(ctx, hipSetDevice(ctx->device));
-    const bool cleared = (flags & RZ_FLAG_CLEARED) != 0;
+    return RZ_OK;
+}
 
+// Builds matrices and light directions (written back to `lights`, as drawing.cpp:238
+// does) and fills every pass's parameters over the working buffers `buf`.
+void fill_frame(const rz_mesh* mesh, rz_light* lights, int32_t nl, const rz_args& a, uint32_t flags,
+                const FrameLayout& lay, void* const buf[kNumBufs], FrameLaunch& L) {
+    const bool cleared = (flags & RZ_FLAG_CLEARED) != 0;
     rz_matrices mats;
-    build_matrices(*args, mats);
+    build_matrices(a, mats);
+    L.lay = lay;
     std::memset(&L.sp, 0, sizeof L.sp);
     for (int32_t l = 0; l < nl; ++l) {
         light_direction(mats.view, lights[l].direction, lights[l].trans_dir);   // drawing.cpp:238
@@ -199,32 +250,10 @@ rz_status prepare_frame(rz_ctx* ctx, const rz_mesh* mesh, rz_light* lights, int3
             L.sp.light_nd[l][c] = -lights[l].trans_dir[c];
         }
     }
-    const int ts = ctx->tile_size, tsl = log2_exact(ts);
-    const int tiles_x = (int)((W + ts - 1) / ts), tiles_y = (int)((H + ts - 1) / ts);
-    const int ntiles = tiles_x * tiles_y;
-    const size_t np = (size_t)W * H;
-    const int32_t nfn = args->flat ? mesh->nf : mesh->nn;
-    rz_status st;
-    if ((st = ensure(ctx, ctx->raster, (size_t)mesh->nv * 16)) != RZ_OK) return st;
-    if ((st = ensure(ctx, ctx->cnormal, (size_t)nfn * 16)) != RZ_OK) return st;
-    if ((st = ensure(ctx, ctx->rec, (size_t)mesh->nf * sizeof(FaceRecord))) != RZ_OK) return st;
-    if ((st = ensure(ctx, ctx->tile_count, (size_t)ntiles * 4)) != RZ_OK) return st;
-    if ((st = ensure(ctx, ctx->bins, (size_t)ntiles * ctx->bin_cap * 4)) != RZ_OK) return st;
-    if ((st = ensure(ctx, ctx->vis, np * 8)) != RZ_OK) return st;
-    if ((st = ensure(ctx, ctx->counters, kNumCounters * 8)) != RZ_OK) return st;
-    L.off_front = 0;
-    L.off_pairs = L.off_front + bin_blocks(mesh->nf);
-    L.off_frag = L.off_pairs + bin_blocks(mesh->nf);
-    L.off_ovf = L.off_frag + (size_t)ntiles;
-    L.off_cov = L.off_ovf + overflow_blocks(mesh->nf);
-    L.n_part = L.off_cov + (size_t)shade_blocks_x(W) * shade_blocks_y(H);
-    if ((st = ensure(ctx, ctx->partials, L.n_part * 4)) != RZ_OK) return st;
-    auto* counters = (unsigned long long*)ctx->counters.ptr;
-    auto* part = (uint32_t*)ctx->partials.ptr;
+    auto* counters = (unsigned long long*)buf[kBufCounters];
+    auto* part = (uint32_t*)buf[kBufPart];
     L.part = part;
-    L.tile_size = ts;
-    L.ts_log2 = tsl;
-    L.np = np;
+    L.counters = counters;
 
     VertexParams& vp = L.vp;
     std::memset(&vp, 0, sizeof vp);
@@ -232,40 +261,42 @@ rz_status prepare_frame(rz_ctx* ctx, const rz_mesh* mesh, rz_light* lights, int3
     std::memcpy(vp.normal, mats.normal, 64);
     vp.px = mesh->px; vp.py = mesh->py; vp.pz = mesh->pz;
     vp.raster_in = mesh->raster_space ? mesh->rin : nullptr;
-    vp.nx = args->flat ? mesh->fnx : mesh->nx;
-    vp.ny = args->flat ? mesh->fny : mesh->ny;
-    vp.nz = args->flat ? mesh->fnz : mesh->nz;
-    vp.raster = (float4*)ctx->raster.ptr;
-    vp.cnormal = (float4*)ctx->cnormal.ptr;
-    vp.tile_count = (uint32_t*)ctx->tile_count.ptr;
+    vp.nx = a.flat ? mesh->fnx : mesh->nx;
+    vp.ny = a.flat ? mesh->fny : mesh->ny;
+    vp.nz = a.flat ? mesh->fnz : mesh->nz;
+    vp.raster = (float4*)buf[kBufRaster];
+    vp.cnormal = (float4*)buf[kBufCnormal];
+    vp.tile_count = (uint32_t*)buf[kBufTiles];
     vp.counters = counters;
-    vp.nv = mesh->nv; vp.nn = nfn; vp.ntiles = ntiles;
-    vp.Wf = (float)W; vp.Hf = (float)H;
+    vp.nv = mesh->nv; vp.nn = lay.nfn; vp.ntiles = lay.ntiles;
+    vp.Wf = (float)lay.W; vp.Hf = (float)lay.H;
 
     BinParams& bp = L.bp;
     std::memset(&bp, 0, sizeof bp);
     bp.fv = mesh->fv;
     bp.raster = vp.raster;
-    bp.rec = (FaceRecord*)ctx->rec.ptr;
+    bp.rec = (FaceRecord*)buf[kBufRec];
     bp.tile_count = vp.tile_count;
-    bp.bins = (uint32_t*)ctx->bins.ptr;
+    bp.bins = (uint32_t*)buf[kBufBins];
     bp.counters = counters;
-    bp.part_front = part + L.off_front;
-    bp.part_pairs = part + L.off_pairs;
-    bp.nf = mesh->nf; bp.W = W; bp.H = H;
-    bp.tiles_x = tiles_x; bp.ts_log2 = tsl; bp.cap = (uint32_t)ctx->bin_cap;
-    bp.wind_cw = args->wind_clockwise != 0;
+    bp.part_front = part + lay.off_front;
+    bp.part_pairs = part + lay.off_pairs;
+    bp.nf = mesh->nf; bp.W = lay.W; bp.H = lay.H;
+    bp.tiles_x = lay.tiles_x; bp.ts_log2 = lay.tsl; bp.cap = lay.cap;
+    bp.wind_cw = a.wind_clockwise != 0;
 
     RasterParams& rp = L.rp;
     std::memset(&rp, 0, sizeof rp);
     rp.rec = bp.rec;
     rp.tile_count = bp.tile_count;
     rp.bins = bp.bins;
-    rp.vis = (unsigned long long*)ctx->vis.ptr;
+    rp.vis = (unsigned long long*)buf[kBufVis];
     rp.counters = counters;
-    rp.part_frag = part + L.off_frag;
-    rp.nf = mesh->nf; rp.W = W; rp.H = H;
-    rp.tiles_x = tiles_x; rp.ntiles = ntiles; rp.ts_log2 = tsl; rp.cap = bp.cap;
+    rp.part_frag = part + lay.off_frag;
+    rp.nf = mesh->nf; rp.W = lay.W; rp.H = lay.H;
+    rp.tiles_x = lay.tiles_x; rp.ntiles = lay.ntiles; rp.ts_log2 = lay.tsl; rp.cap = lay.cap;
+    L.orp = rp;
+    L.orp.part_frag = part + lay.off_ovf;
 
     ShadeParams& sp = L.sp;
     sp.vis = rp.vis;
@@ -274,18 +305,36 @@ rz_status prepare_frame(rz_ctx* ctx, const rz_mesh* mesh, rz_light* lights, int3
     sp.cnormal = vp.cnormal;
     sp.uvs = mesh->uvs;
     sp.mats = mesh->mats; sp.texels = mesh->texels; sp.luts = mesh->luts;
-    sp.part_cov = part + L.off_cov;
-    sp.W = W; sp.H = H;
+    sp.part_cov = part + lay.off_cov;
+    sp.W = lay.W; sp.H = lay.H;
     sp.cleared = cleared;
-    sp.wind_cw = args->wind_clockwise != 0;
-    sp.flat = args->flat != 0;
+    sp.wind_cw = a.wind_clockwise != 0;
+    sp.flat = a.flat != 0;
     sp.nl = nl;
     sp.nmat = mesh->nm;
+}
+
+// Single frame: validates, sizes the context's working buffers and fills L.
+rz_status prepare_frame(rz_ctx* ctx, const rz_mesh* mesh, rz_light* lights, int32_t nl, const rz_args* args,
+                        uint32_t flags, FrameLaunch& L) {
+    rz_status st = check_frame(ctx, mesh, lights, nl, args);
+    if (st != RZ_OK) return st;
+    RZ_HIP(ctx, hipSetDevice(ctx->device));
+    const FrameLayout lay = layout_frame(mesh, *args, ctx->tile_size, ctx->bin_cap);
+    DevBuf* bufs[kNumBufs] = {&ctx->raster, &ctx->cnormal, &ctx->rec, &ctx->tile_count,
+                              &ctx->bins,   &ctx->vis,     &ctx->counters, &ctx->partials};
+    void* ptrs[kNumBufs];
+    for (int i = 0; i < kNumBufs; ++i) {
+        if ((st = ensure(ctx, *bufs[i], lay.bytes[i])) != RZ_OK) return st;
+    }
+    for (int i = 0; i < kNumBufs; ++i) ptrs[i] = bufs[i]->ptr;
+    fill_frame(mesh, lights, nl, *args, flags, lay, ptrs, L);
     return RZ_OK;
 }
 
 void bind_outputs(FrameLaunch& L, bool cleared, uint8_t* frame_d, float* depth_d, int32_t* ids_d) {
     L.rp.depth_in = cleared ? nullptr : depth_d;
+    L.orp.depth_in = L.rp.depth_in;
     L.sp.frame = frame_d;
     L.sp.depth = depth_d;
     L.sp.ids = ids_d;
@@ -297,14 +346,200 @@ rz_status enqueue_frame(rz_ctx* ctx, const FrameLaunch& L, hipStream_t s, bool p
     if (prof) RZ_HIP(ctx, hipEventRecord(ctx->ev[1], s));
     RZ_HIP(ctx, launch_bin(L.bp, s));
     if (prof) RZ_HIP(ctx, hipEventRecord(ctx->ev[2], s));
-    RZ_HIP(ctx, launch_raster(L.rp, L.tile_size, s));
+    RZ_HIP(ctx, launch_raster(L.rp, L.lay.ts, s));
     if (prof) RZ_HIP(ctx, hipEventRecord(ctx->ev[3], s));
-    RasterParams orp = L.rp;   // the overflow pass keeps its own statistics section
-    orp.part_frag = L.part + L.off_ovf;
-    RZ_HIP(ctx, launch_overflow(orp, L.ts_log2, s));
+    RZ_HIP(ctx, launch_overflow(L.orp, s));
     if (prof) RZ_HIP(ctx, hipEventRecord(ctx->ev[4], s));
     RZ_HIP(ctx, launch_shade(L.sp, s));
     if (prof) RZ_HIP(ctx, hipEventRecord(ctx->ev[5], s));
+    return RZ_OK;
+}
+
+// Statistics of one frame from its counters and per-workgroup partial sums (host copies).
+void reduce_stats(const FrameLaunch& L, const unsigned long long* counters, const uint32_t* part,
+                  uint64_t faces, rz_frame_stats* st) {
+    auto sum = [&](size_t a, size_t b) {
+        uint64_t t = 0;
+        for (size_t i = a; i < b; ++i) t += part[i];
+        return t;
+    };
+    const FrameLayout& y = L.lay;
+    st->triangles += faces;
+    st->front_triangles += sum(y.off_front, y.off_pairs);
+    st->bin_pairs += sum(y.off_pairs, y.off_frag);
+    st->overflow_tiles += counters[kCntOverflowTiles];
+    st->fragments += sum(y.off_frag, y.off_cov);
+    st->covered_pixels += sum(y.off_cov, y.n_part);
+}
+
+// ---------------------------------------------------------------------------------
+// Batches: every mesh's working buffers carved from one arena, the per-mesh
+// parameters and the block maps of the five passes in one table buffer.
+
+size_t align256(size_t v) { return (v + 255) & ~(size_t)255; }
+
+struct BatchLaunch {
+    std::vector<FrameLaunch> frames;
+    std::vector<uint64_t> faces;
+    int ts = 16;
+    const VertexParams* d_vp = nullptr;
+    const BinParams* d_bp = nullptr;
+    const RasterParams* d_rp = nullptr;
+    const RasterParams* d_orp = nullptr;
+    const ShadeParams* d_sp = nullptr;
+    const uint2* d_map[5] = {};
+    uint32_t blocks[5] = {};
+    size_t arena_part_bytes = 0;   // statistics: first bytes of the arena are every frame's counters+partials
+};
+
+rz_status prepare_batch(rz_ctx* ctx, const rz_batch_item* items, int32_t n, uint32_t flags, DevBuf& arena,
+                        DevBuf& tables, BatchLaunch& B) {
+    if (n < 1 || !items) return fail(ctx, RZ_ERR_INVALID_ARG, "empty batch");
+    if (!(flags & RZ_FLAG_DEVICE_BUFFERS))
+        return fail(ctx, RZ_ERR_INVALID_ARG, "a batch needs device buffers (RZ_FLAG_DEVICE_BUFFERS)");
+    RZ_HIP(ctx, hipSetDevice(ctx->device));
+    const bool cleared = (flags & RZ_FLAG_CLEARED) != 0;
+    std::vector<FrameLayout> lays(n);
+    // arena: [statistics of every frame][the other buffers of every frame]
+    size_t stat_bytes = 0, work_bytes = 0;
+    for (int32_t i = 0; i < n; ++i) {
+        const rz_batch_item& it = items[i];
+        rz_status st = check_frame(ctx, it.mesh, it.lights, it.num_lights, &it.args);
+        if (st != RZ_OK) return st;
+        if (!it.frame || !it.depth) return fail(ctx, RZ_ERR_INVALID_ARG, "batch item %d: null frame or depth", i);
+        lays[i] = layout_frame(it.mesh, it.args, ctx->tile_size, ctx->bin_cap);
+        stat_bytes += align256(lays[i].bytes[kBufCounters]) + align256(lays[i].bytes[kBufPart]);
+        for (int b = 0; b < kBufCounters; ++b) work_bytes += align256(lays[i].bytes[b]);
+    }
+    rz_status st = ensure(ctx, arena, stat_bytes + work_bytes);
+    if (st != RZ_OK) return st;
+    B.frames.assign(n, FrameLaunch());
+    B.faces.assign(n, 0);
+    B.ts = ctx->tile_size;
+    B.arena_part_bytes = stat_bytes;
+    char* base = (char*)arena.ptr;
+    size_t so = 0, wo = stat_bytes;
+    std::vector<uint2> map[5];
+    for (int32_t i = 0; i < n; ++i) {
+        const rz_batch_item& it = items[i];
+        void* ptrs[kNumBufs];
+        for (int b = 0; b < kBufCounters; ++b) {
+            ptrs[b] = base + wo;
+            wo += align256(lays[i].bytes[b]);
+        }
+        ptrs[kBufCounters] = base + so;
+        so += align256(lays[i].bytes[kBufCounters]);
+        ptrs[kBufPart] = base + so;
+        so += align256(lays[i].bytes[kBufPart]);
+        FrameLaunch& L = B.frames[i];
+        fill_frame(it.mesh, it.lights, it.num_lights, it.args, flags, lays[i], ptrs, L);
+        bind_outputs(L, cleared, it.frame, it.depth, it.ids);
+        B.faces[i] = (uint64_t)it.mesh->nf;
+        const uint32_t nb[5] = {vertex_blocks(L.vp.nv, L.vp.nn, L.vp.ntiles), bin_blocks(L.bp.nf),
+                                (uint32_t)L.rp.ntiles, overflow_blocks(L.rp.nf),
+                                shade_blocks_x(L.sp.W) * shade_blocks_y(L.sp.H)};
+        for (int p = 0; p < 5; ++p)
+            for (uint32_t b = 0; b < nb[p]; ++b) map[p].push_back(make_uint2((uint32_t)i, b));
+    }
+    // tables: the five parameter arrays and the five block maps
+    size_t off[10], total = 0;
+    const size_t sz[5] = {sizeof(VertexParams), sizeof(BinParams), sizeof(RasterParams), sizeof(RasterParams),
+                          sizeof(ShadeParams)};
+    for (int p = 0; p < 5; ++p) {
+        off[p] = total;
+        total = align256(total + sz[p] * (size_t)n);
+    }
+    for (int p = 0; p < 5; ++p) {
+        off[5 + p] = total;
+        total = align256(total + map[p].size() * sizeof(uint2));
+        B.blocks[p] = (uint32_t)map[p].size();
+    }
+    if ((st = ensure(ctx, tables, total)) != RZ_OK) return st;
+    std::vector<char> host(total, 0);
+    for (int32_t i = 0; i < n; ++i) {
+        std::memcpy(host.data() + off[0] + i * sz[0], &B.frames[i].vp, sz[0]);
+        std::memcpy(host.data() + off[1] + i * sz[1], &B.frames[i].bp, sz[1]);
+        std::memcpy(host.data() + off[2] + i * sz[2], &B.frames[i].rp, sz[2]);
+        std::memcpy(host.data() + off[3] + i * sz[3], &B.frames[i].orp, sz[3]);
+        std::memcpy(host.data() + off[4] + i * sz[4], &B.frames[i].sp, sz[4]);
+    }
+    for (int p = 0; p < 5; ++p)
+        if (!map[p].empty()) std::memcpy(host.data() + off[5 + p], map[p].data(), map[p].size() * sizeof(uint2));
+    // the table buffer may still be read by queued work of an earlier batch
+    RZ_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    RZ_HIP(ctx, hipMemcpy(tables.ptr, host.data(), total, hipMemcpyHostToDevice));
+    char* t = (char*)tables.ptr;
+    B.d_vp = (const VertexParams*)(t + off[0]);
+    B.d_bp = (const BinParams*)(t + off[1]);
+    B.d_rp = (const RasterParams*)(t + off[2]);
+    B.d_orp = (const RasterParams*)(t + off[3]);
+    B.d_sp = (const ShadeParams*)(t + off[4]);
+    for (int p = 0; p < 5; ++p) B.d_map[p] = (const uint2*)(t + off[5 + p]);
+    return RZ_OK;
+}
+
+rz_status enqueue_batch(rz_ctx* ctx, const BatchLaunch& B, hipStream_t s, bool prof) {
+    if (prof) RZ_HIP(ctx, hipEventRecord(ctx->ev[0], s));
+    RZ_HIP(ctx, launch_vertex_batch(B.d_vp, B.d_map[0], B.blocks[0], s));
+    if (prof) RZ_HIP(ctx, hipEventRecord(ctx->ev[1], s));
+    RZ_HIP(ctx, launch_bin_batch(B.d_bp, B.d_map[1], B.blocks[1], s));
+    if (prof) RZ_HIP(ctx, hipEventRecord(ctx->ev[2], s));
+    RZ_HIP(ctx, launch_raster_batch(B.d_rp, B.d_map[2], B.blocks[2], B.ts, s));
+    if (prof) RZ_HIP(ctx, hipEventRecord(ctx->ev[3], s));
+    RZ_HIP(ctx, launch_overflow_batch(B.d_orp, B.d_map[3], B.blocks[3], s));
+    if (prof) RZ_HIP(ctx, hipEventRecord(ctx->ev[4], s));
+    RZ_HIP(ctx, launch_shade_batch(B.d_sp, B.d_map[4], B.blocks[4], s));
+    if (prof) RZ_HIP(ctx, hipEventRecord(ctx->ev[5], s));
+    return RZ_OK;
+}
+
+rz_status event_times(rz_ctx* ctx, rz_frame_stats* stats) {
+    float t[5];
+    for (int i = 0; i < 5; ++i) RZ_HIP(ctx, hipEventElapsedTime(&t[i], ctx->ev[i], ctx->ev[i + 1]));
+    stats->ms_vertex = t[0];
+    stats->ms_bin = t[1];
+    stats->ms_raster = t[2];
+    stats->ms_overflow = t[3];
+    stats->ms_shade = t[4];
+    RZ_HIP(ctx, hipEventElapsedTime(&stats->ms_frame, ctx->ev[0], ctx->ev[5]));
+    return RZ_OK;
+}
+
+// Captures `frames` repetitions of enqueue(ctx, stream) into an executable graph.
+template <typename F>
+rz_status capture(rz_ctx* ctx, int32_t frames, F enqueue, hipGraphExec_t* out) {
+    hipStream_t s = ctx->stream;
+    RZ_HIP(ctx, hipStreamSynchronize(s));
+    RZ_HIP(ctx, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    for (int32_t k = 0; k < frames; ++k) {
+        rz_status st = enqueue(k, s);
+        if (st != RZ_OK) {
+            hipGraph_t g = nullptr;
+            (void)hipStreamEndCapture(s, &g);
+            if (g) (void)hipGraphDestroy(g);
+            return st;
+        }
+    }
+    hipGraph_t g = nullptr;
+    RZ_HIP(ctx, hipStreamEndCapture(s, &g));
+    hipError_t e = hipGraphInstantiate(out, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess) return fail(ctx, RZ_ERR_HIP, "hipGraphInstantiate: %s", hipGetErrorString(e));
+    return RZ_OK;
+}
+
+rz_status batch_stats(rz_ctx* ctx, const BatchLaunch& B, const DevBuf& arena, rz_frame_stats* stats) {
+    std::vector<char> host(B.arena_part_bytes);
+    RZ_HIP(ctx, hipMemcpyAsync(host.data(), arena.ptr, host.size(), hipMemcpyDeviceToHost, ctx->stream));
+    RZ_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    std::memset(stats, 0, sizeof *stats);
+    for (size_t i = 0; i < B.frames.size(); ++i) {
+        const FrameLaunch& L = B.frames[i];
+        const size_t oc = (size_t)((char*)L.counters - (char*)arena.ptr);
+        const size_t op = (size_t)((char*)L.part - (char*)arena.ptr);
+        reduce_stats(L, (const unsigned long long*)(host.data() + oc), (const uint32_t*)(host.data() + op),
+                     B.faces[i], stats);
+    }
     return RZ_OK;
 }
 
@@ -378,7 +613,7 @@ void rz_destroy(rz_ctx* ctx) {
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     DevBuf* bufs[] = {&ctx->raster, &ctx->cnormal, &ctx->rec,   &ctx->tile_count, &ctx->bins,
                       &ctx->vis,    &ctx->frame,   &ctx->depth, &ctx->ids,        &ctx->counters,
-                      &ctx->partials};
+                      &ctx->partials, &ctx->batch_arena, &ctx->batch_tables};
     for (DevBuf* b : bufs)
         if (b->ptr) (void)hipFree(b->ptr);
     for (hipEvent_t e : ctx->ev)
@@ -585,7 +820,7 @@ rz_status rz_draw_frame(rz_ctx* ctx, const rz_mesh* mesh, rz_light* lights, int3
     rz_status st = prepare_frame(ctx, mesh, lights, nl, args, flags, L);
     if (st != RZ_OK) return st;
     if (!frame_buffer || !depth_buffer) return fail(ctx, RZ_ERR_INVALID_ARG, "null frame or depth buffer");
-    const size_t np = L.np;
+    const size_t np = L.lay.np;
     uint8_t* frame_d = frame_buffer;
     float* depth_d = depth_buffer;
     int32_t* ids_d = id_buffer;
@@ -612,35 +847,15 @@ rz_status rz_draw_frame(rz_ctx* ctx, const rz_mesh* mesh, rz_light* lights, int3
         if (id_buffer) RZ_HIP(ctx, hipMemcpyAsync(id_buffer, ids_d, np * 4, hipMemcpyDeviceToHost, s));
     }
     if (stats) {
-        RZ_HIP(ctx, hipMemcpyAsync(ctx->host_counters, L.vp.counters, kNumCounters * 8, hipMemcpyDeviceToHost, s));
-        ctx->host_partials.resize(L.n_part);
-        RZ_HIP(ctx, hipMemcpyAsync(ctx->host_partials.data(), ctx->partials.ptr, L.n_part * 4,
-                                   hipMemcpyDeviceToHost, s));
+        RZ_HIP(ctx, hipMemcpyAsync(ctx->host_counters, L.counters, kNumCounters * 8, hipMemcpyDeviceToHost, s));
+        ctx->host_partials.resize(L.lay.n_part);
+        RZ_HIP(ctx, hipMemcpyAsync(ctx->host_partials.data(), L.part, L.lay.n_part * 4, hipMemcpyDeviceToHost, s));
     }
     if (stats || !on_device || !(flags & RZ_FLAG_ASYNC)) RZ_HIP(ctx, hipStreamSynchronize(s));
     if (stats) {
-        auto sum = [&](size_t a, size_t b) {
-            uint64_t t = 0;
-            for (size_t i = a; i < b; ++i) t += ctx->host_partials[i];
-            return t;
-        };
         std::memset(stats, 0, sizeof *stats);
-        stats->triangles = (uint64_t)mesh->nf;
-        stats->front_triangles = sum(L.off_front, L.off_pairs);
-        stats->bin_pairs = sum(L.off_pairs, L.off_frag);
-        stats->overflow_tiles = ctx->host_counters[kCntOverflowTiles];
-        stats->fragments = sum(L.off_frag, L.off_cov);
-        stats->covered_pixels = sum(L.off_cov, L.n_part);
-        if (ctx->profiling) {
-            float t[5];
-            for (int i = 0; i < 5; ++i) RZ_HIP(ctx, hipEventElapsedTime(&t[i], ctx->ev[i], ctx->ev[i + 1]));
-            stats->ms_vertex = t[0];
-            stats->ms_bin = t[1];
-            stats->ms_raster = t[2];
-            stats->ms_overflow = t[3];
-            stats->ms_shade = t[4];
-            RZ_HIP(ctx, hipEventElapsedTime(&stats->ms_frame, ctx->ev[0], ctx->ev[5]));
-        }
+        reduce_stats(L, ctx->host_counters, ctx->host_partials.data(), (uint64_t)mesh->nf, stats);
+        if (ctx->profiling && (st = event_times(ctx, stats)) != RZ_OK) return st;
     }
     return RZ_OK;
 }
@@ -662,24 +877,10 @@ rz_status rz_graph_create(rz_ctx* ctx, const rz_mesh* mesh, rz_light* lights, in
         bind_outputs(launches[k], (flags & RZ_FLAG_CLEARED) != 0, frame_d, depth_d, ids_d);
         a.tait_bryan_angles[1] += ry_step;   // the spin loop's rotation about y (renderer.cpp:124)
     }
-    hipStream_t s = ctx->stream;
-    RZ_HIP(ctx, hipStreamSynchronize(s));
-    RZ_HIP(ctx, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-    for (int32_t k = 0; k < frames; ++k) {
-        rz_status st = enqueue_frame(ctx, launches[k], s, false);
-        if (st != RZ_OK) {
-            hipGraph_t g = nullptr;
-            (void)hipStreamEndCapture(s, &g);
-            if (g) (void)hipGraphDestroy(g);
-            return st;
-        }
-    }
-    hipGraph_t g = nullptr;
-    RZ_HIP(ctx, hipStreamEndCapture(s, &g));
     hipGraphExec_t exec = nullptr;
-    hipError_t e = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
-    (void)hipGraphDestroy(g);
-    if (e != hipSuccess) return fail(ctx, RZ_ERR_HIP, "hipGraphInstantiate: %s", hipGetErrorString(e));
+    rz_status st = capture(ctx, frames, [&](int32_t k, hipStream_t s) { return enqueue_frame(ctx, launches[k], s, false); },
+                           &exec);
+    if (st != RZ_OK) return st;
     rz_graph* gr = new rz_graph();
     gr->ctx = ctx;
     gr->exec = exec;
@@ -692,7 +893,7 @@ rz_status rz_graph_create(rz_ctx* ctx, const rz_mesh* mesh, rz_light* lights, in
 rz_status rz_graph_launch(rz_graph* g) {
     if (!g) return RZ_ERR_INVALID_ARG;
     rz_ctx* ctx = g->ctx;
-    if (g->buffer_gen != ctx->buffer_gen)
+    if (!g->owns_buffers && g->buffer_gen != ctx->buffer_gen)
         return fail(ctx, RZ_ERR_INVALID_ARG, "graph is stale: a later frame reallocated the context's buffers");
     RZ_HIP(ctx, hipSetDevice(ctx->device));
     RZ_HIP(ctx, hipGraphLaunch(g->exec, ctx->stream));
@@ -704,7 +905,103 @@ void rz_graph_destroy(rz_graph* g) {
     (void)hipSetDevice(g->ctx->device);
     (void)hipStreamSynchronize(g->ctx->stream);
     (void)hipGraphExecDestroy(g->exec);
+    if (g->arena.ptr) (void)hipFree(g->arena.ptr);
+    if (g->tables.ptr) (void)hipFree(g->tables.ptr);
     delete g;
+}
+
+rz_status rz_draw_batch(rz_ctx* ctx, const rz_batch_item* items, int32_t n, uint32_t flags, rz_frame_stats* stats) {
+    if (!ctx) return RZ_ERR_INVALID_ARG;
+    BatchLaunch B;
+    rz_status st = prepare_batch(ctx, items, n, flags, ctx->batch_arena, ctx->batch_tables, B);
+    if (st != RZ_OK) return st;
+    if ((st = enqueue_batch(ctx, B, ctx->stream, ctx->profiling)) != RZ_OK) return st;
+    if (stats) {
+        if ((st = batch_stats(ctx, B, ctx->batch_arena, stats)) != RZ_OK) return st;
+        if (ctx->profiling && (st = event_times(ctx, stats)) != RZ_OK) return st;
+    } else if (!(flags & RZ_FLAG_ASYNC)) {
+        RZ_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    }
+    return RZ_OK;
+}
+
+rz_status rz_batch_graph_create(rz_ctx* ctx, const rz_batch_item* items, int32_t n, uint32_t flags, int32_t frames,
+                                rz_graph** out) {
+    if (!ctx || !out) return RZ_ERR_INVALID_ARG;
+    *out = nullptr;
+    if (frames < 1 || frames > 100000) return fail(ctx, RZ_ERR_INVALID_ARG, "frames %d not in [1, 100000]", frames);
+    rz_graph* gr = new rz_graph();
+    gr->ctx = ctx;
+    gr->frames = frames;
+    gr->owns_buffers = true;
+    BatchLaunch B;
+    rz_status st = prepare_batch(ctx, items, n, flags, gr->arena, gr->tables, B);
+    if (st == RZ_OK)
+        st = capture(ctx, frames, [&](int32_t, hipStream_t s) { return enqueue_batch(ctx, B, s, false); },
+                     &gr->exec);
+    if (st != RZ_OK) {
+        rz_graph_destroy(gr);
+        return st;
+    }
+    *out = gr;
+    return RZ_OK;
+}
+
+rz_status rz_autotune(rz_ctx* ctx, const rz_batch_item* items, int32_t n, uint32_t flags, int32_t* tile_size,
+                      int32_t* bin_capacity, float* ms_per_frame) {
+    if (!ctx) return RZ_ERR_INVALID_ARG;
+    const int32_t keep_ts = ctx->tile_size, keep_cap = ctx->bin_cap;
+    const size_t kBinBudget = (size_t)4 << 30;   // bytes of bins the tuner may choose
+    int best_ts = keep_ts, best_cap = keep_cap;
+    float best_ms = 0.f;
+    rz_status st = RZ_OK;
+    for (int ts : {8, 16, 32}) {
+        // one probe frame: the per-tile totals give the capacity that never overflows
+        ctx->tile_size = ts;
+        ctx->bin_cap = 64;
+        BatchLaunch B;
+        if ((st = prepare_batch(ctx, items, n, flags, ctx->batch_arena, ctx->batch_tables, B)) != RZ_OK) break;
+        if ((st = enqueue_batch(ctx, B, ctx->stream, false)) != RZ_OK) break;
+        uint32_t need = 1;
+        size_t tiles = 0;
+        std::vector<uint32_t> counts;
+        for (const FrameLaunch& L : B.frames) {
+            counts.resize(L.lay.ntiles);
+            RZ_HIP(ctx, hipMemcpyAsync(counts.data(), L.vp.tile_count, counts.size() * 4, hipMemcpyDeviceToHost,
+                                       ctx->stream));
+            RZ_HIP(ctx, hipStreamSynchronize(ctx->stream));
+            for (uint32_t c : counts) need = c > need ? c : need;
+            tiles += L.lay.ntiles;
+        }
+        uint32_t cap = 16;
+        while (cap < need && cap < (1u << 20)) cap <<= 1;
+        while (cap > 16 && tiles * (size_t)cap * 4 > kBinBudget) cap >>= 1;   // overflow path beyond the budget
+        ctx->bin_cap = (int)cap;
+        if ((st = prepare_batch(ctx, items, n, flags, ctx->batch_arena, ctx->batch_tables, B)) != RZ_OK) break;
+        for (int w = 0; w < 3 && st == RZ_OK; ++w) st = enqueue_batch(ctx, B, ctx->stream, false);
+        if (st != RZ_OK) break;
+        const int reps = 10;
+        RZ_HIP(ctx, hipEventRecord(ctx->ev[0], ctx->stream));
+        for (int r = 0; r < reps && st == RZ_OK; ++r) st = enqueue_batch(ctx, B, ctx->stream, false);
+        if (st != RZ_OK) break;
+        RZ_HIP(ctx, hipEventRecord(ctx->ev[5], ctx->stream));
+        RZ_HIP(ctx, hipEventSynchronize(ctx->ev[5]));
+        float ms = 0.f;
+        RZ_HIP(ctx, hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[5]));
+        ms /= reps;
+        if (best_ms == 0.f || ms < best_ms) {
+            best_ms = ms;
+            best_ts = ts;
+            best_cap = (int)cap;
+        }
+    }
+    ctx->tile_size = st == RZ_OK ? best_ts : keep_ts;
+    ctx->bin_cap = st == RZ_OK ? best_cap : keep_cap;
+    if (st != RZ_OK) return st;
+    if (tile_size) *tile_size = best_ts;
+    if (bin_capacity) *bin_capacity = best_cap;
+    if (ms_per_frame) *ms_per_frame = best_ms;
+    return RZ_OK;
 }
 
 rz_status rz_draw_frame_desc(rz_ctx* ctx, const rz_mesh_desc* desc, rz_light* lights, int32_t nl,

@@ -118,14 +118,21 @@ struct ShadeParams {
     float light_nd[kMaxLights][3];   // -trans_dir per light
 };
 
-// Launch shapes (shared by the launchers and the host's statistics reduction).
-inline uint32_t bin_blocks(int32_t nf) { return (uint32_t)((nf + 255) / 256); }
-inline uint32_t overflow_blocks(int32_t nf) {
+// Launch shapes (shared by the launchers, the batch block maps and the host's
+// statistics reduction).
+__host__ __device__ inline uint32_t vertex_blocks(int32_t nv, int32_t nn, int32_t ntiles) {
+    int32_t n = nv > nn ? nv : nn;
+    if (ntiles > n) n = ntiles;
+    if (kNumCounters > n) n = kNumCounters;
+    return (uint32_t)((n + 255) / 256);
+}
+__host__ __device__ inline uint32_t bin_blocks(int32_t nf) { return (uint32_t)((nf + 255) / 256); }
+__host__ __device__ inline uint32_t overflow_blocks(int32_t nf) {
     const uint32_t b = bin_blocks(nf);
     return b > 1024u ? 1024u : b;
 }
-inline uint32_t shade_blocks_x(uint32_t W) { return (W + 63) / 64; }
-inline uint32_t shade_blocks_y(uint32_t H) { return (H + 3) / 4; }
+__host__ __device__ inline uint32_t shade_blocks_x(uint32_t W) { return (W + 63) / 64; }
+__host__ __device__ inline uint32_t shade_blocks_y(uint32_t H) { return (H + 3) / 4; }
 
 // ---------------------------------------------------------------------------------
 // Arithmetic (host+device so host-side unit checks can call the same code).

@@ -45,8 +45,8 @@ __device__ __forceinline__ void store_block_sum(uint32_t v, uint32_t* out, int t
 }
 
 // ---------------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_vertex(VertexParams p) {
-    const int i = blockIdx.x * 256 + threadIdx.x;
+__device__ __forceinline__ void vertex_body(const VertexParams& p, int blk) {
+    const int i = blk * 256 + threadIdx.x;
     if (i < p.nv) {
         float4 r;
         if (p.raster_in) {
@@ -91,8 +91,8 @@ __device__ __forceinline__ void bin_push(const BinParams& p, uint32_t face, uint
         atomicAdd(&p.counters[kCntOverflowTiles], 1ull);
 }
 
-__global__ __launch_bounds__(256) void k_bin(BinParams p) {
-    const int f = blockIdx.x * 256 + threadIdx.x;
+__device__ __forceinline__ void bin_body(const BinParams& p, int blk) {
+    const int f = blk * 256 + threadIdx.x;
     bool front = false;
     uint32_t tx0 = 1, tx1 = 0, ty0 = 1, ty1 = 0;   // empty tile rectangle
     if (f < p.nf) {
@@ -119,8 +119,8 @@ __global__ __launch_bounds__(256) void k_bin(BinParams p) {
         }
     }
     const uint32_t nt = (tx0 <= tx1 && ty0 <= ty1) ? (tx1 - tx0 + 1) * (ty1 - ty0 + 1) : 0u;
-    store_block_sum(front ? 1u : 0u, p.part_front + blockIdx.x, threadIdx.x, 256);
-    store_block_sum(nt, p.part_pairs + blockIdx.x, threadIdx.x, 256);
+    store_block_sum(front ? 1u : 0u, p.part_front + blk, threadIdx.x, 256);
+    store_block_sum(nt, p.part_pairs + blk, threadIdx.x, 256);
     const bool big = nt > (uint32_t)kBigTriangleTiles;
     const bool small = nt > 0 && !big;
 
@@ -213,7 +213,7 @@ __device__ __forceinline__ bool fragment_key(float pxf, float pyf, float x0, flo
 }
 
 template <int TS, int BLOCK>
-__global__ __launch_bounds__(BLOCK) void k_raster(RasterParams p) {
+__device__ __forceinline__ void raster_body(const RasterParams& p, int blk) {
     constexpr int NPX = TS * TS;
     constexpr int PPT = NPX / BLOCK;
     constexpr int NW = BLOCK / 64;
@@ -230,7 +230,7 @@ __global__ __launch_bounds__(BLOCK) void k_raster(RasterParams p) {
     const int lane = tid & 63, wave = tid >> 6;
     // Consecutive tiles go to different XCDs (workgroups are dealt round robin), which
     // spreads a screen region's heavy tiles over all eight.
-    const int tile = blockIdx.x;
+    const int tile = blk;
     const int tx = tile % p.tiles_x, ty = tile / p.tiles_x;
     const int ox = tx * TS, oy = ty * TS;
 
@@ -321,18 +321,19 @@ __global__ __launch_bounds__(BLOCK) void k_raster(RasterParams p) {
         const uint32_t x = ox + (l % TS), y = oy + (l / TS);
         if (x < p.W && y < p.H) p.vis[(size_t)y * p.W + x] = s_key[l];
     }
-    store_block_sum(frags, p.part_frag + blockIdx.x, tid, BLOCK);
+    store_block_sum(frags, p.part_frag + blk, tid, BLOCK);
 }
 
 // ---------------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_overflow(RasterParams p, int32_t ts_log2) {
+__device__ __forceinline__ void overflow_body(const RasterParams& p, int blk, int nblk) {
+    const int32_t ts_log2 = p.ts_log2;
     if (p.counters[kCntOverflowTiles] == 0ull) {
-        if (threadIdx.x == 0) p.part_frag[blockIdx.x] = 0u;
+        if (threadIdx.x == 0) p.part_frag[blk] = 0u;
         return;
     }
     const uint32_t ts = 1u << ts_log2;
     uint32_t frags = 0;
-    for (int f = blockIdx.x * 256 + threadIdx.x; f < p.nf; f += gridDim.x * 256) {
+    for (int f = blk * 256 + threadIdx.x; f < p.nf; f += nblk * 256) {
         const FaceRecord* r = p.rec + f;
         const float4 b = r->r1, c = r->r2;
         const uint32_t lo = __float_as_uint(b.w), hi = __float_as_uint(c.w);
@@ -359,7 +360,7 @@ __global__ __launch_bounds__(256) void k_overflow(RasterParams p, int32_t ts_log
             }
         }
     }
-    store_block_sum(frags, p.part_frag + blockIdx.x, threadIdx.x, 256);
+    store_block_sum(frags, p.part_frag + blk, threadIdx.x, 256);
 }
 
 // ---------------------------------------------------------------------------------
@@ -387,7 +388,7 @@ __device__ __forceinline__ void sample_texture(const DevMaterial& m, const uint3
 }
 
 // 64 x 4 pixel blocks: each wave shades 64 consecutive pixels of one row.
-__global__ __launch_bounds__(256) void k_shade(ShadeParams p) {
+__device__ __forceinline__ void shade_body(const ShadeParams& p, int blk) {
     // normalisation tables of the first kLdsLuts materials, staged in LDS
     constexpr int kLdsLuts = 4;
     __shared__ float s_lut[kLdsLuts * 256];
@@ -400,7 +401,8 @@ __global__ __launch_bounds__(256) void k_shade(ShadeParams p) {
         __syncthreads();
     }
     const size_t np = (size_t)p.W * p.H;
-    const uint32_t px = blockIdx.x * 64 + threadIdx.x, py = blockIdx.y * 4 + threadIdx.y;
+    const uint32_t nbx = shade_blocks_x(p.W);
+    const uint32_t px = (blk % nbx) * 64 + threadIdx.x, py = (blk / nbx) * 4 + threadIdx.y;
     const bool valid = px < p.W && py < p.H;
     const size_t i = (size_t)py * p.W + px;
     const unsigned long long key = valid ? p.vis[i] : kEmptyKey;
@@ -474,18 +476,53 @@ __global__ __launch_bounds__(256) void k_shade(ShadeParams p) {
         p.depth[i] = z;
         if (p.ids) p.ids[i] = (int32_t)f;
     }
-    store_block_sum(hit ? 1u : 0u, p.part_cov + blockIdx.y * gridDim.x + blockIdx.x,
-                    threadIdx.y * 64 + threadIdx.x, 256);
+    store_block_sum(hit ? 1u : 0u, p.part_cov + blk, threadIdx.y * 64 + threadIdx.x, 256);
+}
+
+// ---------------------------------------------------------------------------------
+// Entry kernels.  Single frame: parameters by value, one block per unit of work.
+// Batch: per-mesh parameter tables in device memory and a map from block to
+// (mesh, block within that mesh's pass), so one launch covers every mesh.
+
+__global__ __launch_bounds__(256) void k_vertex(VertexParams p) { vertex_body(p, blockIdx.x); }
+__global__ __launch_bounds__(256) void k_bin(BinParams p) { bin_body(p, blockIdx.x); }
+template <int TS, int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_raster(RasterParams p) { raster_body<TS, BLOCK>(p, blockIdx.x); }
+__global__ __launch_bounds__(256) void k_overflow(RasterParams p) { overflow_body(p, blockIdx.x, gridDim.x); }
+__global__ __launch_bounds__(256) void k_shade(ShadeParams p) { shade_body(p, blockIdx.x); }
+
+__global__ __launch_bounds__(256) void k_vertex_batch(const VertexParams* __restrict__ items,
+                                                      const uint2* __restrict__ map) {
+    const uint2 m = map[blockIdx.x];
+    vertex_body(items[m.x], (int)m.y);
+}
+__global__ __launch_bounds__(256) void k_bin_batch(const BinParams* __restrict__ items, const uint2* __restrict__ map) {
+    const uint2 m = map[blockIdx.x];
+    bin_body(items[m.x], (int)m.y);
+}
+template <int TS, int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_raster_batch(const RasterParams* __restrict__ items,
+                                                        const uint2* __restrict__ map) {
+    const uint2 m = map[blockIdx.x];
+    raster_body<TS, BLOCK>(items[m.x], (int)m.y);
+}
+__global__ __launch_bounds__(256) void k_overflow_batch(const RasterParams* __restrict__ items,
+                                                        const uint2* __restrict__ map) {
+    const uint2 m = map[blockIdx.x];
+    const RasterParams& p = items[m.x];
+    overflow_body(p, (int)m.y, (int)overflow_blocks(p.nf));
+}
+__global__ __launch_bounds__(256) void k_shade_batch(const ShadeParams* __restrict__ items,
+                                                     const uint2* __restrict__ map) {
+    const uint2 m = map[blockIdx.x];
+    shade_body(items[m.x], (int)m.y);
 }
 
 // ---------------------------------------------------------------------------------
 // Launchers (host side).
 
 hipError_t launch_vertex(const VertexParams& p, hipStream_t s) {
-    int n = p.nv > p.nn ? p.nv : p.nn;
-    if (p.ntiles > n) n = p.ntiles;
-    if (kNumCounters > n) n = kNumCounters;
-    hipLaunchKernelGGL(k_vertex, dim3((n + 255) / 256), dim3(256), 0, s, p);
+    hipLaunchKernelGGL(k_vertex, dim3(vertex_blocks(p.nv, p.nn, p.ntiles)), dim3(256), 0, s, p);
     return hipGetLastError();
 }
 
@@ -504,13 +541,43 @@ hipError_t launch_raster(const RasterParams& p, int tile_size, hipStream_t s) {
     return hipGetLastError();
 }
 
-hipError_t launch_overflow(const RasterParams& p, int ts_log2, hipStream_t s) {
-    if (p.nf > 0) hipLaunchKernelGGL(k_overflow, dim3(overflow_blocks(p.nf)), dim3(256), 0, s, p, ts_log2);
+hipError_t launch_overflow(const RasterParams& p, hipStream_t s) {
+    if (p.nf > 0) hipLaunchKernelGGL(k_overflow, dim3(overflow_blocks(p.nf)), dim3(256), 0, s, p);
     return hipGetLastError();
 }
 
 hipError_t launch_shade(const ShadeParams& p, hipStream_t s) {
-    hipLaunchKernelGGL(k_shade, dim3(shade_blocks_x(p.W), shade_blocks_y(p.H)), dim3(64, 4), 0, s, p);
+    hipLaunchKernelGGL(k_shade, dim3(shade_blocks_x(p.W) * shade_blocks_y(p.H)), dim3(64, 4), 0, s, p);
+    return hipGetLastError();
+}
+
+// Batch launchers: `items` and `map` are device arrays, `blocks` the summed block
+// count of the pass over all meshes.
+hipError_t launch_vertex_batch(const VertexParams* items, const uint2* map, uint32_t blocks, hipStream_t s) {
+    if (blocks) hipLaunchKernelGGL(k_vertex_batch, dim3(blocks), dim3(256), 0, s, items, map);
+    return hipGetLastError();
+}
+hipError_t launch_bin_batch(const BinParams* items, const uint2* map, uint32_t blocks, hipStream_t s) {
+    if (blocks) hipLaunchKernelGGL(k_bin_batch, dim3(blocks), dim3(256), 0, s, items, map);
+    return hipGetLastError();
+}
+hipError_t launch_raster_batch(const RasterParams* items, const uint2* map, uint32_t blocks, int tile_size,
+                               hipStream_t s) {
+    if (!blocks) return hipSuccess;
+    switch (tile_size) {
+        case 8: hipLaunchKernelGGL((k_raster_batch<8, 64>), dim3(blocks), dim3(64), 0, s, items, map); break;
+        case 16: hipLaunchKernelGGL((k_raster_batch<16, 256>), dim3(blocks), dim3(256), 0, s, items, map); break;
+        case 32: hipLaunchKernelGGL((k_raster_batch<32, 256>), dim3(blocks), dim3(256), 0, s, items, map); break;
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+hipError_t launch_overflow_batch(const RasterParams* items, const uint2* map, uint32_t blocks, hipStream_t s) {
+    if (blocks) hipLaunchKernelGGL(k_overflow_batch, dim3(blocks), dim3(256), 0, s, items, map);
+    return hipGetLastError();
+}
+hipError_t launch_shade_batch(const ShadeParams* items, const uint2* map, uint32_t blocks, hipStream_t s) {
+    if (blocks) hipLaunchKernelGGL(k_shade_batch, dim3(blocks), dim3(64, 4), 0, s, items, map);
     return hipGetLastError();
 }
 

@@ -10,6 +10,13 @@ Workloads (sizes as the benchmark and parity tests use them):
                       seeded value noise (amplitude 0.1), spherical uvs with seam
                       duplicates, a seeded 1024x1024 RGBA8 noise texture, three lights,
                       1920x1080, seeded YXZ rotation in [-pi, pi]^3; flat or smooth.
+  batch_scenes        256 meshes of log-uniform face counts in [500, 200k]: displaced
+                      icospheres (subdivision chosen to hold the count, then
+                      truncated) or random triangle soups (edges U[0.01, 0.2] in the
+                      unit cube), each with its own square viewport (side 256..1024),
+                      transform (scale U[0.5,2], displacement U[-1,1]^3, rotation
+                      U[-pi,pi]^3; resampled until every vertex lies beyond the near
+                      plane) and 1-8 lights.
   screen_soup_scene   N seeded screen-space triangles (edges 0.5-3 px, eye depth
                       U[0.1, 100] through the fovy-60/0.1/100 projection, about half
                       of them wound backwards) on a square image.
@@ -169,7 +176,10 @@ def vertex_normals(pos: np.ndarray, tri: np.ndarray) -> np.ndarray:
     acc = np.zeros_like(pos)
     for k in range(3):
         np.add.at(acc, tri[:, k], fn)
-    return acc / np.linalg.norm(acc, axis=1, keepdims=True)
+    ln = np.linalg.norm(acc, axis=1, keepdims=True)
+    out = np.tile(np.array([0.0, 0.0, 1.0]), (pos.shape[0], 1))   # vertices no face uses
+    np.divide(acc, ln, out=out, where=ln > 0)
+    return out
 
 
 def spherical_uvs(dirs: np.ndarray, tri: np.ndarray):
@@ -266,6 +276,79 @@ def screen_soup_scene(seed: int = 4, n: int = 4_000_000, size: int = 4096,
                   raster_space=True)
     args = make_args(size, size, fovy=60.0, z_near=near, z_far=far)
     return scene, args
+
+
+_ICO_CACHE = {}
+
+
+def _icosphere_cached(subdivisions):
+    if subdivisions not in _ICO_CACHE:
+        _ICO_CACHE[subdivisions] = icosphere(subdivisions)
+    return _ICO_CACHE[subdivisions]
+
+
+def _eye_depths_ok(pos, args, near):
+    """True when every vertex is in front of the near plane (clip w > near) under args."""
+    import ctypes
+    from .abi import Args
+    ax, ay, az = (float(np.float32(t)) for t in args.tait_bryan_angles)
+    def rot(axis, t):
+        c, s_ = math.cos(t), math.sin(t)
+        if axis == 0:
+            return np.array([[1, 0, 0], [0, c, -s_], [0, s_, c]])
+        if axis == 1:
+            return np.array([[c, 0, s_], [0, 1, 0], [-s_, 0, c]])
+        return np.array([[c, -s_, 0], [s_, c, 0], [0, 0, 1]])
+    R = rot(1, ay) @ rot(0, ax) @ rot(2, az)
+    p = (pos.astype(np.float64) * float(args.scale)) @ R.T + np.array(args.displacement[:], np.float64)
+    z_eye = p[:, 2] - 3.0
+    return bool((-z_eye > near * 1.5).all())
+
+
+def batch_scenes(n=256, seed=3, near=0.1, far=100.0, max_faces=200_000, min_faces=500):
+    """The batched workload: a list of (Scene, Args) pairs (see the module docstring)."""
+    rng = SplitMix64(0x5EED0000 + seed)
+    out = []
+    for i in range(n):
+        t = int(round(math.exp(rng.uniform(1, math.log(min_faces), math.log(max_faces))[0])))
+        kind = rng.uniform(1)[0] < 0.5
+        side = int(256 + (rng.next_u64(1)[0] % np.uint64(769)))
+        if kind:
+            sub = 0
+            while 20 * 4 ** sub < t:
+                sub += 1
+            unit, tri = _icosphere_cached(sub)
+            tri = tri[:t]
+            r = 1.0 + 0.1 * value_noise(unit, seed * 1000 + i)
+            pos = unit * r[:, None]
+            nrm = vertex_normals(pos, tri)
+            faces = make_faces(tri, n=tri, t=None)
+        else:
+            c = rng.uniform(3 * t, 0.0, 1.0).reshape(t, 3)
+            e = rng.uniform(t, 0.01, 0.2)
+            d1 = rng.uniform(3 * t, -1, 1).reshape(t, 3)
+            d2 = rng.uniform(3 * t, -1, 1).reshape(t, 3)
+            d1 /= np.linalg.norm(d1, axis=1, keepdims=True)
+            d2 /= np.linalg.norm(d2, axis=1, keepdims=True)
+            pos = np.concatenate([c, c + e[:, None] * d1, c + e[:, None] * d2], 1).reshape(-1, 3)
+            tri = np.arange(3 * t).reshape(t, 3)
+            fn = np.cross(pos[tri[:, 1]] - pos[tri[:, 0]], pos[tri[:, 2]] - pos[tri[:, 0]])
+            nrm = fn / np.maximum(np.linalg.norm(fn, axis=1, keepdims=True), 1e-30)
+            faces = make_faces(tri, n=np.repeat(np.arange(t)[:, None], 3, 1), t=None)
+        nl = int(1 + (rng.next_u64(1)[0] % np.uint64(8)))
+        lights = random_lights(rng, nl)
+        col = tuple(float(x) for x in rng.uniform(3, 0.3, 1.0))
+        scene = Scene(pos.astype(np.float32), nrm.astype(np.float32), np.zeros((1, 2), np.float32), faces,
+                      [MaterialSpec(col)], lights)
+        for _ in range(100):
+            sc = float(np.float32(rng.uniform(1, 0.5, 2.0)[0]))
+            disp = tuple(float(np.float32(x)) for x in rng.uniform(3, -1.0, 1.0))
+            args = make_args(side, side, fovy=60.0, z_near=near, z_far=far, scale=sc, displacement=disp,
+                             angles=random_angles(rng))
+            if _eye_depths_ok(scene.vertices, args, near):
+                break
+        out.append((scene, args))
+    return out
 
 
 def square_scene():

@@ -110,3 +110,80 @@ def test_graph_stale_after_realloc(ctx):
         g.launch()
     g.close()
     mesh.close()
+
+
+def _device_outputs(ctx, args, with_ids=True):
+    W, H = args.image_width, args.image_height
+    return (ctx.empty((3, H, W), np.uint8), ctx.empty((H, W), np.float32),
+            ctx.empty((H, W), np.int32) if with_ids else None)
+
+
+def test_batch_matches_single_frames_and_oracle(ctx):
+    """Every mesh of one batched launch equals its own single-mesh frame bit for bit,
+    and a few equal the oracle."""
+    import oracle
+    from tests.parity import assert_same_frame
+    work = scenes.batch_scenes(n=12, seed=21, max_faces=20_000)
+    meshes = [ctx.upload(s) for s, a in work]
+    outs = [_device_outputs(ctx, a) for s, a in work]
+    lights = [s.lights_copy() for s, a in work]
+    batch = [(m, l, a, f, d, i) for m, l, (s, a), (f, d, i) in zip(meshes, lights, work, outs)]
+    st = ctx.draw_batch(batch, stats=True)
+    total_front = 0
+    for k, ((scene, args), (f, d, i), m) in enumerate(zip(work, outs, meshes)):
+        fb, db, ib = f.download(), d.download(), i.download()
+        single = ctx.draw(m, scene.lights_copy(), args, ids=True, stats=True)
+        assert np.array_equal(fb, single[0]) and np.array_equal(ib, single[2]), k
+        assert np.array_equal(db.view(np.uint32), single[1].view(np.uint32)), k
+        total_front += single[3]["front_triangles"]
+        if k < 4:
+            ref = oracle.draw_frame(scene, args)
+            assert_same_frame((fb, db, ib, single[3], lights[k]), ref, f"batch item {k}")
+    assert st["front_triangles"] == total_front
+    for m in meshes:
+        m.close()
+
+
+def test_batch_graph_and_autotune(ctx):
+    """The autotuner's choice renders the same pixels; a batch graph replays them."""
+    work = scenes.batch_scenes(n=6, seed=22, max_faces=30_000)
+    meshes = [ctx.upload(s) for s, a in work]
+    outs = [_device_outputs(ctx, a) for s, a in work]
+    batch = [(m, s.lights_copy(), a, f, d, i) for m, (s, a), (f, d, i) in zip(meshes, work, outs)]
+    ctx.draw_batch(batch)
+    before = [(f.download(), d.download(), i.download()) for f, d, i in outs]
+    ts, cap, ms = ctx.autotune(batch)
+    assert ts in (8, 16, 32) and cap >= 16 and ms > 0
+    assert ctx.tuning() == (ts, cap)
+    g = ctx.batch_graph(batch, frames=2)
+    for _ in range(2):
+        for f, d, i in outs:
+            f.upload(np.full(f.shape, 3, np.uint8))
+        g.launch()
+        ctx.sync()
+        for (f, d, i), (fb, db, ib) in zip(outs, before):
+            assert np.array_equal(f.download(), fb) and np.array_equal(i.download(), ib)
+            assert np.array_equal(d.download().view(np.uint32), db.view(np.uint32))
+    g.close()
+    ctx.set_tuning(16, 512)
+    for m in meshes:
+        m.close()
+
+
+@pytest.mark.parametrize("tile,cap", [(16, 2), (8, 1)])
+def test_batch_overflow_path(ctx, tile, cap):
+    work = scenes.batch_scenes(n=5, seed=23, max_faces=8_000)
+    ctx.set_tuning(tile, cap)
+    try:
+        meshes = [ctx.upload(s) for s, a in work]
+        outs = [_device_outputs(ctx, a) for s, a in work]
+        batch = [(m, s.lights_copy(), a, f, d, i) for m, (s, a), (f, d, i) in zip(meshes, work, outs)]
+        st = ctx.draw_batch(batch, stats=True)
+        assert st["overflow_tiles"] > 0
+        import oracle
+        for (scene, args), (f, d, i) in list(zip(work, outs))[:3]:
+            of, od, oi, _, _ = oracle.draw_frame(scene, args)
+            assert np.array_equal(i.download(), oi) and np.array_equal(f.download(), of)
+            assert np.array_equal(d.download().view(np.uint32), od.view(np.uint32))
+    finally:
+        ctx.set_tuning(16, 512)
