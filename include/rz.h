@@ -69,13 +69,25 @@ typedef struct rz_light {
  * 8-bit image exactly as the file stores it (tex_height rows of tex_width texels,
  * tex_channels interleaved bytes each, 3 or 4 channels); librz applies the Material
  * constructor's CImg::normalize(0,1) (material.h:20-23: min and max taken over all
- * channels) and samples channels 0..2 with CImg::_linear_atXY (material.cpp:17-22). */
+ * channels) and samples channels 0..2 with CImg::_linear_atXY (material.cpp:17-22).
+ *
+ * No reference counterpart: an optional tangent-space normal map (normal_map_channels
+ * 3 or 4, 0 = none), 8-bit as in its file.  It is sampled at the same uv with the same
+ * bilinear rule on texel/255 (no min/max normalisation), decoded t = 2*s - 1, and
+ * bends the shading normal: n = normalize(T*t.x + B*t.y + N*t.z) with N the
+ * interpolated normal, T the face tangent (from positions and uvs) made orthogonal
+ * to N, and B = cross(N, T). */
 typedef struct rz_material {
     float diffuse_colour[3];
     int32_t tex_width;
     int32_t tex_height;
     int32_t tex_channels;
     const uint8_t* texels;
+    int32_t normal_map_width;
+    int32_t normal_map_height;
+    int32_t normal_map_channels;
+    int32_t reserved;
+    const uint8_t* normal_map;
 } rz_material;
 
 /* Reference: headers/arguments.h:7-21 (struct Args) — the fields draw_frame reads —

@@ -18,6 +18,11 @@
  *   shading       shading.cpp:20-34     -> shade
  *   texture       material.cpp:11-26 + CImg.h:5184-5186, 13466-13491, 26786-26794
  *
+ * Tangent-space normal maps (rz_material.normal_map) have no reference counterpart;
+ * they follow the definition in include/rz.h: tangent from positions and uvs,
+ * Gram-Schmidt against the interpolated normal, bitangent cross(N, T), sample texel/255
+ * with CImg's bilinear rule, t = 2s-1, n = normalize(T t.x + B t.y + N t.z).
+ *
  * Arithmetic contract: IEEE binary32, round to nearest, no FMA contraction
  * (built with -O2 -ffp-contract=off -fno-fast-math), true divisions and sqrt.
  *
@@ -291,6 +296,18 @@ struct Texture {
     std::vector<float> data;   // [C][H][W]
 };
 
+// A normal map as floats texel/255, planar (no min/max normalisation).
+Texture make_normal_map(const rz_material& m) {
+    Texture t;
+    if (m.normal_map_channels <= 0 || !m.normal_map) return t;
+    t.W = m.normal_map_width; t.H = m.normal_map_height; t.C = m.normal_map_channels;
+    const size_t n = (size_t)t.W * t.H;
+    t.data.resize(n * t.C);
+    for (int c = 0; c < t.C; ++c)
+        for (size_t i = 0; i < n; ++i) t.data[c * n + i] = (float)m.normal_map[i * t.C + c] / 255.0f;
+    return t;
+}
+
 Texture make_texture(const rz_material& m) {
     Texture t;
     if (m.tex_channels <= 0 || !m.texels) return t;
@@ -364,7 +381,9 @@ struct Scene {
     const rz_mesh_desc* mesh;
     std::vector<V4> raster;          // per vertex
     std::vector<V3> cnormals;        // camera-space normals (or face normals when flat)
+    std::vector<V3> ctangents;       // camera-space face tangents (normal maps)
     std::vector<Texture> textures;
+    std::vector<Texture> normal_maps;
     const rz_light* lights;
     int nl;
     uint32_t W, H;
@@ -394,6 +413,19 @@ void update_pixel(const Scene& s, uint32_t px, uint32_t py, const V4 rv[3], cons
     V2 t = {d * (inter.x * uv[0].x + inter.y * uv[1].x + inter.z * uv[2].x),
             d * (inter.x * uv[0].y + inter.y * uv[1].y + inter.z * uv[2].y)};
     V3 albedo = sample(s.mesh->materials[mat], s.textures[mat], t);
+    const Texture& nm = s.normal_maps[mat];
+    if (nm.C > 0) {
+        // N = n, T = tangent orthogonalised against N, B = N x T
+        const V3 T = s.ctangents[face];
+        const float dd = dot3(n, T);
+        V3 P = normalize3({T.x - n.x * dd, T.y - n.y * dd, T.z - n.z * dd});
+        V3 B = {n.y * P.z - P.y * n.z, n.z * P.x - P.z * n.x, n.x * P.y - P.x * n.y};
+        const float u = t.x * (float)nm.W, v = (1.f - t.y) * (float)nm.H;
+        float ts[3];
+        for (int c = 0; c < 3; ++c) ts[c] = 2.0f * linear_atXY(nm, u, v, c) - 1.0f;
+        n = normalize3({(P.x * ts[0] + B.x * ts[1]) + n.x * ts[2], (P.y * ts[0] + B.y * ts[1]) + n.y * ts[2],
+                        (P.z * ts[0] + B.z * ts[1]) + n.z * ts[2]});
+    }
     uint32_t px3[3];
     shade(n, albedo, s.lights, s.nl, px3);
     const size_t plane = (size_t)s.W * s.H;
@@ -439,6 +471,10 @@ int validate(const rz_mesh_desc* m) {
     }
     for (int32_t i = 0; i < m->num_materials; ++i) {
         const rz_material& mt = m->materials[i];
+        if (mt.normal_map_channels != 0 &&
+            ((mt.normal_map_channels != 3 && mt.normal_map_channels != 4) || !mt.normal_map ||
+             mt.normal_map_width <= 0 || mt.normal_map_height <= 0))
+            return RZ_ERR_INVALID_ARG;
         if (mt.tex_channels == 0) continue;
         // channels 0..2 are sampled (material.cpp:22): 1- or 2-channel images would
         // read past the CImg buffer in the reference.
@@ -529,7 +565,38 @@ int oracle_draw_frame(const rz_mesh_desc* mesh, rz_light* lights, int32_t nl, co
         }
     }
     s.textures.resize(mesh->num_materials);
-    for (int32_t i = 0; i < mesh->num_materials; ++i) s.textures[i] = make_texture(mesh->materials[i]);
+    s.normal_maps.resize(mesh->num_materials);
+    bool any_nm = false;
+    for (int32_t i = 0; i < mesh->num_materials; ++i) {
+        s.textures[i] = make_texture(mesh->materials[i]);
+        s.normal_maps[i] = make_normal_map(mesh->materials[i]);
+        any_nm = any_nm || s.normal_maps[i].C > 0;
+    }
+    if (any_nm) {
+        if (mesh->raster_space) return RZ_ERR_INVALID_ARG;
+        // face tangent: (e1*dv2 - e2*dv1)/(du1*dv2 - du2*dv1), normalised; e1 if not finite
+        s.ctangents.resize(mesh->num_faces);
+        for (int32_t f = 0; f < mesh->num_faces; ++f) {
+            const rz_triangle& tr = mesh->faces[f];
+            const float* P = mesh->vertices;
+            V3 p[3];
+            V2 q[3];
+            for (int k = 0; k < 3; ++k) {
+                p[k] = {P[3 * tr.vertices[k]], P[3 * tr.vertices[k] + 1], P[3 * tr.vertices[k] + 2]};
+                q[k] = tr.uvs[k] >= 0 ? V2{mesh->uvs[2 * tr.uvs[k]], mesh->uvs[2 * tr.uvs[k] + 1]} : V2{0.f, 0.f};
+            }
+            const V3 e1 = {p[1].x - p[0].x, p[1].y - p[0].y, p[1].z - p[0].z};
+            const V3 e2 = {p[2].x - p[0].x, p[2].y - p[0].y, p[2].z - p[0].z};
+            const float du1 = q[1].x - q[0].x, dv1 = q[1].y - q[0].y;
+            const float du2 = q[2].x - q[0].x, dv2 = q[2].y - q[0].y;
+            const float den = du1 * dv2 - du2 * dv1;
+            V3 tg = {(e1.x * dv2 - e2.x * dv1) / den, (e1.y * dv2 - e2.y * dv1) / den,
+                     (e1.z * dv2 - e2.z * dv1) / den};
+            const float l2 = dot3(tg, tg);
+            if (!(l2 > 0.f && l2 < INFINITY)) tg = e1;
+            s.ctangents[f] = transform_direction(fr.normal, normalize3(tg));
+        }
+    }
     Counts c;
     for (int32_t f = 0; f < mesh->num_faces; ++f) draw_triangle(s, f, frame, depth, ids, c);
     if (counts) {

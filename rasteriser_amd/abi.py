@@ -35,7 +35,9 @@ class Light(C.Structure):
 class Material(C.Structure):
     _fields_ = [("diffuse_colour", C.c_float * 3), ("tex_width", C.c_int32),
                 ("tex_height", C.c_int32), ("tex_channels", C.c_int32),
-                ("texels", C.c_void_p)]
+                ("texels", C.c_void_p), ("normal_map_width", C.c_int32),
+                ("normal_map_height", C.c_int32), ("normal_map_channels", C.c_int32),
+                ("reserved", C.c_int32), ("normal_map", C.c_void_p)]
 
 
 class Args(C.Structure):

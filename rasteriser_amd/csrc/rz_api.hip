@@ -48,6 +48,7 @@ struct rz_mesh {
     float4* rin = nullptr;
     float *nx = nullptr, *ny = nullptr, *nz = nullptr;      // vertex normals
     float *fnx = nullptr, *fny = nullptr, *fnz = nullptr;   // face normals (flat)
+    float *ftx = nullptr, *fty = nullptr, *ftz = nullptr;   // face tangents (normal maps)
     float2* uvs = nullptr;
     int32_t *fv = nullptr, *fn = nullptr, *fuv = nullptr, *fmat = nullptr;
     DevMaterial* mats = nullptr;
@@ -159,6 +160,12 @@ rz_status validate_mesh(rz_ctx* ctx, const rz_mesh_desc* d) {
     }
     for (int32_t i = 0; i < d->num_materials; ++i) {
         const rz_material& m = d->materials[i];
+        if (m.normal_map_channels != 0 &&
+            ((m.normal_map_channels != 3 && m.normal_map_channels != 4) || !m.normal_map ||
+             m.normal_map_width <= 0 || m.normal_map_height <= 0))
+            return fail(ctx, RZ_ERR_INVALID_ARG, "material %d: normal map needs 3 or 4 channels and a size", i);
+        if (m.normal_map_channels != 0 && d->raster_space)
+            return fail(ctx, RZ_ERR_INVALID_ARG, "material %d: normal maps need model-space positions", i);
         if (m.tex_channels == 0) continue;
         if ((m.tex_channels != 3 && m.tex_channels != 4) || !m.texels || m.tex_width <= 0 || m.tex_height <= 0)
             return fail(ctx, RZ_ERR_INVALID_ARG, "material %d: texture needs 3 or 4 channels and a size", i);
@@ -305,6 +312,8 @@ void fill_frame(const rz_mesh* mesh, rz_light* lights, int32_t nl, const rz_args
     sp.cnormal = vp.cnormal;
     sp.uvs = mesh->uvs;
     sp.mats = mesh->mats; sp.texels = mesh->texels; sp.luts = mesh->luts;
+    sp.ftx = mesh->ftx; sp.fty = mesh->fty; sp.ftz = mesh->ftz;
+    std::memcpy(sp.normal_mat, mats.normal, 64);
     sp.part_cov = part + lay.off_cov;
     sp.W = lay.W; sp.H = lay.H;
     sp.cleared = cleared;
@@ -733,6 +742,23 @@ rz_status rz_mesh_create(rz_ctx* ctx, const rz_mesh_desc* d, rz_mesh** out) {
         if ((st = upload(ctx, m, &m->fnx, fx.data(), F * 4)) != RZ_OK) return cleanup(st);
         if ((st = upload(ctx, m, &m->fny, fy.data(), F * 4)) != RZ_OK) return cleanup(st);
         if ((st = upload(ctx, m, &m->fnz, fz.data(), F * 4)) != RZ_OK) return cleanup(st);
+        // face tangents for normal maps, model space
+        for (size_t f = 0; f < F; ++f) {
+            const rz_triangle& t = d->faces[f];
+            float tc[3][2] = {{0.f, 0.f}, {0.f, 0.f}, {0.f, 0.f}};
+            for (int k = 0; k < 3; ++k)
+                if (t.uvs[k] >= 0) {
+                    tc[k][0] = d->uvs[2 * (size_t)t.uvs[k]];
+                    tc[k][1] = d->uvs[2 * (size_t)t.uvs[k] + 1];
+                }
+            float tg[3];
+            face_tangent(d->vertices + 3 * (size_t)t.vertices[0], d->vertices + 3 * (size_t)t.vertices[1],
+                         d->vertices + 3 * (size_t)t.vertices[2], tc[0], tc[1], tc[2], tg);
+            fx[f] = tg[0]; fy[f] = tg[1]; fz[f] = tg[2];
+        }
+        if ((st = upload(ctx, m, &m->ftx, fx.data(), F * 4)) != RZ_OK) return cleanup(st);
+        if ((st = upload(ctx, m, &m->fty, fy.data(), F * 4)) != RZ_OK) return cleanup(st);
+        if ((st = upload(ctx, m, &m->ftz, fz.data(), F * 4)) != RZ_OK) return cleanup(st);
     }
     {
         std::vector<float> sx(N), sy(N), sz(N);
@@ -775,6 +801,21 @@ rz_status rz_mesh_create(rz_ctx* ctx, const rz_mesh_desc* d, rz_mesh** out) {
             o.diffuse[1] = src.diffuse_colour[1];
             o.diffuse[2] = src.diffuse_colour[2];
             o.lut_offset = (uint32_t)(256 * i);
+            if (src.normal_map_channels != 0) {
+                const size_t n = (size_t)src.normal_map_width * src.normal_map_height;
+                if (pool.size() + n > 0xFFFFFFFFull)
+                    return cleanup(fail(ctx, RZ_ERR_INVALID_ARG, "texture pool above 2^32 texels"));
+                o.nmap_w = src.normal_map_width;
+                o.nmap_h = src.normal_map_height;
+                o.nmap_offset = (uint32_t)pool.size();
+                const int C = src.normal_map_channels;
+                pool.resize(pool.size() + n);
+                uint32_t* dst = pool.data() + o.nmap_offset;
+                for (size_t t = 0; t < n; ++t) {
+                    const uint8_t* s = src.normal_map + t * C;
+                    dst[t] = (uint32_t)s[0] | ((uint32_t)s[1] << 8) | ((uint32_t)s[2] << 16);
+                }
+            }
             if (src.tex_channels == 0) continue;
             const size_t n = (size_t)src.tex_width * src.tex_height;
             if (pool.size() + n > 0xFFFFFFFFull)

@@ -44,7 +44,10 @@ struct DevMaterial {
     int32_t tex_h;
     uint32_t tex_offset;   // first texel (RGBA8 packed in a uint32) in the mesh texel pool
     uint32_t lut_offset;   // first entry of the 256-float normalisation table
-    uint32_t pad;
+    int32_t nmap_w;        // 0: no normal map
+    int32_t nmap_h;
+    uint32_t nmap_offset;  // first normal-map texel in the pool
+    uint32_t pad[2];
 };
 
 struct VertexParams {
@@ -104,6 +107,7 @@ struct ShadeParams {
     const DevMaterial* mats;
     const uint32_t* texels;
     const float* luts;
+    const float* ftx; const float* fty; const float* ftz;   // model-space face tangents
     uint8_t* frame;             // planar [3][H][W]
     float* depth;
     int32_t* ids;               // may be null
@@ -114,6 +118,7 @@ struct ShadeParams {
     int32_t flat;
     int32_t nl;
     int32_t nmat;
+    float normal_mat[16];            // inverse-transpose of the model-view (tangents)
     float light_ic[kMaxLights][3];   // intensity * colour per light
     float light_nd[kMaxLights][3];   // -trans_dir per light
 };

@@ -199,6 +199,21 @@ void face_normal(const float p0[3], const float p1[3], const float p2[3], float 
     unit3(c, out);
 }
 
+void face_tangent(const float p0[3], const float p1[3], const float p2[3], const float t0[2], const float t1[2],
+                  const float t2[2], float out[3]) {
+    const float e1[3] = {p1[0] - p0[0], p1[1] - p0[1], p1[2] - p0[2]};
+    const float e2[3] = {p2[0] - p0[0], p2[1] - p0[1], p2[2] - p0[2]};
+    const float du1 = t1[0] - t0[0], dv1 = t1[1] - t0[1], du2 = t2[0] - t0[0], dv2 = t2[1] - t0[1];
+    const float den = du1 * dv2 - du2 * dv1;
+    float t[3];
+    for (int c = 0; c < 3; ++c) t[c] = (e1[c] * dv2 - e2[c] * dv1) / den;
+    const float len2 = (t[0] * t[0] + t[1] * t[1]) + t[2] * t[2];
+    if (!(len2 > 0.f && len2 < INFINITY)) {
+        t[0] = e1[0]; t[1] = e1[1]; t[2] = e1[2];
+    }
+    unit3(t, out);
+}
+
 // CImg::normalize(0, 1) of an 8-bit image (CImg.h:26786-26794) as a per-byte table:
 // min and max over every channel; equal min and max fill with 0; an image already
 // spanning exactly [0, 1] is left as is.

@@ -18,6 +18,11 @@ void light_direction(const float view[16], const float dir[3], float out[3]);
 // normalize(cross(p1 - p0, p2 - p0)): the flat-shading normal of a face.
 void face_normal(const float p0[3], const float p1[3], const float p2[3], float out[3]);
 
+// Unit tangent of a face along increasing u: ((e1*dv2 - e2*dv1) / (du1*dv2 - du2*dv1))
+// normalised, with e1 = p1-p0, e2 = p2-p0; the edge e1 when that is not finite.
+void face_tangent(const float p0[3], const float p1[3], const float p2[3], const float t0[2], const float t1[2],
+                  const float t2[2], float out[3]);
+
 // Per-byte values of CImg::normalize(0,1) over `count` bytes of texels.
 void normalisation_table(const uint8_t* texels, size_t count, float lut[256]);
 

@@ -364,27 +364,61 @@ __device__ __forceinline__ void overflow_body(const RasterParams& p, int blk, in
 }
 
 // ---------------------------------------------------------------------------------
-// CImg::_linear_atXY on the normalised texture (CImg.h:13475-13491), channels 0..2.
-__device__ __forceinline__ void sample_texture(const DevMaterial& m, const uint32_t* __restrict__ texels,
-                                               const float* __restrict__ L, float uvx, float uvy, float out[3]) {
-    const float u = uvx * (float)m.tex_w;
-    const float v = (1.f - uvy) * (float)m.tex_h;
-    const float wm = (float)(m.tex_w - 1), hm = (float)(m.tex_h - 1);
+// CImg::_linear_atXY (CImg.h:13475-13491) of channels 0..2 of an RGBA8 image at
+// (u*W, (1-v)*H) (material.cpp:19-22); decode(byte) gives the texel value.
+template <typename Decode>
+__device__ __forceinline__ void sample_bilinear(const uint32_t* __restrict__ t, int32_t Wt, int32_t Ht, float uvx,
+                                                float uvy, Decode decode, float out[3]) {
+    const float u = uvx * (float)Wt;
+    const float v = (1.f - uvy) * (float)Ht;
+    const float wm = (float)(Wt - 1), hm = (float)(Ht - 1);
     const float nfx = u < 0.0f ? 0.0f : (u > wm ? wm : u);   // cimg::cut
     const float nfy = v < 0.0f ? 0.0f : (v > hm ? hm : v);
     const uint32_t x = cast_u32(nfx), y = cast_u32(nfy);
     const float dx = nfx - (float)x, dy = nfy - (float)y;
     const uint32_t nx = dx > 0 ? x + 1 : x, ny = dy > 0 ? y + 1 : y;
-    const uint32_t* t = texels + m.tex_offset;
-    const uint32_t W = (uint32_t)m.tex_w;
+    const uint32_t W = (uint32_t)Wt;
     const uint32_t tcc = t[y * W + x], tnc = t[y * W + nx], tcn = t[ny * W + x], tnn = t[ny * W + nx];
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
         const int sh = 8 * c;
-        const float Icc = L[(tcc >> sh) & 255u], Inc = L[(tnc >> sh) & 255u];
-        const float Icn = L[(tcn >> sh) & 255u], Inn = L[(tnn >> sh) & 255u];
+        const float Icc = decode((tcc >> sh) & 255u), Inc = decode((tnc >> sh) & 255u);
+        const float Icn = decode((tcn >> sh) & 255u), Inn = decode((tnn >> sh) & 255u);
         out[c] = Icc + dx * (Inc - Icc + dy * (Icc + Inn - Icn - Inc)) + dy * (Icn - Icc);
     }
+}
+
+// Albedo texel: the Material constructor's normalize(0,1) as a per-byte table L.
+__device__ __forceinline__ void sample_texture(const DevMaterial& m, const uint32_t* __restrict__ texels,
+                                               const float* __restrict__ L, float uvx, float uvy, float out[3]) {
+    sample_bilinear(texels + m.tex_offset, m.tex_w, m.tex_h, uvx, uvy, [&](uint32_t b) { return L[b]; }, out);
+}
+
+// Tangent-space normal map: n = normalize(T*t.x + B*t.y + N*t.z), t = 2*s - 1 with s the
+// bilinear sample of texel/255, T the face tangent (camera space) made orthogonal to N,
+// B = cross(N, T).
+__device__ __forceinline__ void bend_normal(const ShadeParams& p, const DevMaterial& m, uint32_t f, float u, float v,
+                                            float& nx, float& ny, float& nz) {
+    float s[3];
+    sample_bilinear(p.texels + m.nmap_offset, m.nmap_w, m.nmap_h, u, v,
+                    [](uint32_t b) { return (float)b / 255.0f; }, s);
+    const float tx = p.ftx[f], ty = p.fty[f], tz = p.ftz[f];
+    const float* M = p.normal_mat;
+    const float w = 0.0f;
+    const float cx = (M[0] * tx + M[4] * ty) + (M[8] * tz + M[12] * w);
+    const float cy = (M[1] * tx + M[5] * ty) + (M[9] * tz + M[13] * w);
+    const float cz = (M[2] * tx + M[6] * ty) + (M[10] * tz + M[14] * w);
+    const float d = (nx * cx + ny * cy) + nz * cz;
+    float px = cx - nx * d, py = cy - ny * d, pz = cz - nz * d;
+    const float rp = 1.f / sqrtf((px * px + py * py) + pz * pz);
+    px = px * rp; py = py * rp; pz = pz * rp;
+    const float bx = ny * pz - py * nz, by = nz * px - pz * nx, bz = nx * py - px * ny;
+    const float t0 = 2.0f * s[0] - 1.0f, t1 = 2.0f * s[1] - 1.0f, t2 = 2.0f * s[2] - 1.0f;
+    float qx = (px * t0 + bx * t1) + nx * t2;
+    float qy = (py * t0 + by * t1) + ny * t2;
+    float qz = (pz * t0 + bz * t1) + nz * t2;
+    const float rq = 1.f / sqrtf((qx * qx + qy * qy) + qz * qz);
+    nx = qx * rq; ny = qy * rq; nz = qz * rq;
 }
 
 // 64 x 4 pixel blocks: each wave shades 64 consecutive pixels of one row.
@@ -444,15 +478,18 @@ __device__ __forceinline__ void shade_body(const ShadeParams& p, int blk) {
         const int mi = p.fmat[f];
         const DevMaterial& m = p.mats[mi];
         float alb[3];
-        if (m.tex_w > 0) {
+        float u = 0.f, v = 0.f;
+        if (m.tex_w > 0 || m.nmap_w > 0) {
             float2 t[3];
 #pragma unroll
             for (int k = 0; k < 3; ++k) {
                 const int ui = p.fuv[3 * f + k];
                 t[k] = ui >= 0 ? p.uvs[ui] : make_float2(0.f, 0.f);
             }
-            const float u = d * (i0 * t[0].x + i1 * t[1].x + i2 * t[2].x);
-            const float v = d * (i0 * t[0].y + i1 * t[1].y + i2 * t[2].y);
+            u = d * (i0 * t[0].x + i1 * t[1].x + i2 * t[2].x);
+            v = d * (i0 * t[0].y + i1 * t[1].y + i2 * t[2].y);
+        }
+        if (m.tex_w > 0) {
             if (mi < kLdsLuts)
                 sample_texture(m, p.texels, s_lut + mi * 256, u, v, alb);
             else
@@ -460,6 +497,7 @@ __device__ __forceinline__ void shade_body(const ShadeParams& p, int blk) {
         } else {
             alb[0] = m.diffuse[0]; alb[1] = m.diffuse[1]; alb[2] = m.diffuse[2];
         }
+        if (m.nmap_w > 0) bend_normal(p, m, f, u, v, nx, ny, nz);
         // shading.cpp:20-34
         constexpr float kOneOverPi = 0.318309886183790671537767526745028724f;
         float acc0 = 0.f, acc1 = 0.f, acc2 = 0.f;

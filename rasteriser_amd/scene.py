@@ -20,6 +20,7 @@ class MaterialSpec:
     """One reference ``Material`` (headers/material.h:11-25)."""
     diffuse: tuple = (1.0, 1.0, 1.0)
     texture: np.ndarray | None = None     # (H, W, C) uint8 as the image file holds it, C in {3, 4}
+    normal_map: np.ndarray | None = None  # (H, W, C) uint8 tangent-space normals, C in {3, 4}
 
 
 @dataclass
@@ -62,6 +63,13 @@ class Scene:
                 keep.append(tex)
                 mats[i].tex_height, mats[i].tex_width, mats[i].tex_channels = tex.shape
                 mats[i].texels = tex.ctypes.data
+            if m.normal_map is not None:
+                nm = np.ascontiguousarray(m.normal_map, np.uint8)
+                if nm.ndim != 3 or nm.shape[2] not in (3, 4):
+                    raise ValueError("normal map must be (H, W, 3|4) uint8")
+                keep.append(nm)
+                mats[i].normal_map_height, mats[i].normal_map_width, mats[i].normal_map_channels = nm.shape
+                mats[i].normal_map = nm.ctypes.data
         d = MeshDesc()
         d.vertices = self.vertices.ctypes.data
         d.normals = self.normals.ctypes.data

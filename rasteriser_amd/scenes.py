@@ -17,6 +17,9 @@ Workloads (sizes as the benchmark and parity tests use them):
                       transform (scale U[0.5,2], displacement U[-1,1]^3, rotation
                       U[-pi,pi]^3; resampled until every vertex lies beyond the near
                       plane) and 1-8 lights.
+  learned_shading_scene  the icosphere workload with 64 lights and a seeded 1024x1024
+                      tangent-space normal map whose unit normals lie within 30 degrees
+                      of +z (the learned-shader comparison).
   screen_soup_scene   N seeded screen-space triangles (edges 0.5-3 px, eye depth
                       U[0.1, 100] through the fovy-60/0.1/100 projection, about half
                       of them wound backwards) on a square image.
@@ -214,6 +217,16 @@ def noise_texture(rng: SplitMix64, width: int = 1024, height: int = 1024, channe
     return tex
 
 
+def cone_normal_map(rng: SplitMix64, width: int = 1024, height: int = 1024, cone_deg: float = 30.0):
+    """Unit normals uniform on the cap within cone_deg of +z, stored as round((n+1)/2*255)."""
+    n = width * height
+    cz = rng.uniform(n, math.cos(math.radians(cone_deg)), 1.0)
+    phi = rng.uniform(n, 0.0, 2.0 * math.pi)
+    sz = np.sqrt(np.maximum(0.0, 1.0 - cz * cz))
+    nrm = np.stack([sz * np.cos(phi), sz * np.sin(phi), cz], 1)
+    return np.clip(np.rint((nrm + 1.0) * 0.5 * 255.0), 0, 255).astype(np.uint8).reshape(height, width, 3)
+
+
 def random_angles(rng: SplitMix64):
     return tuple(float(np.float32(a)) for a in rng.uniform(3, -math.pi, math.pi))
 
@@ -275,6 +288,16 @@ def screen_soup_scene(seed: int = 4, n: int = 4_000_000, size: int = 4096,
                   make_lights([[0.0, 0.0, -1.0]], [255.0 * math.pi], [[1.0, 1.0, 1.0]]),
                   raster_space=True)
     args = make_args(size, size, fovy=60.0, z_near=near, z_far=far)
+    return scene, args
+
+
+def learned_shading_scene(seed: int = 5, width: int = 1920, height: int = 1080, lights: int = 64,
+                          subdivisions: int = 6, map_size: int = 1024):
+    """Icosphere workload with `lights` lights and a cone normal map."""
+    scene, args = icosphere_scene(seed=seed, width=width, height=height, subdivisions=subdivisions,
+                                  texture_size=map_size, lights=lights)
+    rng = SplitMix64(0x5EED1000 + seed)
+    scene.materials[0].normal_map = cone_normal_map(rng, map_size, map_size)
     return scene, args
 
 

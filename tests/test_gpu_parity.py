@@ -187,3 +187,11 @@ def test_batch_overflow_path(ctx, tile, cap):
             assert np.array_equal(d.download().view(np.uint32), od.view(np.uint32))
     finally:
         ctx.set_tuning(16, 512)
+
+
+def test_normal_map_and_64_lights(ctx):
+    """The learned-shading workload's plain shader (64 lights, cone normal map) at a
+    reduced size, bit for bit against the oracle."""
+    scene, args = scenes.learned_shading_scene(width=640, height=360, subdivisions=5, map_size=256)
+    gpu, ref = check(ctx, scene, args, label="normal map 64 lights")
+    assert (gpu[2] >= 0).sum() > 20000

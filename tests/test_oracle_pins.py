@@ -481,3 +481,32 @@ def test_invalid_indices_are_rejected():
     bad.faces[5, 3] = 0
     bad.faces[7, 9] = 3
     assert oracle.draw_frame_status(bad, args) == 2
+
+
+def test_flat_normal_map_matches_smooth_shading():
+    """A normal map of +z texels (128,128,255) bends the normal by under 0.4 degrees:
+    every covered pixel stays within 1/255 of the unmapped frame (the definition's
+    invariant), and ids and depth do not move."""
+    scene, args = scenes.icosphere_scene(subdivisions=4, width=320, height=240, texture_size=64)
+    f0, d0, i0, _, _ = oracle.draw_frame(scene, args)
+    scene.materials[0].normal_map = np.tile(np.array([128, 128, 255], np.uint8), (32, 32, 1))
+    f1, d1, i1, _, _ = oracle.draw_frame(scene, args)
+    assert np.array_equal(i0, i1) and np.array_equal(d0, d1)
+    cov = i0 >= 0
+    diff = np.abs(f1.astype(int) - f0.astype(int))[:, cov]
+    assert diff.max() <= 1 and cov.sum() > 10000
+
+
+def test_normal_map_tilts_toward_the_tangent():
+    """A map of normals tilted toward +x (tangent) brightens faces whose tangent points
+    at a light and darkens the rest, compared with +z; both are Lambert closed forms
+    through the same definition."""
+    scene, args = scenes.icosphere_scene(subdivisions=4, width=320, height=240, texture_size=64)
+    tilt = np.tile(np.array([255, 128, 128], np.uint8), (16, 16, 1))       # t = (1, ~0, ~0)
+    flat = np.tile(np.array([128, 128, 255], np.uint8), (16, 16, 1))
+    scene.materials[0].normal_map = flat
+    fa, _, ia, _, _ = oracle.draw_frame(scene, args)
+    scene.materials[0].normal_map = tilt
+    fb, _, ib, _, _ = oracle.draw_frame(scene, args)
+    assert np.array_equal(ia, ib)
+    assert np.abs(fa.astype(int) - fb.astype(int))[:, ia >= 0].mean() > 5
