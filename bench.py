@@ -17,6 +17,14 @@ bytes per launch (counts from the frame itself) divided by that duration.
 `cpu_baseline` is the CPU oracle (a single-threaded restatement of the reference's
 draw_frame) timed on this host for a bounded number of full frames of the same
 workload.
+
+`learned_shading` (skipped with --no-mlp): the same mesh with 64 lights and a
+1024x1024 tangent-space normal map at 1920x1080.  The plain shader renders frames at
+seeded rotations until 2^22 fragments are collected; the BF16 MLP is trained on them
+by the HIP training kernels; then the shade pass is timed with the plain 64-light loop
+and with the network (HIP events, eager frames), and the MLP frame's error against the
+plain frame is reported per channel over the covered pixels of the workload's own
+rotation (not among the training rotations).
 """
 from __future__ import annotations
 
@@ -88,6 +96,91 @@ def cpu_baseline(scene, args, seconds):
     }
 
 
+MLP_FLOPS_PER_FRAGMENT = 2 * (12 * 64 + 64 * 64 + 64 * 3)   # 10,112, unpadded
+
+
+def shade_ms(ctx, mesh, lights, args, frame, depth, flags, frames):
+    """Average shade-pass and frame durations (ms) over `frames` eager profiled frames."""
+    ctx.set_profiling(True)
+    sh = fr = 0.0
+    for _ in range(frames):
+        s = ctx.draw_device(mesh, lights, args, frame.ptr, depth.ptr, 0, flags, stats=True)
+        sh += s["ms_shade"]
+        fr += s["ms_frame"]
+    ctx.set_profiling(False)
+    return sh / frames, fr / frames
+
+
+def learned_shading(ctx, rz, scenes, np, steps, batch, frames):
+    import ctypes
+
+    scene, args = scenes.learned_shading_scene()
+    W, H = args.image_width, args.image_height
+    mesh = ctx.upload(scene)
+    frame = ctx.empty((3, H, W), np.uint8)
+    depth = ctx.empty((H, W), np.float32)
+    ids = ctx.empty((H, W), np.int32)
+    n_target = 1 << 22
+    samples = ctx.empty((n_target, rz.RZ_MLP_SAMPLE_FLOATS), np.float32)
+    try:
+        # training fragments: the plain shader at seeded rotations
+        rng = np.random.default_rng(0x5EED2000)
+        n, rotations = 0, 0
+        t0 = time.perf_counter()
+        while n < n_target:
+            a_k = rz.make_args(W, H, angles=tuple(rng.uniform(-np.pi, np.pi, 3)))
+            a_k.fovy_degrees, a_k.z_near, a_k.z_far = args.fovy_degrees, args.z_near, args.z_far
+            a_k.scale, a_k.displacement[:] = args.scale, list(args.displacement)
+            n = ctx.mlp_collect(mesh, scene.lights_copy(), a_k, samples, n)
+            rotations += 1
+        t_collect = time.perf_counter() - t0
+        ctx.mlp_init(0x5EED3000)
+        t0 = time.perf_counter()
+        losses = ctx.mlp_train(samples, n, steps=steps, batch=batch, lr=3e-3, seed=0x5EED4000)
+        t_train = time.perf_counter() - t0
+        log(f"mlp: {n} samples from {rotations} rotations in {t_collect:.2f} s; {steps} steps x {batch} "
+            f"in {t_train:.2f} s; loss {losses[0]:.5f} -> {losses[-100:].mean():.6f}")
+
+        lights = scene.lights_copy()
+        flags = rz.RZ_FLAG_CLEARED | rz.RZ_FLAG_ASYNC
+        shade_ms(ctx, mesh, lights, args, frame, depth, flags, 10)
+        plain_ms, plain_frame_ms = shade_ms(ctx, mesh, lights, args, frame, depth, flags, frames)
+        shade_ms(ctx, mesh, lights, args, frame, depth, flags | rz.RZ_FLAG_MLP_SHADE, 10)
+        mlp_ms, mlp_frame_ms = shade_ms(ctx, mesh, lights, args, frame, depth, flags | rz.RZ_FLAG_MLP_SHADE,
+                                        frames)
+
+        ctx.draw_device(mesh, scene.lights_copy(), args, frame.ptr, depth.ptr, ids.ptr, flags)
+        ctx.sync()
+        fp, idp = frame.download(), ids.download()
+        ctx.draw_device(mesh, scene.lights_copy(), args, frame.ptr, depth.ptr, ids.ptr,
+                        flags | rz.RZ_FLAG_MLP_SHADE)
+        ctx.sync()
+        fm, idm = frame.download(), ids.download()
+        cov = idp >= 0
+        if not np.array_equal(idp, idm):
+            raise RuntimeError("MLP frame changed coverage")
+        err = np.abs(fm[:, cov].astype(np.int32) - fp[:, cov].astype(np.int32))
+        frags = int(cov.sum())
+        return {
+            "workload": "icosphere-s6-1080p-64-lights-normal-map-1024",
+            "training_samples": int(n), "training_rotations": rotations,
+            "train_steps": steps, "batch": batch, "train_s": t_train, "collect_s": t_collect,
+            "loss_first": float(losses[0]), "loss_last100": float(losses[-100:].mean()),
+            "fragments": frags,
+            "shade_plain_ms": plain_ms, "shade_mlp_ms": mlp_ms,
+            "frame_plain_ms": plain_frame_ms, "frame_mlp_ms": mlp_frame_ms,
+            "speedup": plain_ms / mlp_ms if mlp_ms > 0 else None,
+            "mlp_tflops": frags * MLP_FLOPS_PER_FRAGMENT / (mlp_ms * 1e-3) / 1e12 if mlp_ms > 0 else None,
+            "err_mean_per_channel_255": [float(e) for e in err.mean(axis=1)],
+            "err_max_per_channel_255": [int(e) for e in err.max(axis=1)],
+            "err_mean_255": float(err.mean()),
+        }
+    finally:
+        for b in (samples, frame, depth, ids):
+            b.close()
+        mesh.close()
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -98,6 +191,9 @@ def main():
     ap.add_argument("--flat", action="store_true", help="flat instead of smooth normals")
     ap.add_argument("--tile", type=int, default=16)
     ap.add_argument("--cap", type=int, default=512)
+    ap.add_argument("--no-mlp", action="store_true", help="skip the learned-shading measurement")
+    ap.add_argument("--mlp-steps", type=int, default=6000)
+    ap.add_argument("--mlp-batch", type=int, default=16384)
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {a.gpus} is not supported (the frame path is "
@@ -182,6 +278,12 @@ def main():
     mesh.close()
     for buf in (frame, depth, ids):
         buf.close()
+
+    mlp = None
+    if not a.no_mlp:
+        ctx.set_tuning(16, 512)
+        mlp = learned_shading(ctx, rz, scenes, np, a.mlp_steps, a.mlp_batch, 200)
+        log(f"learned shading: {mlp}")
     ctx.close()
     out = {
         "metric": METRIC, "value": value, "unit": "Mtri/s", "n_gpus": 1, "steps": a.steps,
@@ -197,8 +299,9 @@ def main():
         "cpu_baseline": cpu,
         "gpu_vs_cpu_frame_rate": (cpu["ms_per_frame"] / ms_per_step) if cpu else None,
         "passes": per_pass,
-        "shade64_plain_ms": None,
-        "shade64_mlp_ms": None,
+        "shade64_plain_ms": mlp["shade_plain_ms"] if mlp else None,
+        "shade64_mlp_ms": mlp["shade_mlp_ms"] if mlp else None,
+        "learned_shading": mlp,
     }
     print(json.dumps(out), flush=True)
     return 0

@@ -160,6 +160,7 @@ typedef struct rz_mesh rz_mesh;
 #define RZ_FLAG_CLEARED        1u  /* frame holds 0 and depth holds 1.0 (the caller's clear, renderer.cpp:85-86,107-108): neither is read */
 #define RZ_FLAG_DEVICE_BUFFERS 2u  /* frame/depth/ids are device pointers on the context's device */
 #define RZ_FLAG_ASYNC          4u  /* with DEVICE_BUFFERS: return without waiting for the stream */
+#define RZ_FLAG_MLP_SHADE      8u  /* shade with the context's learned network instead of the light loop */
 
 /* ---- context ---------------------------------------------------------------- */
 
@@ -264,6 +265,34 @@ rz_status rz_batch_graph_create(rz_ctx* ctx, const rz_batch_item* items, int32_t
  * (as rz_set_tuning would) and returned. */
 rz_status rz_autotune(rz_ctx* ctx, const rz_batch_item* items, int32_t num_items, uint32_t flags,
                       int32_t* tile_size, int32_t* bin_capacity, float* ms_per_frame);
+
+/* ---- learned shader ------------------------------------------------------------ */
+
+/* No reference counterpart.  A 12 -> 64 -> 64 -> 3 MLP (ReLU, ReLU, linear) that
+ * replaces the per-fragment light loop when RZ_FLAG_MLP_SHADE is set: BF16 operands on
+ * the matrix cores, FP32 accumulation.  Inputs per fragment: interpolated normal (3),
+ * unit view vector (3), albedo (3), decoded normal-map sample (3, (0,0,1) without a
+ * map); output colour/255 (3), rounded to 8 bits.  Coverage and depth are unchanged.
+ * Parameters (RZ_MLP_PARAMS floats): W1[64][12], b1[64], W2[64][64], b2[64], W3[3][64],
+ * b3[3].  The network belongs to the context. */
+#define RZ_MLP_PARAMS 5187
+#define RZ_MLP_SAMPLE_FLOATS 15   /* training sample: 12 inputs, 3 targets (colour/255) */
+
+rz_status rz_mlp_init(rz_ctx* ctx, uint64_t seed);                     /* He-uniform, zero biases */
+rz_status rz_mlp_set_weights(rz_ctx* ctx, const float* params);
+rz_status rz_mlp_get_weights(rz_ctx* ctx, float* params);
+/* Renders one frame with the plain shader and appends every shaded fragment's inputs
+ * and colour/255 to samples_device (capacity samples); *count is the fill level before
+ * (in) and after (out). */
+rz_status rz_mlp_collect(rz_ctx* ctx, const rz_mesh* mesh, rz_light* lights, int32_t num_lights,
+                         const rz_args* args, float* samples_device, int64_t capacity, int64_t* count);
+/* `steps` Adam steps of `batch` samples (a multiple of 64) drawn at random from the n
+ * device samples; learning rate lr with cosine decay to lr/100.  losses (may be NULL)
+ * receives the mean squared error of every step. */
+rz_status rz_mlp_train(rz_ctx* ctx, const float* samples_device, int64_t n, int32_t steps, int32_t batch,
+                       float lr, uint64_t seed, float* losses);
+/* The network on n input rows (12 floats each) in device memory -> n rows of 3. */
+rz_status rz_mlp_infer(rz_ctx* ctx, const float* features_device, int64_t n, float* out_device);
 
 /* ---- device memory ------------------------------------------------------------ */
 

@@ -12,13 +12,14 @@ import os
 import numpy as np
 
 from . import abi
-from .abi import (RZ_FLAG_ASYNC, RZ_FLAG_CLEARED, RZ_FLAG_DEVICE_BUFFERS, Args, BatchItem,
-                  FrameStats, Matrices, make_args)
+from .abi import (RZ_FLAG_ASYNC, RZ_FLAG_CLEARED, RZ_FLAG_DEVICE_BUFFERS, RZ_FLAG_MLP_SHADE, RZ_MLP_PARAMS,
+                  RZ_MLP_SAMPLE_FLOATS, Args, BatchItem, FrameStats, Matrices, make_args)
 from .scene import MaterialSpec, Scene, make_faces, make_lights
 
 __all__ = ["Context", "Mesh", "RzError", "Scene", "MaterialSpec", "make_faces", "make_lights",
            "make_args", "library_path", "load_library", "frame_matrices",
-           "RZ_FLAG_CLEARED", "RZ_FLAG_DEVICE_BUFFERS", "RZ_FLAG_ASYNC"]
+           "RZ_FLAG_CLEARED", "RZ_FLAG_DEVICE_BUFFERS", "RZ_FLAG_ASYNC", "RZ_FLAG_MLP_SHADE",
+           "RZ_MLP_PARAMS", "RZ_MLP_SAMPLE_FLOATS"]
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB = None
@@ -192,13 +193,14 @@ class Context:
         return Mesh(self, scene)
 
     def draw(self, mesh: Mesh, lights: np.ndarray, args: Args, frame=None, depth=None,
-             ids: bool = False, stats: bool = False):
+             ids: bool = False, stats: bool = False, flags: int = 0):
         """draw_frame into host buffers.  With frame/depth None the buffers start cleared
         (frame 0, depth 1.0, renderer.cpp:85-86); otherwise they are updated in place.
         ``lights`` (L,10) float32 gets trans_dir written back.  Returns
-        (frame [3,H,W] u8, depth [H,W] f32, ids [H,W] i32 or None, stats dict or None)."""
+        (frame [3,H,W] u8, depth [H,W] f32, ids [H,W] i32 or None, stats dict or None).
+        ``flags`` adds e.g. RZ_FLAG_MLP_SHADE."""
         W, H = args.image_width, args.image_height
-        flags = 0
+        flags = int(flags) & ~(RZ_FLAG_CLEARED | RZ_FLAG_DEVICE_BUFFERS | RZ_FLAG_ASYNC)
         if frame is None and depth is None:
             flags |= RZ_FLAG_CLEARED
             frame = np.zeros((3, H, W), np.uint8)
@@ -279,6 +281,53 @@ class Context:
         _check(self.lib.rz_autotune(self.handle, C.addressof(arr), len(batch), flags | RZ_FLAG_DEVICE_BUFFERS,
                                     C.byref(t), C.byref(c), C.byref(ms)), self.handle)
         return t.value, c.value, ms.value
+
+    # ---- learned shader -------------------------------------------------------------
+    def mlp_init(self, seed: int = 0):
+        _check(self.lib.rz_mlp_init(self.handle, int(seed)), self.handle)
+
+    def mlp_set_weights(self, params: np.ndarray):
+        params = np.ascontiguousarray(params, np.float32).reshape(-1)
+        if params.size != RZ_MLP_PARAMS:
+            raise ValueError(f"expected {RZ_MLP_PARAMS} parameters")
+        _check(self.lib.rz_mlp_set_weights(self.handle, params.ctypes.data), self.handle)
+
+    def mlp_get_weights(self) -> np.ndarray:
+        out = np.empty(RZ_MLP_PARAMS, np.float32)
+        _check(self.lib.rz_mlp_get_weights(self.handle, out.ctypes.data), self.handle)
+        return out
+
+    def mlp_collect(self, mesh: Mesh, lights: np.ndarray, args: Args, samples: DeviceArray, count: int = 0) -> int:
+        """Appends the plain shader's fragments of one frame to `samples` ((cap, 15) float32 on
+        the device) starting at `count`; returns the new fill level (clamped to capacity)."""
+        lights = _lights(lights)
+        n = C.c_int64(int(count))
+        _check(self.lib.rz_mlp_collect(self.handle, mesh.handle, lights.ctypes.data, lights.shape[0],
+                                       C.byref(args), C.c_void_p(samples.ptr), samples.shape[0], C.byref(n)),
+               self.handle)
+        return n.value
+
+    def mlp_train(self, samples: DeviceArray, n: int, steps: int, batch: int = 4096, lr: float = 3e-3,
+                  seed: int = 1) -> np.ndarray:
+        """Adam steps on the first n device samples; returns the per-step mean squared error."""
+        losses = np.empty(int(steps), np.float32)
+        _check(self.lib.rz_mlp_train(self.handle, C.c_void_p(samples.ptr), int(n), int(steps), int(batch),
+                                     float(lr), int(seed), losses.ctypes.data), self.handle)
+        return losses
+
+    def mlp_infer(self, features: np.ndarray) -> np.ndarray:
+        """The network on host rows (n, 12) -> (n, 3), through device buffers."""
+        x = np.ascontiguousarray(features, np.float32).reshape(-1, 12)
+        n = x.shape[0]
+        dx, dy = self.empty((max(n, 1), 12), np.float32), self.empty((max(n, 1), 3), np.float32)
+        try:
+            if n:
+                dx.upload(x)
+            _check(self.lib.rz_mlp_infer(self.handle, C.c_void_p(dx.ptr), n, C.c_void_p(dy.ptr)), self.handle)
+            return dy.download()[:n] if n else np.zeros((0, 3), np.float32)
+        finally:
+            dx.close()
+            dy.close()
 
     def draw_scene(self, scene: Scene, args: Args, frame=None, depth=None):
         """rz_draw_frame_desc: upload, draw and free in one call (draw_frame's by-value form)."""

@@ -20,6 +20,10 @@ RZ_ERR_NO_DEVICE = 5
 RZ_FLAG_CLEARED = 1
 RZ_FLAG_DEVICE_BUFFERS = 2
 RZ_FLAG_ASYNC = 4
+RZ_FLAG_MLP_SHADE = 8
+
+RZ_MLP_PARAMS = 5187
+RZ_MLP_SAMPLE_FLOATS = 15
 
 
 class Triangle(C.Structure):
@@ -128,6 +132,13 @@ def declare(lib):
         "rz_batch_graph_create": (C.c_int, [P, P, C.c_int32, C.c_uint32, C.c_int32, C.POINTER(P)]),
         "rz_autotune": (C.c_int, [P, P, C.c_int32, C.c_uint32, C.POINTER(C.c_int32), C.POINTER(C.c_int32),
                                   C.POINTER(C.c_float)]),
+        "rz_mlp_init": (C.c_int, [P, C.c_uint64]),
+        "rz_mlp_set_weights": (C.c_int, [P, P]),
+        "rz_mlp_get_weights": (C.c_int, [P, P]),
+        "rz_mlp_collect": (C.c_int, [P, P, P, C.c_int32, C.POINTER(Args), P, C.c_int64,
+                                     C.POINTER(C.c_int64)]),
+        "rz_mlp_train": (C.c_int, [P, P, C.c_int64, C.c_int32, C.c_int32, C.c_float, C.c_uint64, P]),
+        "rz_mlp_infer": (C.c_int, [P, P, C.c_int64, P]),
         "rz_device_alloc": (C.c_int, [P, C.c_size_t, C.POINTER(P)]),
         "rz_device_free": (C.c_int, [P, P]),
         "rz_copy_to_device": (C.c_int, [P, P, P, C.c_size_t]),
@@ -147,5 +158,7 @@ EXPORTED = ("rz_abi_version", "rz_status_string", "rz_last_error", "rz_create", 
             "rz_mesh_create", "rz_mesh_destroy", "rz_draw_frame", "rz_draw_frame_desc",
             "rz_graph_create", "rz_graph_launch", "rz_graph_destroy",
             "rz_draw_batch", "rz_batch_graph_create", "rz_autotune",
+            "rz_mlp_init", "rz_mlp_set_weights", "rz_mlp_get_weights", "rz_mlp_collect",
+            "rz_mlp_train", "rz_mlp_infer",
             "rz_device_alloc", "rz_device_free", "rz_copy_to_device", "rz_copy_to_host",
             "rz_default_args", "rz_frame_matrices")

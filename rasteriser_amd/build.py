@@ -15,7 +15,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OUT_DIR = os.path.join(HERE, "_lib")
 LIB = os.path.join(OUT_DIR, "librz.so")
-SOURCES = ["rz_kernels.hip", "rz_api.hip", "rz_host_math.cpp"]
+SOURCES = ["rz_kernels.hip", "rz_mlp.hip", "rz_api.hip", "rz_host_math.cpp"]
 HOST_DIR = os.path.join(HERE, "host")
 HOST_SOURCES = ["drawing.cpp", "fileloader.cpp", "image_io.cpp"]
 HOST_LIB = os.path.join(OUT_DIR, "librz_host.so")

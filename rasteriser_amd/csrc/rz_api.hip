@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
@@ -12,6 +13,7 @@
 #include "../../include/rz.h"
 #include "rz_device.h"
 #include "rz_host_math.h"
+#include "rz_mlp.h"
 
 namespace rz {
 hipError_t launch_vertex(const VertexParams& p, hipStream_t s);
@@ -24,7 +26,13 @@ hipError_t launch_bin_batch(const BinParams* items, const uint2* map, uint32_t b
 hipError_t launch_raster_batch(const RasterParams* items, const uint2* map, uint32_t blocks, int tile_size,
                                hipStream_t s);
 hipError_t launch_overflow_batch(const RasterParams* items, const uint2* map, uint32_t blocks, hipStream_t s);
-hipError_t launch_shade_batch(const ShadeParams* items, const uint2* map, uint32_t blocks, hipStream_t s);
+hipError_t launch_shade_batch(const ShadeParams* items, const uint2* map, uint32_t blocks, int mode,
+                              hipStream_t s);
+hipError_t launch_mlp_pack(const float* W, uint16_t* packed, hipStream_t s);
+hipError_t launch_mlp_infer(const float* x, int64_t n, const uint16_t* packed, float* y, hipStream_t s);
+hipError_t launch_mlp_step(const float* samples, int64_t n, float* W, float* m, float* v, float* gpart,
+                           float* lpart, float* loss_out, int nblocks, uint64_t seed, uint32_t step, float lr,
+                           float c1, float c2, hipStream_t s);
 }  // namespace rz
 
 using namespace rz;
@@ -76,6 +84,10 @@ struct rz_ctx {
     std::string err;
     DevBuf raster, cnormal, rec, tile_count, bins, vis, frame, depth, ids, counters, partials;
     DevBuf batch_arena, batch_tables;
+    // learned shader: FP32 parameters, Adam moments, packed BF16 network, training scratch
+    DevBuf mlp_w, mlp_m, mlp_v, mlp_packed, mlp_gpart, mlp_lpart, mlp_loss, mlp_count;
+    bool mlp_ready = false;
+    uint32_t mlp_steps = 0;
     std::vector<uint32_t> host_partials;
     unsigned long long* host_counters = nullptr;   // pinned
     uint64_t buffer_gen = 0;                       // bumped whenever a per-frame buffer is reallocated
@@ -321,6 +333,10 @@ void fill_frame(const rz_mesh* mesh, rz_light* lights, int32_t nl, const rz_args
     sp.flat = a.flat != 0;
     sp.nl = nl;
     sp.nmat = mesh->nm;
+    sp.mode = (flags & RZ_FLAG_MLP_SHADE) ? kShadeMlp : kShadePlain;
+    const float th = std::tan(a.fovy_degrees * 0.01745329251994329576923690768489f / 2.0f);
+    sp.inv_p00 = a.aspect_ratio * th;
+    sp.inv_p11 = th;
 }
 
 // Single frame: validates, sizes the context's working buffers and fills L.
@@ -337,7 +353,10 @@ rz_status prepare_frame(rz_ctx* ctx, const rz_mesh* mesh, rz_light* lights, int3
         if ((st = ensure(ctx, *bufs[i], lay.bytes[i])) != RZ_OK) return st;
     }
     for (int i = 0; i < kNumBufs; ++i) ptrs[i] = bufs[i]->ptr;
+    if ((flags & RZ_FLAG_MLP_SHADE) && !ctx->mlp_ready)
+        return fail(ctx, RZ_ERR_INVALID_ARG, "RZ_FLAG_MLP_SHADE without network weights (rz_mlp_init / rz_mlp_set_weights)");
     fill_frame(mesh, lights, nl, *args, flags, lay, ptrs, L);
+    L.sp.mlp = (const uint16_t*)ctx->mlp_packed.ptr;
     return RZ_OK;
 }
 
@@ -391,6 +410,7 @@ struct BatchLaunch {
     std::vector<FrameLaunch> frames;
     std::vector<uint64_t> faces;
     int ts = 16;
+    int mode = kShadePlain;
     const VertexParams* d_vp = nullptr;
     const BinParams* d_bp = nullptr;
     const RasterParams* d_rp = nullptr;
@@ -408,6 +428,8 @@ rz_status prepare_batch(rz_ctx* ctx, const rz_batch_item* items, int32_t n, uint
         return fail(ctx, RZ_ERR_INVALID_ARG, "a batch needs device buffers (RZ_FLAG_DEVICE_BUFFERS)");
     RZ_HIP(ctx, hipSetDevice(ctx->device));
     const bool cleared = (flags & RZ_FLAG_CLEARED) != 0;
+    if ((flags & RZ_FLAG_MLP_SHADE) && !ctx->mlp_ready)
+        return fail(ctx, RZ_ERR_INVALID_ARG, "RZ_FLAG_MLP_SHADE without network weights (rz_mlp_init / rz_mlp_set_weights)");
     std::vector<FrameLayout> lays(n);
     // arena: [statistics of every frame][the other buffers of every frame]
     size_t stat_bytes = 0, work_bytes = 0;
@@ -425,6 +447,7 @@ rz_status prepare_batch(rz_ctx* ctx, const rz_batch_item* items, int32_t n, uint
     B.frames.assign(n, FrameLaunch());
     B.faces.assign(n, 0);
     B.ts = ctx->tile_size;
+    B.mode = (flags & RZ_FLAG_MLP_SHADE) ? kShadeMlp : kShadePlain;
     B.arena_part_bytes = stat_bytes;
     char* base = (char*)arena.ptr;
     size_t so = 0, wo = stat_bytes;
@@ -442,6 +465,7 @@ rz_status prepare_batch(rz_ctx* ctx, const rz_batch_item* items, int32_t n, uint
         so += align256(lays[i].bytes[kBufPart]);
         FrameLaunch& L = B.frames[i];
         fill_frame(it.mesh, it.lights, it.num_lights, it.args, flags, lays[i], ptrs, L);
+        L.sp.mlp = (const uint16_t*)ctx->mlp_packed.ptr;
         bind_outputs(L, cleared, it.frame, it.depth, it.ids);
         B.faces[i] = (uint64_t)it.mesh->nf;
         const uint32_t nb[5] = {vertex_blocks(L.vp.nv, L.vp.nn, L.vp.ntiles), bin_blocks(L.bp.nf),
@@ -497,7 +521,7 @@ rz_status enqueue_batch(rz_ctx* ctx, const BatchLaunch& B, hipStream_t s, bool p
     if (prof) RZ_HIP(ctx, hipEventRecord(ctx->ev[3], s));
     RZ_HIP(ctx, launch_overflow_batch(B.d_orp, B.d_map[3], B.blocks[3], s));
     if (prof) RZ_HIP(ctx, hipEventRecord(ctx->ev[4], s));
-    RZ_HIP(ctx, launch_shade_batch(B.d_sp, B.d_map[4], B.blocks[4], s));
+    RZ_HIP(ctx, launch_shade_batch(B.d_sp, B.d_map[4], B.blocks[4], B.mode, s));
     if (prof) RZ_HIP(ctx, hipEventRecord(ctx->ev[5], s));
     return RZ_OK;
 }
@@ -622,7 +646,9 @@ void rz_destroy(rz_ctx* ctx) {
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     DevBuf* bufs[] = {&ctx->raster, &ctx->cnormal, &ctx->rec,   &ctx->tile_count, &ctx->bins,
                       &ctx->vis,    &ctx->frame,   &ctx->depth, &ctx->ids,        &ctx->counters,
-                      &ctx->partials, &ctx->batch_arena, &ctx->batch_tables};
+                      &ctx->partials, &ctx->batch_arena, &ctx->batch_tables, &ctx->mlp_w, &ctx->mlp_m,
+                      &ctx->mlp_v, &ctx->mlp_packed, &ctx->mlp_gpart, &ctx->mlp_lpart, &ctx->mlp_loss,
+                      &ctx->mlp_count};
     for (DevBuf* b : bufs)
         if (b->ptr) (void)hipFree(b->ptr);
     for (hipEvent_t e : ctx->ev)
@@ -1042,6 +1068,137 @@ rz_status rz_autotune(rz_ctx* ctx, const rz_batch_item* items, int32_t n, uint32
     if (tile_size) *tile_size = best_ts;
     if (bin_capacity) *bin_capacity = best_cap;
     if (ms_per_frame) *ms_per_frame = best_ms;
+    return RZ_OK;
+}
+
+// ---- learned shader ----------------------------------------------------------------
+
+namespace {
+
+rz_status mlp_buffers(rz_ctx* ctx) {
+    rz_status st;
+    if ((st = ensure(ctx, ctx->mlp_w, kMlpParams * 4)) != RZ_OK) return st;
+    if ((st = ensure(ctx, ctx->mlp_m, kMlpParams * 4)) != RZ_OK) return st;
+    if ((st = ensure(ctx, ctx->mlp_v, kMlpParams * 4)) != RZ_OK) return st;
+    if ((st = ensure(ctx, ctx->mlp_packed, kPackedBytes)) != RZ_OK) return st;
+    return RZ_OK;
+}
+
+rz_status mlp_reset_adam(rz_ctx* ctx) {
+    std::vector<float> z(kMlpParams, 0.f);
+    RZ_HIP(ctx, hipMemcpyAsync(ctx->mlp_m.ptr, z.data(), kMlpParams * 4, hipMemcpyHostToDevice, ctx->stream));
+    RZ_HIP(ctx, hipMemcpyAsync(ctx->mlp_v.ptr, z.data(), kMlpParams * 4, hipMemcpyHostToDevice, ctx->stream));
+    RZ_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    ctx->mlp_steps = 0;
+    return RZ_OK;
+}
+
+}  // namespace
+
+rz_status rz_mlp_set_weights(rz_ctx* ctx, const float* params) {
+    if (!ctx || !params) return RZ_ERR_INVALID_ARG;
+    RZ_HIP(ctx, hipSetDevice(ctx->device));
+    rz_status st = mlp_buffers(ctx);
+    if (st != RZ_OK) return st;
+    RZ_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    RZ_HIP(ctx, hipMemcpy(ctx->mlp_w.ptr, params, kMlpParams * 4, hipMemcpyHostToDevice));
+    RZ_HIP(ctx, launch_mlp_pack((const float*)ctx->mlp_w.ptr, (uint16_t*)ctx->mlp_packed.ptr, ctx->stream));
+    if ((st = mlp_reset_adam(ctx)) != RZ_OK) return st;
+    ctx->mlp_ready = true;
+    return RZ_OK;
+}
+
+rz_status rz_mlp_get_weights(rz_ctx* ctx, float* params) {
+    if (!ctx || !params) return RZ_ERR_INVALID_ARG;
+    if (!ctx->mlp_ready) return fail(ctx, RZ_ERR_INVALID_ARG, "no network weights");
+    RZ_HIP(ctx, hipSetDevice(ctx->device));
+    RZ_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    RZ_HIP(ctx, hipMemcpy(params, ctx->mlp_w.ptr, kMlpParams * 4, hipMemcpyDeviceToHost));
+    return RZ_OK;
+}
+
+rz_status rz_mlp_init(rz_ctx* ctx, uint64_t seed) {
+    if (!ctx) return RZ_ERR_INVALID_ARG;
+    // He-uniform weights from a SplitMix64 stream, zero biases
+    std::vector<float> w(kMlpParams, 0.f);
+    uint64_t state = seed;
+    auto uniform = [&]() {
+        uint64_t z = (state += 0x9E3779B97F4A7C15ull);
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        z ^= z >> 31;
+        return (double)(z >> 11) * (1.0 / 9007199254740992.0) * 2.0 - 1.0;
+    };
+    const struct { int off, n, fan_in; } layers[3] = {{kOffW1, 768, kMlpIn}, {kOffW2, 4096, kMlpHidden},
+                                                       {kOffW3, 192, kMlpHidden}};
+    for (const auto& L : layers) {
+        const double r = std::sqrt(6.0 / L.fan_in);
+        for (int i = 0; i < L.n; ++i) w[L.off + i] = (float)(r * uniform());
+    }
+    return rz_mlp_set_weights(ctx, w.data());
+}
+
+rz_status rz_mlp_collect(rz_ctx* ctx, const rz_mesh* mesh, rz_light* lights, int32_t nl, const rz_args* args,
+                         float* samples_device, int64_t capacity, int64_t* count) {
+    if (!ctx || !samples_device || capacity < 0 || !count) return RZ_ERR_INVALID_ARG;
+    FrameLaunch L;
+    rz_status st = prepare_frame(ctx, mesh, lights, nl, args, RZ_FLAG_CLEARED, L);
+    if (st != RZ_OK) return st;
+    const size_t np = L.lay.np;
+    if ((st = ensure(ctx, ctx->frame, np * 3)) != RZ_OK) return st;
+    if ((st = ensure(ctx, ctx->depth, np * 4)) != RZ_OK) return st;
+    if ((st = ensure(ctx, ctx->mlp_count, 8)) != RZ_OK) return st;
+    bind_outputs(L, true, (uint8_t*)ctx->frame.ptr, (float*)ctx->depth.ptr, nullptr);
+    const unsigned long long start = (unsigned long long)*count;
+    RZ_HIP(ctx, hipMemcpyAsync(ctx->mlp_count.ptr, &start, 8, hipMemcpyHostToDevice, ctx->stream));
+    L.sp.mode = kShadeCollect;
+    L.sp.samples = samples_device;
+    L.sp.sample_count = (unsigned long long*)ctx->mlp_count.ptr;
+    L.sp.sample_cap = (unsigned long long)capacity;
+    if ((st = enqueue_frame(ctx, L, ctx->stream, false)) != RZ_OK) return st;
+    unsigned long long end = 0;
+    RZ_HIP(ctx, hipMemcpyAsync(&end, ctx->mlp_count.ptr, 8, hipMemcpyDeviceToHost, ctx->stream));
+    RZ_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    *count = (int64_t)(end < (unsigned long long)capacity ? end : (unsigned long long)capacity);
+    return RZ_OK;
+}
+
+rz_status rz_mlp_train(rz_ctx* ctx, const float* samples_device, int64_t n, int32_t steps, int32_t batch, float lr,
+                       uint64_t seed, float* losses) {
+    if (!ctx || !samples_device || n <= 0 || steps < 1 || batch < 64) return RZ_ERR_INVALID_ARG;
+    if (!ctx->mlp_ready) return fail(ctx, RZ_ERR_INVALID_ARG, "no network weights (rz_mlp_init first)");
+    RZ_HIP(ctx, hipSetDevice(ctx->device));
+    const int nblocks = batch / 64;
+    rz_status st;
+    if ((st = ensure(ctx, ctx->mlp_gpart, (size_t)nblocks * kMlpParams * 4)) != RZ_OK) return st;
+    if ((st = ensure(ctx, ctx->mlp_lpart, (size_t)nblocks * 4)) != RZ_OK) return st;
+    if ((st = ensure(ctx, ctx->mlp_loss, (size_t)steps * 4)) != RZ_OK) return st;
+    for (int32_t k = 0; k < steps; ++k) {
+        // cosine decay from lr to lr/100
+        const float frac = (float)k / (float)steps;
+        const float lr_k = lr * (0.01f + 0.99f * 0.5f * (1.0f + std::cos(3.14159265f * frac)));
+        const uint32_t t = ++ctx->mlp_steps;
+        const float c1 = 1.0f / (1.0f - std::pow(0.9f, (float)t));
+        const float c2 = 1.0f / (1.0f - std::pow(0.999f, (float)t));
+        RZ_HIP(ctx, launch_mlp_step(samples_device, n, (float*)ctx->mlp_w.ptr, (float*)ctx->mlp_m.ptr,
+                                    (float*)ctx->mlp_v.ptr, (float*)ctx->mlp_gpart.ptr, (float*)ctx->mlp_lpart.ptr,
+                                    (float*)ctx->mlp_loss.ptr + k, nblocks, seed, t, lr_k, c1, c2, ctx->stream));
+    }
+    RZ_HIP(ctx, launch_mlp_pack((const float*)ctx->mlp_w.ptr, (uint16_t*)ctx->mlp_packed.ptr, ctx->stream));
+    if (losses)
+        RZ_HIP(ctx, hipMemcpyAsync(losses, ctx->mlp_loss.ptr, (size_t)steps * 4, hipMemcpyDeviceToHost, ctx->stream));
+    RZ_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    if (losses)
+        for (int32_t k = 0; k < steps; ++k) losses[k] /= (float)(nblocks * 64);
+    return RZ_OK;
+}
+
+rz_status rz_mlp_infer(rz_ctx* ctx, const float* features_device, int64_t n, float* out_device) {
+    if (!ctx || (n > 0 && (!features_device || !out_device)) || n < 0) return RZ_ERR_INVALID_ARG;
+    if (!ctx->mlp_ready) return fail(ctx, RZ_ERR_INVALID_ARG, "no network weights");
+    RZ_HIP(ctx, hipSetDevice(ctx->device));
+    RZ_HIP(ctx, launch_mlp_infer(features_device, n, (const uint16_t*)ctx->mlp_packed.ptr, out_device, ctx->stream));
+    RZ_HIP(ctx, hipStreamSynchronize(ctx->stream));
     return RZ_OK;
 }
 

@@ -22,6 +22,8 @@ constexpr int kBigTriangleTiles = 16;           // above this many tiles a face 
 // the overflow pass tests); every other statistic is a per-workgroup partial sum
 // written with a plain store and reduced only when a caller asks for statistics
 // (same-address device-scope atomics serialise across the eight XCDs).
+enum ShadeMode : int { kShadePlain = 0, kShadeMlp = 1, kShadeCollect = 2 };
+
 enum Counter : int {
     kCntOverflowTiles = 0,
     kNumCounters = 8
@@ -118,6 +120,12 @@ struct ShadeParams {
     int32_t flat;
     int32_t nl;
     int32_t nmat;
+    int32_t mode;                    // kShadePlain, kShadeMlp or kShadeCollect
+    const uint16_t* mlp;             // packed network (kShadeMlp)
+    float* samples;                  // training samples out (kShadeCollect)
+    unsigned long long* sample_count;
+    unsigned long long sample_cap;
+    float inv_p00, inv_p11;          // 1 / projection[0][0], 1 / projection[1][1] (view vector)
     float normal_mat[16];            // inverse-transpose of the model-view (tangents)
     float light_ic[kMaxLights][3];   // intensity * colour per light
     float light_nd[kMaxLights][3];   // -trans_dir per light

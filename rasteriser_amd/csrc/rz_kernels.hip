@@ -19,6 +19,7 @@
 // over covered fragments with depth below the initial depth: the same fragment the
 // reference's in-order strict `<` test (drawing.cpp:119, faces in file order) leaves.
 #include "rz_device.h"
+#include "rz_mlp.h"
 
 namespace rz {
 
@@ -397,11 +398,16 @@ __device__ __forceinline__ void sample_texture(const DevMaterial& m, const uint3
 // Tangent-space normal map: n = normalize(T*t.x + B*t.y + N*t.z), t = 2*s - 1 with s the
 // bilinear sample of texel/255, T the face tangent (camera space) made orthogonal to N,
 // B = cross(N, T).
-__device__ __forceinline__ void bend_normal(const ShadeParams& p, const DevMaterial& m, uint32_t f, float u, float v,
-                                            float& nx, float& ny, float& nz) {
+__device__ __forceinline__ void normal_map_sample(const ShadeParams& p, const DevMaterial& m, float u, float v,
+                                                  float ts[3]) {
     float s[3];
     sample_bilinear(p.texels + m.nmap_offset, m.nmap_w, m.nmap_h, u, v,
                     [](uint32_t b) { return (float)b / 255.0f; }, s);
+    ts[0] = 2.0f * s[0] - 1.0f; ts[1] = 2.0f * s[1] - 1.0f; ts[2] = 2.0f * s[2] - 1.0f;
+}
+
+__device__ __forceinline__ void bend_normal(const ShadeParams& p, uint32_t f, const float ts[3], float& nx, float& ny,
+                                            float& nz) {
     const float tx = p.ftx[f], ty = p.fty[f], tz = p.ftz[f];
     const float* M = p.normal_mat;
     const float w = 0.0f;
@@ -413,7 +419,7 @@ __device__ __forceinline__ void bend_normal(const ShadeParams& p, const DevMater
     const float rp = 1.f / sqrtf((px * px + py * py) + pz * pz);
     px = px * rp; py = py * rp; pz = pz * rp;
     const float bx = ny * pz - py * nz, by = nz * px - pz * nx, bz = nx * py - px * ny;
-    const float t0 = 2.0f * s[0] - 1.0f, t1 = 2.0f * s[1] - 1.0f, t2 = 2.0f * s[2] - 1.0f;
+    const float t0 = ts[0], t1 = ts[1], t2 = ts[2];
     float qx = (px * t0 + bx * t1) + nx * t2;
     float qy = (py * t0 + by * t1) + ny * t2;
     float qz = (pz * t0 + bz * t1) + nz * t2;
@@ -422,6 +428,10 @@ __device__ __forceinline__ void bend_normal(const ShadeParams& p, const DevMater
 }
 
 // 64 x 4 pixel blocks: each wave shades 64 consecutive pixels of one row.
+// MODE: kShadePlain (the reference's light loop), kShadeMlp (the learned surrogate on
+// matrix cores), kShadeCollect (light loop, and each shaded fragment's MLP inputs and
+// colour/255 appended to the sample buffer for training).
+template <int MODE>
 __device__ __forceinline__ void shade_body(const ShadeParams& p, int blk) {
     // normalisation tables of the first kLdsLuts materials, staged in LDS
     constexpr int kLdsLuts = 4;
@@ -441,6 +451,8 @@ __device__ __forceinline__ void shade_body(const ShadeParams& p, int blk) {
     const size_t i = (size_t)py * p.W + px;
     const unsigned long long key = valid ? p.vis[i] : kEmptyKey;
     const bool hit = key != kEmptyKey;
+    float feat[kMlpIn];
+    uint8_t rgb[3] = {0, 0, 0};
     if (valid && !hit) {
         if (p.cleared) {
             p.frame[i] = 0; p.frame[i + np] = 0; p.frame[i + 2 * np] = 0;
@@ -497,22 +509,75 @@ __device__ __forceinline__ void shade_body(const ShadeParams& p, int blk) {
         } else {
             alb[0] = m.diffuse[0]; alb[1] = m.diffuse[1]; alb[2] = m.diffuse[2];
         }
-        if (m.nmap_w > 0) bend_normal(p, m, f, u, v, nx, ny, nz);
-        // shading.cpp:20-34
-        constexpr float kOneOverPi = 0.318309886183790671537767526745028724f;
-        float acc0 = 0.f, acc1 = 0.f, acc2 = 0.f;
-        for (int l = 0; l < p.nl; ++l) {
-            const float dd = (nx * p.light_nd[l][0] + ny * p.light_nd[l][1]) + nz * p.light_nd[l][2];
-            const float mm = gmax(0.f, dd);
-            acc0 = acc0 + p.light_ic[l][0] * alb[0] * mm * kOneOverPi;
-            acc1 = acc1 + p.light_ic[l][1] * alb[1] * mm * kOneOverPi;
-            acc2 = acc2 + p.light_ic[l][2] * alb[2] * mm * kOneOverPi;
+        float ts[3] = {0.f, 0.f, 1.f};
+        feat[0] = nx; feat[1] = ny; feat[2] = nz;
+        if (m.nmap_w > 0) {
+            normal_map_sample(p, m, u, v, ts);
+            bend_normal(p, f, ts, nx, ny, nz);
         }
-        p.frame[i] = (uint8_t)cast_u32(gmin(acc0, 255.f));
-        p.frame[i + np] = (uint8_t)cast_u32(gmin(acc1, 255.f));
-        p.frame[i + 2 * np] = (uint8_t)cast_u32(gmin(acc2, 255.f));
         p.depth[i] = z;
         if (p.ids) p.ids[i] = (int32_t)f;
+        if (MODE != kShadeMlp) {
+            // shading.cpp:20-34
+            constexpr float kOneOverPi = 0.318309886183790671537767526745028724f;
+            float acc0 = 0.f, acc1 = 0.f, acc2 = 0.f;
+            for (int l = 0; l < p.nl; ++l) {
+                const float dd = (nx * p.light_nd[l][0] + ny * p.light_nd[l][1]) + nz * p.light_nd[l][2];
+                const float mm = gmax(0.f, dd);
+                acc0 = acc0 + p.light_ic[l][0] * alb[0] * mm * kOneOverPi;
+                acc1 = acc1 + p.light_ic[l][1] * alb[1] * mm * kOneOverPi;
+                acc2 = acc2 + p.light_ic[l][2] * alb[2] * mm * kOneOverPi;
+            }
+            rgb[0] = (uint8_t)cast_u32(gmin(acc0, 255.f));
+            rgb[1] = (uint8_t)cast_u32(gmin(acc1, 255.f));
+            rgb[2] = (uint8_t)cast_u32(gmin(acc2, 255.f));
+        }
+        if (MODE != kShadePlain) {
+            // view vector: from the surface point (eye space, depth d) to the eye
+            const float xn = 2.0f * pxf / (float)p.W - 1.0f, yn = 1.0f - 2.0f * pyf / (float)p.H;
+            float vx = -(xn * d) * p.inv_p00, vy = -(yn * d) * p.inv_p11, vz = d;
+            const float rv = 1.f / sqrtf((vx * vx + vy * vy) + vz * vz);
+            feat[3] = vx * rv; feat[4] = vy * rv; feat[5] = vz * rv;
+            feat[6] = alb[0]; feat[7] = alb[1]; feat[8] = alb[2];
+            feat[9] = ts[0]; feat[10] = ts[1]; feat[11] = ts[2];
+        }
+    }
+    if (MODE == kShadeCollect) {
+        // append (inputs, colour/255) of this wave's shaded fragments
+        const unsigned long long bal = __ballot(hit);
+        const int lane = threadIdx.x;
+        unsigned long long base = 0;
+        if (lane == 0 && bal) base = atomicAdd(p.sample_count, (unsigned long long)__popcll(bal));
+        base = __shfl(base, 0, 64);
+        const unsigned long long slot = base + __popcll(bal & ((1ull << lane) - 1ull));
+        if (hit && slot < p.sample_cap) {
+            float* o = p.samples + slot * kMlpSampleFloats;
+            for (int k = 0; k < kMlpIn; ++k) o[k] = feat[k];
+            for (int c = 0; c < 3; ++c) o[kMlpIn + c] = (float)rgb[c] / 255.0f;
+        }
+    }
+    if (MODE == kShadeMlp) {
+        __shared__ uint16_t s_in[4][64 * 16];
+        __shared__ float s_out[4][64 * 4];
+        const int wave = threadIdx.y, lane = threadIdx.x;
+#pragma unroll
+        for (int k = 0; k < 16; ++k) s_in[wave][lane * 16 + k] = to_bf16((hit && k < kMlpIn) ? feat[k] : 0.f);
+        if (__ballot(hit)) {   // uniform over the wave: waves with no fragment skip the network
+            __builtin_amdgcn_wave_barrier();
+            mlp_wave(p.mlp, s_in[wave], s_out[wave], lane);
+            __builtin_amdgcn_wave_barrier();
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                const float y = s_out[wave][lane * 4 + c] * 255.0f;
+                const float yc = y > 0.f ? (y < 255.f ? y : 255.f) : 0.f;
+                rgb[c] = (uint8_t)(uint32_t)(yc + 0.5f);
+            }
+        }
+    }
+    if (hit) {
+        p.frame[i] = rgb[0];
+        p.frame[i + np] = rgb[1];
+        p.frame[i + 2 * np] = rgb[2];
     }
     store_block_sum(hit ? 1u : 0u, p.part_cov + blk, threadIdx.y * 64 + threadIdx.x, 256);
 }
@@ -527,7 +592,8 @@ __global__ __launch_bounds__(256) void k_bin(BinParams p) { bin_body(p, blockIdx
 template <int TS, int BLOCK>
 __global__ __launch_bounds__(BLOCK) void k_raster(RasterParams p) { raster_body<TS, BLOCK>(p, blockIdx.x); }
 __global__ __launch_bounds__(256) void k_overflow(RasterParams p) { overflow_body(p, blockIdx.x, gridDim.x); }
-__global__ __launch_bounds__(256) void k_shade(ShadeParams p) { shade_body(p, blockIdx.x); }
+template <int MODE>
+__global__ __launch_bounds__(256) void k_shade(ShadeParams p) { shade_body<MODE>(p, blockIdx.x); }
 
 __global__ __launch_bounds__(256) void k_vertex_batch(const VertexParams* __restrict__ items,
                                                       const uint2* __restrict__ map) {
@@ -550,10 +616,11 @@ __global__ __launch_bounds__(256) void k_overflow_batch(const RasterParams* __re
     const RasterParams& p = items[m.x];
     overflow_body(p, (int)m.y, (int)overflow_blocks(p.nf));
 }
+template <int MODE>
 __global__ __launch_bounds__(256) void k_shade_batch(const ShadeParams* __restrict__ items,
                                                      const uint2* __restrict__ map) {
     const uint2 m = map[blockIdx.x];
-    shade_body(items[m.x], (int)m.y);
+    shade_body<MODE>(items[m.x], (int)m.y);
 }
 
 // ---------------------------------------------------------------------------------
@@ -585,7 +652,13 @@ hipError_t launch_overflow(const RasterParams& p, hipStream_t s) {
 }
 
 hipError_t launch_shade(const ShadeParams& p, hipStream_t s) {
-    hipLaunchKernelGGL(k_shade, dim3(shade_blocks_x(p.W) * shade_blocks_y(p.H)), dim3(64, 4), 0, s, p);
+    const dim3 grid(shade_blocks_x(p.W) * shade_blocks_y(p.H)), block(64, 4);
+    switch (p.mode) {
+        case kShadePlain: hipLaunchKernelGGL(k_shade<kShadePlain>, grid, block, 0, s, p); break;
+        case kShadeMlp: hipLaunchKernelGGL(k_shade<kShadeMlp>, grid, block, 0, s, p); break;
+        case kShadeCollect: hipLaunchKernelGGL(k_shade<kShadeCollect>, grid, block, 0, s, p); break;
+        default: return hipErrorInvalidValue;
+    }
     return hipGetLastError();
 }
 
@@ -614,8 +687,14 @@ hipError_t launch_overflow_batch(const RasterParams* items, const uint2* map, ui
     if (blocks) hipLaunchKernelGGL(k_overflow_batch, dim3(blocks), dim3(256), 0, s, items, map);
     return hipGetLastError();
 }
-hipError_t launch_shade_batch(const ShadeParams* items, const uint2* map, uint32_t blocks, hipStream_t s) {
-    if (blocks) hipLaunchKernelGGL(k_shade_batch, dim3(blocks), dim3(64, 4), 0, s, items, map);
+hipError_t launch_shade_batch(const ShadeParams* items, const uint2* map, uint32_t blocks, int mode,
+                              hipStream_t s) {
+    if (!blocks) return hipSuccess;
+    switch (mode) {
+        case kShadePlain: hipLaunchKernelGGL(k_shade_batch<kShadePlain>, dim3(blocks), dim3(64, 4), 0, s, items, map); break;
+        case kShadeMlp: hipLaunchKernelGGL(k_shade_batch<kShadeMlp>, dim3(blocks), dim3(64, 4), 0, s, items, map); break;
+        default: return hipErrorInvalidValue;
+    }
     return hipGetLastError();
 }
 

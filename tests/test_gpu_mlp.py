@@ -1,0 +1,131 @@
+"""GPU tests of the learned shader: the matrix-core network against its numpy statement
+(tests/mlp_ref.py), one training step against the numpy gradient, sample collection
+against the plain shader, and the MLP-shaded frame against the network evaluated on
+the collected inputs.  Coverage, depth and face ids of an MLP-shaded frame must equal
+the plain frame's bit for bit; colour is an approximation whose error is measured,
+not bounded by parity."""
+import numpy as np
+import pytest
+
+import mlp_ref as M
+import rasteriser_amd as rz
+from rasteriser_amd import scenes
+
+pytestmark = pytest.mark.gpu
+
+
+def _colour_bytes(y: np.ndarray) -> np.ndarray:
+    """The shading pass's conversion of network outputs to 8 bits, in float32."""
+    v = y.astype(np.float32) * np.float32(255.0)
+    v = np.minimum(np.maximum(v, np.float32(0.0)), np.float32(255.0))
+    return np.floor(v + np.float32(0.5)).astype(np.uint8)
+
+
+def test_weights_roundtrip(ctx):
+    p = M.random_params(11)
+    ctx.mlp_set_weights(p)
+    assert np.array_equal(ctx.mlp_get_weights().view(np.uint32), p.view(np.uint32))
+
+
+@pytest.mark.parametrize("n", [1, 63, 1000, 4096 + 17])
+def test_infer_matches_numpy(ctx, n):
+    p = M.random_params(12, scale=0.7)
+    ctx.mlp_set_weights(p)
+    x = np.random.default_rng(n).uniform(-1, 1, (n, M.N_IN)).astype(np.float32)
+    y = ctx.mlp_infer(x)
+    ref = M.forward(p, x)
+    err = np.abs(y - ref)
+    scale = 1.0 + np.abs(ref)
+    # FP32 accumulation against float64: a hidden unit whose sum lands next to a bf16
+    # rounding boundary may round the other way, so a few rows differ by more
+    assert (err <= 1e-2 * scale).all(), err.max()
+    assert np.mean(err <= 1e-4 * scale) > 0.9
+
+
+def test_infer_empty(ctx):
+    ctx.mlp_init(1)
+    assert ctx.mlp_infer(np.zeros((0, 12), np.float32)).shape == (0, 3)
+
+
+def test_one_training_step_matches_numpy_gradient(ctx):
+    p = M.random_params(13, scale=0.5)
+    ctx.mlp_set_weights(p)
+    rng = np.random.default_rng(14)
+    x = rng.uniform(-1, 1, M.N_IN).astype(np.float32)
+    t = rng.uniform(0, 1, M.N_OUT).astype(np.float32)
+    buf = ctx.empty((1, rz.RZ_MLP_SAMPLE_FLOATS), np.float32)
+    try:
+        buf.upload(np.concatenate([x, t])[None, :])
+        lr = 1e-3
+        losses = ctx.mlp_train(buf, 1, steps=1, batch=128, lr=lr, seed=3)
+    finally:
+        buf.close()
+    loss, g = M.loss_and_grad(p, x[None, :], t[None, :])
+    assert abs(losses[0] - loss) <= 1e-4 * (1 + loss), (losses[0], loss)
+    delta = ctx.mlp_get_weights().astype(np.float64) - p.astype(np.float64)
+    # first Adam step: delta = -lr * g / (|g| + 1e-8) = -lr * sign(g) where g is not tiny
+    big = np.abs(g) > 1e-4
+    assert big.sum() > 1000
+    agree = np.abs(delta[big] + lr * np.sign(g[big])) <= 2e-3 * lr
+    assert agree.mean() > 0.995, agree.mean()
+    zero = g == 0
+    assert (np.abs(delta[zero]) <= 1e-6).mean() > 0.995
+
+
+def test_collect_train_and_shade(ctx):
+    scene, args = scenes.learned_shading_scene(width=320, height=180, subdivisions=4, map_size=128)
+    mesh = ctx.upload(scene)
+    cap = 200_000
+    samples = ctx.empty((cap, rz.RZ_MLP_SAMPLE_FLOATS), np.float32)
+    try:
+        frame_p, depth_p, ids_p, _ = ctx.draw(mesh, scene.lights_copy(), args, ids=True)
+        covered = ids_p >= 0
+        n = ctx.mlp_collect(mesh, scene.lights_copy(), args, samples, 0)
+        assert n == int(covered.sum()) > 5000
+        S = samples.download()[:n]
+        feat, target = S[:, :12], S[:, 12:]
+        # targets are the plain shader's bytes / 255, one per covered pixel
+        got = np.sort(np.round(target * 255).astype(np.int64).view([("", np.int64)] * 3).ravel())
+        want = np.sort(frame_p[:, covered].T.astype(np.int64).copy().view([("", np.int64)] * 3).ravel())
+        assert np.array_equal(got, want)
+        assert np.abs(np.linalg.norm(feat[:, 0:3], axis=1) - 1).max() < 1e-5
+        assert np.abs(np.linalg.norm(feat[:, 3:6], axis=1) - 1).max() < 1e-5
+        assert (feat[:, 5] > 0).all()   # the eye looks down -z: the view vector has +z
+        assert (feat[:, 6:9] >= 0).all() and (feat[:, 6:9] <= 1).all()
+        assert (np.abs(feat[:, 9:12]) <= 1).all()
+        # a second collect appends behind the first
+        n2 = ctx.mlp_collect(mesh, scene.lights_copy(), args, samples, n)
+        assert n2 == 2 * n
+
+        ctx.mlp_init(7)
+        losses = ctx.mlp_train(samples, n, steps=400, batch=4096, lr=3e-3, seed=9)
+        assert np.isfinite(losses).all()
+        assert losses[-20:].mean() < 0.5 * losses[:20].mean(), (losses[:5], losses[-5:])
+
+        frame_m, depth_m, ids_m, _ = ctx.draw(mesh, scene.lights_copy(), args, ids=True,
+                                              flags=rz.RZ_FLAG_MLP_SHADE)
+        assert np.array_equal(ids_m, ids_p)
+        assert np.array_equal(depth_m.view(np.uint32), depth_p.view(np.uint32))
+        assert (frame_m[:, ~covered] == 0).all()
+        # the shaded colours are the network on the collected inputs, fragment for fragment
+        y = ctx.mlp_infer(feat)
+        want = np.sort(_colour_bytes(y).astype(np.int64).view([("", np.int64)] * 3).ravel())
+        got = np.sort(frame_m[:, covered].T.astype(np.int64).copy().view([("", np.int64)] * 3).ravel())
+        assert np.array_equal(got, want)
+        err = np.abs(frame_m[:, covered].astype(np.int32) - frame_p[:, covered].astype(np.int32))
+        assert err.mean() < 12.0, err.mean()
+    finally:
+        samples.close()
+        mesh.close()
+
+
+def test_mlp_flag_without_weights_is_an_error():
+    c = rz.Context(0)
+    try:
+        scene, args = scenes.uv_sphere_scene()
+        mesh = c.upload(scene)
+        with pytest.raises(rz.RzError):
+            c.draw(mesh, scene.lights_copy(), args, flags=rz.RZ_FLAG_MLP_SHADE)
+        mesh.close()
+    finally:
+        c.close()
