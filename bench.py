@@ -10,9 +10,11 @@ Timed region: `steps` frames captured in one HIP graph, launched once between tw
 device synchronisations.  `value` = faces / frame time in Mtri/s.
 
 `roofline` is for the dominant kernel of the frame: the pass with the longest
-average duration, measured with HIP events recorded on the librz stream around
-each pass over `steps` further eager frames; `achieved` is the pass's algorithmic
-bytes per launch (counts from the frame itself) divided by that duration.
+average duration.  Each pass is timed as a HIP graph of min(steps, 500) back-to-back
+launches of its kernel on the librz stream, with HIP events around the graph
+(rz_time_passes); `achieved` is the pass's algorithmic bytes per launch (counts from
+the frame itself) divided by that duration; `traffic` is the HBM bytes per launch read
+from the committed hardware-counter summary under profiles/.
 
 `cpu_baseline` is the CPU oracle (a single-threaded restatement of the reference's
 draw_frame) timed on this host for a bounded number of full frames of the same
@@ -21,10 +23,11 @@ workload.
 `learned_shading` (skipped with --no-mlp): the same mesh with 64 lights and a
 1024x1024 tangent-space normal map at 1920x1080.  The plain shader renders frames at
 seeded rotations until 2^22 fragments are collected; the BF16 MLP is trained on them
-by the HIP training kernels; then the shade pass is timed with the plain 64-light loop
-and with the network (HIP events, eager frames), and the MLP frame's error against the
-plain frame is reported per channel over the covered pixels of the workload's own
-rotation (not among the training rotations).
+by the HIP training kernels; then the deferred shade pass (run as its own pass) is
+timed with the plain 64-light loop and with the network (rz_time_passes), whole frames
+are timed both ways in both pipelines, and the MLP frame's error against the plain
+frame is reported per channel over the covered pixels of the workload's own rotation
+(not among the training rotations).
 """
 from __future__ import annotations
 
@@ -46,9 +49,11 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def algorithmic_bytes(counts, scene, args, cleared=True):
-    """Per-pass algorithmic bytes of one frame (input reads + output writes that the
-    pass must do at minimum)."""
+def algorithmic_bytes(counts, scene, args, fused, cleared=True):
+    """Per-pass algorithmic bytes of one frame: the input reads and output writes a pass
+    must make at minimum.  With the fused pipeline the raster pass also shades (no
+    visibility buffer round trip) and the separate shade pass only covers overflowed
+    tiles (none in this frame)."""
     W, H = args.image_width, args.image_height
     V = scene.vertices.shape[0]
     Vn = scene.normals.shape[0]
@@ -56,13 +61,29 @@ def algorithmic_bytes(counts, scene, args, cleared=True):
     L = scene.lights.shape[0]
     tex = sum(int(m.texture.shape[0] * m.texture.shape[1]) for m in scene.materials if m.texture is not None)
     P16, Tf, Tvis = counts["bin_pairs16"], counts["front_triangles"], counts["visible_faces"]
-    return {
+    depth_in = 0 if cleared else 4 * W * H
+    raster_in = 48 * Tf + 4 * P16 + depth_in
+    shade_out = 7 * W * H + 108 * Tvis + 4 * tex + 40 * L   # frame + depth, winners' data, texture, lights
+    out = {
         "vertex": 28 * V + 24 * Vn,
         "bin": 12 * T + 16 * V + 4 * P16,
-        "raster": 48 * Tf + 4 * P16 + 8 * W * H + (0 if cleared else 4 * W * H),
         "overflow": 0,
-        "shade": 8 * W * H + 7 * W * H + 108 * Tvis + 4 * tex + 40 * L,
     }
+    if fused:
+        out["raster"] = raster_in + shade_out
+        out["shade"] = 0
+    else:
+        out["raster"] = raster_in + 8 * W * H        # + visibility words written
+        out["shade"] = 8 * W * H + shade_out         # + visibility words read
+    return out
+
+
+def per_pass_table(us, nbytes):
+    table = {}
+    for p, t in us.items():
+        gbs = nbytes[p] / (t * 1e-6) / 1e9 if t > 0 else 0.0
+        table[p] = {"avg_us": t, "alg_bytes": nbytes[p], "GBps": gbs, "frac": gbs / HBM_PEAK_GBS}
+    return table
 
 
 def cpu_baseline(scene, args, seconds):
@@ -99,16 +120,33 @@ def cpu_baseline(scene, args, seconds):
 MLP_FLOPS_PER_FRAGMENT = 2 * (12 * 64 + 64 * 64 + 64 * 3)   # 10,112, unpadded
 
 
-def shade_ms(ctx, mesh, lights, args, frame, depth, flags, frames):
-    """Average shade-pass and frame durations (ms) over `frames` eager profiled frames."""
-    ctx.set_profiling(True)
-    sh = fr = 0.0
-    for _ in range(frames):
-        s = ctx.draw_device(mesh, lights, args, frame.ptr, depth.ptr, 0, flags, stats=True)
-        sh += s["ms_shade"]
-        fr += s["ms_frame"]
-    ctx.set_profiling(False)
-    return sh / frames, fr / frames
+def graph_ms(ctx, mesh, lights, args, frame, depth, frames, flags):
+    """ms per frame of a graph of `frames` whole frames (second launch)."""
+    g = ctx.graph(mesh, lights, args, frame.ptr, depth.ptr, frames, flags=flags)
+    try:
+        g.launch()
+        ctx.sync()
+        t0 = time.perf_counter()
+        g.launch()
+        ctx.sync()
+        return (time.perf_counter() - t0) * 1000.0 / frames
+    finally:
+        g.close()
+
+
+def traffic_bytes(kernel):
+    """HBM bytes per launch of `kernel` from the committed hardware-counter summary
+    (profiles/hbm_traffic_icosphere_1080p.md: FETCH_SIZE and WRITE_SIZE per dispatch, each
+    scaled by what the counter reads for a copy of known size), or None."""
+    path = os.path.join(ROOT, "profiles", "hbm_traffic_icosphere_1080p.md")
+    try:
+        with open(path) as f:
+            for line in f:
+                if line.startswith("traffic_bytes_per_launch ") and line.split()[1] == kernel:
+                    return float(line.split()[2])
+    except OSError:
+        pass
+    return None
 
 
 def learned_shading(ctx, rz, scenes, np, steps, batch, frames):
@@ -143,12 +181,18 @@ def learned_shading(ctx, rz, scenes, np, steps, batch, frames):
 
         lights = scene.lights_copy()
         flags = rz.RZ_FLAG_CLEARED | rz.RZ_FLAG_ASYNC
-        shade_ms(ctx, mesh, lights, args, frame, depth, flags, 10)
-        plain_ms, plain_frame_ms = shade_ms(ctx, mesh, lights, args, frame, depth, flags, frames)
-        shade_ms(ctx, mesh, lights, args, frame, depth, flags | rz.RZ_FLAG_MLP_SHADE, 10)
-        mlp_ms, mlp_frame_ms = shade_ms(ctx, mesh, lights, args, frame, depth, flags | rz.RZ_FLAG_MLP_SHADE,
-                                        frames)
-
+        # the shade pass on its own (separate-pass pipeline), plain light loop vs network
+        fused = ctx.fused_shade
+        ctx.set_fused_shade(False)
+        plain_ms = ctx.time_passes(mesh, lights, args, frame, depth, reps=frames)["shade"] / 1000.0
+        mlp_ms = ctx.time_passes(mesh, lights, args, frame, depth, reps=frames,
+                                 flags=rz.RZ_FLAG_MLP_SHADE)["shade"] / 1000.0
+        split_plain_frame_ms = graph_ms(ctx, mesh, lights, args, frame, depth, frames, flags)
+        split_mlp_frame_ms = graph_ms(ctx, mesh, lights, args, frame, depth, frames, flags | rz.RZ_FLAG_MLP_SHADE)
+        ctx.set_fused_shade(True)
+        plain_frame_ms = graph_ms(ctx, mesh, lights, args, frame, depth, frames, flags)
+        mlp_frame_ms = graph_ms(ctx, mesh, lights, args, frame, depth, frames, flags | rz.RZ_FLAG_MLP_SHADE)
+        ctx.set_fused_shade(fused)
         ctx.draw_device(mesh, scene.lights_copy(), args, frame.ptr, depth.ptr, ids.ptr, flags)
         ctx.sync()
         fp, idp = frame.download(), ids.download()
@@ -169,6 +213,8 @@ def learned_shading(ctx, rz, scenes, np, steps, batch, frames):
             "fragments": frags,
             "shade_plain_ms": plain_ms, "shade_mlp_ms": mlp_ms,
             "frame_plain_ms": plain_frame_ms, "frame_mlp_ms": mlp_frame_ms,
+            "frame_plain_ms_separate_shade": split_plain_frame_ms,
+            "frame_mlp_ms_separate_shade": split_mlp_frame_ms,
             "speedup": plain_ms / mlp_ms if mlp_ms > 0 else None,
             "mlp_tflops": frags * MLP_FLOPS_PER_FRAGMENT / (mlp_ms * 1e-3) / 1e12 if mlp_ms > 0 else None,
             "err_mean_per_channel_255": [float(e) for e in err.mean(axis=1)],
@@ -192,6 +238,9 @@ def main():
     ap.add_argument("--tile", type=int, default=16)
     ap.add_argument("--cap", type=int, default=512)
     ap.add_argument("--no-mlp", action="store_true", help="skip the learned-shading measurement")
+    ap.add_argument("--copy-mib", type=int, default=0,
+                    help="first run one device-to-device copy of this many MiB (k_copy), the "
+                         "known-size reference for hardware byte counters")
     ap.add_argument("--mlp-steps", type=int, default=6000)
     ap.add_argument("--mlp-batch", type=int, default=16384)
     a = ap.parse_args()
@@ -208,6 +257,13 @@ def main():
     scene, args = scenes.icosphere_scene(flat=a.flat)
     W, H = args.image_width, args.image_height
     ctx = rz.Context(0)
+    if a.copy_mib > 0:
+        n = a.copy_mib << 20
+        src, dst = ctx.empty((n,), np.uint8), ctx.empty((n,), np.uint8)
+        dst.copy_from(src)
+        log(f"k_copy: {n} bytes read, {n} bytes written")
+        src.close()
+        dst.close()
     ctx.set_tuning(a.tile, a.cap)
     mesh = ctx.upload(scene)
     frame = ctx.empty((3, H, W), np.uint8)
@@ -244,28 +300,27 @@ def main():
     value = scene.num_faces / (ms_per_step / 1000.0) / 1e6
     log(f"graph: {a.steps} frames in {(t1 - t0) * 1000:.2f} ms -> {ms_per_step * 1000:.2f} us/frame")
 
-    # per-pass durations: HIP events on the librz stream around every pass
-    ctx.set_profiling(True)
-    passes = ("vertex", "bin", "raster", "overflow", "shade")
-    acc = {p: 0.0 for p in passes}
-    nprof = max(1, min(a.steps, 500))
-    for _ in range(nprof):
-        s = ctx.draw_device(mesh, lights, args, frame.ptr, depth.ptr, 0, flags, stats=True)
-        for p in passes:
-            acc[p] += s["ms_" + p]
-    ctx.set_profiling(False)
-    avg_ms = {p: acc[p] / nprof for p in passes}
-    nbytes = algorithmic_bytes(counts, scene, args)
-    per_pass = {p: {"avg_us": avg_ms[p] * 1000.0, "alg_bytes": nbytes[p],
-                    "GBps": (nbytes[p] / (avg_ms[p] * 1e-3) / 1e9) if avg_ms[p] > 0 else 0.0}
-                for p in passes}
-    for p in passes:
-        per_pass[p]["frac"] = per_pass[p]["GBps"] / HBM_PEAK_GBS
-    dom = max(passes, key=lambda p: avg_ms[p])
+    # per-pass durations: for each pass a HIP graph of `reps` launches of that kernel on the
+    # librz stream, timed with HIP events (rz_time_passes); first the default fused
+    # pipeline, then with the deferred shade as its own pass over the visibility buffer
+    reps = max(1, min(a.steps, 500))
+    fused = ctx.fused_shade
+    us = ctx.time_passes(mesh, lights, args, frame, depth, reps=reps)
+    per_pass = per_pass_table(us, algorithmic_bytes(counts, scene, args, fused))
+    ctx.set_fused_shade(not fused)
+    us_other = ctx.time_passes(mesh, lights, args, frame, depth, reps=reps)
+    per_pass_other = per_pass_table(us_other, algorithmic_bytes(counts, scene, args, not fused))
+    ctx.set_fused_shade(fused)
     log("per pass:", json.dumps(per_pass))
+    log("per pass, other pipeline:", json.dumps(per_pass_other))
+    dom = max(per_pass, key=lambda p: per_pass[p]["avg_us"])
+    kernel = {"vertex": "k_vertex", "bin": "k_bin", "overflow": "k_overflow",
+              "raster": "k_raster_shade" if fused else "k_raster",
+              "shade": "k_shade_tiles" if fused else "k_shade"}[dom]
     roofline = {"bound": "hbm", "achieved": per_pass[dom]["GBps"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": per_pass[dom]["frac"], "traffic": None, "kernel": f"k_{dom}",
-                "alg_bytes_per_launch": nbytes[dom], "avg_launch_us": per_pass[dom]["avg_us"]}
+                "frac": per_pass[dom]["frac"], "traffic": traffic_bytes(kernel), "kernel": kernel,
+                "alg_bytes_per_launch": per_pass[dom]["alg_bytes"], "avg_launch_us": per_pass[dom]["avg_us"]}
+    avg_ms = {p: us[p] / 1000.0 for p in us}
     gfrag = counts["fragments"] / (avg_ms["raster"] * 1e-3) / 1e9 if avg_ms["raster"] > 0 else None
 
     cpu = None
@@ -298,7 +353,9 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
         "gpu_vs_cpu_frame_rate": (cpu["ms_per_frame"] / ms_per_step) if cpu else None,
+        "pipeline": "fused raster+shade" if fused else "separate shade pass",
         "passes": per_pass,
+        "passes_other_pipeline": per_pass_other,
         "shade64_plain_ms": mlp["shade_plain_ms"] if mlp else None,
         "shade64_mlp_ms": mlp["shade_mlp_ms"] if mlp else None,
         "learned_shading": mlp,

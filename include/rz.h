@@ -176,6 +176,12 @@ int32_t rz_abi_version(void);
  * the next frames.  Defaults: 16 and 512. */
 rz_status rz_set_tuning(rz_ctx* ctx, int32_t tile_size, int32_t bin_capacity);
 rz_status rz_get_tuning(const rz_ctx* ctx, int32_t* tile_size, int32_t* bin_capacity);
+/* Single frames and frame graphs: 1 (default) shades each tile in the raster pass from
+ * its LDS visibility words, so only tiles whose bin overflowed use the HBM visibility
+ * buffer and a separate shade pass; 0 runs the deferred shade over the whole visibility
+ * buffer as its own pass.  Results are identical.  Batches always use the latter. */
+rz_status rz_set_fused_shade(rz_ctx* ctx, int32_t enable);
+int32_t rz_get_fused_shade(const rz_ctx* ctx);
 /* 1: bracket every pass with HIP events so rz_frame_stats carries GPU times. */
 rz_status rz_set_profiling(rz_ctx* ctx, int32_t enable);
 /* The context's hipStream_t, for callers that order their own work against it. */
@@ -266,6 +272,16 @@ rz_status rz_batch_graph_create(rz_ctx* ctx, const rz_batch_item* items, int32_t
 rz_status rz_autotune(rz_ctx* ctx, const rz_batch_item* items, int32_t num_items, uint32_t flags,
                       int32_t* tile_size, int32_t* bin_capacity, float* ms_per_frame);
 
+/* No reference counterpart.  Times each pass of one frame on the context's stream: a
+ * HIP graph of `reps` back-to-back launches of the pass, timed by HIP events around its
+ * second launch.  us[0..4] = vertex, bin, raster, overflow, shade in microseconds per
+ * launch (bin = vertex+bin minus vertex, since binning appends to the tile lists).
+ * Device buffers and RZ_FLAG_CLEARED required; RZ_FLAG_MLP_SHADE times the learned
+ * shader.  Leaves the frame rendered in frame_device/depth_device. */
+rz_status rz_time_passes(rz_ctx* ctx, const rz_mesh* mesh, rz_light* lights, int32_t num_lights,
+                         const rz_args* args, uint8_t* frame_device, float* depth_device, uint32_t flags,
+                         int32_t reps, float us[5]);
+
 /* ---- learned shader ------------------------------------------------------------ */
 
 /* No reference counterpart.  A 12 -> 64 -> 64 -> 3 MLP (ReLU, ReLU, linear) that
@@ -303,6 +319,9 @@ rz_status rz_device_alloc(rz_ctx* ctx, size_t bytes, void** out_ptr);
 rz_status rz_device_free(rz_ctx* ctx, void* ptr);
 rz_status rz_copy_to_device(rz_ctx* ctx, void* dst_device, const void* src_host, size_t bytes);
 rz_status rz_copy_to_host(rz_ctx* ctx, void* dst_host, const void* src_device, size_t bytes);
+/* Device-to-device copy on the context's stream by a plain streaming kernel (16-byte
+ * words; any unaligned remainder by hipMemcpy); waits for it. */
+rz_status rz_copy_device(rz_ctx* ctx, void* dst_device, const void* src_device, size_t bytes);
 
 /* ---- host helpers -------------------------------------------------------------- */
 

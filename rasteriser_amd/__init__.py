@@ -131,6 +131,12 @@ class DeviceArray:
                                             self.nbytes), self.ctx.handle)
         return out
 
+    def copy_from(self, other: "DeviceArray"):
+        """Device-to-device copy of `other`'s bytes into this array (rz_copy_device)."""
+        assert other.nbytes == self.nbytes
+        _check(self.ctx.lib.rz_copy_device(self.ctx.handle, C.c_void_p(self.ptr), C.c_void_p(other.ptr),
+                                           self.nbytes), self.ctx.handle)
+
     def close(self):
         if self.ptr:
             self.ctx.lib.rz_device_free(self.ctx.handle, C.c_void_p(self.ptr))
@@ -176,6 +182,14 @@ class Context:
         t, c = C.c_int32(), C.c_int32()
         _check(self.lib.rz_get_tuning(self.handle, C.byref(t), C.byref(c)), self.handle)
         return t.value, c.value
+
+    def set_fused_shade(self, on: bool):
+        """rz_set_fused_shade: shade in the raster pass (default) or as a separate pass."""
+        _check(self.lib.rz_set_fused_shade(self.handle, int(bool(on))), self.handle)
+
+    @property
+    def fused_shade(self) -> bool:
+        return bool(self.lib.rz_get_fused_shade(self.handle))
 
     def set_profiling(self, on: bool):
         _check(self.lib.rz_set_profiling(self.handle, int(bool(on))), self.handle)
@@ -281,6 +295,17 @@ class Context:
         _check(self.lib.rz_autotune(self.handle, C.addressof(arr), len(batch), flags | RZ_FLAG_DEVICE_BUFFERS,
                                     C.byref(t), C.byref(c), C.byref(ms)), self.handle)
         return t.value, c.value, ms.value
+
+    def time_passes(self, mesh: Mesh, lights: np.ndarray, args: Args, frame: "DeviceArray",
+                    depth: "DeviceArray", reps: int = 200, flags: int = RZ_FLAG_CLEARED) -> dict:
+        """rz_time_passes: microseconds per launch of each pass (graph of `reps` launches)."""
+        lights = _lights(lights)
+        us = (C.c_float * 5)()
+        _check(self.lib.rz_time_passes(self.handle, mesh.handle, lights.ctypes.data, lights.shape[0],
+                                       C.byref(args), C.c_void_p(frame.ptr), C.c_void_p(depth.ptr),
+                                       flags | RZ_FLAG_CLEARED | RZ_FLAG_DEVICE_BUFFERS, int(reps), us),
+               self.handle)
+        return dict(zip(("vertex", "bin", "raster", "overflow", "shade"), [float(u) for u in us]))
 
     # ---- learned shader -------------------------------------------------------------
     def mlp_init(self, seed: int = 0):

@@ -115,6 +115,8 @@ def declare(lib):
         "rz_destroy": (None, [P]),
         "rz_set_tuning": (C.c_int, [P, C.c_int32, C.c_int32]),
         "rz_get_tuning": (C.c_int, [P, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
+        "rz_set_fused_shade": (C.c_int, [P, C.c_int32]),
+        "rz_get_fused_shade": (C.c_int32, [P]),
         "rz_set_profiling": (C.c_int, [P, C.c_int32]),
         "rz_stream": (P, [P]),
         "rz_sync": (C.c_int, [P]),
@@ -132,6 +134,8 @@ def declare(lib):
         "rz_batch_graph_create": (C.c_int, [P, P, C.c_int32, C.c_uint32, C.c_int32, C.POINTER(P)]),
         "rz_autotune": (C.c_int, [P, P, C.c_int32, C.c_uint32, C.POINTER(C.c_int32), C.POINTER(C.c_int32),
                                   C.POINTER(C.c_float)]),
+        "rz_time_passes": (C.c_int, [P, P, P, C.c_int32, C.POINTER(Args), P, P, C.c_uint32, C.c_int32,
+                                     C.POINTER(C.c_float)]),
         "rz_mlp_init": (C.c_int, [P, C.c_uint64]),
         "rz_mlp_set_weights": (C.c_int, [P, P]),
         "rz_mlp_get_weights": (C.c_int, [P, P]),
@@ -143,6 +147,7 @@ def declare(lib):
         "rz_device_free": (C.c_int, [P, P]),
         "rz_copy_to_device": (C.c_int, [P, P, P, C.c_size_t]),
         "rz_copy_to_host": (C.c_int, [P, P, P, C.c_size_t]),
+        "rz_copy_device": (C.c_int, [P, P, P, C.c_size_t]),
         "rz_default_args": (None, [C.POINTER(Args), C.c_uint32, C.c_uint32]),
         "rz_frame_matrices": (C.c_int, [C.POINTER(Args), C.POINTER(Matrices)]),
     }
@@ -154,11 +159,11 @@ def declare(lib):
 
 
 EXPORTED = ("rz_abi_version", "rz_status_string", "rz_last_error", "rz_create", "rz_destroy",
-            "rz_set_tuning", "rz_get_tuning", "rz_set_profiling", "rz_stream", "rz_sync",
+            "rz_set_tuning", "rz_get_tuning", "rz_set_fused_shade", "rz_get_fused_shade", "rz_set_profiling", "rz_stream", "rz_sync",
             "rz_mesh_create", "rz_mesh_destroy", "rz_draw_frame", "rz_draw_frame_desc",
             "rz_graph_create", "rz_graph_launch", "rz_graph_destroy",
             "rz_draw_batch", "rz_batch_graph_create", "rz_autotune",
-            "rz_mlp_init", "rz_mlp_set_weights", "rz_mlp_get_weights", "rz_mlp_collect",
+            "rz_time_passes", "rz_mlp_init", "rz_mlp_set_weights", "rz_mlp_get_weights", "rz_mlp_collect",
             "rz_mlp_train", "rz_mlp_infer",
-            "rz_device_alloc", "rz_device_free", "rz_copy_to_device", "rz_copy_to_host",
+            "rz_device_alloc", "rz_device_free", "rz_copy_to_device", "rz_copy_to_host", "rz_copy_device",
             "rz_default_args", "rz_frame_matrices")

@@ -19,6 +19,8 @@ namespace rz {
 hipError_t launch_vertex(const VertexParams& p, hipStream_t s);
 hipError_t launch_bin(const BinParams& p, hipStream_t s);
 hipError_t launch_raster(const RasterParams& p, int tile_size, hipStream_t s);
+hipError_t launch_raster_shade(const RasterParams& p, const ShadeParams& sp, int tile_size, hipStream_t s);
+hipError_t launch_shade_tiles(const RasterParams& p, const ShadeParams& sp, int tile_size, hipStream_t s);
 hipError_t launch_overflow(const RasterParams& p, hipStream_t s);
 hipError_t launch_shade(const ShadeParams& p, hipStream_t s);
 hipError_t launch_vertex_batch(const VertexParams* items, const uint2* map, uint32_t blocks, hipStream_t s);
@@ -28,6 +30,8 @@ hipError_t launch_raster_batch(const RasterParams* items, const uint2* map, uint
 hipError_t launch_overflow_batch(const RasterParams* items, const uint2* map, uint32_t blocks, hipStream_t s);
 hipError_t launch_shade_batch(const ShadeParams* items, const uint2* map, uint32_t blocks, int mode,
                               hipStream_t s);
+hipError_t launch_hold(uint64_t ticks, hipStream_t s);
+hipError_t launch_copy(const void* src, void* dst, size_t n16, hipStream_t s);
 hipError_t launch_mlp_pack(const float* W, uint16_t* packed, hipStream_t s);
 hipError_t launch_mlp_infer(const float* x, int64_t n, const uint16_t* packed, float* y, hipStream_t s);
 hipError_t launch_mlp_step(const float* samples, int64_t n, float* W, float* m, float* v, float* gpart,
@@ -79,6 +83,7 @@ struct rz_ctx {
     hipStream_t stream = nullptr;
     int tile_size = 16;
     int bin_cap = 512;
+    bool fused = true;
     bool profiling = false;
     hipEvent_t ev[6] = {};
     std::string err;
@@ -189,6 +194,7 @@ rz_status validate_mesh(rz_ctx* ctx, const rz_mesh_desc* d) {
 // Sizes and statistics sections of one frame's working buffers.
 struct FrameLayout {
     int ts = 16, tsl = 4, tiles_x = 0, tiles_y = 0, ntiles = 0;
+    bool fused = false;     // raster pass shades non-overflowed tiles; coverage per tile
     uint32_t W = 0, H = 0, cap = 0;
     size_t np = 0;
     int32_t nfn = 0;
@@ -199,9 +205,10 @@ struct FrameLayout {
 
 enum { kBufRaster, kBufCnormal, kBufRec, kBufTiles, kBufBins, kBufVis, kBufCounters, kBufPart, kNumBufs };
 
-FrameLayout layout_frame(const rz_mesh* mesh, const rz_args& a, int ts, int cap) {
+FrameLayout layout_frame(const rz_mesh* mesh, const rz_args& a, int ts, int cap, bool fused) {
     FrameLayout L;
     L.ts = ts;
+    L.fused = fused;
     L.tsl = log2_exact(ts);
     L.W = a.image_width;
     L.H = a.image_height;
@@ -216,11 +223,11 @@ FrameLayout layout_frame(const rz_mesh* mesh, const rz_args& a, int ts, int cap)
     L.off_frag = L.off_pairs + bin_blocks(mesh->nf);
     L.off_ovf = L.off_frag + (size_t)L.ntiles;
     L.off_cov = L.off_ovf + overflow_blocks(mesh->nf);
-    L.n_part = L.off_cov + (size_t)shade_blocks_x(L.W) * shade_blocks_y(L.H);
+    L.n_part = L.off_cov + (fused ? (size_t)L.ntiles : (size_t)shade_blocks_x(L.W) * shade_blocks_y(L.H));
     L.bytes[kBufRaster] = (size_t)mesh->nv * 16;
     L.bytes[kBufCnormal] = (size_t)L.nfn * 16;
     L.bytes[kBufRec] = (size_t)mesh->nf * sizeof(FaceRecord);
-    L.bytes[kBufTiles] = (size_t)L.ntiles * 4;
+    L.bytes[kBufTiles] = (size_t)L.ntiles * 4 * kCountStride;
     L.bytes[kBufBins] = (size_t)L.ntiles * L.cap * 4;
     L.bytes[kBufVis] = L.np * 8;
     L.bytes[kBufCounters] = kNumCounters * 8;
@@ -345,7 +352,7 @@ rz_status prepare_frame(rz_ctx* ctx, const rz_mesh* mesh, rz_light* lights, int3
     rz_status st = check_frame(ctx, mesh, lights, nl, args);
     if (st != RZ_OK) return st;
     RZ_HIP(ctx, hipSetDevice(ctx->device));
-    const FrameLayout lay = layout_frame(mesh, *args, ctx->tile_size, ctx->bin_cap);
+    const FrameLayout lay = layout_frame(mesh, *args, ctx->tile_size, ctx->bin_cap, ctx->fused);
     DevBuf* bufs[kNumBufs] = {&ctx->raster, &ctx->cnormal, &ctx->rec, &ctx->tile_count,
                               &ctx->bins,   &ctx->vis,     &ctx->counters, &ctx->partials};
     void* ptrs[kNumBufs];
@@ -368,17 +375,27 @@ void bind_outputs(FrameLaunch& L, bool cleared, uint8_t* frame_d, float* depth_d
     L.sp.ids = ids_d;
 }
 
+// 200 us of the 100 MHz clock: longer than the host takes to queue one frame
+constexpr uint64_t kHoldTicks = 20000;
+
 rz_status enqueue_frame(rz_ctx* ctx, const FrameLaunch& L, hipStream_t s, bool prof) {
+    if (prof) RZ_HIP(ctx, launch_hold(kHoldTicks, s));
     if (prof) RZ_HIP(ctx, hipEventRecord(ctx->ev[0], s));
     RZ_HIP(ctx, launch_vertex(L.vp, s));
     if (prof) RZ_HIP(ctx, hipEventRecord(ctx->ev[1], s));
     RZ_HIP(ctx, launch_bin(L.bp, s));
     if (prof) RZ_HIP(ctx, hipEventRecord(ctx->ev[2], s));
-    RZ_HIP(ctx, launch_raster(L.rp, L.lay.ts, s));
+    if (L.lay.fused)
+        RZ_HIP(ctx, launch_raster_shade(L.rp, L.sp, L.lay.ts, s));
+    else
+        RZ_HIP(ctx, launch_raster(L.rp, L.lay.ts, s));
     if (prof) RZ_HIP(ctx, hipEventRecord(ctx->ev[3], s));
     RZ_HIP(ctx, launch_overflow(L.orp, s));
     if (prof) RZ_HIP(ctx, hipEventRecord(ctx->ev[4], s));
-    RZ_HIP(ctx, launch_shade(L.sp, s));
+    if (L.lay.fused)
+        RZ_HIP(ctx, launch_shade_tiles(L.rp, L.sp, L.lay.ts, s));
+    else
+        RZ_HIP(ctx, launch_shade(L.sp, s));
     if (prof) RZ_HIP(ctx, hipEventRecord(ctx->ev[5], s));
     return RZ_OK;
 }
@@ -438,7 +455,7 @@ rz_status prepare_batch(rz_ctx* ctx, const rz_batch_item* items, int32_t n, uint
         rz_status st = check_frame(ctx, it.mesh, it.lights, it.num_lights, &it.args);
         if (st != RZ_OK) return st;
         if (!it.frame || !it.depth) return fail(ctx, RZ_ERR_INVALID_ARG, "batch item %d: null frame or depth", i);
-        lays[i] = layout_frame(it.mesh, it.args, ctx->tile_size, ctx->bin_cap);
+        lays[i] = layout_frame(it.mesh, it.args, ctx->tile_size, ctx->bin_cap, false);
         stat_bytes += align256(lays[i].bytes[kBufCounters]) + align256(lays[i].bytes[kBufPart]);
         for (int b = 0; b < kBufCounters; ++b) work_bytes += align256(lays[i].bytes[b]);
     }
@@ -512,6 +529,7 @@ rz_status prepare_batch(rz_ctx* ctx, const rz_batch_item* items, int32_t n, uint
 }
 
 rz_status enqueue_batch(rz_ctx* ctx, const BatchLaunch& B, hipStream_t s, bool prof) {
+    if (prof) RZ_HIP(ctx, launch_hold(kHoldTicks, s));
     if (prof) RZ_HIP(ctx, hipEventRecord(ctx->ev[0], s));
     RZ_HIP(ctx, launch_vertex_batch(B.d_vp, B.d_map[0], B.blocks[0], s));
     if (prof) RZ_HIP(ctx, hipEventRecord(ctx->ev[1], s));
@@ -669,6 +687,14 @@ rz_status rz_set_tuning(rz_ctx* ctx, int32_t tile_size, int32_t bin_capacity) {
     return RZ_OK;
 }
 
+rz_status rz_set_fused_shade(rz_ctx* ctx, int32_t enable) {
+    if (!ctx) return RZ_ERR_INVALID_ARG;
+    ctx->fused = enable != 0;
+    return RZ_OK;
+}
+
+int32_t rz_get_fused_shade(const rz_ctx* ctx) { return ctx && ctx->fused ? 1 : 0; }
+
 rz_status rz_get_tuning(const rz_ctx* ctx, int32_t* tile_size, int32_t* bin_capacity) {
     if (!ctx) return RZ_ERR_INVALID_ARG;
     if (tile_size) *tile_size = ctx->tile_size;
@@ -720,6 +746,19 @@ rz_status rz_copy_to_host(rz_ctx* ctx, void* dst, const void* src, size_t bytes)
     if (!ctx || (bytes && (!dst || !src))) return RZ_ERR_INVALID_ARG;
     RZ_HIP(ctx, hipSetDevice(ctx->device));
     RZ_HIP(ctx, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
+    RZ_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return RZ_OK;
+}
+
+rz_status rz_copy_device(rz_ctx* ctx, void* dst, const void* src, size_t bytes) {
+    if (!ctx || (bytes && (!dst || !src))) return RZ_ERR_INVALID_ARG;
+    RZ_HIP(ctx, hipSetDevice(ctx->device));
+    const bool aligned = ((uintptr_t)dst % 16 == 0) && ((uintptr_t)src % 16 == 0);
+    const size_t head = aligned ? bytes / 16 * 16 : 0;
+    RZ_HIP(ctx, launch_copy(src, dst, head / 16, ctx->stream));
+    if (bytes > head)
+        RZ_HIP(ctx, hipMemcpyAsync((uint8_t*)dst + head, (const uint8_t*)src + head, bytes - head,
+                                   hipMemcpyDeviceToDevice, ctx->stream));
     RZ_HIP(ctx, hipStreamSynchronize(ctx->stream));
     return RZ_OK;
 }
@@ -875,6 +914,72 @@ void rz_mesh_destroy(rz_ctx* ctx, rz_mesh* m) {
     }
     for (void* p : m->allocs) (void)hipFree(p);
     delete m;
+}
+
+rz_status rz_time_passes(rz_ctx* ctx, const rz_mesh* mesh, rz_light* lights, int32_t nl, const rz_args* args,
+                         uint8_t* frame_device, float* depth_device, uint32_t flags, int32_t reps, float us[5]) {
+    if (!ctx) return RZ_ERR_INVALID_ARG;
+    if (!frame_device || !depth_device || !us || reps < 1 || !(flags & RZ_FLAG_CLEARED))
+        return fail(ctx, RZ_ERR_INVALID_ARG, "rz_time_passes: device buffers, reps >= 1 and RZ_FLAG_CLEARED required");
+    if ((flags & RZ_FLAG_MLP_SHADE) && !ctx->mlp_ready)
+        return fail(ctx, RZ_ERR_INVALID_ARG, "RZ_FLAG_MLP_SHADE without network weights");
+    FrameLaunch L;
+    rz_status st = prepare_frame(ctx, mesh, lights, nl, args, flags, L);
+    if (st != RZ_OK) return st;
+    bind_outputs(L, true, frame_device, depth_device, nullptr);
+    hipStream_t s = ctx->stream;
+    // one graph of `reps` launches of the pass(es) `enqueue_one` issues; ms of the second launch
+    auto timed = [&](auto enqueue_one, float& ms) -> rz_status {
+        hipGraphExec_t exec = nullptr;
+        rz_status r = capture(ctx, reps, [&](int32_t, hipStream_t cs) { return enqueue_one(cs); }, &exec);
+        if (r != RZ_OK) return r;
+        hipError_t e = hipGraphLaunch(exec, s);
+        if (e == hipSuccess) e = hipEventRecord(ctx->ev[0], s);
+        if (e == hipSuccess) e = hipGraphLaunch(exec, s);
+        if (e == hipSuccess) e = hipEventRecord(ctx->ev[5], s);
+        if (e == hipSuccess) e = hipStreamSynchronize(s);
+        if (e == hipSuccess) e = hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[5]);
+        (void)hipGraphExecDestroy(exec);
+        if (e != hipSuccess) return fail(ctx, RZ_ERR_HIP, "rz_time_passes: %s", hipGetErrorString(e));
+        return RZ_OK;
+    };
+    auto vertex = [&](hipStream_t cs) -> rz_status { RZ_HIP(ctx, launch_vertex(L.vp, cs)); return RZ_OK; };
+    auto vertex_bin = [&](hipStream_t cs) -> rz_status {
+        RZ_HIP(ctx, launch_vertex(L.vp, cs));
+        RZ_HIP(ctx, launch_bin(L.bp, cs));
+        return RZ_OK;
+    };
+    auto raster = [&](hipStream_t cs) -> rz_status {
+        if (L.lay.fused)
+            RZ_HIP(ctx, launch_raster_shade(L.rp, L.sp, L.lay.ts, cs));
+        else
+            RZ_HIP(ctx, launch_raster(L.rp, L.lay.ts, cs));
+        return RZ_OK;
+    };
+    auto overflow = [&](hipStream_t cs) -> rz_status { RZ_HIP(ctx, launch_overflow(L.orp, cs)); return RZ_OK; };
+    auto shade = [&](hipStream_t cs) -> rz_status {
+        if (L.lay.fused)
+            RZ_HIP(ctx, launch_shade_tiles(L.rp, L.sp, L.lay.ts, cs));
+        else
+            RZ_HIP(ctx, launch_shade(L.sp, cs));
+        return RZ_OK;
+    };
+    float ms_v = 0, ms_vb = 0, ms_r = 0, ms_o = 0, ms_s = 0;
+    // vertex alone, then vertex+bin (each repetition leaves a whole binned frame behind);
+    // raster, overflow and shade are idempotent on that state
+    if ((st = enqueue_frame(ctx, L, s, false)) != RZ_OK) return st;
+    if ((st = timed(vertex, ms_v)) != RZ_OK) return st;
+    if ((st = timed(vertex_bin, ms_vb)) != RZ_OK) return st;
+    if ((st = timed(raster, ms_r)) != RZ_OK) return st;
+    if ((st = timed(overflow, ms_o)) != RZ_OK) return st;
+    if ((st = timed(shade, ms_s)) != RZ_OK) return st;
+    const float k = 1000.0f / (float)reps;
+    us[0] = ms_v * k;
+    us[1] = (ms_vb - ms_v) * k;
+    us[2] = ms_r * k;
+    us[3] = ms_o * k;
+    us[4] = ms_s * k;
+    return RZ_OK;
 }
 
 rz_status rz_draw_frame(rz_ctx* ctx, const rz_mesh* mesh, rz_light* lights, int32_t nl, const rz_args* args,
@@ -1034,8 +1139,8 @@ rz_status rz_autotune(rz_ctx* ctx, const rz_batch_item* items, int32_t n, uint32
         std::vector<uint32_t> counts;
         for (const FrameLaunch& L : B.frames) {
             counts.resize(L.lay.ntiles);
-            RZ_HIP(ctx, hipMemcpyAsync(counts.data(), L.vp.tile_count, counts.size() * 4, hipMemcpyDeviceToHost,
-                                       ctx->stream));
+            RZ_HIP(ctx, hipMemcpy2DAsync(counts.data(), 4, L.vp.tile_count, 4 * kCountStride, 4, counts.size(),
+                                         hipMemcpyDeviceToHost, ctx->stream));
             RZ_HIP(ctx, hipStreamSynchronize(ctx->stream));
             for (uint32_t c : counts) need = c > need ? c : need;
             tiles += L.lay.ntiles;

@@ -16,6 +16,9 @@ constexpr int kMaxLights = 64;
 constexpr uint64_t kEmptyKey = ~0ull;           // visibility word of a pixel with no fragment
 constexpr uint32_t kNoFace = 0xFFFFFFFFu;
 constexpr int kBigTriangleTiles = 16;           // above this many tiles a face is binned by its whole wave
+// Tile counters sit one per 128-byte line: device-scope atomics on one line serialise,
+// so packing 32 counters per line would queue every face of 32 tiles behind each other.
+constexpr uint32_t kCountStride = 32;
 
 // Global counter slots (uint64) cleared by the vertex pass each frame.  Only the
 // overflow-tile count is a device-wide atomic (it is rare and doubles as the flag
@@ -146,6 +149,29 @@ __host__ __device__ inline uint32_t overflow_blocks(int32_t nf) {
 }
 __host__ __device__ inline uint32_t shade_blocks_x(uint32_t W) { return (W + 63) / 64; }
 __host__ __device__ inline uint32_t shade_blocks_y(uint32_t H) { return (H + 3) / 4; }
+
+// XCD-aware placement of a 2-D grid of work units.  Workgroups are dealt to the eight
+// XCDs round robin (workgroup b runs on XCD b % 8), and each XCD has its own L2.  Units
+// are grouped into 4x4 (raster tiles) or 1x16 (shade strips) groups of 16; group g runs
+// on XCD g % 8, its 16 units as consecutive workgroups of that XCD, so neighbouring
+// units share one L2 while neighbouring groups spread over all XCDs.
+struct XcdGroups {
+    uint32_t gw, gh;        // group shape in units (gw * gh == 16)
+    uint32_t nx, ny;        // units
+    __host__ __device__ uint32_t groups_x() const { return (nx + gw - 1) / gw; }
+    __host__ __device__ uint32_t groups() const { return groups_x() * ((ny + gh - 1) / gh); }
+    // workgroups to launch: every XCD gets the same number of whole groups
+    __host__ __device__ uint32_t grid() const { return (groups() + 7) / 8 * 8 * 16; }
+    // unit index (y * nx + x) of workgroup b, or -1 for padding
+    __host__ __device__ int unit(uint32_t b) const {
+        const uint32_t xcd = b & 7u, k = b >> 3, g = (k >> 4) * 8u + xcd, t = k & 15u;
+        if (g >= groups()) return -1;
+        const uint32_t gx = groups_x();
+        const uint32_t x = (g % gx) * gw + t % gw, y = (g / gx) * gh + t / gw;
+        if (x >= nx || y >= ny) return -1;
+        return (int)(y * nx + x);
+    }
+};
 
 // ---------------------------------------------------------------------------------
 // Arithmetic (host+device so host-side unit checks can call the same code).

@@ -79,13 +79,13 @@ __device__ __forceinline__ void vertex_body(const VertexParams& p, int blk) {
         n.w = 0.0f;
         p.cnormal[i] = n;
     }
-    if (i < p.ntiles) p.tile_count[i] = 0u;
+    if (i < p.ntiles) p.tile_count[(size_t)i * kCountStride] = 0u;
     if (i < kNumCounters) p.counters[i] = 0ull;
 }
 
 // ---------------------------------------------------------------------------------
 __device__ __forceinline__ void bin_push(const BinParams& p, uint32_t face, uint32_t tile) {
-    const uint32_t slot = atomicAdd(&p.tile_count[tile], 1u);
+    const uint32_t slot = atomicAdd(&p.tile_count[(size_t)tile * kCountStride], 1u);
     if (slot < p.cap)
         p.bins[(size_t)tile * p.cap + slot] = face;
     else if (slot == p.cap)
@@ -165,7 +165,7 @@ __device__ __forceinline__ void bin_body(const BinParams& p, int blk) {
             const uint32_t c = s_cnt[i];
             if (!c) continue;
             const uint32_t tile = (uint32_t)(uy0 + i / uw) * p.tiles_x + (uint32_t)(ux0 + i % uw);
-            const uint32_t base = atomicAdd(&p.tile_count[tile], c);
+            const uint32_t base = atomicAdd(&p.tile_count[(size_t)tile * kCountStride], c);
             s_base[i] = base;
             if (base <= p.cap && base + c > p.cap) atomicAdd(&p.counters[kCntOverflowTiles], 1ull);
         }
@@ -197,171 +197,6 @@ __device__ __forceinline__ void bin_body(const BinParams& p, int blk) {
             bin_push(p, bf, ty * p.tiles_x + tx);
         }
     }
-}
-
-// ---------------------------------------------------------------------------------
-// Inside + depth test of one (face, pixel) candidate; returns true when inside.
-__device__ __forceinline__ bool fragment_key(float pxf, float pyf, float x0, float y0, float z0, float x1,
-                                             float y1, float z1, float x2, float y2, float z2, float area,
-                                             float& z) {
-    const float e0 = edge(pxf, pyf, x1, y1, x2, y2);
-    const float e1 = edge(pxf, pyf, x2, y2, x0, y0);
-    const float e2 = edge(pxf, pyf, x0, y0, x1, y1);
-    const float b0 = e0 / area, b1 = e1 / area, b2 = e2 / area;
-    if (!(b0 >= 0.0f && b1 >= 0.0f && b2 >= 0.0f)) return false;
-    z = (z0 * b0 + z1 * b1) + z2 * b2;
-    return true;
-}
-
-template <int TS, int BLOCK>
-__device__ __forceinline__ void raster_body(const RasterParams& p, int blk) {
-    constexpr int NPX = TS * TS;
-    constexpr int PPT = NPX / BLOCK;
-    constexpr int NW = BLOCK / 64;
-    static_assert(PPT >= 1 && NPX % BLOCK == 0, "tile must cover the block");
-    __shared__ unsigned long long s_key[NPX];
-    __shared__ float s_d0[NPX];
-    __shared__ float s_v[10][BLOCK];   // x0 y0 z0 x1 y1 z1 x2 y2 z2 area
-    __shared__ uint32_t s_rect[BLOCK]; // rx0 | ry0 << 8 | rw << 16 (tile-local)
-    __shared__ uint32_t s_face[BLOCK];
-    __shared__ uint32_t s_off[BLOCK];
-    __shared__ uint32_t s_wsum[NW];
-
-    const int tid = threadIdx.x;
-    const int lane = tid & 63, wave = tid >> 6;
-    // Consecutive tiles go to different XCDs (workgroups are dealt round robin), which
-    // spreads a screen region's heavy tiles over all eight.
-    const int tile = blk;
-    const int tx = tile % p.tiles_x, ty = tile / p.tiles_x;
-    const int ox = tx * TS, oy = ty * TS;
-
-#pragma unroll
-    for (int j = 0; j < PPT; ++j) {
-        const int l = tid + j * BLOCK;
-        const uint32_t x = ox + (l % TS), y = oy + (l / TS);
-        float d0 = 1.0f;
-        if (p.depth_in && x < p.W && y < p.H) d0 = p.depth_in[(size_t)y * p.W + x];
-        s_key[l] = kEmptyKey;
-        s_d0[l] = d0;
-    }
-    const uint32_t count = p.tile_count[tile];
-    const uint32_t n = count > p.cap ? 0u : count;   // overflowed tiles go to k_overflow
-    uint32_t frags = 0;
-    __syncthreads();
-
-    for (uint32_t base = 0; base < n; base += BLOCK) {
-        const uint32_t i = base + tid;
-        uint32_t cnt = 0;
-        if (i < n) {
-            const uint32_t f = p.bins[(size_t)tile * p.cap + i];
-            const FaceRecord* r = p.rec + f;
-            const float4 a = r->r0, b = r->r1, c = r->r2;
-            const uint32_t lo = __float_as_uint(b.w), hi = __float_as_uint(c.w);
-            const int rx0 = max((int)(lo & 0xFFFFu) - ox, 0), rx1 = min((int)(hi & 0xFFFFu) - ox, TS - 1);
-            const int ry0 = max((int)(lo >> 16) - oy, 0), ry1 = min((int)(hi >> 16) - oy, TS - 1);
-            uint32_t rw = 0;
-            if (rx0 <= rx1 && ry0 <= ry1) {
-                rw = rx1 - rx0 + 1;
-                cnt = rw * (ry1 - ry0 + 1);
-            }
-            s_v[0][tid] = a.x; s_v[1][tid] = a.y; s_v[2][tid] = a.z;
-            s_v[3][tid] = b.x; s_v[4][tid] = b.y; s_v[5][tid] = b.z;
-            s_v[6][tid] = c.x; s_v[7][tid] = c.y; s_v[8][tid] = c.z;
-            s_v[9][tid] = a.w;
-            s_rect[tid] = (uint32_t)rx0 | ((uint32_t)ry0 << 8) | (rw << 16);
-            s_face[tid] = f;
-        }
-        // block-wide exclusive prefix sum of candidate counts
-        uint32_t v = cnt;
-#pragma unroll
-        for (int d = 1; d < 64; d <<= 1) {
-            const uint32_t t = __shfl_up(v, d, 64);
-            if (lane >= d) v += t;
-        }
-        if (lane == 63) s_wsum[wave] = v;
-        __syncthreads();
-        uint32_t wbase = 0, total = 0;
-#pragma unroll
-        for (int w = 0; w < NW; ++w) {
-            const uint32_t s = s_wsum[w];
-            if (w < wave) wbase += s;
-            total += s;
-        }
-        s_off[tid] = wbase + v - cnt;
-        __syncthreads();
-
-        const int nc = (int)min(n - base, (uint32_t)BLOCK);
-        for (uint32_t k = tid; k < total; k += BLOCK) {
-            int lo = 0, hi = nc - 1;
-            while (lo < hi) {
-                const int mid = (lo + hi + 1) >> 1;
-                if (s_off[mid] <= k) lo = mid; else hi = mid - 1;
-            }
-            const int j = lo;
-            const uint32_t rect = s_rect[j];
-            const uint32_t rw = rect >> 16;
-            const uint32_t l = k - s_off[j];
-            // l / rw exactly: |error| of the product < 1/(2*rw) for l < 1024
-            const uint32_t dy = (uint32_t)(((float)l + 0.5f) * __builtin_amdgcn_rcpf((float)rw));
-            const uint32_t dx = l - dy * rw;
-            const int lx = (int)(rect & 0xFFu) + (int)dx, ly = (int)((rect >> 8) & 0xFFu) + (int)dy;
-            float z;
-            if (fragment_key((float)(ox + lx), (float)(oy + ly), s_v[0][j], s_v[1][j], s_v[2][j], s_v[3][j],
-                             s_v[4][j], s_v[5][j], s_v[6][j], s_v[7][j], s_v[8][j], s_v[9][j], z)) {
-                ++frags;
-                const int pl = ly * TS + lx;
-                if (z < s_d0[pl]) atomicMin(&s_key[pl], vis_key(z, s_face[j]));
-            }
-        }
-        __syncthreads();
-    }
-
-#pragma unroll
-    for (int j = 0; j < PPT; ++j) {
-        const int l = tid + j * BLOCK;
-        const uint32_t x = ox + (l % TS), y = oy + (l / TS);
-        if (x < p.W && y < p.H) p.vis[(size_t)y * p.W + x] = s_key[l];
-    }
-    store_block_sum(frags, p.part_frag + blk, tid, BLOCK);
-}
-
-// ---------------------------------------------------------------------------------
-__device__ __forceinline__ void overflow_body(const RasterParams& p, int blk, int nblk) {
-    const int32_t ts_log2 = p.ts_log2;
-    if (p.counters[kCntOverflowTiles] == 0ull) {
-        if (threadIdx.x == 0) p.part_frag[blk] = 0u;
-        return;
-    }
-    const uint32_t ts = 1u << ts_log2;
-    uint32_t frags = 0;
-    for (int f = blk * 256 + threadIdx.x; f < p.nf; f += nblk * 256) {
-        const FaceRecord* r = p.rec + f;
-        const float4 b = r->r1, c = r->r2;
-        const uint32_t lo = __float_as_uint(b.w), hi = __float_as_uint(c.w);
-        const uint32_t bx0 = lo & 0xFFFFu, by0 = lo >> 16, bx1 = hi & 0xFFFFu, by1 = hi >> 16;
-        if (bx0 > bx1 || by0 > by1) continue;
-        const float4 a = r->r0;
-        for (uint32_t ty = by0 >> ts_log2; ty <= (by1 >> ts_log2); ++ty) {
-            for (uint32_t tx = bx0 >> ts_log2; tx <= (bx1 >> ts_log2); ++tx) {
-                const uint32_t tile = ty * p.tiles_x + tx;
-                if (p.tile_count[tile] <= p.cap) continue;
-                const uint32_t x0 = max(bx0, tx * ts), x1 = min(bx1, tx * ts + ts - 1);
-                const uint32_t y0 = max(by0, ty * ts), y1 = min(by1, ty * ts + ts - 1);
-                for (uint32_t y = y0; y <= y1; ++y) {
-                    for (uint32_t x = x0; x <= x1; ++x) {
-                        float z;
-                        if (!fragment_key((float)x, (float)y, a.x, a.y, a.z, b.x, b.y, b.z, c.x, c.y, c.z, a.w, z))
-                            continue;
-                        ++frags;
-                        const size_t pi = (size_t)y * p.W + x;
-                        const float d0 = p.depth_in ? p.depth_in[pi] : 1.0f;
-                        if (z < d0) atomicMin(&p.vis[pi], vis_key(z, (uint32_t)f));
-                    }
-                }
-            }
-        }
-    }
-    store_block_sum(frags, p.part_frag + blk, threadIdx.x, 256);
 }
 
 // ---------------------------------------------------------------------------------
@@ -427,30 +262,27 @@ __device__ __forceinline__ void bend_normal(const ShadeParams& p, uint32_t f, co
     nx = qx * rq; ny = qy * rq; nz = qz * rq;
 }
 
-// 64 x 4 pixel blocks: each wave shades 64 consecutive pixels of one row.
+// Normalisation tables of the first kLdsLuts materials, staged in LDS by the whole
+// workgroup (a barrier must follow before they are read).
+constexpr int kLdsLuts = 4;
+__device__ __forceinline__ void stage_luts(const ShadeParams& p, float* s_lut, int tid, int nthreads) {
+    const int n = (p.nmat < kLdsLuts ? p.nmat : kLdsLuts) * 256;
+    for (int e = tid; e < n; e += nthreads) s_lut[e] = p.luts[e];
+}
+
+// Shades one pixel from its visibility word; every lane of the wave calls it together
+// (the learned shader and sample collection work per wave).  Writes frame, depth and
+// ids; returns whether the pixel has a winning fragment.
 // MODE: kShadePlain (the reference's light loop), kShadeMlp (the learned surrogate on
 // matrix cores), kShadeCollect (light loop, and each shaded fragment's MLP inputs and
 // colour/255 appended to the sample buffer for training).
 template <int MODE>
-__device__ __forceinline__ void shade_body(const ShadeParams& p, int blk) {
-    // normalisation tables of the first kLdsLuts materials, staged in LDS
-    constexpr int kLdsLuts = 4;
-    __shared__ float s_lut[kLdsLuts * 256];
-    {
-        const int t = threadIdx.y * 64 + threadIdx.x;
-        const int nl = p.nmat < kLdsLuts ? p.nmat : kLdsLuts;
-#pragma unroll
-        for (int j = 0; j < kLdsLuts; ++j)
-            if (j < nl) s_lut[j * 256 + t] = p.luts[j * 256 + t];
-        __syncthreads();
-    }
+__device__ __forceinline__ bool shade_pixel(const ShadeParams& p, const float* s_lut, const uint16_t* s_net,
+                                            uint32_t px, uint32_t py, bool valid, unsigned long long key, int lane,
+                                            uint16_t* s_in, float* s_out) {
     const size_t np = (size_t)p.W * p.H;
-    const uint32_t nbx = shade_blocks_x(p.W);
-    const uint32_t px = (blk % nbx) * 64 + threadIdx.x, py = (blk / nbx) * 4 + threadIdx.y;
-    const bool valid = px < p.W && py < p.H;
     const size_t i = (size_t)py * p.W + px;
-    const unsigned long long key = valid ? p.vis[i] : kEmptyKey;
-    const bool hit = key != kEmptyKey;
+    const bool hit = valid && key != kEmptyKey;
     float feat[kMlpIn];
     uint8_t rgb[3] = {0, 0, 0};
     if (valid && !hit) {
@@ -513,7 +345,7 @@ __device__ __forceinline__ void shade_body(const ShadeParams& p, int blk) {
         feat[0] = nx; feat[1] = ny; feat[2] = nz;
         if (m.nmap_w > 0) {
             normal_map_sample(p, m, u, v, ts);
-            bend_normal(p, f, ts, nx, ny, nz);
+            if (MODE != kShadeMlp) bend_normal(p, f, ts, nx, ny, nz);   // the network takes the sample itself
         }
         p.depth[i] = z;
         if (p.ids) p.ids[i] = (int32_t)f;
@@ -545,7 +377,6 @@ __device__ __forceinline__ void shade_body(const ShadeParams& p, int blk) {
     if (MODE == kShadeCollect) {
         // append (inputs, colour/255) of this wave's shaded fragments
         const unsigned long long bal = __ballot(hit);
-        const int lane = threadIdx.x;
         unsigned long long base = 0;
         if (lane == 0 && bal) base = atomicAdd(p.sample_count, (unsigned long long)__popcll(bal));
         base = __shfl(base, 0, 64);
@@ -557,18 +388,15 @@ __device__ __forceinline__ void shade_body(const ShadeParams& p, int blk) {
         }
     }
     if (MODE == kShadeMlp) {
-        __shared__ uint16_t s_in[4][64 * 16];
-        __shared__ float s_out[4][64 * 4];
-        const int wave = threadIdx.y, lane = threadIdx.x;
 #pragma unroll
-        for (int k = 0; k < 16; ++k) s_in[wave][lane * 16 + k] = to_bf16((hit && k < kMlpIn) ? feat[k] : 0.f);
+        for (int k = 0; k < 16; ++k) s_in[lane * 16 + k] = to_bf16((hit && k < kMlpIn) ? feat[k] : 0.f);
         if (__ballot(hit)) {   // uniform over the wave: waves with no fragment skip the network
             __builtin_amdgcn_wave_barrier();
-            mlp_wave(p.mlp, s_in[wave], s_out[wave], lane);
+            mlp_wave(s_net, s_in, s_out, lane);
             __builtin_amdgcn_wave_barrier();
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
-                const float y = s_out[wave][lane * 4 + c] * 255.0f;
+                const float y = s_out[lane * 4 + c] * 255.0f;
                 const float yc = y > 0.f ? (y < 255.f ? y : 255.f) : 0.f;
                 rgb[c] = (uint8_t)(uint32_t)(yc + 0.5f);
             }
@@ -579,7 +407,299 @@ __device__ __forceinline__ void shade_body(const ShadeParams& p, int blk) {
         p.frame[i + np] = rgb[1];
         p.frame[i + 2 * np] = rgb[2];
     }
-    store_block_sum(hit ? 1u : 0u, p.part_cov + blk, threadIdx.y * 64 + threadIdx.x, 256);
+    return hit;
+}
+
+// Deferred shade pass: 64 x 4 pixel blocks, each wave 64 consecutive pixels of a row.
+template <int MODE>
+__device__ __forceinline__ void shade_body(const ShadeParams& p, int blk) {
+    __shared__ float s_lut[kLdsLuts * 256];
+    __shared__ __align__(16) uint16_t s_net[MODE == kShadeMlp ? kPackedBytes / 2 : 8];
+    __shared__ __align__(16) uint16_t s_in[MODE == kShadeMlp ? 4 : 1][64 * 16];
+    __shared__ __align__(16) float s_out[MODE == kShadeMlp ? 4 : 1][64 * 4];
+    const int tid = threadIdx.y * 64 + threadIdx.x;
+    stage_luts(p, s_lut, tid, 256);
+    if (MODE == kShadeMlp) stage_net(p.mlp, s_net, tid, 256);
+    __syncthreads();
+    const int ws = MODE == kShadeMlp ? threadIdx.y : 0;
+    const uint32_t nbx = shade_blocks_x(p.W);
+    const uint32_t px = (blk % nbx) * 64 + threadIdx.x, py = (blk / nbx) * 4 + threadIdx.y;
+    const bool valid = px < p.W && py < p.H;
+    const unsigned long long key = valid ? p.vis[(size_t)py * p.W + px] : kEmptyKey;
+    const bool hit = shade_pixel<MODE>(p, s_lut, s_net, px, py, valid, key, threadIdx.x, s_in[ws], s_out[ws]);
+    store_block_sum(hit ? 1u : 0u, p.part_cov + blk, tid, 256);
+}
+
+// Deferred shade of the tiles whose bin overflowed, when the raster pass shades the
+// others itself (one workgroup per tile, visibility words from HBM).
+template <int TS, int BLOCK, int MODE>
+__device__ __forceinline__ void shade_tile_body(const RasterParams& rp, const ShadeParams& p, int tile) {
+    if (rp.tile_count[(size_t)tile * kCountStride] <= rp.cap) return;   // shaded by the raster pass
+    constexpr int NW = BLOCK / 64;
+    __shared__ float s_lut[kLdsLuts * 256];
+    __shared__ __align__(16) uint16_t s_net[MODE == kShadeMlp ? kPackedBytes / 2 : 8];
+    __shared__ __align__(16) uint16_t s_in[MODE == kShadeMlp ? NW : 1][64 * 16];
+    __shared__ __align__(16) float s_out[MODE == kShadeMlp ? NW : 1][64 * 4];
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int ws = MODE == kShadeMlp ? tid >> 6 : 0;
+    stage_luts(p, s_lut, tid, BLOCK);
+    if (MODE == kShadeMlp) stage_net(p.mlp, s_net, tid, BLOCK);
+    __syncthreads();
+    const int ox = (tile % rp.tiles_x) * TS, oy = (tile / rp.tiles_x) * TS;
+    uint32_t cov = 0;
+#pragma unroll
+    for (int l0 = 0; l0 < TS * TS; l0 += BLOCK) {
+        const int l = l0 + tid;
+        const uint32_t x = ox + (l % TS), y = oy + (l / TS);
+        const bool valid = x < p.W && y < p.H;
+        const unsigned long long key = valid ? p.vis[(size_t)y * p.W + x] : kEmptyKey;
+        cov += shade_pixel<MODE>(p, s_lut, s_net, x, y, valid, key, lane, s_in[ws], s_out[ws]) ? 1u : 0u;
+    }
+    store_block_sum(cov, p.part_cov + tile, tid, BLOCK);
+}
+
+// ---------------------------------------------------------------------------------
+// Inside + depth test of one (face, pixel) candidate; returns true when inside.
+// The reference tests b_i = e_i / area >= 0 (drawing.cpp:105-113).  When |area| lies in
+// [2^-100, 2^100] and |e_i| >= 2^-20, the quotient is a normal number far from zero and
+// infinity, so its sign is sign(e_i) * sign(area): such a candidate with a sign mismatch
+// is outside without dividing.  Every other candidate takes the three divisions, so the
+// result is the reference's predicate bit for bit.
+__device__ __forceinline__ bool area_in_range(float area) {
+    const float a = fabsf(area);
+    return a >= 0x1p-100f && a <= 0x1p100f;
+}
+
+__device__ __forceinline__ bool surely_outside(float e, bool area_pos) {
+    return fabsf(e) >= 0x1p-20f && ((e > 0.0f) != area_pos);
+}
+
+__device__ __forceinline__ bool fragment_key(float pxf, float pyf, float x0, float y0, float z0, float x1,
+                                             float y1, float z1, float x2, float y2, float z2, float area,
+                                             float& z) {
+    const float e0 = edge(pxf, pyf, x1, y1, x2, y2);
+    const float e1 = edge(pxf, pyf, x2, y2, x0, y0);
+    const float e2 = edge(pxf, pyf, x0, y0, x1, y1);
+    if (area_in_range(area)) {
+        const bool ap = area > 0.0f;
+        if (surely_outside(e0, ap) || surely_outside(e1, ap) || surely_outside(e2, ap)) return false;
+    }
+    const float b0 = e0 / area, b1 = e1 / area, b2 = e2 / area;
+    if (!(b0 >= 0.0f && b1 >= 0.0f && b2 >= 0.0f)) return false;
+    z = (z0 * b0 + z1 * b1) + z2 * b2;
+    return true;
+}
+
+// One workgroup per tile.  The tile's bin is taken BLOCK faces at a time: the faces
+// whose bbox meets the tile are compacted (ballot + popcount ranks, wave totals in LDS)
+// and their candidate pixel counts (bbox inside the tile) prefix-summed across the
+// workgroup.  Candidates are then handed out one per thread, up to 4096 per round: a
+// bitmask in LDS marks each face's first candidate, and a candidate's face is found by
+// counting the marks at or before it (per-word prefix counts + one popcount), with no
+// search.
+//
+// FUSED: the workgroup then shades its own pixels from the LDS visibility words (the
+// deferred shade, without the round trip of the visibility buffer through HBM); tiles
+// whose bin overflowed still write their words for k_overflow and k_shade_tiles.
+template <int TS, int BLOCK, bool FUSED, int MODE>
+__device__ __forceinline__ void raster_body(const RasterParams& p, const ShadeParams* sp, int tile) {
+    constexpr int NPX = TS * TS;
+    constexpr int PPT = NPX / BLOCK;
+    constexpr int NW = BLOCK / 64;
+    constexpr int kWords = 64;                 // mask words per round: 4096 candidates
+    static_assert(PPT >= 1 && NPX % BLOCK == 0, "tile must cover the block");
+    constexpr bool kNet = FUSED && MODE == kShadeMlp;
+    __shared__ unsigned long long s_key[NPX];
+    __shared__ float s_d0[NPX];
+    // raster scratch (face data by rank), reused by the learned shader's staging after
+    // the raster loop
+    constexpr int kRasterScratch = 13 * BLOCK * 4;
+    constexpr int kMlpScratch = kNet ? NW * (64 * 16 * 2 + 64 * 4 * 4) : 0;
+    __shared__ __align__(16) unsigned char s_raw[kRasterScratch > kMlpScratch ? kRasterScratch : kMlpScratch];
+    float (*s_v)[BLOCK] = reinterpret_cast<float (*)[BLOCK]>(s_raw);   // x0 y0 z0 x1 y1 z1 x2 y2 z2 area
+    uint32_t* s_rect = reinterpret_cast<uint32_t*>(s_raw + 10 * BLOCK * 4);   // rx0 | ry0 << 8 | rw << 16
+    uint32_t* s_face = s_rect + BLOCK;
+    uint32_t* s_start = s_face + BLOCK;
+    __shared__ uint32_t s_wsum[NW], s_wlive[NW];
+    __shared__ unsigned long long s_mask[kWords];
+    __shared__ uint32_t s_wpre[kWords];
+    __shared__ float s_lut[FUSED ? kLdsLuts * 256 : 1];
+    __shared__ __align__(16) uint16_t s_net[kNet ? kPackedBytes / 2 : 8];
+
+    const int tid = threadIdx.x;
+    const int lane = tid & 63, wave = tid >> 6;
+    const int tx = tile % p.tiles_x, ty = tile / p.tiles_x;
+    const int ox = tx * TS, oy = ty * TS;
+
+#pragma unroll
+    for (int j = 0; j < PPT; ++j) {
+        const int l = tid + j * BLOCK;
+        const uint32_t x = ox + (l % TS), y = oy + (l / TS);
+        float d0 = 1.0f;
+        if (p.depth_in && x < p.W && y < p.H) d0 = p.depth_in[(size_t)y * p.W + x];
+        s_key[l] = kEmptyKey;
+        s_d0[l] = d0;
+    }
+    if (FUSED) stage_luts(*sp, s_lut, tid, BLOCK);
+    if (kNet) stage_net(sp->mlp, s_net, tid, BLOCK);
+    const uint32_t count = p.tile_count[(size_t)tile * kCountStride];
+    const uint32_t n = count > p.cap ? 0u : count;   // overflowed tiles go to k_overflow
+    uint32_t frags = 0;
+    __syncthreads();
+
+    const unsigned long long below = (1ull << lane) - 1ull;
+    for (uint32_t base = 0; base < n; base += BLOCK) {
+        const uint32_t i = base + tid;
+        uint32_t cnt = 0, rect = 0, f = 0;
+        float4 a = make_float4(0.f, 0.f, 0.f, 0.f), b = a, c = a;
+        if (i < n) {
+            f = p.bins[(size_t)tile * p.cap + i];
+            const FaceRecord* r = p.rec + f;
+            a = r->r0; b = r->r1; c = r->r2;
+            const uint32_t lo = __float_as_uint(b.w), hi = __float_as_uint(c.w);
+            const int rx0 = max((int)(lo & 0xFFFFu) - ox, 0), rx1 = min((int)(hi & 0xFFFFu) - ox, TS - 1);
+            const int ry0 = max((int)(lo >> 16) - oy, 0), ry1 = min((int)(hi >> 16) - oy, TS - 1);
+            if (rx0 <= rx1 && ry0 <= ry1) {
+                const uint32_t rw = rx1 - rx0 + 1;
+                cnt = rw * (ry1 - ry0 + 1);
+                rect = (uint32_t)rx0 | ((uint32_t)ry0 << 8) | (rw << 16);
+            }
+        }
+        // workgroup-wide exclusive prefix sums of the candidate counts and of the live faces
+        const unsigned long long live = __ballot(cnt > 0);
+        uint32_t v = cnt;
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+            const uint32_t t = __shfl_up(v, d, 64);
+            if (lane >= d) v += t;
+        }
+        if (lane == 63) {
+            s_wsum[wave] = v;
+            s_wlive[wave] = (uint32_t)__popcll(live);
+        }
+        __syncthreads();
+        uint32_t wbase = 0, lbase = 0, total = 0;
+#pragma unroll
+        for (int w = 0; w < NW; ++w) {
+            const uint32_t sw = s_wsum[w], lw = s_wlive[w];
+            if (w < wave) { wbase += sw; lbase += lw; }
+            total += sw;
+        }
+        const uint32_t start = wbase + v - cnt;
+        if (cnt > 0) {
+            const int rank = (int)(lbase + (uint32_t)__popcll(live & below));
+            s_v[0][rank] = a.x; s_v[1][rank] = a.y; s_v[2][rank] = a.z;
+            s_v[3][rank] = b.x; s_v[4][rank] = b.y; s_v[5][rank] = b.z;
+            s_v[6][rank] = c.x; s_v[7][rank] = c.y; s_v[8][rank] = c.z;
+            s_v[9][rank] = a.w;
+            s_rect[rank] = rect;
+            s_face[rank] = f;
+            s_start[rank] = start;
+        }
+        uint32_t before = 0;   // live faces whose first candidate precedes the round
+        for (uint32_t r0 = 0; r0 < total; r0 += kWords * 64) {
+            if (tid < kWords) s_mask[tid] = 0ull;
+            __syncthreads();
+            if (cnt > 0 && start >= r0 && start - r0 < kWords * 64u)
+                atomicOr(&s_mask[(start - r0) >> 6], 1ull << ((start - r0) & 63u));
+            __syncthreads();
+            if (wave == 0) {
+                const uint32_t pc = (uint32_t)__popcll(s_mask[lane]);
+                uint32_t e = pc;
+#pragma unroll
+                for (int d = 1; d < 64; d <<= 1) {
+                    const uint32_t t = __shfl_up(e, d, 64);
+                    if (lane >= d) e += t;
+                }
+                s_wpre[lane] = e - pc;
+            }
+            __syncthreads();
+            const uint32_t r1 = min(total, r0 + kWords * 64u);
+            for (uint32_t k = r0 + tid; k < r1; k += BLOCK) {
+                const uint32_t q = k - r0, wi = q >> 6, bit = q & 63u;
+                const unsigned long long upto = bit == 63u ? ~0ull : (2ull << bit) - 1ull;
+                const int j = (int)(before + s_wpre[wi] + (uint32_t)__popcll(s_mask[wi] & upto)) - 1;
+                const uint32_t rc = s_rect[j];
+                const uint32_t rw = rc >> 16;
+                const uint32_t l = k - s_start[j];
+                // l / rw exactly: |error| of the product < 1/(2*rw) for l < 1024
+                const uint32_t dy = (uint32_t)(((float)l + 0.5f) * __builtin_amdgcn_rcpf((float)rw));
+                const uint32_t dx = l - dy * rw;
+                const int lx = (int)(rc & 0xFFu) + (int)dx, ly = (int)((rc >> 8) & 0xFFu) + (int)dy;
+                float z;
+                if (fragment_key((float)(ox + lx), (float)(oy + ly), s_v[0][j], s_v[1][j], s_v[2][j], s_v[3][j],
+                                 s_v[4][j], s_v[5][j], s_v[6][j], s_v[7][j], s_v[8][j], s_v[9][j], z)) {
+                    ++frags;
+                    const int pl = ly * TS + lx;
+                    if (z < s_d0[pl]) atomicMin(&s_key[pl], vis_key(z, s_face[j]));
+                }
+            }
+            before += s_wpre[kWords - 1] + (uint32_t)__popcll(s_mask[kWords - 1]);
+            __syncthreads();   // the mask is cleared for the next round
+        }
+        __syncthreads();       // ranks and wave totals are rewritten by the next chunk
+    }
+
+    if (FUSED && count <= p.cap) {
+        uint16_t* s_in = reinterpret_cast<uint16_t*>(s_raw) + (kNet ? wave : 0) * 64 * 16;
+        float* s_out = reinterpret_cast<float*>(s_raw + NW * 64 * 16 * 2) + (kNet ? wave : 0) * 64 * 4;
+        uint32_t cov = 0;
+#pragma unroll
+        for (int j = 0; j < PPT; ++j) {
+            const int l = tid + j * BLOCK;
+            const uint32_t x = ox + (l % TS), y = oy + (l / TS);
+            const bool valid = x < p.W && y < p.H;
+            cov += shade_pixel<MODE>(*sp, s_lut, s_net, x, y, valid, s_key[l], lane, s_in, s_out) ? 1u : 0u;
+        }
+        store_block_sum(cov, sp->part_cov + tile, tid, BLOCK);
+    } else {
+#pragma unroll
+        for (int j = 0; j < PPT; ++j) {
+            const int l = tid + j * BLOCK;
+            const uint32_t x = ox + (l % TS), y = oy + (l / TS);
+            if (x < p.W && y < p.H) p.vis[(size_t)y * p.W + x] = s_key[l];
+        }
+    }
+    store_block_sum(frags, p.part_frag + tile, tid, BLOCK);
+}
+
+// ---------------------------------------------------------------------------------
+__device__ __forceinline__ void overflow_body(const RasterParams& p, int blk, int nblk) {
+    const int32_t ts_log2 = p.ts_log2;
+    if (p.counters[kCntOverflowTiles] == 0ull) {
+        if (threadIdx.x == 0) p.part_frag[blk] = 0u;
+        return;
+    }
+    const uint32_t ts = 1u << ts_log2;
+    uint32_t frags = 0;
+    for (int f = blk * 256 + threadIdx.x; f < p.nf; f += nblk * 256) {
+        const FaceRecord* r = p.rec + f;
+        const float4 b = r->r1, c = r->r2;
+        const uint32_t lo = __float_as_uint(b.w), hi = __float_as_uint(c.w);
+        const uint32_t bx0 = lo & 0xFFFFu, by0 = lo >> 16, bx1 = hi & 0xFFFFu, by1 = hi >> 16;
+        if (bx0 > bx1 || by0 > by1) continue;
+        const float4 a = r->r0;
+        for (uint32_t ty = by0 >> ts_log2; ty <= (by1 >> ts_log2); ++ty) {
+            for (uint32_t tx = bx0 >> ts_log2; tx <= (bx1 >> ts_log2); ++tx) {
+                const uint32_t tile = ty * p.tiles_x + tx;
+                if (p.tile_count[(size_t)tile * kCountStride] <= p.cap) continue;
+                const uint32_t x0 = max(bx0, tx * ts), x1 = min(bx1, tx * ts + ts - 1);
+                const uint32_t y0 = max(by0, ty * ts), y1 = min(by1, ty * ts + ts - 1);
+                for (uint32_t y = y0; y <= y1; ++y) {
+                    for (uint32_t x = x0; x <= x1; ++x) {
+                        float z;
+                        if (!fragment_key((float)x, (float)y, a.x, a.y, a.z, b.x, b.y, b.z, c.x, c.y, c.z, a.w, z))
+                            continue;
+                        ++frags;
+                        const size_t pi = (size_t)y * p.W + x;
+                        const float d0 = p.depth_in ? p.depth_in[pi] : 1.0f;
+                        if (z < d0) atomicMin(&p.vis[pi], vis_key(z, (uint32_t)f));
+                    }
+                }
+            }
+        }
+    }
+    store_block_sum(frags, p.part_frag + blk, threadIdx.x, 256);
 }
 
 // ---------------------------------------------------------------------------------
@@ -590,10 +710,32 @@ __device__ __forceinline__ void shade_body(const ShadeParams& p, int blk) {
 __global__ __launch_bounds__(256) void k_vertex(VertexParams p) { vertex_body(p, blockIdx.x); }
 __global__ __launch_bounds__(256) void k_bin(BinParams p) { bin_body(p, blockIdx.x); }
 template <int TS, int BLOCK>
-__global__ __launch_bounds__(BLOCK) void k_raster(RasterParams p) { raster_body<TS, BLOCK>(p, blockIdx.x); }
+__global__ __launch_bounds__(BLOCK) void k_raster(RasterParams p) {
+    const XcdGroups g{4, 4, (uint32_t)p.tiles_x, (uint32_t)(p.ntiles / p.tiles_x)};
+    const int tile = g.unit(blockIdx.x);
+    if (tile >= 0) raster_body<TS, BLOCK, false, kShadePlain>(p, nullptr, tile);
+}
+// Raster and shade in one pass (the tile groups are 128 pixels wide so that a line of
+// a frame plane is written from one XCD).
+template <int TS, int BLOCK, int MODE>
+__global__ __launch_bounds__(BLOCK) void k_raster_shade(RasterParams p, ShadeParams sp) {
+    const XcdGroups g{128u / TS, 16u * TS / 128u, (uint32_t)p.tiles_x, (uint32_t)(p.ntiles / p.tiles_x)};
+    const int tile = g.unit(blockIdx.x);
+    if (tile >= 0) raster_body<TS, BLOCK, true, MODE>(p, &sp, tile);
+}
+template <int TS, int BLOCK, int MODE>
+__global__ __launch_bounds__(BLOCK) void k_shade_tiles(RasterParams p, ShadeParams sp) {
+    const XcdGroups g{4, 4, (uint32_t)p.tiles_x, (uint32_t)(p.ntiles / p.tiles_x)};
+    const int tile = g.unit(blockIdx.x);
+    if (tile >= 0) shade_tile_body<TS, BLOCK, MODE>(p, sp, tile);
+}
 __global__ __launch_bounds__(256) void k_overflow(RasterParams p) { overflow_body(p, blockIdx.x, gridDim.x); }
 template <int MODE>
-__global__ __launch_bounds__(256) void k_shade(ShadeParams p) { shade_body<MODE>(p, blockIdx.x); }
+__global__ __launch_bounds__(256) void k_shade(ShadeParams p) {
+    const XcdGroups g{1, 16, shade_blocks_x(p.W), shade_blocks_y(p.H)};
+    const int blk = g.unit(blockIdx.x);
+    if (blk >= 0) shade_body<MODE>(p, blk);
+}
 
 __global__ __launch_bounds__(256) void k_vertex_batch(const VertexParams* __restrict__ items,
                                                       const uint2* __restrict__ map) {
@@ -608,7 +750,7 @@ template <int TS, int BLOCK>
 __global__ __launch_bounds__(BLOCK) void k_raster_batch(const RasterParams* __restrict__ items,
                                                         const uint2* __restrict__ map) {
     const uint2 m = map[blockIdx.x];
-    raster_body<TS, BLOCK>(items[m.x], (int)m.y);
+    raster_body<TS, BLOCK, false, kShadePlain>(items[m.x], nullptr, (int)m.y);
 }
 __global__ __launch_bounds__(256) void k_overflow_batch(const RasterParams* __restrict__ items,
                                                         const uint2* __restrict__ map) {
@@ -623,8 +765,35 @@ __global__ __launch_bounds__(256) void k_shade_batch(const ShadeParams* __restri
     shade_body<MODE>(items[m.x], (int)m.y);
 }
 
+// Holds the stream for about `ticks` of the 100 MHz constant clock (bounded wait), so
+// that the host has queued the passes behind it before they start: the HIP events
+// between passes then time the passes back to back rather than the host's launch rate.
+__global__ __launch_bounds__(64) void k_hold(uint64_t ticks) {
+    const uint64_t t0 = wall_clock64();
+    while (wall_clock64() - t0 < ticks) __builtin_amdgcn_s_sleep(32);
+}
+
 // ---------------------------------------------------------------------------------
 // Launchers (host side).
+
+// Device-to-device copy of n16 16-byte words (one read and one write per byte).
+__global__ __launch_bounds__(256) void k_copy(const uint4* __restrict__ src, uint4* __restrict__ dst, size_t n16) {
+    const size_t stride = (size_t)gridDim.x * 256;
+    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n16; i += stride) dst[i] = src[i];
+}
+
+hipError_t launch_copy(const void* src, void* dst, size_t n16, hipStream_t s) {
+    if (n16 == 0) return hipSuccess;
+    const size_t blocks = (n16 + 255) / 256;
+    hipLaunchKernelGGL(k_copy, dim3((unsigned)(blocks < 8192 ? blocks : 8192)), dim3(256), 0, s,
+                       (const uint4*)src, (uint4*)dst, n16);
+    return hipGetLastError();
+}
+
+hipError_t launch_hold(uint64_t ticks, hipStream_t s) {
+    hipLaunchKernelGGL(k_hold, dim3(1), dim3(64), 0, s, ticks);
+    return hipGetLastError();
+}
 
 hipError_t launch_vertex(const VertexParams& p, hipStream_t s) {
     hipLaunchKernelGGL(k_vertex, dim3(vertex_blocks(p.nv, p.nn, p.ntiles)), dim3(256), 0, s, p);
@@ -637,13 +806,67 @@ hipError_t launch_bin(const BinParams& p, hipStream_t s) {
 }
 
 hipError_t launch_raster(const RasterParams& p, int tile_size, hipStream_t s) {
+    const uint32_t grid = XcdGroups{4, 4, (uint32_t)p.tiles_x, (uint32_t)(p.ntiles / p.tiles_x)}.grid();
     switch (tile_size) {
-        case 8: hipLaunchKernelGGL((k_raster<8, 64>), dim3(p.ntiles), dim3(64), 0, s, p); break;
-        case 16: hipLaunchKernelGGL((k_raster<16, 256>), dim3(p.ntiles), dim3(256), 0, s, p); break;
-        case 32: hipLaunchKernelGGL((k_raster<32, 256>), dim3(p.ntiles), dim3(256), 0, s, p); break;
+        case 8: hipLaunchKernelGGL((k_raster<8, 64>), dim3(grid), dim3(64), 0, s, p); break;
+        case 16: hipLaunchKernelGGL((k_raster<16, 256>), dim3(grid), dim3(256), 0, s, p); break;
+        case 32: hipLaunchKernelGGL((k_raster<32, 256>), dim3(grid), dim3(256), 0, s, p); break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
+}
+
+template <int MODE>
+hipError_t launch_raster_shade_mode(const RasterParams& p, const ShadeParams& sp, int tile_size, hipStream_t s) {
+    switch (tile_size) {
+        case 8: {
+            const uint32_t grid = XcdGroups{16, 1, (uint32_t)p.tiles_x, (uint32_t)(p.ntiles / p.tiles_x)}.grid();
+            hipLaunchKernelGGL((k_raster_shade<8, 64, MODE>), dim3(grid), dim3(64), 0, s, p, sp);
+            break;
+        }
+        case 16: {
+            const uint32_t grid = XcdGroups{8, 2, (uint32_t)p.tiles_x, (uint32_t)(p.ntiles / p.tiles_x)}.grid();
+            hipLaunchKernelGGL((k_raster_shade<16, 256, MODE>), dim3(grid), dim3(256), 0, s, p, sp);
+            break;
+        }
+        case 32: {
+            const uint32_t grid = XcdGroups{4, 4, (uint32_t)p.tiles_x, (uint32_t)(p.ntiles / p.tiles_x)}.grid();
+            hipLaunchKernelGGL((k_raster_shade<32, 256, MODE>), dim3(grid), dim3(256), 0, s, p, sp);
+            break;
+        }
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_raster_shade(const RasterParams& p, const ShadeParams& sp, int tile_size, hipStream_t s) {
+    switch (sp.mode) {
+        case kShadePlain: return launch_raster_shade_mode<kShadePlain>(p, sp, tile_size, s);
+        case kShadeMlp: return launch_raster_shade_mode<kShadeMlp>(p, sp, tile_size, s);
+        case kShadeCollect: return launch_raster_shade_mode<kShadeCollect>(p, sp, tile_size, s);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+template <int MODE>
+hipError_t launch_shade_tiles_mode(const RasterParams& p, const ShadeParams& sp, int tile_size, hipStream_t s) {
+    const uint32_t grid = XcdGroups{4, 4, (uint32_t)p.tiles_x, (uint32_t)(p.ntiles / p.tiles_x)}.grid();
+    switch (tile_size) {
+        case 8: hipLaunchKernelGGL((k_shade_tiles<8, 64, MODE>), dim3(grid), dim3(64), 0, s, p, sp); break;
+        case 16: hipLaunchKernelGGL((k_shade_tiles<16, 256, MODE>), dim3(grid), dim3(256), 0, s, p, sp); break;
+        case 32: hipLaunchKernelGGL((k_shade_tiles<32, 256, MODE>), dim3(grid), dim3(256), 0, s, p, sp); break;
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_shade_tiles(const RasterParams& p, const ShadeParams& sp, int tile_size, hipStream_t s) {
+    switch (sp.mode) {
+        case kShadePlain: return launch_shade_tiles_mode<kShadePlain>(p, sp, tile_size, s);
+        case kShadeMlp: return launch_shade_tiles_mode<kShadeMlp>(p, sp, tile_size, s);
+        case kShadeCollect: return launch_shade_tiles_mode<kShadeCollect>(p, sp, tile_size, s);
+        default: return hipErrorInvalidValue;
+    }
 }
 
 hipError_t launch_overflow(const RasterParams& p, hipStream_t s) {
@@ -652,7 +875,7 @@ hipError_t launch_overflow(const RasterParams& p, hipStream_t s) {
 }
 
 hipError_t launch_shade(const ShadeParams& p, hipStream_t s) {
-    const dim3 grid(shade_blocks_x(p.W) * shade_blocks_y(p.H)), block(64, 4);
+    const dim3 grid(XcdGroups{1, 16, shade_blocks_x(p.W), shade_blocks_y(p.H)}.grid()), block(64, 4);
     switch (p.mode) {
         case kShadePlain: hipLaunchKernelGGL(k_shade<kShadePlain>, grid, block, 0, s, p); break;
         case kShadeMlp: hipLaunchKernelGGL(k_shade<kShadeMlp>, grid, block, 0, s, p); break;

@@ -48,45 +48,35 @@ __host__ __device__ __forceinline__ float from_bf16(uint16_t h) {
 }
 __host__ __device__ __forceinline__ float round_bf16(float f) { return from_bf16(to_bf16(f)); }
 
-// Evaluates the MLP for the 64 fragments of one wave.  s_in: this wave's LDS staging of
-// [64][16] bf16 inputs (row = lane that owns the fragment, inputs 12..15 zero);
-// s_out: [64][4] floats receiving the three outputs.  Every lane of the wave calls it.
-__device__ __forceinline__ void mlp_wave(const uint16_t* __restrict__ packed, const uint16_t* s_in, float* s_out,
-                                         int lane) {
-    const mlp_s4* fr = reinterpret_cast<const mlp_s4*>(packed);
-    const float* bias = reinterpret_cast<const float*>(packed + kFragEnd);
-    mlp_s4 w1[4], w2[4][4], w3[4];
-#pragma unroll
-    for (int h = 0; h < 4; ++h) w1[h] = fr[kFragL1 / 4 + h * 64 + lane];
-#pragma unroll
-    for (int h = 0; h < 4; ++h)
-#pragma unroll
-        for (int kt = 0; kt < 4; ++kt) w2[h][kt] = fr[kFragL2 / 4 + (h * 4 + kt) * 64 + lane];
-#pragma unroll
-    for (int kt = 0; kt < 4; ++kt) w3[kt] = fr[kFragL3 / 4 + kt * 64 + lane];
-    const int row = 4 * (lane >> 4), col = lane & 15;
-    float b1[4][4], b2[4][4], b3[4];
-#pragma unroll
-    for (int h = 0; h < 4; ++h)
-#pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            b1[h][j] = bias[16 * h + row + j];
-            b2[h][j] = bias[64 + 16 * h + row + j];
-        }
-#pragma unroll
-    for (int j = 0; j < 4; ++j) b3[j] = bias[128 + row + j];
+// Copies the packed network (kPackedBytes) into LDS; the whole workgroup calls it and
+// a barrier must follow before mlp_wave reads it.
+__device__ __forceinline__ void stage_net(const uint16_t* __restrict__ net, uint16_t* s_net, int tid,
+                                          int nthreads) {
+    const uint4* src = reinterpret_cast<const uint4*>(net);
+    uint4* dst = reinterpret_cast<uint4*>(s_net);
+    for (int e = tid; e < kPackedBytes / 16; e += nthreads) dst[e] = src[e];
+}
 
-#pragma unroll
+// Evaluates the MLP for the 64 fragments of one wave.  net: the packed network (in LDS,
+// staged by stage_net); s_in: this wave's LDS staging of [64][16] bf16 inputs (row =
+// lane that owns the fragment, inputs 12..15 zero); s_out: [64][4] floats receiving the
+// three outputs.  Every lane of the wave calls it.  Weight fragments are read where
+// each product needs them, so the network does not occupy registers.
+__device__ __forceinline__ void mlp_wave(const uint16_t* net, const uint16_t* s_in, float* s_out, int lane) {
+    const mlp_s4* fr = reinterpret_cast<const mlp_s4*>(net);
+    const float* bias = reinterpret_cast<const float*>(net + kFragEnd);
+    const int row = 4 * (lane >> 4), col = lane & 15;
+#pragma unroll 1
     for (int pt = 0; pt < 4; ++pt) {
         const mlp_s4 xb = *reinterpret_cast<const mlp_s4*>(s_in + (16 * pt + col) * 16 + row);
         mlp_s4 h1[4], h2[4];
 #pragma unroll
         for (int h = 0; h < 4; ++h) {
             mlp_f4 acc = {0.f, 0.f, 0.f, 0.f};
-            acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(w1[h], xb, acc, 0, 0, 0);
+            acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(fr[kFragL1 / 4 + h * 64 + lane], xb, acc, 0, 0, 0);
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
-                const float v = acc[j] + b1[h][j];
+                const float v = acc[j] + bias[16 * h + row + j];
                 h1[h][j] = (short)to_bf16(v > 0.f ? v : 0.f);
             }
         }
@@ -94,19 +84,22 @@ __device__ __forceinline__ void mlp_wave(const uint16_t* __restrict__ packed, co
         for (int h = 0; h < 4; ++h) {
             mlp_f4 acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-            for (int kt = 0; kt < 4; ++kt) acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(w2[h][kt], h1[kt], acc, 0, 0, 0);
+            for (int kt = 0; kt < 4; ++kt)
+                acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(fr[kFragL2 / 4 + (h * 4 + kt) * 64 + lane], h1[kt],
+                                                               acc, 0, 0, 0);
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
-                const float v = acc[j] + b2[h][j];
+                const float v = acc[j] + bias[64 + 16 * h + row + j];
                 h2[h][j] = (short)to_bf16(v > 0.f ? v : 0.f);
             }
         }
         mlp_f4 acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-        for (int kt = 0; kt < 4; ++kt) acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(w3[kt], h2[kt], acc, 0, 0, 0);
+        for (int kt = 0; kt < 4; ++kt)
+            acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(fr[kFragL3 / 4 + kt * 64 + lane], h2[kt], acc, 0, 0, 0);
         if (lane < 16) {
 #pragma unroll
-            for (int j = 0; j < 3; ++j) s_out[(16 * pt + col) * 4 + j] = acc[j] + b3[j];
+            for (int j = 0; j < 3; ++j) s_out[(16 * pt + col) * 4 + j] = acc[j] + bias[128 + row + j];
         }
     }
 }

@@ -37,15 +37,17 @@ __global__ __launch_bounds__(256) void k_mlp_pack(const float* __restrict__ W, u
 // mlp_wave on fragments it computes itself).
 __global__ __launch_bounds__(256) void k_mlp_infer(const float* __restrict__ x, int64_t n,
                                                    const uint16_t* __restrict__ packed, float* __restrict__ y) {
-    __shared__ uint16_t s_in[4][64 * 16];
-    __shared__ float s_out[4][64 * 4];
+    __shared__ __align__(16) uint16_t s_net[kPackedBytes / 2];
+    __shared__ __align__(16) uint16_t s_in[4][64 * 16];
+    __shared__ __align__(16) float s_out[4][64 * 4];
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const int64_t i = (int64_t)blockIdx.x * 256 + tid;
+    stage_net(packed, s_net, tid, 256);
 #pragma unroll
     for (int k = 0; k < 16; ++k)
         s_in[wave][lane * 16 + k] = to_bf16((i < n && k < kMlpIn) ? x[i * kMlpIn + k] : 0.f);
     __syncthreads();
-    mlp_wave(packed, s_in[wave], s_out[wave], lane);
+    mlp_wave(s_net, s_in[wave], s_out[wave], lane);
     __syncthreads();
     if (i < n)
         for (int c = 0; c < kMlpOut; ++c) y[i * kMlpOut + c] = s_out[wave][lane * 4 + c];

@@ -114,6 +114,14 @@ def test_collect_train_and_shade(ctx):
         assert np.array_equal(got, want)
         err = np.abs(frame_m[:, covered].astype(np.int32) - frame_p[:, covered].astype(np.int32))
         assert err.mean() < 12.0, err.mean()
+        # the separate shade pass evaluates the same network on the same inputs
+        ctx.set_fused_shade(False)
+        try:
+            frame_s, depth_s, ids_s, _ = ctx.draw(mesh, scene.lights_copy(), args, ids=True,
+                                                  flags=rz.RZ_FLAG_MLP_SHADE)
+        finally:
+            ctx.set_fused_shade(True)
+        assert np.array_equal(frame_s, frame_m) and np.array_equal(ids_s, ids_m)
     finally:
         samples.close()
         mesh.close()

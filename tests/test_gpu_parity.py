@@ -55,6 +55,37 @@ def test_screen_soup_small(ctx):
     assert gpu[3]["front_triangles"] > 80_000
 
 
+def test_screen_soup_full_4m(ctx):
+    """The small-triangle depth stress at its full size: 4M screen-space triangles at
+    4096x4096, about half back-facing, bit for bit against the oracle — once with the
+    default bins and once with bins of 40 so that crowded tiles take the overflow path."""
+    import oracle
+    from tests.parity import assert_same_frame, render_gpu
+    scene, args = scenes.screen_soup_scene()
+    assert scene.num_faces == 4_000_000 and args.image_width == 4096
+    ref = oracle.draw_frame(scene, args)
+    gpu = render_gpu(ctx, scene, args)
+    assert_same_frame(gpu, ref, "screen soup 4M")
+    assert gpu[3]["overflow_tiles"] == 0
+    assert 1_900_000 < gpu[3]["front_triangles"] < 2_100_000
+    gpu = render_gpu(ctx, scene, args, tuning=(16, 40))
+    assert_same_frame(gpu, ref, "screen soup 4M, bins of 40")
+    assert gpu[3]["overflow_tiles"] > 1000
+
+
+@pytest.mark.parametrize("tuning", [(16, 512), (16, 4), (8, 64), (32, 2048)])
+def test_split_shade_pass(ctx, tuning):
+    """The deferred shade as its own pass over the HBM visibility buffer (the default
+    shades non-overflowed tiles inside the raster pass): same bits."""
+    ctx.set_fused_shade(False)
+    try:
+        scene, args = scenes.icosphere_scene(subdivisions=5, width=800, height=600)
+        check(ctx, scene, args, tuning=tuning, label=f"split shade {tuning}")
+    finally:
+        ctx.set_fused_shade(True)
+    assert ctx.fused_shade
+
+
 def test_repeatable(ctx):
     """Two renders of the same frame give identical visibility (order-free atomics)."""
     scene, args = scenes.icosphere_scene(subdivisions=5, width=800, height=600)
