@@ -8,6 +8,7 @@ from __future__ import annotations
 
 import ctypes as C
 import os
+import weakref
 
 import numpy as np
 
@@ -73,10 +74,12 @@ class Mesh:
         _check(ctx.lib.rz_mesh_create(ctx.handle, C.byref(desc), C.byref(h)), ctx.handle)
         del keep
         self.handle = h
+        ctx._children.add(self)
 
     def close(self):
         if self.handle:
-            self.ctx.lib.rz_mesh_destroy(self.ctx.handle, self.handle)
+            if self.ctx.handle:
+                self.ctx.lib.rz_mesh_destroy(self.ctx.handle, self.handle)
             self.handle = None
 
     def __del__(self):
@@ -91,13 +94,15 @@ class FrameGraph:
 
     def __init__(self, ctx: "Context", handle, frames: int):
         self.ctx, self.handle, self.frames = ctx, handle, frames
+        ctx._children.add(self)
 
     def launch(self):
         _check(self.ctx.lib.rz_graph_launch(self.handle), self.ctx.handle)
 
     def close(self):
         if self.handle:
-            self.ctx.lib.rz_graph_destroy(self.handle)
+            if self.ctx.handle:
+                self.ctx.lib.rz_graph_destroy(self.handle)
             self.handle = None
 
     def __del__(self):
@@ -118,6 +123,7 @@ class DeviceArray:
         p = C.c_void_p()
         _check(ctx.lib.rz_device_alloc(ctx.handle, self.nbytes, C.byref(p)), ctx.handle)
         self.ptr = int(p.value)
+        ctx._children.add(self)
 
     def upload(self, host: np.ndarray):
         host = np.ascontiguousarray(host, self.dtype)
@@ -139,7 +145,8 @@ class DeviceArray:
 
     def close(self):
         if self.ptr:
-            self.ctx.lib.rz_device_free(self.ctx.handle, C.c_void_p(self.ptr))
+            if self.ctx.handle:
+                self.ctx.lib.rz_device_free(self.ctx.handle, C.c_void_p(self.ptr))
             self.ptr = 0
 
     def __del__(self):
@@ -154,12 +161,17 @@ class Context:
 
     def __init__(self, device: int = 0):
         self.lib = load_library()
+        self.handle = None
+        # meshes, graphs and device arrays of this context, released before it is
+        self._children = weakref.WeakSet()
         h = C.c_void_p()
         _check(self.lib.rz_create(int(device), C.byref(h)))
         self.handle = h
 
     def close(self):
         if self.handle:
+            for child in list(self._children):
+                child.close()
             self.lib.rz_destroy(self.handle)
             self.handle = None
 

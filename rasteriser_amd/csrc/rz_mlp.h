@@ -6,13 +6,19 @@
 // vector from the surface to the eye (3), the albedo (3), and the decoded normal-map
 // sample t = 2s-1 ((0,0,1) without a map) (3).  Output: the colour / 255 (3).
 //
-// Every layer is computed transposed, Y^T = W X^T, with v_mfma_f32_16x16x16_bf16:
-//   A (16x16, rows m, cols k): lane l holds A[l%16][4*(l/16) + j], j = 0..3
-//   B (16x16, rows k, cols n): lane l holds B[4*(l/16) + j][l%16]
-//   D (16x16):                 lane l holds D[4*(l/16) + j][l%16]
-// so a layer's accumulator tile (hidden unit 4*(l/16)+j of tile h, fragment l%16) is
-// already the next layer's B operand for k-tile h: no data moves between layers.
-// Weights are packed once into per-lane A fragments (k_mlp_pack).
+// Every layer is computed transposed, Y^T = W X^T.  Layer 1 (K = 12, padded to 16)
+// uses v_mfma_f32_16x16x16_bf16, layers 2 and 3 (K = 64) v_mfma_f32_16x16x32_bf16:
+//   x16 A (16x16): lane l holds A[l%16][4*(l/16) + j], j = 0..3
+//   x16 B (16x16): lane l holds B[4*(l/16) + j][l%16]
+//   x32 A (16x32): lane l holds A[l%16][8*(l/16) + j], j = 0..7
+//   x32 B (32x16): lane l holds B[8*(l/16) + j][l%16]
+//   D (16x16, both): lane l holds D[4*(l/16) + j][l%16]
+// A layer's accumulator tile h leaves hidden unit 16h + 4*(l/16) + j of fragment l%16
+// in lane l.  The x32 B operand of k-tile t is the lane's own tiles 2t and 2t+1
+// concatenated, so its row 8g + j (g = l/16) is hidden unit
+//   u(t, g, j) = 16*(2t + j/4) + 4g + j%4,
+// and the A fragments are packed with the same permutation of K (k_mlp_pack): no data
+// moves between layers.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -28,14 +34,26 @@ constexpr int kOffW1 = 0, kOffB1 = kOffW1 + 768, kOffW2 = kOffB1 + 64, kOffB2 = 
               kOffW3 = kOffB2 + 64, kOffB3 = kOffW3 + 192;
 constexpr int kMlpSampleFloats = kMlpIn + kMlpOut;   // training sample: inputs, target colour/255
 
-// Packed BF16 fragments: layer 1 [4 h][64 lanes][4], layer 2 [4 h][4 kt][64][4],
-// layer 3 [4 kt][64][4] (uint16 each), then biases b1[64], b2[64], b3[16] as floats.
+// Packed BF16 fragments: layer 1 [4 h][64 lanes][4], layer 2 [4 h][2 t][64][8],
+// layer 3 [2 t][64][8] (uint16 each), then biases b1[64], b2[64], b3[16] as floats.
 constexpr int kFragL1 = 0, kFragL2 = kFragL1 + 4 * 64 * 4, kFragL3 = kFragL2 + 16 * 64 * 4,
               kFragEnd = kFragL3 + 4 * 64 * 4;   // uint16 count: 6144
 constexpr int kPackedBytes = kFragEnd * 2 + (64 + 64 + 16) * 4;
 
 typedef short mlp_s4 __attribute__((ext_vector_type(4)));
+typedef short mlp_s8 __attribute__((ext_vector_type(8)));
+typedef __bf16 mlp_b8 __attribute__((ext_vector_type(8)));
 typedef float mlp_f4 __attribute__((ext_vector_type(4)));
+
+// Hidden unit held in row 8g + j of the x32 B operand of k-tile t.
+__host__ __device__ __forceinline__ int mlp_k32_unit(int t, int g, int j) {
+    return 16 * (2 * t + j / 4) + 4 * g + j % 4;
+}
+
+__device__ __forceinline__ mlp_f4 mfma_k32(mlp_s8 a, mlp_s8 b, mlp_f4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(mlp_b8, a), __builtin_bit_cast(mlp_b8, b), c,
+                                                   0, 0, 0);
+}
 
 // float -> bf16, round to nearest even (finite inputs).
 __host__ __device__ __forceinline__ uint16_t to_bf16(float f) {
@@ -64,6 +82,7 @@ __device__ __forceinline__ void stage_net(const uint16_t* __restrict__ net, uint
 // each product needs them, so the network does not occupy registers.
 __device__ __forceinline__ void mlp_wave(const uint16_t* net, const uint16_t* s_in, float* s_out, int lane) {
     const mlp_s4* fr = reinterpret_cast<const mlp_s4*>(net);
+    const mlp_s8* fr8 = reinterpret_cast<const mlp_s8*>(net);
     const float* bias = reinterpret_cast<const float*>(net + kFragEnd);
     const int row = 4 * (lane >> 4), col = lane & 15;
 #pragma unroll 1
@@ -80,13 +99,13 @@ __device__ __forceinline__ void mlp_wave(const uint16_t* net, const uint16_t* s_
                 h1[h][j] = (short)to_bf16(v > 0.f ? v : 0.f);
             }
         }
+        const mlp_s8 hb0 = __builtin_shufflevector(h1[0], h1[1], 0, 1, 2, 3, 4, 5, 6, 7);
+        const mlp_s8 hb1 = __builtin_shufflevector(h1[2], h1[3], 0, 1, 2, 3, 4, 5, 6, 7);
 #pragma unroll
         for (int h = 0; h < 4; ++h) {
             mlp_f4 acc = {0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-            for (int kt = 0; kt < 4; ++kt)
-                acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(fr[kFragL2 / 4 + (h * 4 + kt) * 64 + lane], h1[kt],
-                                                               acc, 0, 0, 0);
+            acc = mfma_k32(fr8[kFragL2 / 8 + (h * 2 + 0) * 64 + lane], hb0, acc);
+            acc = mfma_k32(fr8[kFragL2 / 8 + (h * 2 + 1) * 64 + lane], hb1, acc);
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
                 const float v = acc[j] + bias[64 + 16 * h + row + j];
@@ -94,9 +113,9 @@ __device__ __forceinline__ void mlp_wave(const uint16_t* net, const uint16_t* s_
             }
         }
         mlp_f4 acc = {0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-        for (int kt = 0; kt < 4; ++kt)
-            acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(fr[kFragL3 / 4 + kt * 64 + lane], h2[kt], acc, 0, 0, 0);
+        acc = mfma_k32(fr8[kFragL3 / 8 + lane], __builtin_shufflevector(h2[0], h2[1], 0, 1, 2, 3, 4, 5, 6, 7), acc);
+        acc = mfma_k32(fr8[kFragL3 / 8 + 64 + lane], __builtin_shufflevector(h2[2], h2[3], 0, 1, 2, 3, 4, 5, 6, 7),
+                       acc);
         if (lane < 16) {
 #pragma unroll
             for (int j = 0; j < 3; ++j) s_out[(16 * pt + col) * 4 + j] = acc[j] + bias[128 + row + j];

@@ -18,13 +18,15 @@ __global__ __launch_bounds__(256) void k_mlp_pack(const float* __restrict__ W, u
         const int o = 16 * h + lane % 16, k = 4 * (lane / 16) + j;
         packed[i] = to_bf16(k < kMlpIn ? W[kOffW1 + o * kMlpIn + k] : 0.f);
     } else if (i < kFragL3) {
-        const int q = i - kFragL2, t = q / 256, lane = (q / 4) % 64, j = q % 4;
-        const int h = t / 4, kt = t % 4;
-        const int o = 16 * h + lane % 16, k = 16 * kt + 4 * (lane / 16) + j;
+        // [h][t][lane][8]: A[m][8g + j] = W2[16h + m][u(t, g, j)]
+        const int q = i - kFragL2, ht = q / 512, lane = (q / 8) % 64, j = q % 8;
+        const int h = ht / 2, t = ht % 2;
+        const int o = 16 * h + lane % 16, k = mlp_k32_unit(t, lane / 16, j);
         packed[i] = to_bf16(W[kOffW2 + o * kMlpHidden + k]);
     } else if (i < kFragEnd) {
-        const int q = i - kFragL3, kt = q / 256, lane = (q / 4) % 64, j = q % 4;
-        const int o = lane % 16, k = 16 * kt + 4 * (lane / 16) + j;
+        // [t][lane][8]: A[m][8g + j] = W3[m][u(t, g, j)] (rows 3..15 zero)
+        const int q = i - kFragL3, t = q / 512, lane = (q / 8) % 64, j = q % 8;
+        const int o = lane % 16, k = mlp_k32_unit(t, lane / 16, j);
         packed[i] = to_bf16(o < kMlpOut ? W[kOffW3 + o * kMlpHidden + k] : 0.f);
     } else if (i < kFragEnd + 144) {
         float* bias = reinterpret_cast<float*>(packed + kFragEnd);

@@ -125,22 +125,40 @@ def test_frame_graph_spin(ctx):
     mesh.close()
 
 
-def test_graph_stale_after_realloc(ctx):
-    """A frame that grows the context buffers makes an older graph refuse to launch."""
+def test_graph_stale_after_realloc():
+    """A frame that grows the context buffers makes an older graph refuse to launch (a
+    fresh context, whose buffers are sized by this test's frames alone)."""
     import rasteriser_amd as rz
+    with rz.Context(0) as ctx:
+        scene, args = scenes.uv_sphere_scene(width=64, height=64)
+        mesh = ctx.upload(scene)
+        frame = ctx.empty((3, 64, 64), np.uint8)
+        depth = ctx.empty((64, 64), np.float32)
+        g = ctx.graph(mesh, scene.lights_copy(), args, frame.ptr, depth.ptr, 1)
+        g.launch()
+        ctx.sync()
+        big, bargs = scenes.uv_sphere_scene(width=2500, height=2500)
+        ctx.draw(mesh, scene.lights_copy(), bargs)
+        with pytest.raises(rz.RzError):
+            g.launch()
+
+
+def test_context_close_releases_children():
+    """Closing a context first releases its meshes, graphs and device arrays; objects
+    collected afterwards do not touch the destroyed context."""
+    import gc
+    import rasteriser_amd as rz
+    ctx = rz.Context(0)
     scene, args = scenes.uv_sphere_scene(width=64, height=64)
     mesh = ctx.upload(scene)
     frame = ctx.empty((3, 64, 64), np.uint8)
     depth = ctx.empty((64, 64), np.float32)
-    g = ctx.graph(mesh, scene.lights_copy(), args, frame.ptr, depth.ptr, 1)
+    g = ctx.graph(mesh, scene.lights_copy(), args, frame.ptr, depth.ptr, 2)
     g.launch()
-    ctx.sync()
-    big, bargs = scenes.uv_sphere_scene(width=2500, height=2500)
-    ctx.draw(mesh, scene.lights_copy(), bargs)
-    with pytest.raises(rz.RzError):
-        g.launch()
-    g.close()
-    mesh.close()
+    ctx.close()
+    assert mesh.handle is None and g.handle is None and frame.ptr == 0 and depth.ptr == 0
+    del mesh, frame, depth, g
+    gc.collect()
 
 
 def _device_outputs(ctx, args, with_ids=True):
