@@ -51,9 +51,9 @@ def log(*a):
 
 def algorithmic_bytes(counts, scene, args, fused, cleared=True):
     """Per-pass algorithmic bytes of one frame: the input reads and output writes a pass
-    must make at minimum.  With the fused pipeline the raster pass also shades (no
-    visibility buffer round trip) and the separate shade pass only covers overflowed
-    tiles (none in this frame)."""
+    must make at minimum.  With the fused pipeline the bin launch also clears the frame
+    and depth, the raster pass also shades (no visibility buffer round trip) and the
+    separate shade pass only covers overflowed tiles (none in this frame)."""
     W, H = args.image_width, args.image_height
     V = scene.vertices.shape[0]
     Vn = scene.normals.shape[0]
@@ -70,7 +70,11 @@ def algorithmic_bytes(counts, scene, args, fused, cleared=True):
         "overflow": 0,
     }
     if fused:
-        out["raster"] = raster_in + shade_out
+        # cleared buffers: the bin launch writes the cleared frame and depth (7 B/px), the
+        # raster pass shades the covered pixels only
+        C = counts["covered_pixels"]
+        out["bin"] += 7 * W * H
+        out["raster"] = raster_in + 7 * C + 108 * Tvis + 4 * tex + 40 * L
         out["shade"] = 0
     else:
         out["raster"] = raster_in + 8 * W * H        # + visibility words written
@@ -227,6 +231,48 @@ def learned_shading(ctx, rz, scenes, np, steps, batch, frames):
         mesh.close()
 
 
+def batched(ctx, rz, scenes, np, frames):
+    """The 256-mesh batched workload: autotuned tile size and bin capacity, then
+    `frames` batch frames (every mesh, one launch per pass) in one HIP graph."""
+    t0 = time.perf_counter()
+    work = scenes.batch_scenes()
+    t_gen = time.perf_counter() - t0
+    meshes = [ctx.upload(sc) for sc, _ in work]
+    outs = []
+    for _, a in work:
+        W, H = a.image_width, a.image_height
+        outs.append((ctx.empty((3, H, W), np.uint8), ctx.empty((H, W), np.float32)))
+    batch = [(m, sc.lights_copy(), a, f, d, None) for m, (sc, a), (f, d) in zip(meshes, work, outs)]
+    try:
+        t0 = time.perf_counter()
+        ts, cap, tuned_ms = ctx.autotune(batch)
+        t_tune = time.perf_counter() - t0
+        st = ctx.draw_batch(batch, stats=True)
+        g = ctx.batch_graph(batch, frames)
+        g.launch()
+        ctx.sync()
+        t0 = time.perf_counter()
+        g.launch()
+        ctx.sync()
+        ms = (time.perf_counter() - t0) * 1000.0 / frames
+        g.close()
+        faces = sum(sc.num_faces for sc, _ in work)
+        pixels = sum(a.image_width * a.image_height for _, a in work)
+        return {"workload": "256 meshes, log-uniform 500-200k faces, viewports 256^2-1024^2",
+                "meshes": len(work), "faces": faces, "pixels": pixels, "tile": ts, "bin_capacity": cap,
+                "autotune_ms_per_frame": tuned_ms, "ms_per_batch_frame": ms,
+                "mtri_per_s": faces / (ms * 1e-3) / 1e6, "gpix_per_s": pixels / (ms * 1e-3) / 1e9,
+                "fragments": st["fragments"], "overflow_tiles": st["overflow_tiles"],
+                "generate_s": t_gen, "autotune_s": t_tune, "frames_per_graph": frames}
+    finally:
+        for f, d in outs:
+            f.close()
+            d.close()
+        for m in meshes:
+            m.close()
+        ctx.set_tuning(16, 512)
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -238,6 +284,8 @@ def main():
     ap.add_argument("--tile", type=int, default=16)
     ap.add_argument("--cap", type=int, default=512)
     ap.add_argument("--no-mlp", action="store_true", help="skip the learned-shading measurement")
+    ap.add_argument("--batch", action="store_true",
+                    help="also measure the 256-mesh batched workload (autotuned, one graph)")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="first run one device-to-device copy of this many MiB (k_copy), the "
                          "known-size reference for hardware byte counters")
@@ -346,6 +394,11 @@ def main():
     for buf in (frame, depth, ids):
         buf.close()
 
+    batch = None
+    if a.batch:
+        batch = batched(ctx, rz, scenes, np, 20)
+        log(f"batched: {batch}")
+
     mlp = None
     if not a.no_mlp:
         ctx.set_tuning(16, 512)
@@ -372,6 +425,7 @@ def main():
         "shade64_plain_ms": mlp["shade_plain_ms"] if mlp else None,
         "shade64_mlp_ms": mlp["shade_mlp_ms"] if mlp else None,
         "learned_shading": mlp,
+        "batched": batch,
     }
     print(json.dumps(out), flush=True)
     return 0

@@ -31,6 +31,7 @@ hipError_t launch_overflow_batch(const RasterParams* items, const uint2* map, ui
 hipError_t launch_shade_batch(const ShadeParams* items, const uint2* map, uint32_t blocks, int mode,
                               hipStream_t s);
 hipError_t launch_hold(uint64_t ticks, hipStream_t s);
+uint32_t raster_blocks_per_cu(int tile_size, bool fused, int mode);
 hipError_t launch_copy(const void* src, void* dst, size_t n16, hipStream_t s);
 hipError_t launch_mlp_pack(const float* W, uint16_t* packed, hipStream_t s);
 hipError_t launch_mlp_infer(const float* x, int64_t n, const uint16_t* packed, float* y, hipStream_t s);
@@ -44,6 +45,9 @@ using namespace rz;
 namespace {
 
 constexpr uint32_t kMaxImageSide = 32767;   // bbox corners are packed in 16 bits
+// Workgroups per CU of the overflow shade launch (persistent: it has work only when a
+// bin overflowed).
+constexpr int kPersistentPerCU = 4;
 
 struct DevBuf {
     void* ptr = nullptr;
@@ -62,7 +66,11 @@ struct rz_mesh {
     float *fnx = nullptr, *fny = nullptr, *fnz = nullptr;   // face normals (flat)
     float *ftx = nullptr, *fty = nullptr, *ftz = nullptr;   // face tangents (normal maps)
     float2* uvs = nullptr;
-    int32_t *fv = nullptr, *fn = nullptr, *fuv = nullptr, *fmat = nullptr;
+    int32_t *fn = nullptr, *fuv = nullptr, *fmat = nullptr;
+    // binning order: faces sorted by the Morton code of their centroid, with their
+    // vertex indices in that order (the bin pass's workgroups then cover compact
+    // screen regions whatever the input order; visibility does not depend on it)
+    int32_t *bin_order = nullptr, *bin_fv = nullptr;
     DevMaterial* mats = nullptr;
     uint32_t* texels = nullptr;
     float* luts = nullptr;
@@ -85,9 +93,11 @@ struct rz_ctx {
     int bin_cap = 512;
     bool fused = true;
     bool profiling = false;
+    uint32_t persistent_blocks = 0;   // workgroups of the overflow shade launch (a few per CU)
+    int cus = 0;
     hipEvent_t ev[6] = {};
     std::string err;
-    DevBuf raster, cnormal, rec, tile_count, bins, vis, frame, depth, ids, counters, partials;
+    DevBuf raster, cnormal, rec, tile_count, bins, vis, lists, frame, depth, ids, counters, partials;
     DevBuf batch_arena, batch_tables;
     // learned shader: FP32 parameters, Adam moments, packed BF16 network, training scratch
     DevBuf mlp_w, mlp_m, mlp_v, mlp_packed, mlp_gpart, mlp_lpart, mlp_loss, mlp_count;
@@ -194,21 +204,29 @@ rz_status validate_mesh(rz_ctx* ctx, const rz_mesh_desc* d) {
 // Sizes and statistics sections of one frame's working buffers.
 struct FrameLayout {
     int ts = 16, tsl = 4, tiles_x = 0, tiles_y = 0, ntiles = 0;
-    bool fused = false;     // raster pass shades non-overflowed tiles; coverage per tile
+    bool fused = false;     // raster pass shades non-overflowed tiles
+    bool lists = false;     // raster pass driven by the bin pass's per-XCD tile lists
+    uint32_t raster_grid = 0, tiles_grid = 0;   // workgroups of the list-driven launches
     uint32_t W = 0, H = 0, cap = 0;
     size_t np = 0;
     int32_t nfn = 0;
     // per-workgroup statistics sections (uint32 each)
     size_t off_front = 0, off_pairs = 0, off_frag = 0, off_ovf = 0, off_cov = 0, n_part = 0;
-    size_t bytes[8] = {};   // raster, cnormal, rec, tile_count, bins, vis, counters, partials
+    size_t bytes[9] = {};   // raster, cnormal, rec, tile_count, bins, vis, lists, counters, partials
 };
 
-enum { kBufRaster, kBufCnormal, kBufRec, kBufTiles, kBufBins, kBufVis, kBufCounters, kBufPart, kNumBufs };
+enum { kBufRaster, kBufCnormal, kBufRec, kBufTiles, kBufBins, kBufVis, kBufLists, kBufCounters, kBufPart, kNumBufs };
 
-FrameLayout layout_frame(const rz_mesh* mesh, const rz_args& a, int ts, int cap, bool fused) {
+// raster_grid > 0: single frame, list-driven raster (raster_grid workgroups) and
+// overflow shade (tiles_grid); 0: one raster workgroup per tile (batches).
+FrameLayout layout_frame(const rz_mesh* mesh, const rz_args& a, int ts, int cap, bool fused, uint32_t raster_grid,
+                         uint32_t tiles_grid) {
     FrameLayout L;
     L.ts = ts;
-    L.fused = fused;
+    L.fused = fused && raster_grid > 0;
+    L.lists = raster_grid > 0;
+    L.raster_grid = raster_grid;
+    L.tiles_grid = tiles_grid;
     L.tsl = log2_exact(ts);
     L.W = a.image_width;
     L.H = a.image_height;
@@ -221,13 +239,15 @@ FrameLayout layout_frame(const rz_mesh* mesh, const rz_args& a, int ts, int cap,
     L.off_front = 0;
     L.off_pairs = L.off_front + bin_blocks(mesh->nf);
     L.off_frag = L.off_pairs + bin_blocks(mesh->nf);
-    L.off_ovf = L.off_frag + (size_t)L.ntiles;
+    L.off_ovf = L.off_frag + (L.lists ? (size_t)raster_grid : (size_t)L.ntiles);
     L.off_cov = L.off_ovf + overflow_blocks(mesh->nf);
-    L.n_part = L.off_cov + (fused ? (size_t)L.ntiles : (size_t)shade_blocks_x(L.W) * shade_blocks_y(L.H));
+    L.n_part = L.off_cov + (L.fused ? (size_t)raster_grid + tiles_grid
+                                    : (size_t)shade_blocks_x(L.W) * shade_blocks_y(L.H));
     L.bytes[kBufRaster] = (size_t)mesh->nv * 16;
     L.bytes[kBufCnormal] = (size_t)L.nfn * 16;
     L.bytes[kBufRec] = (size_t)mesh->nf * sizeof(FaceRecord);
-    L.bytes[kBufTiles] = (size_t)L.ntiles * 4 * kCountStride;
+    L.bytes[kBufTiles] = ((size_t)L.ntiles + 8) * 4 * kCountStride;   // + the 8 list counters
+    L.bytes[kBufLists] = L.lists ? (size_t)8 * L.ntiles * 4 : 0;
     L.bytes[kBufBins] = (size_t)L.ntiles * L.cap * 4;
     L.bytes[kBufVis] = L.np * 8;
     L.bytes[kBufCounters] = kNumCounters * 8;
@@ -294,12 +314,13 @@ void fill_frame(const rz_mesh* mesh, rz_light* lights, int32_t nl, const rz_args
     vp.cnormal = (float4*)buf[kBufCnormal];
     vp.tile_count = (uint32_t*)buf[kBufTiles];
     vp.counters = counters;
-    vp.nv = mesh->nv; vp.nn = lay.nfn; vp.ntiles = lay.ntiles;
+    vp.nv = mesh->nv; vp.nn = lay.nfn; vp.ntiles = lay.ntiles + 8;   // clears the list counters too
     vp.Wf = (float)lay.W; vp.Hf = (float)lay.H;
 
     BinParams& bp = L.bp;
     std::memset(&bp, 0, sizeof bp);
-    bp.fv = mesh->fv;
+    bp.fv = mesh->bin_fv;
+    bp.order = mesh->bin_order;
     bp.raster = vp.raster;
     bp.rec = (FaceRecord*)buf[kBufRec];
     bp.tile_count = vp.tile_count;
@@ -310,6 +331,11 @@ void fill_frame(const rz_mesh* mesh, rz_light* lights, int32_t nl, const rz_args
     bp.nf = mesh->nf; bp.W = lay.W; bp.H = lay.H;
     bp.tiles_x = lay.tiles_x; bp.ts_log2 = lay.tsl; bp.cap = lay.cap;
     bp.wind_cw = a.wind_clockwise != 0;
+    bp.ntiles = lay.ntiles;
+    bp.lists = lay.lists ? (uint32_t*)buf[kBufLists] : nullptr;
+    bp.list_count = bp.tile_count + (size_t)lay.ntiles * kCountStride;
+    bp.xcd_gw = lay.fused ? 128u / (uint32_t)lay.ts : 4u;
+    bp.xcd_gh = lay.fused ? 16u * (uint32_t)lay.ts / 128u : 4u;
 
     RasterParams& rp = L.rp;
     std::memset(&rp, 0, sizeof rp);
@@ -321,6 +347,9 @@ void fill_frame(const rz_mesh* mesh, rz_light* lights, int32_t nl, const rz_args
     rp.part_frag = part + lay.off_frag;
     rp.nf = mesh->nf; rp.W = lay.W; rp.H = lay.H;
     rp.tiles_x = lay.tiles_x; rp.ntiles = lay.ntiles; rp.ts_log2 = lay.tsl; rp.cap = lay.cap;
+    rp.lists = bp.lists;
+    rp.list_count = bp.list_count;
+    rp.max_blocks = lay.raster_grid;
     L.orp = rp;
     L.orp.part_frag = part + lay.off_ovf;
 
@@ -352,9 +381,14 @@ rz_status prepare_frame(rz_ctx* ctx, const rz_mesh* mesh, rz_light* lights, int3
     rz_status st = check_frame(ctx, mesh, lights, nl, args);
     if (st != RZ_OK) return st;
     RZ_HIP(ctx, hipSetDevice(ctx->device));
-    const FrameLayout lay = layout_frame(mesh, *args, ctx->tile_size, ctx->bin_cap, ctx->fused);
-    DevBuf* bufs[kNumBufs] = {&ctx->raster, &ctx->cnormal, &ctx->rec, &ctx->tile_count,
-                              &ctx->bins,   &ctx->vis,     &ctx->counters, &ctx->partials};
+    const int mode = (flags & RZ_FLAG_MLP_SHADE) ? kShadeMlp : kShadePlain;
+    const uint32_t per_cu = raster_blocks_per_cu(ctx->tile_size, ctx->fused, mode);
+    uint32_t grid = (uint32_t)ctx->cus * per_cu / 8u * 8u;
+    if (grid < 8) grid = 8;
+    const FrameLayout lay =
+        layout_frame(mesh, *args, ctx->tile_size, ctx->bin_cap, ctx->fused, grid, ctx->persistent_blocks);
+    DevBuf* bufs[kNumBufs] = {&ctx->raster, &ctx->cnormal, &ctx->rec,   &ctx->tile_count, &ctx->bins,
+                              &ctx->vis,    &ctx->lists,   &ctx->counters, &ctx->partials};
     void* ptrs[kNumBufs];
     for (int i = 0; i < kNumBufs; ++i) {
         if ((st = ensure(ctx, *bufs[i], lay.bytes[i])) != RZ_OK) return st;
@@ -364,8 +398,19 @@ rz_status prepare_frame(rz_ctx* ctx, const rz_mesh* mesh, rz_light* lights, int3
         return fail(ctx, RZ_ERR_INVALID_ARG, "RZ_FLAG_MLP_SHADE without network weights (rz_mlp_init / rz_mlp_set_weights)");
     fill_frame(mesh, lights, nl, *args, flags, lay, ptrs, L);
     L.sp.mlp = (const uint16_t*)ctx->mlp_packed.ptr;
+    L.sp.max_blocks = lay.tiles_grid;
     return RZ_OK;
 }
+
+// The overflow shade's parameters: its coverage partials follow the raster pass's.
+ShadeParams tiles_params(const FrameLaunch& L) {
+    ShadeParams t = L.sp;
+    t.part_cov += L.lay.raster_grid;
+    return t;
+}
+
+// Workgroups of the bin launch that clear buffers.
+constexpr uint32_t kClearBlocks = 1024;
 
 void bind_outputs(FrameLaunch& L, bool cleared, uint8_t* frame_d, float* depth_d, int32_t* ids_d) {
     L.rp.depth_in = cleared ? nullptr : depth_d;
@@ -373,6 +418,22 @@ void bind_outputs(FrameLaunch& L, bool cleared, uint8_t* frame_d, float* depth_d
     L.sp.frame = frame_d;
     L.sp.depth = depth_d;
     L.sp.ids = ids_d;
+    // fused pipeline on cleared buffers: the bin launch clears the outputs, so tiles
+    // with nothing binned need no raster workgroup work at all
+    const bool pre = L.lay.fused && cleared;
+    // the list-driven raster visits listed tiles only: with the fused pipeline the face
+    // ids of the others must still read -1, with the separate-shade pipeline their
+    // visibility words must be empty
+    const bool ids = L.lay.fused && ids_d;
+    const bool vis = L.lay.lists && !L.lay.fused;
+    L.bp.clear_frame = pre ? frame_d : nullptr;
+    L.bp.clear_depth = pre ? depth_d : nullptr;
+    L.bp.clear_ids = ids ? ids_d : nullptr;
+    L.bp.clear_vis = vis ? L.rp.vis : nullptr;
+    L.bp.clear_np = (pre || ids || vis) ? L.lay.np : 0;
+    L.bp.clear_blocks = (pre || ids || vis) ? kClearBlocks : 0u;
+    L.rp.pre_cleared = L.orp.pre_cleared = pre ? 1 : 0;
+    L.sp.pre_cleared = pre ? 1 : 0;
 }
 
 // 200 us of the 100 MHz clock: longer than the host takes to queue one frame
@@ -393,7 +454,7 @@ rz_status enqueue_frame(rz_ctx* ctx, const FrameLaunch& L, hipStream_t s, bool p
     RZ_HIP(ctx, launch_overflow(L.orp, s));
     if (prof) RZ_HIP(ctx, hipEventRecord(ctx->ev[4], s));
     if (L.lay.fused)
-        RZ_HIP(ctx, launch_shade_tiles(L.rp, L.sp, L.lay.ts, s));
+        RZ_HIP(ctx, launch_shade_tiles(L.rp, tiles_params(L), L.lay.ts, s));
     else
         RZ_HIP(ctx, launch_shade(L.sp, s));
     if (prof) RZ_HIP(ctx, hipEventRecord(ctx->ev[5], s));
@@ -455,7 +516,7 @@ rz_status prepare_batch(rz_ctx* ctx, const rz_batch_item* items, int32_t n, uint
         rz_status st = check_frame(ctx, it.mesh, it.lights, it.num_lights, &it.args);
         if (st != RZ_OK) return st;
         if (!it.frame || !it.depth) return fail(ctx, RZ_ERR_INVALID_ARG, "batch item %d: null frame or depth", i);
-        lays[i] = layout_frame(it.mesh, it.args, ctx->tile_size, ctx->bin_cap, false);
+        lays[i] = layout_frame(it.mesh, it.args, ctx->tile_size, ctx->bin_cap, false, 0, 0);
         stat_bytes += align256(lays[i].bytes[kBufCounters]) + align256(lays[i].bytes[kBufPart]);
         for (int b = 0; b < kBufCounters; ++b) work_bytes += align256(lays[i].bytes[b]);
     }
@@ -648,6 +709,10 @@ rz_status rz_create(int32_t device, rz_ctx** out) {
         if (e == hipSuccess)
             e = hipHostMalloc((void**)&ctx->host_counters, sizeof(unsigned long long) * kNumCounters,
                               hipHostMallocDefault);
+        int cus = 0;
+        if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device);
+        if (e == hipSuccess) ctx->persistent_blocks = (uint32_t)(cus * kPersistentPerCU) / 8u * 8u;
+        ctx->cus = cus;
         if (e != hipSuccess) st = fail(ctx, RZ_ERR_HIP, "context setup: %s", hipGetErrorString(e));
     } while (0);
     if (st != RZ_OK) {
@@ -663,7 +728,7 @@ void rz_destroy(rz_ctx* ctx) {
     (void)hipSetDevice(ctx->device);
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     DevBuf* bufs[] = {&ctx->raster, &ctx->cnormal, &ctx->rec,   &ctx->tile_count, &ctx->bins,
-                      &ctx->vis,    &ctx->frame,   &ctx->depth, &ctx->ids,        &ctx->counters,
+                      &ctx->vis,    &ctx->lists,   &ctx->frame,   &ctx->depth, &ctx->ids,        &ctx->counters,
                       &ctx->partials, &ctx->batch_arena, &ctx->batch_tables, &ctx->mlp_w, &ctx->mlp_m,
                       &ctx->mlp_v, &ctx->mlp_packed, &ctx->mlp_gpart, &ctx->mlp_lpart, &ctx->mlp_loss,
                       &ctx->mlp_count};
@@ -763,6 +828,64 @@ rz_status rz_copy_device(rz_ctx* ctx, void* dst, const void* src, size_t bytes) 
     return RZ_OK;
 }
 
+namespace {
+
+// 10 bits of each coordinate interleaved: z2 y2 x2 z1 y1 x1 ...
+uint32_t morton30(uint32_t x, uint32_t y, uint32_t z) {
+    auto spread = [](uint32_t v) {
+        v &= 0x3FFu;
+        v = (v | (v << 16)) & 0x030000FFu;
+        v = (v | (v << 8)) & 0x0300F00Fu;
+        v = (v | (v << 4)) & 0x030C30C3u;
+        v = (v | (v << 2)) & 0x09249249u;
+        return v;
+    };
+    return spread(x) | (spread(y) << 1) | (spread(z) << 2);
+}
+
+// Face processing order of the bin pass: by the Morton code of the face centroid
+// (positions as given: model space, or raster x, y for raster-space meshes), ties in
+// input order.  fv: validated vertex indices in input order.
+void bin_order(const rz_mesh_desc* d, const std::vector<int32_t>& fv, std::vector<int32_t>& order,
+               std::vector<int32_t>& bin_fv) {
+    const size_t F = (size_t)d->num_faces;
+    const int stride = d->raster_space ? 4 : 3;
+    const int dims = d->raster_space ? 2 : 3;
+    std::vector<float> c(3 * F, 0.f);
+    float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (size_t f = 0; f < F; ++f)
+        for (int a = 0; a < dims; ++a) {
+            float v = 0.f;
+            for (int k = 0; k < 3; ++k) v += d->vertices[(size_t)stride * fv[3 * f + k] + a];
+            v /= 3.0f;
+            c[3 * f + a] = v;
+            if (std::isfinite(v)) {
+                lo[a] = std::min(lo[a], v);
+                hi[a] = std::max(hi[a], v);
+            }
+        }
+    std::vector<uint64_t> key(F);
+    for (size_t f = 0; f < F; ++f) {
+        uint32_t q[3] = {0, 0, 0};
+        for (int a = 0; a < dims; ++a) {
+            const float span = hi[a] - lo[a];
+            const float t = span > 0.f ? (c[3 * f + a] - lo[a]) / span : 0.f;
+            q[a] = t >= 0.f ? (t < 1.f ? (uint32_t)(t * 1023.0f) : 1023u) : 0u;   // NaN -> 0
+        }
+        key[f] = ((uint64_t)morton30(q[0], q[1], q[2]) << 32) | (uint64_t)f;
+    }
+    std::sort(key.begin(), key.end());
+    order.resize(F);
+    bin_fv.resize(3 * F);
+    for (size_t i = 0; i < F; ++i) {
+        const size_t f = (size_t)(key[i] & 0xFFFFFFFFu);
+        order[i] = (int32_t)f;
+        for (int k = 0; k < 3; ++k) bin_fv[3 * i + k] = fv[3 * f + k];
+    }
+}
+
+}  // namespace
+
 rz_status rz_mesh_create(rz_ctx* ctx, const rz_mesh_desc* d, rz_mesh** out) {
     if (!ctx || !out) return RZ_ERR_INVALID_ARG;
     *out = nullptr;
@@ -848,7 +971,10 @@ rz_status rz_mesh_create(rz_ctx* ctx, const rz_mesh_desc* d, rz_mesh** out) {
             }
             fmat[f] = t.material;
         }
-        if ((st = upload(ctx, m, &m->fv, fv.data(), 12 * F)) != RZ_OK) return cleanup(st);
+        std::vector<int32_t> order, bin_fv;
+        bin_order(d, fv, order, bin_fv);
+        if ((st = upload(ctx, m, &m->bin_order, order.data(), 4 * F)) != RZ_OK) return cleanup(st);
+        if ((st = upload(ctx, m, &m->bin_fv, bin_fv.data(), 12 * F)) != RZ_OK) return cleanup(st);
         if ((st = upload(ctx, m, &m->fn, fn.data(), 12 * F)) != RZ_OK) return cleanup(st);
         if ((st = upload(ctx, m, &m->fuv, fuv.data(), 12 * F)) != RZ_OK) return cleanup(st);
         if ((st = upload(ctx, m, &m->fmat, fmat.data(), 4 * F)) != RZ_OK) return cleanup(st);
@@ -959,7 +1085,7 @@ rz_status rz_time_passes(rz_ctx* ctx, const rz_mesh* mesh, rz_light* lights, int
     auto overflow = [&](hipStream_t cs) -> rz_status { RZ_HIP(ctx, launch_overflow(L.orp, cs)); return RZ_OK; };
     auto shade = [&](hipStream_t cs) -> rz_status {
         if (L.lay.fused)
-            RZ_HIP(ctx, launch_shade_tiles(L.rp, L.sp, L.lay.ts, cs));
+            RZ_HIP(ctx, launch_shade_tiles(L.rp, tiles_params(L), L.lay.ts, cs));
         else
             RZ_HIP(ctx, launch_shade(L.sp, cs));
         return RZ_OK;

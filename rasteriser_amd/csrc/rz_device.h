@@ -70,7 +70,8 @@ struct VertexParams {
 };
 
 struct BinParams {
-    const int32_t* fv;          // 3 vertex indices per face
+    const int32_t* fv;          // 3 vertex indices per face, in binning order
+    const int32_t* order;       // binning order -> face index
     const float4* raster;
     FaceRecord* rec;
     uint32_t* tile_count;
@@ -84,6 +85,21 @@ struct BinParams {
     int32_t ts_log2;
     uint32_t cap;
     int32_t wind_cw;
+    int32_t ntiles;
+    // per-XCD lists of the tiles that received a face (null: not built); a tile goes to
+    // list (group % 8) of the raster pass's XcdGroups{xcd_gw, xcd_gh}
+    uint32_t* lists;
+    uint32_t* list_count;       // 8 counters, kCountStride apart
+    uint32_t xcd_gw, xcd_gh;
+    // extra workgroups of the bin launch that clear buffers (clear_np = 0: none): the
+    // visibility words (separate-shade pipeline) and, for the fused pipeline on cleared
+    // buffers, frame 0, depth 1.0, ids -1
+    uint8_t* clear_frame;
+    float* clear_depth;
+    int32_t* clear_ids;
+    unsigned long long* clear_vis;
+    size_t clear_np;
+    uint32_t clear_blocks;
 };
 
 struct RasterParams {
@@ -99,6 +115,10 @@ struct RasterParams {
     int32_t tiles_x, ntiles;
     int32_t ts_log2;
     uint32_t cap;
+    uint32_t max_blocks;        // workgroups of the list-driven launch (a multiple of 8)
+    int32_t pre_cleared;        // outputs cleared by the bin launch
+    const uint32_t* lists;      // per-XCD lists of tiles with faces (see BinParams)
+    const uint32_t* list_count;
 };
 
 struct ShadeParams {
@@ -129,6 +149,8 @@ struct ShadeParams {
     unsigned long long* sample_count;
     unsigned long long sample_cap;
     float inv_p00, inv_p11;          // 1 / projection[0][0], 1 / projection[1][1] (view vector)
+    uint32_t max_blocks;             // persistent strip kernel: workgroups to launch (0 = one per strip)
+    int32_t pre_cleared;             // empty pixels already hold 0 / 1.0 / -1 (bin launch cleared them)
     float normal_mat[16];            // inverse-transpose of the model-view (tangents)
     float light_ic[kMaxLights][3];   // intensity * colour per light
     float light_nd[kMaxLights][3];   // -trans_dir per light
