@@ -10,7 +10,7 @@ Timed region: `steps` frames captured in one HIP graph, launched once between tw
 device synchronisations.  `value` = faces / frame time in Mtri/s.
 
 `roofline` is for the dominant kernel of the frame: the pass with the longest
-average duration.  Each pass is timed as a HIP graph of min(steps, 500) back-to-back
+average duration.  Each pass is timed as a HIP graph of min(steps, 200) back-to-back
 launches of its kernel on the librz stream, with HIP events around the graph
 (rz_time_passes); `achieved` is the pass's algorithmic bytes per launch (counts from
 the frame itself) divided by that duration; `traffic` is the HBM bytes per launch read
@@ -351,7 +351,7 @@ def main():
     # per-pass durations: for each pass a HIP graph of `reps` launches of that kernel on the
     # librz stream, timed with HIP events (rz_time_passes); first the default fused
     # pipeline, then with the deferred shade as its own pass over the visibility buffer
-    reps = max(1, min(a.steps, 500))
+    reps = max(1, min(a.steps, 200))
     fused = ctx.fused_shade
     us = ctx.time_passes(mesh, lights, args, frame, depth, reps=reps)
     per_pass = per_pass_table(us, algorithmic_bytes(counts, scene, args, fused))
@@ -402,7 +402,7 @@ def main():
     mlp = None
     if not a.no_mlp:
         ctx.set_tuning(16, 512)
-        mlp = learned_shading(ctx, rz, scenes, np, a.mlp_steps, a.mlp_batch, 200)
+        mlp = learned_shading(ctx, rz, scenes, np, a.mlp_steps, a.mlp_batch, max(10, min(a.steps, 100)))
         log(f"learned shading: {mlp}")
     ctx.close()
     out = {

@@ -334,8 +334,6 @@ void fill_frame(const rz_mesh* mesh, rz_light* lights, int32_t nl, const rz_args
     bp.ntiles = lay.ntiles;
     bp.lists = lay.lists ? (uint32_t*)buf[kBufLists] : nullptr;
     bp.list_count = bp.tile_count + (size_t)lay.ntiles * kCountStride;
-    bp.xcd_gw = lay.fused ? 128u / (uint32_t)lay.ts : 4u;
-    bp.xcd_gh = lay.fused ? 16u * (uint32_t)lay.ts / 128u : 4u;
 
     RasterParams& rp = L.rp;
     std::memset(&rp, 0, sizeof rp);

@@ -86,11 +86,9 @@ struct BinParams {
     uint32_t cap;
     int32_t wind_cw;
     int32_t ntiles;
-    // per-XCD lists of the tiles that received a face (null: not built); a tile goes to
-    // list (group % 8) of the raster pass's XcdGroups{xcd_gw, xcd_gh}
+    // per-XCD lists of the tiles that received a face (null: not built)
     uint32_t* lists;
     uint32_t* list_count;       // 8 counters, kCountStride apart
-    uint32_t xcd_gw, xcd_gh;
     // extra workgroups of the bin launch that clear buffers (clear_np = 0: none): the
     // visibility words (separate-shade pipeline) and, for the fused pipeline on cleared
     // buffers, frame 0, depth 1.0, ids -1

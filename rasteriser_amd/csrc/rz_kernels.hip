@@ -84,12 +84,13 @@ __device__ __forceinline__ void vertex_body(const VertexParams& p, int blk) {
 }
 
 // ---------------------------------------------------------------------------------
-// A tile's first face puts the tile on its XCD's list for the raster pass.
+// A tile's first face puts the tile on an XCD's list for the raster pass.  Tiles are
+// dealt to XCDs by (tx + 3 ty) mod 8, so any screen region splits almost evenly (each
+// XCD's list must fit its resident workgroups, or its last tiles start a round late).
 __device__ __forceinline__ void list_tile(const BinParams& p, uint32_t tile) {
     if (!p.lists) return;
     const uint32_t tx = tile % (uint32_t)p.tiles_x, ty = tile / (uint32_t)p.tiles_x;
-    const uint32_t gx = ((uint32_t)p.tiles_x + p.xcd_gw - 1) / p.xcd_gw;
-    const uint32_t x = ((ty / p.xcd_gh) * gx + tx / p.xcd_gw) & 7u;
+    const uint32_t x = (tx + 3u * ty) & 7u;
     const uint32_t k = atomicAdd(&p.list_count[x * kCountStride], 1u);
     p.lists[(size_t)x * p.ntiles + k] = tile;
 }
@@ -541,7 +542,10 @@ __device__ __forceinline__ void raster_body(const RasterParams& p, const ShadePa
     __shared__ unsigned long long s_mask[kWords];
     __shared__ uint32_t s_wpre[kWords];
 
-    const int tid = threadIdx.x;
+    // an opaque copy of the thread index: values derived from it are recomputed for each
+    // tile instead of being hoisted out of the kernels' tile loops into registers
+    int tid = threadIdx.x;
+    asm volatile("" : "+v"(tid));
     const int lane = tid & 63, wave = tid >> 6;
     const int tx = tile % p.tiles_x, ty = tile / p.tiles_x;
     const int ox = tx * TS, oy = ty * TS;
@@ -756,8 +760,10 @@ __global__ __launch_bounds__(256) void k_bin(BinParams p) {
 // gridDim.x / 8.  Empty tiles cost no workgroup dispatch at all: dispatching one
 // workgroup per tile measured ~27 us for 8,160 tiles at 1080p, longer than the work.
 // Statistics are per-workgroup partial sums.
+// (minimum waves per SIMD in the launch bounds: 8 for the raster kernel, 6 for the fused
+// one, 3 for the fused learned shader, the most each reaches without register spills)
 template <int TS, int BLOCK>
-__global__ __launch_bounds__(BLOCK) void k_raster(RasterParams p) {
+__global__ __launch_bounds__(BLOCK, 8) void k_raster(RasterParams p) {
     const uint32_t xcd = blockIdx.x & 7u, step = gridDim.x >> 3;
     const uint32_t n = p.list_count[xcd * kCountStride];
     uint32_t frags = 0, cov = 0;
@@ -767,11 +773,12 @@ __global__ __launch_bounds__(BLOCK) void k_raster(RasterParams p) {
     store_block_sum(frags, p.part_frag + blockIdx.x, threadIdx.x, BLOCK);
 }
 // Raster and shade in one pass.
+// (the normalisation tables are read through the caches here: their LDS copy would cost
+// a resident workgroup per CU)
 template <int TS, int BLOCK, int MODE>
-__global__ __launch_bounds__(BLOCK) void k_raster_shade(RasterParams p, ShadeParams sp) {
-    __shared__ float s_lut[kLdsLuts * 256];
+__global__ __launch_bounds__(BLOCK, MODE == kShadeMlp ? 3 : 6) void k_raster_shade(RasterParams p, ShadeParams sp) {
+    const float* s_lut = sp.luts;
     __shared__ __align__(16) uint16_t s_net[MODE == kShadeMlp ? kPackedBytes / 2 : 8];
-    stage_luts(sp, s_lut, threadIdx.x, BLOCK);
     if (MODE == kShadeMlp) stage_net(sp.mlp, s_net, threadIdx.x, BLOCK);
     // (raster_body's first barrier orders these stores before any read)
     const uint32_t xcd = blockIdx.x & 7u, step = gridDim.x >> 3;
