@@ -85,8 +85,11 @@ def algorithmic_bytes(counts, scene, args, fused, cleared=True):
 def per_pass_table(us, nbytes):
     table = {}
     for p, t in us.items():
-        gbs = nbytes[p] / (t * 1e-6) / 1e9 if t > 0 else 0.0
-        table[p] = {"avg_us": t, "alg_bytes": nbytes[p], "GBps": gbs, "frac": gbs / HBM_PEAK_GBS}
+        if p == "network" and t == 0:
+            continue   # the learned shader's pass: not part of the plain frame
+        b = nbytes.get(p, 0)
+        gbs = b / (t * 1e-6) / 1e9 if t > 0 else 0.0
+        table[p] = {"avg_us": t, "alg_bytes": b, "GBps": gbs, "frac": gbs / HBM_PEAK_GBS}
     return table
 
 
@@ -189,8 +192,10 @@ def learned_shading(ctx, rz, scenes, np, steps, batch, frames):
         fused = ctx.fused_shade
         ctx.set_fused_shade(False)
         plain_ms = ctx.time_passes(mesh, lights, args, frame, depth, reps=frames)["shade"] / 1000.0
-        mlp_ms = ctx.time_passes(mesh, lights, args, frame, depth, reps=frames,
-                                 flags=rz.RZ_FLAG_MLP_SHADE)["shade"] / 1000.0
+        t_mlp = ctx.time_passes(mesh, lights, args, frame, depth, reps=frames, flags=rz.RZ_FLAG_MLP_SHADE)
+        mlp_records_ms = t_mlp["shade"] / 1000.0      # shade pass writing the fragment records
+        mlp_network_ms = t_mlp["network"] / 1000.0    # the network on the matrix cores
+        mlp_ms = mlp_records_ms + mlp_network_ms
         split_plain_frame_ms = graph_ms(ctx, mesh, lights, args, frame, depth, frames, flags)
         split_mlp_frame_ms = graph_ms(ctx, mesh, lights, args, frame, depth, frames, flags | rz.RZ_FLAG_MLP_SHADE)
         ctx.set_fused_shade(True)
@@ -216,11 +221,13 @@ def learned_shading(ctx, rz, scenes, np, steps, batch, frames):
             "loss_first": float(losses[0]), "loss_last100": float(losses[-100:].mean()),
             "fragments": frags,
             "shade_plain_ms": plain_ms, "shade_mlp_ms": mlp_ms,
+            "shade_mlp_records_ms": mlp_records_ms, "shade_mlp_network_ms": mlp_network_ms,
             "frame_plain_ms": plain_frame_ms, "frame_mlp_ms": mlp_frame_ms,
             "frame_plain_ms_separate_shade": split_plain_frame_ms,
             "frame_mlp_ms_separate_shade": split_mlp_frame_ms,
             "speedup": plain_ms / mlp_ms if mlp_ms > 0 else None,
-            "mlp_tflops": frags * MLP_FLOPS_PER_FRAGMENT / (mlp_ms * 1e-3) / 1e12 if mlp_ms > 0 else None,
+            "mlp_network_tflops": (frags * MLP_FLOPS_PER_FRAGMENT / (mlp_network_ms * 1e-3) / 1e12
+                                   if mlp_network_ms > 0 else None),
             "err_mean_per_channel_255": [float(e) for e in err.mean(axis=1)],
             "err_max_per_channel_255": [int(e) for e in err.max(axis=1)],
             "err_mean_255": float(err.mean()),

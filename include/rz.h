@@ -274,13 +274,14 @@ rz_status rz_autotune(rz_ctx* ctx, const rz_batch_item* items, int32_t num_items
 
 /* No reference counterpart.  Times each pass of one frame on the context's stream: a
  * HIP graph of `reps` back-to-back launches of the pass, timed by HIP events around its
- * second launch.  us[0..4] = vertex, bin, raster, overflow, shade in microseconds per
- * launch (bin = vertex+bin minus vertex, since binning appends to the tile lists).
- * Device buffers and RZ_FLAG_CLEARED required; RZ_FLAG_MLP_SHADE times the learned
- * shader.  Leaves the frame rendered in frame_device/depth_device. */
+ * second launch.  us[0..5] = vertex, bin, raster, overflow, shade, network in
+ * microseconds per launch (bin = vertex+bin minus vertex, since binning appends to the
+ * tile lists; network = the learned shader's second pass, 0 without
+ * RZ_FLAG_MLP_SHADE).  Device buffers and RZ_FLAG_CLEARED required.  Leaves the frame
+ * rendered in frame_device/depth_device. */
 rz_status rz_time_passes(rz_ctx* ctx, const rz_mesh* mesh, rz_light* lights, int32_t num_lights,
                          const rz_args* args, uint8_t* frame_device, float* depth_device, uint32_t flags,
-                         int32_t reps, float us[5]);
+                         int32_t reps, float us[6]);
 
 /* ---- learned shader ------------------------------------------------------------ */
 

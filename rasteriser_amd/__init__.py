@@ -310,14 +310,15 @@ class Context:
 
     def time_passes(self, mesh: Mesh, lights: np.ndarray, args: Args, frame: "DeviceArray",
                     depth: "DeviceArray", reps: int = 200, flags: int = RZ_FLAG_CLEARED) -> dict:
-        """rz_time_passes: microseconds per launch of each pass (graph of `reps` launches)."""
+        """rz_time_passes: microseconds per launch of each pass (graph of `reps` launches);
+        "network" is the learned shader's second pass (0 without RZ_FLAG_MLP_SHADE)."""
         lights = _lights(lights)
-        us = (C.c_float * 5)()
+        us = (C.c_float * 6)()
         _check(self.lib.rz_time_passes(self.handle, mesh.handle, lights.ctypes.data, lights.shape[0],
                                        C.byref(args), C.c_void_p(frame.ptr), C.c_void_p(depth.ptr),
                                        flags | RZ_FLAG_CLEARED | RZ_FLAG_DEVICE_BUFFERS, int(reps), us),
                self.handle)
-        return dict(zip(("vertex", "bin", "raster", "overflow", "shade"), [float(u) for u in us]))
+        return dict(zip(("vertex", "bin", "raster", "overflow", "shade", "network"), [float(u) for u in us]))
 
     # ---- learned shader -------------------------------------------------------------
     def mlp_init(self, seed: int = 0):

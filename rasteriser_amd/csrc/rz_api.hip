@@ -31,6 +31,9 @@ hipError_t launch_overflow_batch(const RasterParams* items, const uint2* map, ui
 hipError_t launch_shade_batch(const ShadeParams* items, const uint2* map, uint32_t blocks, int mode,
                               hipStream_t s);
 hipError_t launch_hold(uint64_t ticks, hipStream_t s);
+hipError_t launch_mlp_shade(const void* records, const uint32_t* counts, uint32_t region, const uint32_t* lists,
+                            const uint32_t* list_count, uint32_t ntiles, const uint16_t* packed, uint8_t* frame,
+                            size_t np, uint32_t blocks, hipStream_t s);
 uint32_t raster_blocks_per_cu(int tile_size, bool fused, int mode);
 hipError_t launch_copy(const void* src, void* dst, size_t n16, hipStream_t s);
 hipError_t launch_mlp_pack(const float* W, uint16_t* packed, hipStream_t s);
@@ -101,6 +104,8 @@ struct rz_ctx {
     DevBuf batch_arena, batch_tables;
     // learned shader: FP32 parameters, Adam moments, packed BF16 network, training scratch
     DevBuf mlp_w, mlp_m, mlp_v, mlp_packed, mlp_gpart, mlp_lpart, mlp_loss, mlp_count;
+    DevBuf frag_records;   // the learned shader's fragment records (kFragRecordBytes per pixel)
+    DevBuf frag_counts;    // records per region
     bool mlp_ready = false;
     uint32_t mlp_steps = 0;
     std::vector<uint32_t> host_partials;
@@ -265,6 +270,7 @@ struct FrameLaunch {
     FrameLayout lay;
     uint32_t* part = nullptr;
     unsigned long long* counters = nullptr;
+    uint32_t mlp_blocks = 0;   // learned shader: workgroups of k_mlp_shade (0: not used)
 };
 
 rz_status check_frame(rz_ctx* ctx, const rz_mesh* mesh, const rz_light* lights, int32_t nl, const rz_args* args) {
@@ -379,7 +385,8 @@ rz_status prepare_frame(rz_ctx* ctx, const rz_mesh* mesh, rz_light* lights, int3
     rz_status st = check_frame(ctx, mesh, lights, nl, args);
     if (st != RZ_OK) return st;
     RZ_HIP(ctx, hipSetDevice(ctx->device));
-    const int mode = (flags & RZ_FLAG_MLP_SHADE) ? kShadeMlp : kShadePlain;
+    // single frames run the learned shader in two phases: fragment records, then k_mlp_shade
+    const int mode = (flags & RZ_FLAG_MLP_SHADE) ? kShadeFeat : kShadePlain;
     const uint32_t per_cu = raster_blocks_per_cu(ctx->tile_size, ctx->fused, mode);
     uint32_t grid = (uint32_t)ctx->cus * per_cu / 8u * 8u;
     if (grid < 8) grid = 8;
@@ -397,7 +404,28 @@ rz_status prepare_frame(rz_ctx* ctx, const rz_mesh* mesh, rz_light* lights, int3
     fill_frame(mesh, lights, nl, *args, flags, lay, ptrs, L);
     L.sp.mlp = (const uint16_t*)ctx->mlp_packed.ptr;
     L.sp.max_blocks = lay.tiles_grid;
+    if (mode == kShadeFeat) {
+        // one region of records per tile
+        const uint32_t region = (uint32_t)(lay.ts * lay.ts);
+        if ((st = ensure(ctx, ctx->frag_records, (size_t)lay.ntiles * region * kFragRecordBytes)) != RZ_OK) return st;
+        if ((st = ensure(ctx, ctx->frag_counts, (size_t)lay.ntiles * 4 * kCountStride)) != RZ_OK) return st;
+        L.sp.mode = kShadeFeat;
+        L.sp.samples = (float*)ctx->frag_records.ptr;
+        L.sp.rec_counts = (uint32_t*)ctx->frag_counts.ptr;
+        L.sp.rec_region = region;
+        L.sp.tiles_x = lay.tiles_x;
+        L.sp.ts_log2 = lay.tsl;
+        if (!lay.fused) L.rp.rec_counts = L.sp.rec_counts;   // zeroed per listed tile by k_raster
+        L.mlp_blocks = (uint32_t)ctx->cus * 8u;
+    }
     return RZ_OK;
+}
+
+// The learned shader's second phase, after the pass that appended the fragment records.
+hipError_t launch_mlp_phase(const FrameLaunch& L, hipStream_t s) {
+    if (L.sp.mode != kShadeFeat) return hipSuccess;
+    return launch_mlp_shade(L.sp.samples, L.sp.rec_counts, L.sp.rec_region, L.rp.lists, L.rp.list_count,
+                            (uint32_t)L.lay.ntiles, L.sp.mlp, L.sp.frame, L.lay.np, L.mlp_blocks, s);
 }
 
 // The overflow shade's parameters: its coverage partials follow the raster pass's.
@@ -455,6 +483,7 @@ rz_status enqueue_frame(rz_ctx* ctx, const FrameLaunch& L, hipStream_t s, bool p
         RZ_HIP(ctx, launch_shade_tiles(L.rp, tiles_params(L), L.lay.ts, s));
     else
         RZ_HIP(ctx, launch_shade(L.sp, s));
+    RZ_HIP(ctx, launch_mlp_phase(L, s));
     if (prof) RZ_HIP(ctx, hipEventRecord(ctx->ev[5], s));
     return RZ_OK;
 }
@@ -729,7 +758,7 @@ void rz_destroy(rz_ctx* ctx) {
                       &ctx->vis,    &ctx->lists,   &ctx->frame,   &ctx->depth, &ctx->ids,        &ctx->counters,
                       &ctx->partials, &ctx->batch_arena, &ctx->batch_tables, &ctx->mlp_w, &ctx->mlp_m,
                       &ctx->mlp_v, &ctx->mlp_packed, &ctx->mlp_gpart, &ctx->mlp_lpart, &ctx->mlp_loss,
-                      &ctx->mlp_count};
+                      &ctx->mlp_count, &ctx->frag_records, &ctx->frag_counts};
     for (DevBuf* b : bufs)
         if (b->ptr) (void)hipFree(b->ptr);
     for (hipEvent_t e : ctx->ev)
@@ -1041,7 +1070,7 @@ void rz_mesh_destroy(rz_ctx* ctx, rz_mesh* m) {
 }
 
 rz_status rz_time_passes(rz_ctx* ctx, const rz_mesh* mesh, rz_light* lights, int32_t nl, const rz_args* args,
-                         uint8_t* frame_device, float* depth_device, uint32_t flags, int32_t reps, float us[5]) {
+                         uint8_t* frame_device, float* depth_device, uint32_t flags, int32_t reps, float us[6]) {
     if (!ctx) return RZ_ERR_INVALID_ARG;
     if (!frame_device || !depth_device || !us || reps < 1 || !(flags & RZ_FLAG_CLEARED))
         return fail(ctx, RZ_ERR_INVALID_ARG, "rz_time_passes: device buffers, reps >= 1 and RZ_FLAG_CLEARED required");
@@ -1067,6 +1096,7 @@ rz_status rz_time_passes(rz_ctx* ctx, const rz_mesh* mesh, rz_light* lights, int
         if (e != hipSuccess) return fail(ctx, RZ_ERR_HIP, "rz_time_passes: %s", hipGetErrorString(e));
         return RZ_OK;
     };
+    const bool feat = L.sp.mode == kShadeFeat;
     auto vertex = [&](hipStream_t cs) -> rz_status { RZ_HIP(ctx, launch_vertex(L.vp, cs)); return RZ_OK; };
     auto vertex_bin = [&](hipStream_t cs) -> rz_status {
         RZ_HIP(ctx, launch_vertex(L.vp, cs));
@@ -1088,21 +1118,37 @@ rz_status rz_time_passes(rz_ctx* ctx, const rz_mesh* mesh, rz_light* lights, int
             RZ_HIP(ctx, launch_shade(L.sp, cs));
         return RZ_OK;
     };
-    float ms_v = 0, ms_vb = 0, ms_r = 0, ms_o = 0, ms_s = 0;
+    auto network = [&](hipStream_t cs) -> rz_status { RZ_HIP(ctx, launch_mlp_phase(L, cs)); return RZ_OK; };
+    // the separate shade pass of the learned shader accumulates into per-tile record counts
+    // that its raster pass zeroes: timed as raster + shade minus raster
+    auto raster_shade = [&](hipStream_t cs) -> rz_status {
+        rz_status r = raster(cs);
+        return r != RZ_OK ? r : shade(cs);
+    };
+    const bool with_raster = feat && !L.lay.fused;
+    float ms_v = 0, ms_vb = 0, ms_r = 0, ms_o = 0, ms_s = 0, ms_n = 0;
     // vertex alone, then vertex+bin (each repetition leaves a whole binned frame behind);
-    // raster, overflow and shade are idempotent on that state
+    // raster, overflow and shade are idempotent on that state; the network pass last,
+    // on the records of one whole frame
     if ((st = enqueue_frame(ctx, L, s, false)) != RZ_OK) return st;
     if ((st = timed(vertex, ms_v)) != RZ_OK) return st;
     if ((st = timed(vertex_bin, ms_vb)) != RZ_OK) return st;
     if ((st = timed(raster, ms_r)) != RZ_OK) return st;
     if ((st = timed(overflow, ms_o)) != RZ_OK) return st;
-    if ((st = timed(shade, ms_s)) != RZ_OK) return st;
+    if (with_raster) {
+        if ((st = timed(raster_shade, ms_s)) != RZ_OK) return st;
+        ms_s -= ms_r;
+    } else if ((st = timed(shade, ms_s)) != RZ_OK) {
+        return st;
+    }
+    if (feat && (st = timed(network, ms_n)) != RZ_OK) return st;
     const float k = 1000.0f / (float)reps;
     us[0] = ms_v * k;
     us[1] = (ms_vb - ms_v) * k;
     us[2] = ms_r * k;
     us[3] = ms_o * k;
     us[4] = ms_s * k;
+    us[5] = ms_n * k;
     return RZ_OK;
 }
 

@@ -25,7 +25,8 @@ constexpr uint32_t kCountStride = 32;
 // the overflow pass tests); every other statistic is a per-workgroup partial sum
 // written with a plain store and reduced only when a caller asks for statistics
 // (same-address device-scope atomics serialise across the eight XCDs).
-enum ShadeMode : int { kShadePlain = 0, kShadeMlp = 1, kShadeCollect = 2 };
+// kShadeFeat: the learned shader's first phase (fragment records for k_mlp_shade)
+enum ShadeMode : int { kShadePlain = 0, kShadeMlp = 1, kShadeCollect = 2, kShadeFeat = 3 };
 
 enum Counter : int {
     kCntOverflowTiles = 0,
@@ -115,6 +116,7 @@ struct RasterParams {
     uint32_t cap;
     uint32_t max_blocks;        // workgroups of the list-driven launch (a multiple of 8)
     int32_t pre_cleared;        // outputs cleared by the bin launch
+    uint32_t* rec_counts;       // separate shade pipeline, learned shader: zeroed per listed tile
     const uint32_t* lists;      // per-XCD lists of tiles with faces (see BinParams)
     const uint32_t* list_count;
 };
@@ -141,9 +143,15 @@ struct ShadeParams {
     int32_t flat;
     int32_t nl;
     int32_t nmat;
-    int32_t mode;                    // kShadePlain, kShadeMlp or kShadeCollect
+    int32_t mode;                    // ShadeMode
     const uint16_t* mlp;             // packed network (kShadeMlp)
-    float* samples;                  // training samples out (kShadeCollect)
+    float* samples;                  // training samples (kShadeCollect) or fragment records (kShadeFeat) out
+    // kShadeFeat: the records of a tile's covered pixels are compacted into the tile's
+    // region of rec_region (= tile pixels) records, their number in
+    // rec_counts[tile * kCountStride]
+    uint32_t* rec_counts;
+    uint32_t rec_region;
+    int32_t tiles_x, ts_log2;
     unsigned long long* sample_count;
     unsigned long long sample_cap;
     float inv_p00, inv_p11;          // 1 / projection[0][0], 1 / projection[1][1] (view vector)

@@ -302,10 +302,11 @@ __device__ __forceinline__ void stage_luts(const ShadeParams& p, float* s_lut, i
 // MODE: kShadePlain (the reference's light loop), kShadeMlp (the learned surrogate on
 // matrix cores), kShadeCollect (light loop, and each shaded fragment's MLP inputs and
 // colour/255 appended to the sample buffer for training).
+// kShadeFeat: rec receives the fragment's record words 0-5 (the caller stores them).
 template <int MODE>
 __device__ __forceinline__ bool shade_pixel(const ShadeParams& p, const float* s_lut, const uint16_t* s_net,
                                             uint32_t px, uint32_t py, bool valid, unsigned long long key, int lane,
-                                            uint16_t* s_in, float* s_out) {
+                                            uint16_t* s_in, float* s_out, uint32_t* rec = nullptr) {
     const size_t np = (size_t)p.W * p.H;
     const size_t i = (size_t)py * p.W + px;
     const bool hit = valid && key != kEmptyKey;
@@ -371,11 +372,12 @@ __device__ __forceinline__ bool shade_pixel(const ShadeParams& p, const float* s
         feat[0] = nx; feat[1] = ny; feat[2] = nz;
         if (m.nmap_w > 0) {
             normal_map_sample(p, m, u, v, ts);
-            if (MODE != kShadeMlp) bend_normal(p, f, ts, nx, ny, nz);   // the network takes the sample itself
+            if (MODE != kShadeMlp && MODE != kShadeFeat)   // the network takes the sample itself
+                bend_normal(p, f, ts, nx, ny, nz);
         }
         p.depth[i] = z;
         if (p.ids) p.ids[i] = (int32_t)f;
-        if (MODE != kShadeMlp) {
+        if (MODE == kShadePlain || MODE == kShadeCollect) {
             // shading.cpp:20-34
             constexpr float kOneOverPi = 0.318309886183790671537767526745028724f;
             float acc0 = 0.f, acc1 = 0.f, acc2 = 0.f;
@@ -413,6 +415,10 @@ __device__ __forceinline__ bool shade_pixel(const ShadeParams& p, const float* s
             for (int c = 0; c < 3; ++c) o[kMlpIn + c] = (float)rgb[c] / 255.0f;
         }
     }
+    if (MODE == kShadeFeat && hit) {
+#pragma unroll
+        for (int k = 0; k < 6; ++k) rec[k] = bf16x2(feat[2 * k], feat[2 * k + 1]);
+    }
     if (MODE == kShadeMlp) {
 #pragma unroll
         for (int k = 0; k < 16; ++k) s_in[lane * 16 + k] = to_bf16((hit && k < kMlpIn) ? feat[k] : 0.f);
@@ -428,12 +434,38 @@ __device__ __forceinline__ bool shade_pixel(const ShadeParams& p, const float* s
             }
         }
     }
-    if (hit) {
+    if (hit && MODE != kShadeFeat) {   // (kShadeFeat: k_mlp_shade writes the colour)
         p.frame[i] = rgb[0];
         p.frame[i + np] = rgb[1];
         p.frame[i + 2 * np] = rgb[2];
     }
     return hit;
+}
+
+// kShadeFeat: stores the records of the workgroup's covered pixels compactly in its
+// region, after `base` earlier ones (ballot ranks within a wave, wave totals through LDS;
+// no global atomics); returns the new count.  Every thread of the workgroup calls it.
+template <int NW>
+__device__ __forceinline__ uint32_t region_append(bool hit, const uint32_t w[6], uint32_t pixel, uint4* region,
+                                                  uint32_t base, int tid, uint32_t* s_w) {
+    const int lane = tid & 63, wave = tid >> 6;
+    const unsigned long long bal = __ballot(hit);
+    if (lane == 0) s_w[wave] = (uint32_t)__popcll(bal);
+    __syncthreads();
+    uint32_t off = base, total = base;
+#pragma unroll
+    for (int k = 0; k < NW; ++k) {
+        const uint32_t v = s_w[k];
+        if (k < wave) off += v;
+        total += v;
+    }
+    __syncthreads();   // s_w is rewritten by the next call
+    if (hit) {
+        const uint32_t slot = off + (uint32_t)__popcll(bal & ((1ull << lane) - 1ull));
+        region[2 * slot] = make_uint4(w[0], w[1], w[2], w[3]);
+        region[2 * slot + 1] = make_uint4(w[4], w[5], pixel, 0u);
+    }
+    return total;
 }
 
 // Deferred shade pass: 64 x 4 pixel blocks, each wave 64 consecutive pixels of a row.
@@ -449,7 +481,31 @@ __device__ __forceinline__ void shade_body(const ShadeParams& p, const float* s_
     const uint32_t px = (blk % nbx) * 64 + threadIdx.x, py = (blk / nbx) * 4 + threadIdx.y;
     const bool valid = px < p.W && py < p.H;
     const unsigned long long key = valid ? p.vis[(size_t)py * p.W + px] : kEmptyKey;
-    const bool hit = shade_pixel<MODE>(p, s_lut, s_net, px, py, valid, key, threadIdx.x, s_in[ws], s_out[ws]);
+    uint32_t rec[6];
+    const bool hit = shade_pixel<MODE>(p, s_lut, s_net, px, py, valid, key, threadIdx.x, s_in[ws], s_out[ws], rec);
+    if (MODE == kShadeFeat) {
+        // the wave's pixels lie in up to 64 / tile-edge tiles: one slot reservation per tile
+        const uint32_t tile = (py >> p.ts_log2) * (uint32_t)p.tiles_x + (px >> p.ts_log2);
+        const int lane = threadIdx.x;
+        unsigned long long todo = __ballot(hit);
+        while (todo) {   // uniform over the wave
+            const int leader = __ffsll((long long)todo) - 1;
+            const uint32_t t = __shfl(tile, leader, 64);
+            const unsigned long long same = __ballot(hit && tile == t);
+            uint32_t base = 0;
+            if (lane == leader) base = atomicAdd(&p.rec_counts[(size_t)t * kCountStride], (uint32_t)__popcll(same));
+            base = __shfl(base, leader, 64);
+            const uint32_t slot = base + (uint32_t)__popcll(same & ((1ull << lane) - 1ull));
+            // (a frame's records never exceed the tile's pixels; the bound keeps any
+            // launch sequence that skips the raster pass's reset inside the region)
+            if (((same >> lane) & 1ull) && slot < p.rec_region) {
+                uint4* o = reinterpret_cast<uint4*>(p.samples) + 2 * ((size_t)t * p.rec_region + slot);
+                o[0] = make_uint4(rec[0], rec[1], rec[2], rec[3]);
+                o[1] = make_uint4(rec[4], rec[5], (uint32_t)((size_t)py * p.W + px), 0u);
+            }
+            todo &= ~same;
+        }
+    }
     store_block_sum(hit ? 1u : 0u, p.part_cov + blk, tid, 256);
 }
 
@@ -464,14 +520,23 @@ __device__ __forceinline__ void shade_tile_body(const RasterParams& rp, const Sh
     const int tid = threadIdx.x, lane = tid & 63;
     const int ws = MODE == kShadeMlp ? tid >> 6 : 0;
     const int ox = (tile % rp.tiles_x) * TS, oy = (tile / rp.tiles_x) * TS;
+    __shared__ uint32_t s_w[NW];
+    uint32_t nrec = 0;
 #pragma unroll
     for (int l0 = 0; l0 < TS * TS; l0 += BLOCK) {
         const int l = l0 + tid;
         const uint32_t x = ox + (l % TS), y = oy + (l / TS);
         const bool valid = x < p.W && y < p.H;
         const unsigned long long key = valid ? p.vis[(size_t)y * p.W + x] : kEmptyKey;
-        cov += shade_pixel<MODE>(p, s_lut, s_net, x, y, valid, key, lane, s_in[ws], s_out[ws]) ? 1u : 0u;
+        uint32_t rec[6];
+        const bool hit = shade_pixel<MODE>(p, s_lut, s_net, x, y, valid, key, lane, s_in[ws], s_out[ws], rec);
+        cov += hit ? 1u : 0u;
+        if (MODE == kShadeFeat)
+            nrec = region_append<NW>(hit, rec, (uint32_t)((size_t)y * p.W + x),
+                                     reinterpret_cast<uint4*>(p.samples) + 2 * (size_t)tile * p.rec_region, nrec,
+                                     tid, s_w);
     }
+    if (MODE == kShadeFeat && tid == 0) p.rec_counts[(size_t)tile * kCountStride] = nrec;
     __syncthreads();   // the staging buffers are reused by the next tile
 }
 
@@ -538,7 +603,7 @@ __device__ __forceinline__ void raster_body(const RasterParams& p, const ShadePa
     uint32_t* s_rect = reinterpret_cast<uint32_t*>(s_raw + 10 * BLOCK * 4);   // rx0 | ry0 << 8 | rw << 16
     uint32_t* s_face = s_rect + BLOCK;
     uint32_t* s_start = s_face + BLOCK;
-    __shared__ uint32_t s_wsum[NW], s_wlive[NW];
+    __shared__ uint32_t s_wsum[NW], s_wlive[NW], s_wfeat[NW];
     __shared__ unsigned long long s_mask[kWords];
     __shared__ uint32_t s_wpre[kWords];
 
@@ -552,6 +617,7 @@ __device__ __forceinline__ void raster_body(const RasterParams& p, const ShadePa
 
     const uint32_t count = p.tile_count[(size_t)tile * kCountStride];
     const uint32_t n = count > p.cap ? 0u : count;   // overflowed tiles go to k_overflow
+    if (!FUSED && p.rec_counts && tid == 0) p.rec_counts[(size_t)tile * kCountStride] = 0u;   // filled by k_shade
 #pragma unroll
     for (int j = 0; j < PPT; ++j) {
         const int l = tid + j * BLOCK;
@@ -659,13 +725,21 @@ __device__ __forceinline__ void raster_body(const RasterParams& p, const ShadePa
     if (FUSED && count <= p.cap) {
         uint16_t* s_in = reinterpret_cast<uint16_t*>(s_raw) + (kNet ? wave : 0) * 64 * 16;
         float* s_out = reinterpret_cast<float*>(s_raw + NW * 64 * 16 * 2) + (kNet ? wave : 0) * 64 * 4;
+        uint32_t nrec = 0;
 #pragma unroll
         for (int j = 0; j < PPT; ++j) {
             const int l = tid + j * BLOCK;
             const uint32_t x = ox + (l % TS), y = oy + (l / TS);
             const bool valid = x < p.W && y < p.H;
-            cov += shade_pixel<MODE>(*sp, s_lut, s_net, x, y, valid, s_key[l], lane, s_in, s_out) ? 1u : 0u;
+            uint32_t rec[6];
+            const bool hit = shade_pixel<MODE>(*sp, s_lut, s_net, x, y, valid, s_key[l], lane, s_in, s_out, rec);
+            cov += hit ? 1u : 0u;
+            if (MODE == kShadeFeat)
+                nrec = region_append<NW>(hit, rec, y * p.W + x,
+                                         reinterpret_cast<uint4*>(sp->samples) + 2 * (size_t)tile * sp->rec_region,
+                                         nrec, tid, s_wfeat);
         }
+        if (MODE == kShadeFeat && tid == 0) sp->rec_counts[(size_t)tile * kCountStride] = nrec;
     } else {
 #pragma unroll
         for (int j = 0; j < PPT; ++j) {
@@ -760,10 +834,11 @@ __global__ __launch_bounds__(256) void k_bin(BinParams p) {
 // gridDim.x / 8.  Empty tiles cost no workgroup dispatch at all: dispatching one
 // workgroup per tile measured ~27 us for 8,160 tiles at 1080p, longer than the work.
 // Statistics are per-workgroup partial sums.
-// (minimum waves per SIMD in the launch bounds: 8 for the raster kernel, 6 for the fused
-// one, 3 for the fused learned shader, the most each reaches without register spills)
+// (minimum waves per SIMD in the launch bounds: 8 for the raster kernel (6 for 32-pixel
+// tiles, whose LDS allows no more), 6 for the fused one, 3 for the fused learned shader:
+// the most each reaches without register spills)
 template <int TS, int BLOCK>
-__global__ __launch_bounds__(BLOCK, 8) void k_raster(RasterParams p) {
+__global__ __launch_bounds__(BLOCK, TS == 32 ? 6 : 8) void k_raster(RasterParams p) {
     const uint32_t xcd = blockIdx.x & 7u, step = gridDim.x >> 3;
     const uint32_t n = p.list_count[xcd * kCountStride];
     uint32_t frags = 0, cov = 0;
@@ -915,6 +990,8 @@ uint32_t raster_blocks_per_cu(int tile_size, bool fused, int mode) {
         f = reinterpret_cast<const void*>(&k_raster_shade<TS, B, kShadeMlp>);                        \
     else if (mode == kShadeCollect)                                                                 \
         f = reinterpret_cast<const void*>(&k_raster_shade<TS, B, kShadeCollect>);                    \
+    else if (mode == kShadeFeat)                                                                    \
+        f = reinterpret_cast<const void*>(&k_raster_shade<TS, B, kShadeFeat>);                       \
     else                                                                                            \
         f = reinterpret_cast<const void*>(&k_raster_shade<TS, B, kShadePlain>);
     switch (tile_size) {
@@ -967,6 +1044,7 @@ hipError_t launch_raster_shade(const RasterParams& p, const ShadeParams& sp, int
         case kShadePlain: return launch_raster_shade_mode<kShadePlain>(p, sp, tile_size, s);
         case kShadeMlp: return launch_raster_shade_mode<kShadeMlp>(p, sp, tile_size, s);
         case kShadeCollect: return launch_raster_shade_mode<kShadeCollect>(p, sp, tile_size, s);
+        case kShadeFeat: return launch_raster_shade_mode<kShadeFeat>(p, sp, tile_size, s);
         default: return hipErrorInvalidValue;
     }
 }
@@ -988,6 +1066,7 @@ hipError_t launch_shade_tiles(const RasterParams& p, const ShadeParams& sp, int 
         case kShadePlain: return launch_shade_tiles_mode<kShadePlain>(p, sp, tile_size, s);
         case kShadeMlp: return launch_shade_tiles_mode<kShadeMlp>(p, sp, tile_size, s);
         case kShadeCollect: return launch_shade_tiles_mode<kShadeCollect>(p, sp, tile_size, s);
+        case kShadeFeat: return launch_shade_tiles_mode<kShadeFeat>(p, sp, tile_size, s);
         default: return hipErrorInvalidValue;
     }
 }
@@ -1003,6 +1082,7 @@ hipError_t launch_shade(const ShadeParams& p, hipStream_t s) {
         case kShadePlain: hipLaunchKernelGGL(k_shade<kShadePlain>, grid, block, 0, s, p); break;
         case kShadeMlp: hipLaunchKernelGGL(k_shade<kShadeMlp>, grid, block, 0, s, p); break;
         case kShadeCollect: hipLaunchKernelGGL(k_shade<kShadeCollect>, grid, block, 0, s, p); break;
+        case kShadeFeat: hipLaunchKernelGGL(k_shade<kShadeFeat>, grid, block, 0, s, p); break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();

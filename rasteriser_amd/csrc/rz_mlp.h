@@ -39,6 +39,9 @@ constexpr int kMlpSampleFloats = kMlpIn + kMlpOut;   // training sample: inputs,
 constexpr int kFragL1 = 0, kFragL2 = kFragL1 + 4 * 64 * 4, kFragL3 = kFragL2 + 16 * 64 * 4,
               kFragEnd = kFragL3 + 4 * 64 * 4;   // uint16 count: 6144
 constexpr int kPackedBytes = kFragEnd * 2 + (64 + 64 + 16) * 4;
+// A fragment record of the two-phase learned shader: the 12 inputs as bf16 (words 0-5,
+// input 2k in the low half of word k), the pixel index (word 6), word 7 zero.
+constexpr int kFragRecordBytes = 32;
 
 typedef short mlp_s4 __attribute__((ext_vector_type(4)));
 typedef short mlp_s8 __attribute__((ext_vector_type(8)));
@@ -65,6 +68,18 @@ __host__ __device__ __forceinline__ float from_bf16(uint16_t h) {
     return __builtin_bit_cast(float, (uint32_t)h << 16);
 }
 __host__ __device__ __forceinline__ float round_bf16(float f) { return from_bf16(to_bf16(f)); }
+
+// Two floats to packed bf16 (a in the low half) with gfx950's conversion instruction:
+// round to nearest even, so equal to to_bf16 on finite values.
+__device__ __forceinline__ uint32_t bf16x2(float a, float b) {
+    return (uint32_t)__builtin_bit_cast(uint16_t, (__bf16)a) | ((uint32_t)__builtin_bit_cast(uint16_t, (__bf16)b) << 16);
+}
+
+__device__ __forceinline__ mlp_s4 relu_bf16x4(mlp_f4 acc, const float* bias) {
+    const float v0 = fmaxf(acc[0] + bias[0], 0.f), v1 = fmaxf(acc[1] + bias[1], 0.f);
+    const float v2 = fmaxf(acc[2] + bias[2], 0.f), v3 = fmaxf(acc[3] + bias[3], 0.f);
+    return __builtin_bit_cast(mlp_s4, make_uint2(bf16x2(v0, v1), bf16x2(v2, v3)));
+}
 
 // Copies the packed network (kPackedBytes) into LDS; the whole workgroup calls it and
 // a barrier must follow before mlp_wave reads it.
@@ -93,11 +108,7 @@ __device__ __forceinline__ void mlp_wave(const uint16_t* net, const uint16_t* s_
         for (int h = 0; h < 4; ++h) {
             mlp_f4 acc = {0.f, 0.f, 0.f, 0.f};
             acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(fr[kFragL1 / 4 + h * 64 + lane], xb, acc, 0, 0, 0);
-#pragma unroll
-            for (int j = 0; j < 4; ++j) {
-                const float v = acc[j] + bias[16 * h + row + j];
-                h1[h][j] = (short)to_bf16(v > 0.f ? v : 0.f);
-            }
+            h1[h] = relu_bf16x4(acc, bias + 16 * h + row);
         }
         const mlp_s8 hb0 = __builtin_shufflevector(h1[0], h1[1], 0, 1, 2, 3, 4, 5, 6, 7);
         const mlp_s8 hb1 = __builtin_shufflevector(h1[2], h1[3], 0, 1, 2, 3, 4, 5, 6, 7);
@@ -106,11 +117,7 @@ __device__ __forceinline__ void mlp_wave(const uint16_t* net, const uint16_t* s_
             mlp_f4 acc = {0.f, 0.f, 0.f, 0.f};
             acc = mfma_k32(fr8[kFragL2 / 8 + (h * 2 + 0) * 64 + lane], hb0, acc);
             acc = mfma_k32(fr8[kFragL2 / 8 + (h * 2 + 1) * 64 + lane], hb1, acc);
-#pragma unroll
-            for (int j = 0; j < 4; ++j) {
-                const float v = acc[j] + bias[64 + 16 * h + row + j];
-                h2[h][j] = (short)to_bf16(v > 0.f ? v : 0.f);
-            }
+            h2[h] = relu_bf16x4(acc, bias + 64 + 16 * h + row);
         }
         mlp_f4 acc = {0.f, 0.f, 0.f, 0.f};
         acc = mfma_k32(fr8[kFragL3 / 8 + lane], __builtin_shufflevector(h2[0], h2[1], 0, 1, 2, 3, 4, 5, 6, 7), acc);

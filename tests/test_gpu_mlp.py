@@ -137,3 +137,30 @@ def test_mlp_flag_without_weights_is_an_error():
         mesh.close()
     finally:
         c.close()
+
+
+def test_batch_network_equals_single_frames(ctx):
+    """Batches evaluate the network per wave inside the shade pass, single frames in a
+    separate pass over compacted fragment records: the same network on the same inputs,
+    so the same bytes."""
+    ctx.mlp_set_weights(M.random_params(21, scale=0.4))
+    work = scenes.batch_scenes(n=4, seed=31, max_faces=20_000)
+    meshes = [ctx.upload(s) for s, a in work]
+    outs = []
+    for s, a in work:
+        W, H = a.image_width, a.image_height
+        outs.append((ctx.empty((3, H, W), np.uint8), ctx.empty((H, W), np.float32), ctx.empty((H, W), np.int32)))
+    try:
+        batch = [(m, s.lights_copy(), a, f, d, i) for m, (s, a), (f, d, i) in zip(meshes, work, outs)]
+        ctx.draw_batch(batch, flags=rz.RZ_FLAG_CLEARED | rz.RZ_FLAG_MLP_SHADE)
+        for m, (s, a), (f, d, i) in zip(meshes, work, outs):
+            fs, ds, ids, _ = ctx.draw(m, s.lights_copy(), a, ids=True, flags=rz.RZ_FLAG_MLP_SHADE)
+            assert np.array_equal(i.download(), ids)
+            assert np.array_equal(d.download().view(np.uint32), ds.view(np.uint32))
+            assert np.array_equal(f.download(), fs)
+            assert (ids >= 0).sum() > 1000
+    finally:
+        for f, d, i in outs:
+            f.close(); d.close(); i.close()
+        for m in meshes:
+            m.close()
