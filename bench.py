@@ -280,6 +280,42 @@ def batched(ctx, rz, scenes, np, frames):
         ctx.set_tuning(16, 512)
 
 
+def stress(ctx, rz, scenes, np, frames):
+    """The small-triangle depth stress: 4M screen-space triangles at 4096x4096 (about half
+    back-facing), default tuning; frame time from a graph of `frames` frames, per-pass
+    times from rz_time_passes."""
+    scene, args = scenes.screen_soup_scene()
+    W, H = args.image_width, args.image_height
+    mesh = ctx.upload(scene)
+    frame, depth = ctx.empty((3, H, W), np.uint8), ctx.empty((H, W), np.float32)
+    ids = ctx.empty((H, W), np.int32)
+    try:
+        flags = rz.RZ_FLAG_CLEARED | rz.RZ_FLAG_ASYNC
+        st = ctx.draw_device(mesh, scene.lights_copy(), args, frame.ptr, depth.ptr, ids.ptr, flags, stats=True)
+        idh = ids.download()
+        counts = {"bin_pairs16": st["bin_pairs"], "front_triangles": st["front_triangles"],
+                  "visible_faces": int(np.unique(idh[idh >= 0]).size), "fragments": st["fragments"],
+                  "covered_pixels": st["covered_pixels"]}
+        ms = graph_ms(ctx, mesh, scene.lights_copy(), args, frame, depth, frames, flags)
+        us = ctx.time_passes(mesh, scene.lights_copy(), args, frame, depth, reps=frames)
+        fused = ctx.fused_shade
+        table = per_pass_table(us, algorithmic_bytes(counts, scene, args, fused))
+        ctx.set_fused_shade(not fused)
+        us2 = ctx.time_passes(mesh, scene.lights_copy(), args, frame, depth, reps=frames)
+        table2 = per_pass_table(us2, algorithmic_bytes(counts, scene, args, not fused))
+        ctx.set_fused_shade(fused)
+        raster_only = us2["raster"] if fused else us["raster"]
+        return {"workload": "screen-soup-4M-4096x4096", "faces": scene.num_faces, "width": W, "height": H,
+                "ms_per_frame": ms, "mtri_per_s": scene.num_faces / (ms * 1e-3) / 1e6,
+                "gfrag_per_s": counts["fragments"] / (raster_only * 1e-6) / 1e9 if raster_only > 0 else None,
+                "counts": counts, "overflow_tiles": st["overflow_tiles"], "passes": table,
+                "passes_other_pipeline": table2, "frames_per_graph": frames}
+    finally:
+        for b in (frame, depth, ids):
+            b.close()
+        mesh.close()
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -291,6 +327,8 @@ def main():
     ap.add_argument("--tile", type=int, default=16)
     ap.add_argument("--cap", type=int, default=512)
     ap.add_argument("--no-mlp", action="store_true", help="skip the learned-shading measurement")
+    ap.add_argument("--stress", action="store_true",
+                    help="also measure the 4M-triangle 4096x4096 depth stress frame")
     ap.add_argument("--batch", action="store_true",
                     help="also measure the 256-mesh batched workload (autotuned, one graph)")
     ap.add_argument("--copy-mib", type=int, default=0,
@@ -401,6 +439,11 @@ def main():
     for buf in (frame, depth, ids):
         buf.close()
 
+    stressed = None
+    if a.stress:
+        stressed = stress(ctx, rz, scenes, np, 20)
+        log(f"stress: {stressed}")
+
     batch = None
     if a.batch:
         batch = batched(ctx, rz, scenes, np, 20)
@@ -433,6 +476,7 @@ def main():
         "shade64_mlp_ms": mlp["shade_mlp_ms"] if mlp else None,
         "learned_shading": mlp,
         "batched": batch,
+        "stress": stressed,
     }
     print(json.dumps(out), flush=True)
     return 0
