@@ -245,7 +245,7 @@ FrameLayout layout_frame(const rz_mesh* mesh, const rz_args& a, int ts, int cap,
     L.off_pairs = L.off_front + bin_blocks(mesh->nf);
     L.off_frag = L.off_pairs + bin_blocks(mesh->nf);
     L.off_ovf = L.off_frag + (L.lists ? (size_t)raster_grid : (size_t)L.ntiles);
-    L.off_cov = L.off_ovf + overflow_blocks(mesh->nf);
+    L.off_cov = L.off_ovf + (L.lists ? 0 : overflow_blocks(mesh->nf));   // list-driven: in the raster pass
     L.n_part = L.off_cov + (L.fused ? (size_t)raster_grid + tiles_grid
                                     : (size_t)shade_blocks_x(L.W) * shade_blocks_y(L.H));
     L.bytes[kBufRaster] = (size_t)mesh->nv * 16;
@@ -340,6 +340,7 @@ void fill_frame(const rz_mesh* mesh, rz_light* lights, int32_t nl, const rz_args
     bp.ntiles = lay.ntiles;
     bp.lists = lay.lists ? (uint32_t*)buf[kBufLists] : nullptr;
     bp.list_count = bp.tile_count + (size_t)lay.ntiles * kCountStride;
+    bp.ovf_vis = lay.lists ? (unsigned long long*)buf[kBufVis] : nullptr;
 
     RasterParams& rp = L.rp;
     std::memset(&rp, 0, sizeof rp);
@@ -477,7 +478,7 @@ rz_status enqueue_frame(rz_ctx* ctx, const FrameLaunch& L, hipStream_t s, bool p
     else
         RZ_HIP(ctx, launch_raster(L.rp, L.lay.ts, s));
     if (prof) RZ_HIP(ctx, hipEventRecord(ctx->ev[3], s));
-    RZ_HIP(ctx, launch_overflow(L.orp, s));
+    if (!L.lay.lists) RZ_HIP(ctx, launch_overflow(L.orp, s));   // (list-driven: inside the raster pass)
     if (prof) RZ_HIP(ctx, hipEventRecord(ctx->ev[4], s));
     if (L.lay.fused)
         RZ_HIP(ctx, launch_shade_tiles(L.rp, tiles_params(L), L.lay.ts, s));
@@ -1134,7 +1135,7 @@ rz_status rz_time_passes(rz_ctx* ctx, const rz_mesh* mesh, rz_light* lights, int
     if ((st = timed(vertex, ms_v)) != RZ_OK) return st;
     if ((st = timed(vertex_bin, ms_vb)) != RZ_OK) return st;
     if ((st = timed(raster, ms_r)) != RZ_OK) return st;
-    if ((st = timed(overflow, ms_o)) != RZ_OK) return st;
+    if (!L.lay.lists && (st = timed(overflow, ms_o)) != RZ_OK) return st;
     if (with_raster) {
         if ((st = timed(raster_shade, ms_s)) != RZ_OK) return st;
         ms_s -= ms_r;

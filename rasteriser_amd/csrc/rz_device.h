@@ -90,6 +90,9 @@ struct BinParams {
     // per-XCD lists of the tiles that received a face (null: not built)
     uint32_t* lists;
     uint32_t* list_count;       // 8 counters, kCountStride apart
+    // list-driven passes: the visibility words, which the push that overflows a tile's
+    // bin clears for that tile (null: batches, whose raster pass writes every tile's)
+    unsigned long long* ovf_vis;
     // extra workgroups of the bin launch that clear buffers (clear_np = 0: none): the
     // visibility words (separate-shade pipeline) and, for the fused pipeline on cleared
     // buffers, frame 0, depth 1.0, ids -1

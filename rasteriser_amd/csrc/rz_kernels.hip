@@ -95,13 +95,25 @@ __device__ __forceinline__ void list_tile(const BinParams& p, uint32_t tile) {
     p.lists[(size_t)x * p.ntiles + k] = tile;
 }
 
+// Called once per tile, by the push that takes its bin past capacity.
+__device__ __forceinline__ void tile_overflowed(const BinParams& p, uint32_t tile) {
+    atomicAdd(&p.counters[kCntOverflowTiles], 1ull);
+    if (!p.ovf_vis) return;
+    // the raster kernel's overflow loop merges this tile's fragments into these words
+    const uint32_t ts = 1u << p.ts_log2;
+    const uint32_t ox = (tile % (uint32_t)p.tiles_x) * ts, oy = (tile / (uint32_t)p.tiles_x) * ts;
+    const uint32_t x1 = min(ox + ts, p.W), y1 = min(oy + ts, p.H);
+    for (uint32_t y = oy; y < y1; ++y)
+        for (uint32_t x = ox; x < x1; ++x) p.ovf_vis[(size_t)y * p.W + x] = kEmptyKey;
+}
+
 __device__ __forceinline__ void bin_push(const BinParams& p, uint32_t face, uint32_t tile) {
     const uint32_t slot = atomicAdd(&p.tile_count[(size_t)tile * kCountStride], 1u);
     if (slot == 0) list_tile(p, tile);
     if (slot < p.cap)
         p.bins[(size_t)tile * p.cap + slot] = face;
     else if (slot == p.cap)
-        atomicAdd(&p.counters[kCntOverflowTiles], 1ull);
+        tile_overflowed(p, tile);
 }
 
 __device__ __forceinline__ void bin_body(const BinParams& p, int blk) {
@@ -193,7 +205,7 @@ __device__ __forceinline__ void bin_body(const BinParams& p, int blk) {
             const uint32_t base = atomicAdd(&p.tile_count[(size_t)tile * kCountStride], c);
             s_base[i] = base;
             if (base == 0) list_tile(p, tile);
-            if (base <= p.cap && base + c > p.cap) atomicAdd(&p.counters[kCntOverflowTiles], 1ull);
+            if (base <= p.cap && base + c > p.cap) tile_overflowed(p, tile);
         }
         __syncthreads();
         if (small) {
@@ -615,9 +627,15 @@ __device__ __forceinline__ void raster_body(const RasterParams& p, const ShadePa
     const int tx = tile % p.tiles_x, ty = tile / p.tiles_x;
     const int ox = tx * TS, oy = ty * TS;
 
+    // the first chunk's bin entries are read beside the count, not after it (entries past
+    // the count are never used; every index below the capacity is inside the bin)
+    const uint32_t f_first = p.bins[(size_t)tile * p.cap + ((uint32_t)tid < p.cap ? (uint32_t)tid : 0u)];
     const uint32_t count = p.tile_count[(size_t)tile * kCountStride];
-    const uint32_t n = count > p.cap ? 0u : count;   // overflowed tiles go to k_overflow
+    const uint32_t n = count > p.cap ? 0u : count;   // overflowed tiles go to the overflow loop
     if (!FUSED && p.rec_counts && tid == 0) p.rec_counts[(size_t)tile * kCountStride] = 0u;   // filled by k_shade
+    // list-driven passes: the bin pass cleared an overflowed tile's visibility words and
+    // the kernel's overflow loop fills them (uniform over the workgroup)
+    if (count > p.cap && p.lists) return;
 #pragma unroll
     for (int j = 0; j < PPT; ++j) {
         const int l = tid + j * BLOCK;
@@ -635,7 +653,7 @@ __device__ __forceinline__ void raster_body(const RasterParams& p, const ShadePa
         uint32_t cnt = 0, rect = 0, f = 0;
         float4 a = make_float4(0.f, 0.f, 0.f, 0.f), b = a, c = a;
         if (i < n) {
-            f = p.bins[(size_t)tile * p.cap + i];
+            f = base == 0 ? f_first : p.bins[(size_t)tile * p.cap + i];
             const FaceRecord* r = p.rec + f;
             a = r->r0; b = r->r1; c = r->r2;
             const uint32_t lo = __float_as_uint(b.w), hi = __float_as_uint(c.w);
@@ -752,15 +770,13 @@ __device__ __forceinline__ void raster_body(const RasterParams& p, const ShadePa
 }
 
 // ---------------------------------------------------------------------------------
-__device__ __forceinline__ void overflow_body(const RasterParams& p, int blk, int nblk) {
+// Every face meeting an overflowed tile, one thread per face, into the visibility words
+// by global atomicMin; thread t of nthreads.  Returns the fragments passing the inside test.
+__device__ __forceinline__ uint32_t overflow_faces(const RasterParams& p, uint32_t t, uint32_t nthreads) {
     const int32_t ts_log2 = p.ts_log2;
-    if (p.counters[kCntOverflowTiles] == 0ull) {
-        if (threadIdx.x == 0) p.part_frag[blk] = 0u;
-        return;
-    }
     const uint32_t ts = 1u << ts_log2;
     uint32_t frags = 0;
-    for (int f = blk * 256 + threadIdx.x; f < p.nf; f += nblk * 256) {
+    for (uint32_t f = t; f < (uint32_t)p.nf; f += nthreads) {
         const FaceRecord* r = p.rec + f;
         const float4 b = r->r1, c = r->r2;
         const uint32_t lo = __float_as_uint(b.w), hi = __float_as_uint(c.w);
@@ -787,6 +803,13 @@ __device__ __forceinline__ void overflow_body(const RasterParams& p, int blk, in
             }
         }
     }
+    return frags;
+}
+
+__device__ __forceinline__ void overflow_body(const RasterParams& p, int blk, int nblk) {
+    const uint32_t frags = p.counters[kCntOverflowTiles] == 0ull
+                               ? 0u
+                               : overflow_faces(p, (uint32_t)blk * 256u + threadIdx.x, (uint32_t)nblk * 256u);
     store_block_sum(frags, p.part_frag + blk, threadIdx.x, 256);
 }
 
@@ -833,6 +856,9 @@ __global__ __launch_bounds__(256) void k_bin(BinParams p) {
 // 8, where workgroup b takes the tiles of list b % 8 (run on XCD b % 8) with stride
 // gridDim.x / 8.  Empty tiles cost no workgroup dispatch at all: dispatching one
 // workgroup per tile measured ~27 us for 8,160 tiles at 1080p, longer than the work.
+// When a bin overflowed, each raster workgroup then takes its share of the faces for
+// the overflow loop (the tile's visibility words were cleared by the bin pass and the
+// raster loop skipped it), so the frame needs no launch of its own for it.
 // Statistics are per-workgroup partial sums.
 // (minimum waves per SIMD in the launch bounds: 8 for the raster kernel (6 for 32-pixel
 // tiles, whose LDS allows no more), 6 for the fused one, 3 for the fused learned shader:
@@ -845,6 +871,8 @@ __global__ __launch_bounds__(BLOCK, TS == 32 ? 6 : 8) void k_raster(RasterParams
     for (uint32_t k = blockIdx.x >> 3; k < n; k += step)
         raster_body<TS, BLOCK, false, kShadePlain>(p, nullptr, nullptr, nullptr, (int)p.lists[xcd * p.ntiles + k],
                                                    frags, cov);
+    if (p.counters[kCntOverflowTiles] != 0ull)
+        frags += overflow_faces(p, blockIdx.x * BLOCK + threadIdx.x, gridDim.x * BLOCK);
     store_block_sum(frags, p.part_frag + blockIdx.x, threadIdx.x, BLOCK);
 }
 // Raster and shade in one pass.
@@ -861,6 +889,8 @@ __global__ __launch_bounds__(BLOCK, MODE == kShadeMlp ? 3 : 6) void k_raster_sha
     uint32_t frags = 0, cov = 0;
     for (uint32_t k = blockIdx.x >> 3; k < n; k += step)
         raster_body<TS, BLOCK, true, MODE>(p, &sp, s_lut, s_net, (int)p.lists[xcd * p.ntiles + k], frags, cov);
+    if (p.counters[kCntOverflowTiles] != 0ull)
+        frags += overflow_faces(p, blockIdx.x * BLOCK + threadIdx.x, gridDim.x * BLOCK);
     __syncthreads();
     store_block_sum(frags, p.part_frag + blockIdx.x, threadIdx.x, BLOCK);
     store_block_sum(cov, sp.part_cov + blockIdx.x, threadIdx.x, BLOCK);
