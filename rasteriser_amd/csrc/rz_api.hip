@@ -21,15 +21,14 @@ hipError_t launch_bin(const BinParams& p, hipStream_t s);
 hipError_t launch_raster(const RasterParams& p, int tile_size, hipStream_t s);
 hipError_t launch_raster_shade(const RasterParams& p, const ShadeParams& sp, int tile_size, hipStream_t s);
 hipError_t launch_shade_tiles(const RasterParams& p, const ShadeParams& sp, int tile_size, hipStream_t s);
-hipError_t launch_overflow(const RasterParams& p, hipStream_t s);
 hipError_t launch_shade(const ShadeParams& p, hipStream_t s);
 hipError_t launch_vertex_batch(const VertexParams* items, const uint2* map, uint32_t blocks, hipStream_t s);
 hipError_t launch_bin_batch(const BinParams* items, const uint2* map, uint32_t blocks, hipStream_t s);
-hipError_t launch_raster_batch(const RasterParams* items, const uint2* map, uint32_t blocks, int tile_size,
-                               hipStream_t s);
 hipError_t launch_overflow_batch(const RasterParams* items, const uint2* map, uint32_t blocks, hipStream_t s);
-hipError_t launch_shade_batch(const ShadeParams* items, const uint2* map, uint32_t blocks, int mode,
-                              hipStream_t s);
+uint32_t raster_batch_blocks_per_cu(int tile_size, int mode);
+hipError_t launch_batch_tiles(const RasterParams* items, const ShadeParams* sitems, const BatchLists& bl,
+                              uint32_t raster_grid, uint32_t tiles_grid, int tile_size, int mode, hipStream_t s,
+                              bool raster);
 hipError_t launch_hold(uint64_t ticks, hipStream_t s);
 hipError_t launch_mlp_shade(const void* records, const uint32_t* counts, uint32_t region, const uint32_t* lists,
                             const uint32_t* list_count, uint32_t ntiles, const uint16_t* packed, uint8_t* frame,
@@ -210,7 +209,7 @@ rz_status validate_mesh(rz_ctx* ctx, const rz_mesh_desc* d) {
 struct FrameLayout {
     int ts = 16, tsl = 4, tiles_x = 0, tiles_y = 0, ntiles = 0;
     bool fused = false;     // raster pass shades non-overflowed tiles
-    bool lists = false;     // raster pass driven by the bin pass's per-XCD tile lists
+    bool batch = false;     // one mesh of a batch (tile lists and raster statistics shared)
     uint32_t raster_grid = 0, tiles_grid = 0;   // workgroups of the list-driven launches
     uint32_t W = 0, H = 0, cap = 0;
     size_t np = 0;
@@ -222,14 +221,16 @@ struct FrameLayout {
 
 enum { kBufRaster, kBufCnormal, kBufRec, kBufTiles, kBufBins, kBufVis, kBufLists, kBufCounters, kBufPart, kNumBufs };
 
-// raster_grid > 0: single frame, list-driven raster (raster_grid workgroups) and
-// overflow shade (tiles_grid); 0: one raster workgroup per tile (batches).
+// Single frame: raster_grid / tiles_grid workgroups of the list-driven raster and
+// overflow-shade launches.  Batch meshes (batch = true) are always fused; the batch owns
+// the tile lists and the statistics of those two launches, and each mesh keeps the
+// partial sums of its overflow loop (k_overflow_batch).
 FrameLayout layout_frame(const rz_mesh* mesh, const rz_args& a, int ts, int cap, bool fused, uint32_t raster_grid,
-                         uint32_t tiles_grid) {
+                         uint32_t tiles_grid, bool batch) {
     FrameLayout L;
     L.ts = ts;
-    L.fused = fused && raster_grid > 0;
-    L.lists = raster_grid > 0;
+    L.fused = fused || batch;
+    L.batch = batch;
     L.raster_grid = raster_grid;
     L.tiles_grid = tiles_grid;
     L.tsl = log2_exact(ts);
@@ -244,15 +245,16 @@ FrameLayout layout_frame(const rz_mesh* mesh, const rz_args& a, int ts, int cap,
     L.off_front = 0;
     L.off_pairs = L.off_front + bin_blocks(mesh->nf);
     L.off_frag = L.off_pairs + bin_blocks(mesh->nf);
-    L.off_ovf = L.off_frag + (L.lists ? (size_t)raster_grid : (size_t)L.ntiles);
-    L.off_cov = L.off_ovf + (L.lists ? 0 : overflow_blocks(mesh->nf));   // list-driven: in the raster pass
-    L.n_part = L.off_cov + (L.fused ? (size_t)raster_grid + tiles_grid
-                                    : (size_t)shade_blocks_x(L.W) * shade_blocks_y(L.H));
+    L.off_ovf = L.off_frag + (batch ? 0 : (size_t)raster_grid);
+    L.off_cov = L.off_ovf + (batch ? overflow_blocks(mesh->nf) : 0);   // single frames: in the raster pass
+    L.n_part = L.off_cov + (batch ? 0
+                                  : L.fused ? (size_t)raster_grid + tiles_grid
+                                            : (size_t)shade_blocks_x(L.W) * shade_blocks_y(L.H));
     L.bytes[kBufRaster] = (size_t)mesh->nv * 16;
     L.bytes[kBufCnormal] = (size_t)L.nfn * 16;
     L.bytes[kBufRec] = (size_t)mesh->nf * sizeof(FaceRecord);
     L.bytes[kBufTiles] = ((size_t)L.ntiles + 8) * 4 * kCountStride;   // + the 8 list counters
-    L.bytes[kBufLists] = L.lists ? (size_t)8 * L.ntiles * 4 : 0;
+    L.bytes[kBufLists] = batch ? 0 : (size_t)8 * L.ntiles * 4;
     L.bytes[kBufBins] = (size_t)L.ntiles * L.cap * 4;
     L.bytes[kBufVis] = L.np * 8;
     L.bytes[kBufCounters] = kNumCounters * 8;
@@ -338,9 +340,9 @@ void fill_frame(const rz_mesh* mesh, rz_light* lights, int32_t nl, const rz_args
     bp.tiles_x = lay.tiles_x; bp.ts_log2 = lay.tsl; bp.cap = lay.cap;
     bp.wind_cw = a.wind_clockwise != 0;
     bp.ntiles = lay.ntiles;
-    bp.lists = lay.lists ? (uint32_t*)buf[kBufLists] : nullptr;
+    bp.lists = (uint32_t*)buf[kBufLists];   // (batches: replaced by the batch's lists)
     bp.list_count = bp.tile_count + (size_t)lay.ntiles * kCountStride;
-    bp.ovf_vis = lay.lists ? (unsigned long long*)buf[kBufVis] : nullptr;
+    bp.ovf_vis = (unsigned long long*)buf[kBufVis];
 
     RasterParams& rp = L.rp;
     std::memset(&rp, 0, sizeof rp);
@@ -355,7 +357,7 @@ void fill_frame(const rz_mesh* mesh, rz_light* lights, int32_t nl, const rz_args
     rp.lists = bp.lists;
     rp.list_count = bp.list_count;
     rp.max_blocks = lay.raster_grid;
-    L.orp = rp;
+    L.orp = rp;   // batches: k_overflow_batch
     L.orp.part_frag = part + lay.off_ovf;
 
     ShadeParams& sp = L.sp;
@@ -392,7 +394,7 @@ rz_status prepare_frame(rz_ctx* ctx, const rz_mesh* mesh, rz_light* lights, int3
     uint32_t grid = (uint32_t)ctx->cus * per_cu / 8u * 8u;
     if (grid < 8) grid = 8;
     const FrameLayout lay =
-        layout_frame(mesh, *args, ctx->tile_size, ctx->bin_cap, ctx->fused, grid, ctx->persistent_blocks);
+        layout_frame(mesh, *args, ctx->tile_size, ctx->bin_cap, ctx->fused, grid, ctx->persistent_blocks, false);
     DevBuf* bufs[kNumBufs] = {&ctx->raster, &ctx->cnormal, &ctx->rec,   &ctx->tile_count, &ctx->bins,
                               &ctx->vis,    &ctx->lists,   &ctx->counters, &ctx->partials};
     void* ptrs[kNumBufs];
@@ -436,8 +438,10 @@ ShadeParams tiles_params(const FrameLaunch& L) {
     return t;
 }
 
-// Workgroups of the bin launch that clear buffers.
+// Workgroups of the bin launch that clear buffers: single frames, and per pixel for a
+// batch's meshes.
 constexpr uint32_t kClearBlocks = 1024;
+constexpr uint32_t kClearPixelsPerBlock = 8192;
 
 void bind_outputs(FrameLaunch& L, bool cleared, uint8_t* frame_d, float* depth_d, int32_t* ids_d) {
     L.rp.depth_in = cleared ? nullptr : depth_d;
@@ -452,13 +456,15 @@ void bind_outputs(FrameLaunch& L, bool cleared, uint8_t* frame_d, float* depth_d
     // ids of the others must still read -1, with the separate-shade pipeline their
     // visibility words must be empty
     const bool ids = L.lay.fused && ids_d;
-    const bool vis = L.lay.lists && !L.lay.fused;
+    const bool vis = !L.lay.fused;
     L.bp.clear_frame = pre ? frame_d : nullptr;
     L.bp.clear_depth = pre ? depth_d : nullptr;
     L.bp.clear_ids = ids ? ids_d : nullptr;
     L.bp.clear_vis = vis ? L.rp.vis : nullptr;
     L.bp.clear_np = (pre || ids || vis) ? L.lay.np : 0;
-    L.bp.clear_blocks = (pre || ids || vis) ? kClearBlocks : 0u;
+    const size_t per = (L.lay.np + kClearPixelsPerBlock - 1) / kClearPixelsPerBlock;
+    const uint32_t nclear = L.lay.batch ? (uint32_t)(per < kClearBlocks ? per : kClearBlocks) : kClearBlocks;
+    L.bp.clear_blocks = (pre || ids || vis) ? nclear : 0u;
     L.rp.pre_cleared = L.orp.pre_cleared = pre ? 1 : 0;
     L.sp.pre_cleared = pre ? 1 : 0;
 }
@@ -478,7 +484,7 @@ rz_status enqueue_frame(rz_ctx* ctx, const FrameLaunch& L, hipStream_t s, bool p
     else
         RZ_HIP(ctx, launch_raster(L.rp, L.lay.ts, s));
     if (prof) RZ_HIP(ctx, hipEventRecord(ctx->ev[3], s));
-    if (!L.lay.lists) RZ_HIP(ctx, launch_overflow(L.orp, s));   // (list-driven: inside the raster pass)
+    // (the overflow loop runs inside the raster pass)
     if (prof) RZ_HIP(ctx, hipEventRecord(ctx->ev[4], s));
     if (L.lay.fused)
         RZ_HIP(ctx, launch_shade_tiles(L.rp, tiles_params(L), L.lay.ts, s));
@@ -507,8 +513,11 @@ void reduce_stats(const FrameLaunch& L, const unsigned long long* counters, cons
 }
 
 // ---------------------------------------------------------------------------------
-// Batches: every mesh's working buffers carved from one arena, the per-mesh
-// parameters and the block maps of the five passes in one table buffer.
+// Batches: every mesh's working buffers carved from one arena, the per-mesh parameters
+// and the block maps of the vertex, bin and overflow passes in one table buffer.  The
+// raster pass is one resident grid over tile lists that all meshes' bin workgroups fill
+// (entries (mesh << kListMeshShift) | tile), raster and shade fused as for single
+// frames; the overflowed tiles of every mesh are shaded by one more list-driven launch.
 
 size_t align256(size_t v) { return (v + 255) & ~(size_t)255; }
 
@@ -522,42 +531,55 @@ struct BatchLaunch {
     const RasterParams* d_rp = nullptr;
     const RasterParams* d_orp = nullptr;
     const ShadeParams* d_sp = nullptr;
-    const uint2* d_map[5] = {};
-    uint32_t blocks[5] = {};
-    size_t arena_part_bytes = 0;   // statistics: first bytes of the arena are every frame's counters+partials
+    const uint2* d_map[3] = {};     // vertex, bin (+ clears), overflow
+    uint32_t blocks[3] = {};
+    BatchLists bl = {};
+    uint32_t raster_grid = 0, tiles_grid = 0;
+    size_t arena_part_bytes = 0;   // statistics: first bytes of the arena (the batch's, then every frame's)
 };
 
 rz_status prepare_batch(rz_ctx* ctx, const rz_batch_item* items, int32_t n, uint32_t flags, DevBuf& arena,
                         DevBuf& tables, BatchLaunch& B) {
     if (n < 1 || !items) return fail(ctx, RZ_ERR_INVALID_ARG, "empty batch");
+    if (n > kMaxBatchMeshes) return fail(ctx, RZ_ERR_INVALID_ARG, "batch of %d meshes: at most %d", n, kMaxBatchMeshes);
     if (!(flags & RZ_FLAG_DEVICE_BUFFERS))
         return fail(ctx, RZ_ERR_INVALID_ARG, "a batch needs device buffers (RZ_FLAG_DEVICE_BUFFERS)");
     RZ_HIP(ctx, hipSetDevice(ctx->device));
     const bool cleared = (flags & RZ_FLAG_CLEARED) != 0;
     if ((flags & RZ_FLAG_MLP_SHADE) && !ctx->mlp_ready)
         return fail(ctx, RZ_ERR_INVALID_ARG, "RZ_FLAG_MLP_SHADE without network weights (rz_mlp_init / rz_mlp_set_weights)");
+    B.ts = ctx->tile_size;
+    B.mode = (flags & RZ_FLAG_MLP_SHADE) ? kShadeMlp : kShadePlain;
+    uint32_t grid = (uint32_t)ctx->cus * raster_batch_blocks_per_cu(B.ts, B.mode) / 8u * 8u;
+    B.raster_grid = grid < 8 ? 8 : grid;
+    B.tiles_grid = (uint32_t)ctx->persistent_blocks;
     std::vector<FrameLayout> lays(n);
-    // arena: [statistics of every frame][the other buffers of every frame]
-    size_t stat_bytes = 0, work_bytes = 0;
+    // arena: [batch statistics][statistics of every frame][tile lists][the other buffers of every frame]
+    const size_t batch_stat_bytes = align256(((size_t)B.raster_grid * 2 + B.tiles_grid) * 4);
+    size_t stat_bytes = batch_stat_bytes, work_bytes = 0, stride = 0;
     for (int32_t i = 0; i < n; ++i) {
         const rz_batch_item& it = items[i];
         rz_status st = check_frame(ctx, it.mesh, it.lights, it.num_lights, &it.args);
         if (st != RZ_OK) return st;
         if (!it.frame || !it.depth) return fail(ctx, RZ_ERR_INVALID_ARG, "batch item %d: null frame or depth", i);
-        lays[i] = layout_frame(it.mesh, it.args, ctx->tile_size, ctx->bin_cap, false, 0, 0);
+        lays[i] = layout_frame(it.mesh, it.args, ctx->tile_size, ctx->bin_cap, true, 0, 0, true);
+        if ((uint32_t)lays[i].ntiles > kListTileMask)
+            return fail(ctx, RZ_ERR_INVALID_ARG, "batch item %d: %d tiles, at most %u", i, lays[i].ntiles, kListTileMask);
         stat_bytes += align256(lays[i].bytes[kBufCounters]) + align256(lays[i].bytes[kBufPart]);
         for (int b = 0; b < kBufCounters; ++b) work_bytes += align256(lays[i].bytes[b]);
+        stride += (size_t)lays[i].ntiles;
     }
-    rz_status st = ensure(ctx, arena, stat_bytes + work_bytes);
+    const size_t list_bytes = align256(8 * stride * 4);
+    rz_status st = ensure(ctx, arena, stat_bytes + list_bytes + work_bytes);
     if (st != RZ_OK) return st;
     B.frames.assign(n, FrameLaunch());
     B.faces.assign(n, 0);
-    B.ts = ctx->tile_size;
-    B.mode = (flags & RZ_FLAG_MLP_SHADE) ? kShadeMlp : kShadePlain;
     B.arena_part_bytes = stat_bytes;
     char* base = (char*)arena.ptr;
-    size_t so = 0, wo = stat_bytes;
-    std::vector<uint2> map[5];
+    uint32_t* batch_part = (uint32_t*)base;
+    uint32_t* lists = (uint32_t*)(base + stat_bytes);
+    size_t so = batch_stat_bytes, wo = stat_bytes + list_bytes;
+    std::vector<uint2> map[3];
     for (int32_t i = 0; i < n; ++i) {
         const rz_batch_item& it = items[i];
         void* ptrs[kNumBufs];
@@ -573,23 +595,38 @@ rz_status prepare_batch(rz_ctx* ctx, const rz_batch_item* items, int32_t n, uint
         fill_frame(it.mesh, it.lights, it.num_lights, it.args, flags, lays[i], ptrs, L);
         L.sp.mlp = (const uint16_t*)ctx->mlp_packed.ptr;
         bind_outputs(L, cleared, it.frame, it.depth, it.ids);
+        // the shared lists; their 8 counters are the first mesh's (cleared by its vertex pass)
+        const FrameLaunch& L0 = B.frames[0];
+        L.bp.lists = lists;
+        L.bp.list_count = L0.bp.list_count;
+        L.bp.ntiles = (int32_t)stride;
+        L.bp.list_tag = (uint32_t)i << kListMeshShift;
+        L.bp.batch_ovf = L0.counters + kCntBatchOverflow;
+        L.rp.lists = L.orp.lists = lists;
+        L.rp.list_count = L.orp.list_count = L0.bp.list_count;
         B.faces[i] = (uint64_t)it.mesh->nf;
-        const uint32_t nb[5] = {vertex_blocks(L.vp.nv, L.vp.nn, L.vp.ntiles), bin_blocks(L.bp.nf),
-                                (uint32_t)L.rp.ntiles, overflow_blocks(L.rp.nf),
-                                shade_blocks_x(L.sp.W) * shade_blocks_y(L.sp.H)};
-        for (int p = 0; p < 5; ++p)
+        const uint32_t nb[3] = {vertex_blocks(L.vp.nv, L.vp.nn, L.vp.ntiles), bin_blocks(L.bp.nf) + L.bp.clear_blocks,
+                                overflow_blocks(L.rp.nf)};
+        for (int p = 0; p < 3; ++p)
             for (uint32_t b = 0; b < nb[p]; ++b) map[p].push_back(make_uint2((uint32_t)i, b));
     }
-    // tables: the five parameter arrays and the five block maps
-    size_t off[10], total = 0;
-    const size_t sz[5] = {sizeof(VertexParams), sizeof(BinParams), sizeof(RasterParams), sizeof(RasterParams),
-                          sizeof(ShadeParams)};
-    for (int p = 0; p < 5; ++p) {
+    B.bl.lists = lists;
+    B.bl.list_count = B.frames[0].bp.list_count;
+    B.bl.stride = (uint32_t)stride;
+    B.bl.overflow = B.frames[0].counters + kCntBatchOverflow;
+    B.bl.part_frag = batch_part;
+    B.bl.part_cov = batch_part + B.raster_grid;
+    // tables: the four parameter arrays and the three block maps
+    size_t off[7], total = 0;
+    const size_t sz[4] = {sizeof(VertexParams), sizeof(BinParams), sizeof(RasterParams), sizeof(ShadeParams)};
+    for (int p = 0; p < 4; ++p) {
         off[p] = total;
         total = align256(total + sz[p] * (size_t)n);
     }
-    for (int p = 0; p < 5; ++p) {
-        off[5 + p] = total;
+    const size_t off_orp = total;
+    total = align256(total + sizeof(RasterParams) * (size_t)n);
+    for (int p = 0; p < 3; ++p) {
+        off[4 + p] = total;
         total = align256(total + map[p].size() * sizeof(uint2));
         B.blocks[p] = (uint32_t)map[p].size();
     }
@@ -599,11 +636,11 @@ rz_status prepare_batch(rz_ctx* ctx, const rz_batch_item* items, int32_t n, uint
         std::memcpy(host.data() + off[0] + i * sz[0], &B.frames[i].vp, sz[0]);
         std::memcpy(host.data() + off[1] + i * sz[1], &B.frames[i].bp, sz[1]);
         std::memcpy(host.data() + off[2] + i * sz[2], &B.frames[i].rp, sz[2]);
-        std::memcpy(host.data() + off[3] + i * sz[3], &B.frames[i].orp, sz[3]);
-        std::memcpy(host.data() + off[4] + i * sz[4], &B.frames[i].sp, sz[4]);
+        std::memcpy(host.data() + off[3] + i * sz[3], &B.frames[i].sp, sz[3]);
+        std::memcpy(host.data() + off_orp + i * sizeof(RasterParams), &B.frames[i].orp, sizeof(RasterParams));
     }
-    for (int p = 0; p < 5; ++p)
-        if (!map[p].empty()) std::memcpy(host.data() + off[5 + p], map[p].data(), map[p].size() * sizeof(uint2));
+    for (int p = 0; p < 3; ++p)
+        if (!map[p].empty()) std::memcpy(host.data() + off[4 + p], map[p].data(), map[p].size() * sizeof(uint2));
     // the table buffer may still be read by queued work of an earlier batch
     RZ_HIP(ctx, hipStreamSynchronize(ctx->stream));
     RZ_HIP(ctx, hipMemcpy(tables.ptr, host.data(), total, hipMemcpyHostToDevice));
@@ -611,9 +648,9 @@ rz_status prepare_batch(rz_ctx* ctx, const rz_batch_item* items, int32_t n, uint
     B.d_vp = (const VertexParams*)(t + off[0]);
     B.d_bp = (const BinParams*)(t + off[1]);
     B.d_rp = (const RasterParams*)(t + off[2]);
-    B.d_orp = (const RasterParams*)(t + off[3]);
-    B.d_sp = (const ShadeParams*)(t + off[4]);
-    for (int p = 0; p < 5; ++p) B.d_map[p] = (const uint2*)(t + off[5 + p]);
+    B.d_sp = (const ShadeParams*)(t + off[3]);
+    B.d_orp = (const RasterParams*)(t + off_orp);
+    for (int p = 0; p < 3; ++p) B.d_map[p] = (const uint2*)(t + off[4 + p]);
     return RZ_OK;
 }
 
@@ -624,11 +661,11 @@ rz_status enqueue_batch(rz_ctx* ctx, const BatchLaunch& B, hipStream_t s, bool p
     if (prof) RZ_HIP(ctx, hipEventRecord(ctx->ev[1], s));
     RZ_HIP(ctx, launch_bin_batch(B.d_bp, B.d_map[1], B.blocks[1], s));
     if (prof) RZ_HIP(ctx, hipEventRecord(ctx->ev[2], s));
-    RZ_HIP(ctx, launch_raster_batch(B.d_rp, B.d_map[2], B.blocks[2], B.ts, s));
+    RZ_HIP(ctx, launch_batch_tiles(B.d_rp, B.d_sp, B.bl, B.raster_grid, B.tiles_grid, B.ts, B.mode, s, true));
     if (prof) RZ_HIP(ctx, hipEventRecord(ctx->ev[3], s));
-    RZ_HIP(ctx, launch_overflow_batch(B.d_orp, B.d_map[3], B.blocks[3], s));
+    RZ_HIP(ctx, launch_overflow_batch(B.d_orp, B.d_map[2], B.blocks[2], s));
     if (prof) RZ_HIP(ctx, hipEventRecord(ctx->ev[4], s));
-    RZ_HIP(ctx, launch_shade_batch(B.d_sp, B.d_map[4], B.blocks[4], B.mode, s));
+    RZ_HIP(ctx, launch_batch_tiles(B.d_rp, B.d_sp, B.bl, B.raster_grid, B.tiles_grid, B.ts, B.mode, s, false));
     if (prof) RZ_HIP(ctx, hipEventRecord(ctx->ev[5], s));
     return RZ_OK;
 }
@@ -673,6 +710,9 @@ rz_status batch_stats(rz_ctx* ctx, const BatchLaunch& B, const DevBuf& arena, rz
     RZ_HIP(ctx, hipMemcpyAsync(host.data(), arena.ptr, host.size(), hipMemcpyDeviceToHost, ctx->stream));
     RZ_HIP(ctx, hipStreamSynchronize(ctx->stream));
     std::memset(stats, 0, sizeof *stats);
+    const uint32_t* bp = (const uint32_t*)host.data();   // the batch's raster and overflow-shade partials
+    for (uint32_t i = 0; i < B.raster_grid; ++i) stats->fragments += bp[i];
+    for (uint32_t i = 0; i < B.raster_grid + B.tiles_grid; ++i) stats->covered_pixels += bp[B.raster_grid + i];
     for (size_t i = 0; i < B.frames.size(); ++i) {
         const FrameLaunch& L = B.frames[i];
         const size_t oc = (size_t)((char*)L.counters - (char*)arena.ptr);
@@ -1111,7 +1151,6 @@ rz_status rz_time_passes(rz_ctx* ctx, const rz_mesh* mesh, rz_light* lights, int
             RZ_HIP(ctx, launch_raster(L.rp, L.lay.ts, cs));
         return RZ_OK;
     };
-    auto overflow = [&](hipStream_t cs) -> rz_status { RZ_HIP(ctx, launch_overflow(L.orp, cs)); return RZ_OK; };
     auto shade = [&](hipStream_t cs) -> rz_status {
         if (L.lay.fused)
             RZ_HIP(ctx, launch_shade_tiles(L.rp, tiles_params(L), L.lay.ts, cs));
@@ -1127,7 +1166,7 @@ rz_status rz_time_passes(rz_ctx* ctx, const rz_mesh* mesh, rz_light* lights, int
         return r != RZ_OK ? r : shade(cs);
     };
     const bool with_raster = feat && !L.lay.fused;
-    float ms_v = 0, ms_vb = 0, ms_r = 0, ms_o = 0, ms_s = 0, ms_n = 0;
+    float ms_v = 0, ms_vb = 0, ms_r = 0, ms_s = 0, ms_n = 0;
     // vertex alone, then vertex+bin (each repetition leaves a whole binned frame behind);
     // raster, overflow and shade are idempotent on that state; the network pass last,
     // on the records of one whole frame
@@ -1135,7 +1174,6 @@ rz_status rz_time_passes(rz_ctx* ctx, const rz_mesh* mesh, rz_light* lights, int
     if ((st = timed(vertex, ms_v)) != RZ_OK) return st;
     if ((st = timed(vertex_bin, ms_vb)) != RZ_OK) return st;
     if ((st = timed(raster, ms_r)) != RZ_OK) return st;
-    if (!L.lay.lists && (st = timed(overflow, ms_o)) != RZ_OK) return st;
     if (with_raster) {
         if ((st = timed(raster_shade, ms_s)) != RZ_OK) return st;
         ms_s -= ms_r;
@@ -1147,7 +1185,7 @@ rz_status rz_time_passes(rz_ctx* ctx, const rz_mesh* mesh, rz_light* lights, int
     us[0] = ms_v * k;
     us[1] = (ms_vb - ms_v) * k;
     us[2] = ms_r * k;
-    us[3] = ms_o * k;
+    us[3] = 0.0f;   // the overflow loop runs inside the raster pass
     us[4] = ms_s * k;
     us[5] = ms_n * k;
     return RZ_OK;

@@ -30,8 +30,14 @@ enum ShadeMode : int { kShadePlain = 0, kShadeMlp = 1, kShadeCollect = 2, kShade
 
 enum Counter : int {
     kCntOverflowTiles = 0,
+    kCntBatchOverflow = 1,      // batches, in the first mesh's counters: overflowed tiles of all meshes
     kNumCounters = 8
 };
+
+// A batch's tile lists hold (mesh << kListMeshShift) | tile.
+constexpr uint32_t kListMeshShift = 22;
+constexpr uint32_t kListTileMask = (1u << kListMeshShift) - 1u;
+constexpr int32_t kMaxBatchMeshes = 1 << (32 - kListMeshShift);
 
 // Per-face setup record written by the bin pass (64 bytes, one per face):
 //   r0 = (x0, y0, z0, area)            raster vertex 0, area = edge(v2, v0, v1)
@@ -87,12 +93,15 @@ struct BinParams {
     uint32_t cap;
     int32_t wind_cw;
     int32_t ntiles;
-    // per-XCD lists of the tiles that received a face (null: not built)
+    // per-XCD lists of the tiles that received a face, `ntiles` entries apart (a batch's
+    // meshes share one set: entries are list_tag | tile)
     uint32_t* lists;
     uint32_t* list_count;       // 8 counters, kCountStride apart
-    // list-driven passes: the visibility words, which the push that overflows a tile's
-    // bin clears for that tile (null: batches, whose raster pass writes every tile's)
+    uint32_t list_tag;
+    // the visibility words, which the push that overflows a tile's bin clears for that
+    // tile (the raster pass skips it and the overflow loop fills them)
     unsigned long long* ovf_vis;
+    unsigned long long* batch_ovf;   // batches: also counts the overflowed tile here
     // extra workgroups of the bin launch that clear buffers (clear_np = 0: none): the
     // visibility words (separate-shade pipeline) and, for the fused pipeline on cleared
     // buffers, frame 0, depth 1.0, ids -1
@@ -122,6 +131,17 @@ struct RasterParams {
     uint32_t* rec_counts;       // separate shade pipeline, learned shader: zeroed per listed tile
     const uint32_t* lists;      // per-XCD lists of tiles with faces (see BinParams)
     const uint32_t* list_count;
+};
+
+// A batch's raster and overflow-shade launches: the shared tile lists and the launches'
+// per-workgroup statistics.
+struct BatchLists {
+    const uint32_t* lists;      // 8 lists, `stride` entries apart
+    uint32_t* list_count;       // 8 counters, kCountStride apart (second word: work cursor)
+    uint32_t stride;
+    const unsigned long long* overflow;   // tiles overflowed over the whole batch
+    uint32_t* part_frag;        // per raster workgroup: fragments passing the inside test
+    uint32_t* part_cov;         // per raster workgroup, then per overflow-shade workgroup: covered pixels
 };
 
 struct ShadeParams {

@@ -80,6 +80,9 @@ __device__ __forceinline__ void vertex_body(const VertexParams& p, int blk) {
         p.cnormal[i] = n;
     }
     if (i < p.ntiles) p.tile_count[(size_t)i * kCountStride] = 0u;
+    // (the last 8 entries are the tile lists' counters; their second word is a batch's
+    // work cursor)
+    if (i + 8 >= p.ntiles && i < p.ntiles) p.tile_count[(size_t)i * kCountStride + 1] = 0u;
     if (i < kNumCounters) p.counters[i] = 0ull;
 }
 
@@ -92,13 +95,13 @@ __device__ __forceinline__ void list_tile(const BinParams& p, uint32_t tile) {
     const uint32_t tx = tile % (uint32_t)p.tiles_x, ty = tile / (uint32_t)p.tiles_x;
     const uint32_t x = (tx + 3u * ty) & 7u;
     const uint32_t k = atomicAdd(&p.list_count[x * kCountStride], 1u);
-    p.lists[(size_t)x * p.ntiles + k] = tile;
+    p.lists[(size_t)x * p.ntiles + k] = p.list_tag | tile;
 }
 
 // Called once per tile, by the push that takes its bin past capacity.
 __device__ __forceinline__ void tile_overflowed(const BinParams& p, uint32_t tile) {
     atomicAdd(&p.counters[kCntOverflowTiles], 1ull);
-    if (!p.ovf_vis) return;
+    if (p.batch_ovf) atomicAdd(p.batch_ovf, 1ull);
     // the raster kernel's overflow loop merges this tile's fragments into these words
     const uint32_t ts = 1u << p.ts_log2;
     const uint32_t ox = (tile % (uint32_t)p.tiles_x) * ts, oy = (tile / (uint32_t)p.tiles_x) * ts;
@@ -593,8 +596,9 @@ __device__ __forceinline__ bool fragment_key(float pxf, float pyf, float x0, flo
 // search.
 //
 // FUSED: the workgroup then shades its own pixels from the LDS visibility words (the
-// deferred shade, without the round trip of the visibility buffer through HBM); tiles
-// whose bin overflowed still write their words for k_overflow and k_shade_tiles.
+// deferred shade, without the round trip of the visibility buffer through HBM).  Tiles
+// whose bin overflowed are skipped: the overflow loop fills their visibility words and
+// k_shade_tiles shades them.
 template <int TS, int BLOCK, bool FUSED, int MODE>
 __device__ __forceinline__ void raster_body(const RasterParams& p, const ShadeParams* sp, const float* s_lut,
                                             const uint16_t* s_net, int tile, uint32_t& frags, uint32_t& cov) {
@@ -631,11 +635,11 @@ __device__ __forceinline__ void raster_body(const RasterParams& p, const ShadePa
     // the count are never used; every index below the capacity is inside the bin)
     const uint32_t f_first = p.bins[(size_t)tile * p.cap + ((uint32_t)tid < p.cap ? (uint32_t)tid : 0u)];
     const uint32_t count = p.tile_count[(size_t)tile * kCountStride];
-    const uint32_t n = count > p.cap ? 0u : count;   // overflowed tiles go to the overflow loop
+    const uint32_t n = count;
     if (!FUSED && p.rec_counts && tid == 0) p.rec_counts[(size_t)tile * kCountStride] = 0u;   // filled by k_shade
-    // list-driven passes: the bin pass cleared an overflowed tile's visibility words and
-    // the kernel's overflow loop fills them (uniform over the workgroup)
-    if (count > p.cap && p.lists) return;
+    // the bin pass cleared an overflowed tile's visibility words and the overflow loop
+    // fills them (uniform over the workgroup)
+    if (count > p.cap) return;
 #pragma unroll
     for (int j = 0; j < PPT; ++j) {
         const int l = tid + j * BLOCK;
@@ -740,7 +744,7 @@ __device__ __forceinline__ void raster_body(const RasterParams& p, const ShadePa
         __syncthreads();       // ranks and wave totals are rewritten by the next chunk
     }
 
-    if (FUSED && count <= p.cap) {
+    if (FUSED) {
         uint16_t* s_in = reinterpret_cast<uint16_t*>(s_raw) + (kNet ? wave : 0) * 64 * 16;
         float* s_out = reinterpret_cast<float*>(s_raw + NW * 64 * 16 * 2) + (kNet ? wave : 0) * 64 * 4;
         uint32_t nrec = 0;
@@ -770,34 +774,56 @@ __device__ __forceinline__ void raster_body(const RasterParams& p, const ShadePa
 }
 
 // ---------------------------------------------------------------------------------
-// Every face meeting an overflowed tile, one thread per face, into the visibility words
-// by global atomicMin; thread t of nthreads.  Returns the fragments passing the inside test.
+// Every face meeting an overflowed tile, into the visibility words by global atomicMin;
+// thread t of nthreads (both multiples of 64 at wave starts).  Each lane tests one face
+// for an overflowed tile in its tile rectangle; the wave then takes the faces found one
+// at a time, its 64 lanes spread over the face's pixels in each overflowed tile (a big
+// face's pixels would otherwise be one lane's serial loop).  Returns the fragments passing
+// the inside test.
 __device__ __forceinline__ uint32_t overflow_faces(const RasterParams& p, uint32_t t, uint32_t nthreads) {
     const int32_t ts_log2 = p.ts_log2;
     const uint32_t ts = 1u << ts_log2;
+    const uint32_t lane = t & 63u;
     uint32_t frags = 0;
-    for (uint32_t f = t; f < (uint32_t)p.nf; f += nthreads) {
-        const FaceRecord* r = p.rec + f;
-        const float4 b = r->r1, c = r->r2;
-        const uint32_t lo = __float_as_uint(b.w), hi = __float_as_uint(c.w);
-        const uint32_t bx0 = lo & 0xFFFFu, by0 = lo >> 16, bx1 = hi & 0xFFFFu, by1 = hi >> 16;
-        if (bx0 > bx1 || by0 > by1) continue;
-        const float4 a = r->r0;
-        for (uint32_t ty = by0 >> ts_log2; ty <= (by1 >> ts_log2); ++ty) {
-            for (uint32_t tx = bx0 >> ts_log2; tx <= (bx1 >> ts_log2); ++tx) {
-                const uint32_t tile = ty * p.tiles_x + tx;
-                if (p.tile_count[(size_t)tile * kCountStride] <= p.cap) continue;
-                const uint32_t x0 = max(bx0, tx * ts), x1 = min(bx1, tx * ts + ts - 1);
-                const uint32_t y0 = max(by0, ty * ts), y1 = min(by1, ty * ts + ts - 1);
-                for (uint32_t y = y0; y <= y1; ++y) {
-                    for (uint32_t x = x0; x <= x1; ++x) {
+    for (uint32_t base = t - lane; base < (uint32_t)p.nf; base += nthreads) {   // uniform over the wave
+        const uint32_t f = base + lane;
+        bool meets = false;
+        if (f < (uint32_t)p.nf) {
+            const FaceRecord* r = p.rec + f;
+            const uint32_t lo = __float_as_uint(r->r1.w), hi = __float_as_uint(r->r2.w);
+            const uint32_t bx0 = lo & 0xFFFFu, by0 = lo >> 16, bx1 = hi & 0xFFFFu, by1 = hi >> 16;
+            if (bx0 <= bx1 && by0 <= by1) {
+                for (uint32_t ty = by0 >> ts_log2; ty <= (by1 >> ts_log2) && !meets; ++ty)
+                    for (uint32_t tx = bx0 >> ts_log2; tx <= (bx1 >> ts_log2) && !meets; ++tx)
+                        meets = p.tile_count[(size_t)(ty * p.tiles_x + tx) * kCountStride] > p.cap;
+            }
+        }
+        unsigned long long todo = __ballot(meets);
+        while (todo) {
+            const uint32_t g = base + (uint32_t)(__ffsll((long long)todo) - 1);
+            todo &= todo - 1ull;
+            const FaceRecord* r = p.rec + g;
+            const float4 a = r->r0, b = r->r1, c = r->r2;
+            const uint32_t lo = __float_as_uint(b.w), hi = __float_as_uint(c.w);
+            const uint32_t bx0 = lo & 0xFFFFu, by0 = lo >> 16, bx1 = hi & 0xFFFFu, by1 = hi >> 16;
+            for (uint32_t ty = by0 >> ts_log2; ty <= (by1 >> ts_log2); ++ty) {
+                for (uint32_t tx = bx0 >> ts_log2; tx <= (bx1 >> ts_log2); ++tx) {
+                    const uint32_t tile = ty * p.tiles_x + tx;
+                    if (p.tile_count[(size_t)tile * kCountStride] <= p.cap) continue;
+                    const uint32_t x0 = max(bx0, tx * ts), x1 = min(bx1, tx * ts + ts - 1);
+                    const uint32_t y0 = max(by0, ty * ts), y1 = min(by1, ty * ts + ts - 1);
+                    const uint32_t rw = x1 - x0 + 1, npx = rw * (y1 - y0 + 1);   // npx <= 1024
+                    for (uint32_t l = lane; l < npx; l += 64) {
+                        // l / rw exactly: |error| of the product < 1/(2*rw) for l < 1024
+                        const uint32_t dy = (uint32_t)(((float)l + 0.5f) * __builtin_amdgcn_rcpf((float)rw));
+                        const uint32_t x = x0 + (l - dy * rw), y = y0 + dy;
                         float z;
                         if (!fragment_key((float)x, (float)y, a.x, a.y, a.z, b.x, b.y, b.z, c.x, c.y, c.z, a.w, z))
                             continue;
                         ++frags;
                         const size_t pi = (size_t)y * p.W + x;
                         const float d0 = p.depth_in ? p.depth_in[pi] : 1.0f;
-                        if (z < d0) atomicMin(&p.vis[pi], vis_key(z, (uint32_t)f));
+                        if (z < d0) atomicMin(&p.vis[pi], vis_key(z, g));
                     }
                 }
             }
@@ -914,7 +940,6 @@ __global__ __launch_bounds__(BLOCK) void k_shade_tiles(RasterParams p, ShadePara
     }
     store_block_sum(cov, sp.part_cov + blockIdx.x, threadIdx.x, BLOCK);
 }
-__global__ __launch_bounds__(256) void k_overflow(RasterParams p) { overflow_body(p, blockIdx.x, gridDim.x); }
 template <int MODE>
 __global__ __launch_bounds__(256) void k_shade(ShadeParams p) {
     __shared__ float s_lut[kLdsLuts * 256];
@@ -935,17 +960,48 @@ __global__ __launch_bounds__(256) void k_vertex_batch(const VertexParams* __rest
     const uint2 m = map[blockIdx.x];
     vertex_body(items[m.x], (int)m.y);
 }
+// Batch bin launch: each mesh's binning workgroups, then its clearing ones.
 __global__ __launch_bounds__(256) void k_bin_batch(const BinParams* __restrict__ items, const uint2* __restrict__ map) {
     const uint2 m = map[blockIdx.x];
-    bin_body(items[m.x], (int)m.y);
+    const BinParams& p = items[m.x];
+    const uint32_t nb = bin_blocks(p.nf);
+    if (m.y < nb)
+        bin_body(p, (int)m.y);
+    else
+        clear_body(p, m.y - nb, p.clear_blocks);
 }
-template <int TS, int BLOCK>
-__global__ __launch_bounds__(BLOCK) void k_raster_batch(const RasterParams* __restrict__ items,
-                                                        const uint2* __restrict__ map) {
-    const uint2 m = map[blockIdx.x];
+// A batch's raster pass: one resident grid over the tile lists the meshes share, entry
+// (mesh << kListMeshShift) | tile, raster and shade of the tile with its mesh's
+// parameters.  A batch's tiles differ in cost by orders of magnitude (overdraw), so
+// workgroups take tiles one at a time from per-list cursors (the second word of each
+// list counter's line): first list b % 8, then the others in turn.  Every cursor only
+// grows, so each workgroup leaves after one failed take per list.
+template <int TS, int BLOCK, int MODE>
+__global__ __launch_bounds__(BLOCK, MODE == kShadeMlp ? 3 : 6) void k_raster_shade_batch(
+    const RasterParams* __restrict__ items, const ShadeParams* __restrict__ sitems, BatchLists bl) {
+    __shared__ __align__(16) uint16_t s_net[MODE == kShadeMlp ? kPackedBytes / 2 : 8];
+    __shared__ uint32_t s_take;
+    if (MODE == kShadeMlp) stage_net(sitems[0].mlp, s_net, threadIdx.x, BLOCK);   // one network per batch
     uint32_t frags = 0, cov = 0;
-    raster_body<TS, BLOCK, false, kShadePlain>(items[m.x], nullptr, nullptr, nullptr, (int)m.y, frags, cov);
-    store_block_sum(frags, items[m.x].part_frag + m.y, threadIdx.x, BLOCK);
+    for (uint32_t j = 0; j < 8; ++j) {
+        const uint32_t x = (blockIdx.x + j) & 7u;
+        const uint32_t n = bl.list_count[x * kCountStride];
+        uint32_t* cursor = bl.list_count + x * kCountStride + 1;
+        for (;;) {
+            if (threadIdx.x == 0) s_take = atomicAdd(cursor, 1u);
+            __syncthreads();
+            const uint32_t k = s_take;
+            __syncthreads();   // (s_take is rewritten by the next take)
+            if (k >= n) break;
+            const uint32_t e = bl.lists[(size_t)x * bl.stride + k];
+            const uint32_t m = e >> kListMeshShift;
+            raster_body<TS, BLOCK, true, MODE>(items[m], &sitems[m], sitems[m].luts, s_net,
+                                               (int)(e & kListTileMask), frags, cov);
+        }
+    }
+    __syncthreads();
+    store_block_sum(frags, bl.part_frag + blockIdx.x, threadIdx.x, BLOCK);
+    store_block_sum(cov, bl.part_cov + blockIdx.x, threadIdx.x, BLOCK);
 }
 __global__ __launch_bounds__(256) void k_overflow_batch(const RasterParams* __restrict__ items,
                                                         const uint2* __restrict__ map) {
@@ -953,23 +1009,30 @@ __global__ __launch_bounds__(256) void k_overflow_batch(const RasterParams* __re
     const RasterParams& p = items[m.x];
     overflow_body(p, (int)m.y, (int)overflow_blocks(p.nf));
 }
-template <int MODE>
-__global__ __launch_bounds__(256) void k_shade_batch(const ShadeParams* __restrict__ items,
-                                                     const uint2* __restrict__ map) {
-    __shared__ float s_lut[kLdsLuts * 256];
-    __shared__ __align__(16) uint16_t s_net[MODE == kShadeMlp ? kPackedBytes / 2 : 8];
-    const uint2 m = map[blockIdx.x];
-    const ShadeParams& p = items[m.x];
-    const int tid = threadIdx.y * 64 + threadIdx.x;
-    stage_luts(p, s_lut, tid, 256);
-    if (MODE == kShadeMlp) stage_net(p.mlp, s_net, tid, 256);
-    __syncthreads();
-    shade_body<MODE>(p, s_lut, s_net, (int)m.y);
+// Shade of the batch's overflowed tiles (after the overflow loop), as k_shade_tiles;
+// `part_cov` is the section after the raster launch's.
+template <int TS, int BLOCK, int MODE>
+__global__ __launch_bounds__(BLOCK) void k_shade_tiles_batch(const RasterParams* __restrict__ items,
+                                                             const ShadeParams* __restrict__ sitems, BatchLists bl,
+                                                             uint32_t* part_cov) {
+    uint32_t cov = 0;
+    if (*bl.overflow != 0ull) {
+        __shared__ __align__(16) uint16_t s_net[MODE == kShadeMlp ? kPackedBytes / 2 : 8];
+        if (MODE == kShadeMlp) stage_net(sitems[0].mlp, s_net, threadIdx.x, BLOCK);
+        __syncthreads();
+        const uint32_t xcd = blockIdx.x & 7u, step = gridDim.x >> 3;
+        const uint32_t n = bl.list_count[xcd * kCountStride];
+        for (uint32_t k = blockIdx.x >> 3; k < n; k += step) {
+            const uint32_t e = bl.lists[(size_t)xcd * bl.stride + k];
+            const uint32_t m = e >> kListMeshShift, tile = e & kListTileMask;
+            const RasterParams& p = items[m];
+            if (p.tile_count[(size_t)tile * kCountStride] > p.cap)
+                shade_tile_body<TS, BLOCK, MODE>(p, sitems[m], sitems[m].luts, s_net, (int)tile, cov);
+        }
+    }
+    store_block_sum(cov, part_cov + blockIdx.x, threadIdx.x, BLOCK);
 }
 
-// Holds the stream for about `ticks` of the 100 MHz constant clock (bounded wait), so
-// that the host has queued the passes behind it before they start: the HIP events
-// between passes then time the passes back to back rather than the host's launch rate.
 __global__ __launch_bounds__(64) void k_hold(uint64_t ticks) {
     const uint64_t t0 = wall_clock64();
     while (wall_clock64() - t0 < ticks) __builtin_amdgcn_s_sleep(32);
@@ -1101,11 +1164,6 @@ hipError_t launch_shade_tiles(const RasterParams& p, const ShadeParams& sp, int 
     }
 }
 
-hipError_t launch_overflow(const RasterParams& p, hipStream_t s) {
-    if (p.nf > 0) hipLaunchKernelGGL(k_overflow, dim3(overflow_blocks(p.nf)), dim3(256), 0, s, p);
-    return hipGetLastError();
-}
-
 hipError_t launch_shade(const ShadeParams& p, hipStream_t s) {
     const dim3 grid(XcdGroups{1, 16, shade_blocks_x(p.W), shade_blocks_y(p.H)}.grid()), block(64, 4);
     switch (p.mode) {
@@ -1128,30 +1186,62 @@ hipError_t launch_bin_batch(const BinParams* items, const uint2* map, uint32_t b
     if (blocks) hipLaunchKernelGGL(k_bin_batch, dim3(blocks), dim3(256), 0, s, items, map);
     return hipGetLastError();
 }
-hipError_t launch_raster_batch(const RasterParams* items, const uint2* map, uint32_t blocks, int tile_size,
-                               hipStream_t s) {
-    if (!blocks) return hipSuccess;
-    switch (tile_size) {
-        case 8: hipLaunchKernelGGL((k_raster_batch<8, 64>), dim3(blocks), dim3(64), 0, s, items, map); break;
-        case 16: hipLaunchKernelGGL((k_raster_batch<16, 256>), dim3(blocks), dim3(256), 0, s, items, map); break;
-        case 32: hipLaunchKernelGGL((k_raster_batch<32, 256>), dim3(blocks), dim3(256), 0, s, items, map); break;
-        default: return hipErrorInvalidValue;
-    }
-    return hipGetLastError();
-}
 hipError_t launch_overflow_batch(const RasterParams* items, const uint2* map, uint32_t blocks, hipStream_t s) {
     if (blocks) hipLaunchKernelGGL(k_overflow_batch, dim3(blocks), dim3(256), 0, s, items, map);
     return hipGetLastError();
 }
-hipError_t launch_shade_batch(const ShadeParams* items, const uint2* map, uint32_t blocks, int mode,
-                              hipStream_t s) {
-    if (!blocks) return hipSuccess;
-    switch (mode) {
-        case kShadePlain: hipLaunchKernelGGL(k_shade_batch<kShadePlain>, dim3(blocks), dim3(64, 4), 0, s, items, map); break;
-        case kShadeMlp: hipLaunchKernelGGL(k_shade_batch<kShadeMlp>, dim3(blocks), dim3(64, 4), 0, s, items, map); break;
+
+// Resident workgroups per CU of the batch raster kernel.
+uint32_t raster_batch_blocks_per_cu(int tile_size, int mode) {
+    const void* f = nullptr;
+    int block = 256;
+#define RZ_PICK(TS, B)                                                                              \
+    block = B;                                                                                      \
+    f = mode == kShadeMlp ? reinterpret_cast<const void*>(&k_raster_shade_batch<TS, B, kShadeMlp>)   \
+                          : reinterpret_cast<const void*>(&k_raster_shade_batch<TS, B, kShadePlain>);
+    switch (tile_size) {
+        case 8: RZ_PICK(8, 64) break;
+        case 16: RZ_PICK(16, 256) break;
+        default: RZ_PICK(32, 256) break;
+    }
+#undef RZ_PICK
+    int n = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, f, block, 0) != hipSuccess || n < 1) n = 1;
+    return (uint32_t)n;
+}
+
+template <int MODE>
+hipError_t launch_batch_tiles_mode(const RasterParams* items, const ShadeParams* sitems, const BatchLists& bl,
+                                   uint32_t raster_grid, uint32_t tiles_grid, int tile_size, hipStream_t s,
+                                   bool raster) {
+    uint32_t* tiles_cov = bl.part_cov + raster_grid;
+#define RZ_LAUNCH(TS, B)                                                                                          \
+    if (raster)                                                                                                   \
+        hipLaunchKernelGGL((k_raster_shade_batch<TS, B, MODE>), dim3(raster_grid), dim3(B), 0, s, items, sitems, bl); \
+    else                                                                                                          \
+        hipLaunchKernelGGL((k_shade_tiles_batch<TS, B, MODE>), dim3(tiles_grid), dim3(B), 0, s, items, sitems, bl,    \
+                           tiles_cov);
+    switch (tile_size) {
+        case 8: RZ_LAUNCH(8, 64) break;
+        case 16: RZ_LAUNCH(16, 256) break;
+        case 32: RZ_LAUNCH(32, 256) break;
         default: return hipErrorInvalidValue;
     }
+#undef RZ_LAUNCH
     return hipGetLastError();
+}
+
+// The batch's raster pass (raster = true) or its overflow shade; grids are multiples of 8.
+hipError_t launch_batch_tiles(const RasterParams* items, const ShadeParams* sitems, const BatchLists& bl,
+                              uint32_t raster_grid, uint32_t tiles_grid, int tile_size, int mode, hipStream_t s,
+                              bool raster) {
+    switch (mode) {
+        case kShadePlain:
+            return launch_batch_tiles_mode<kShadePlain>(items, sitems, bl, raster_grid, tiles_grid, tile_size, s, raster);
+        case kShadeMlp:
+            return launch_batch_tiles_mode<kShadeMlp>(items, sitems, bl, raster_grid, tiles_grid, tile_size, s, raster);
+        default: return hipErrorInvalidValue;
+    }
 }
 
 }  // namespace rz

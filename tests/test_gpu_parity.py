@@ -178,17 +178,19 @@ def test_batch_matches_single_frames_and_oracle(ctx):
     lights = [s.lights_copy() for s, a in work]
     batch = [(m, l, a, f, d, i) for m, l, (s, a), (f, d, i) in zip(meshes, lights, work, outs)]
     st = ctx.draw_batch(batch, stats=True)
-    total_front = 0
+    total = {"front_triangles": 0, "fragments": 0, "covered_pixels": 0}
     for k, ((scene, args), (f, d, i), m) in enumerate(zip(work, outs, meshes)):
         fb, db, ib = f.download(), d.download(), i.download()
         single = ctx.draw(m, scene.lights_copy(), args, ids=True, stats=True)
         assert np.array_equal(fb, single[0]) and np.array_equal(ib, single[2]), k
         assert np.array_equal(db.view(np.uint32), single[1].view(np.uint32)), k
-        total_front += single[3]["front_triangles"]
+        for key in total:
+            total[key] += single[3][key]
         if k < 4:
             ref = oracle.draw_frame(scene, args)
             assert_same_frame((fb, db, ib, single[3], lights[k]), ref, f"batch item {k}")
-    assert st["front_triangles"] == total_front
+    for key in total:   # the batch's statistics are the sums of its meshes'
+        assert st[key] == total[key], key
     for m in meshes:
         m.close()
 
