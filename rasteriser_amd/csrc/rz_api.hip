@@ -68,11 +68,10 @@ struct rz_mesh {
     float *fnx = nullptr, *fny = nullptr, *fnz = nullptr;   // face normals (flat)
     float *ftx = nullptr, *fty = nullptr, *ftz = nullptr;   // face tangents (normal maps)
     float2* uvs = nullptr;
-    int32_t *fn = nullptr, *fuv = nullptr, *fmat = nullptr;
-    // binning order: faces sorted by the Morton code of their centroid, with their
-    // vertex indices in that order (the bin pass's workgroups then cover compact
-    // screen regions whatever the input order; visibility does not depend on it)
-    int32_t *bin_order = nullptr, *bin_fv = nullptr;
+    // the faces' indices in binning order: sorted by the Morton code of their centroid
+    // (the bin pass's workgroups then cover compact screen regions whatever the input
+    // order; visibility does not depend on it)
+    BinFace* bin_faces = nullptr;
     DevMaterial* mats = nullptr;
     uint32_t* texels = nullptr;
     float* luts = nullptr;
@@ -327,9 +326,11 @@ void fill_frame(const rz_mesh* mesh, rz_light* lights, int32_t nl, const rz_args
 
     BinParams& bp = L.bp;
     std::memset(&bp, 0, sizeof bp);
-    bp.fv = mesh->bin_fv;
-    bp.order = mesh->bin_order;
+    bp.faces = mesh->bin_faces;
     bp.raster = vp.raster;
+    bp.cnormal = vp.cnormal;
+    bp.uvs = mesh->uvs;
+    bp.flat = a.flat != 0;
     bp.rec = (FaceRecord*)buf[kBufRec];
     bp.tile_count = vp.tile_count;
     bp.bins = (uint32_t*)buf[kBufBins];
@@ -363,9 +364,6 @@ void fill_frame(const rz_mesh* mesh, rz_light* lights, int32_t nl, const rz_args
     ShadeParams& sp = L.sp;
     sp.vis = rp.vis;
     sp.rec = bp.rec;
-    sp.fn = mesh->fn; sp.fuv = mesh->fuv; sp.fmat = mesh->fmat;
-    sp.cnormal = vp.cnormal;
-    sp.uvs = mesh->uvs;
     sp.mats = mesh->mats; sp.texels = mesh->texels; sp.luts = mesh->luts;
     sp.ftx = mesh->ftx; sp.fty = mesh->fty; sp.ftz = mesh->ftz;
     std::memcpy(sp.normal_mat, mats.normal, 64);
@@ -913,9 +911,11 @@ uint32_t morton30(uint32_t x, uint32_t y, uint32_t z) {
 
 // Face processing order of the bin pass: by the Morton code of the face centroid
 // (positions as given: model space, or raster x, y for raster-space meshes), ties in
-// input order.  fv: validated vertex indices in input order.
-void bin_order(const rz_mesh_desc* d, const std::vector<int32_t>& fv, std::vector<int32_t>& order,
-               std::vector<int32_t>& bin_fv) {
+// input order; the faces' indices (validated) in that order.
+void bin_faces(const rz_mesh_desc* d, std::vector<BinFace>& out) {
+    std::vector<int32_t> fv(3 * (size_t)d->num_faces);
+    for (size_t f = 0; f < fv.size() / 3; ++f)
+        for (int k = 0; k < 3; ++k) fv[3 * f + k] = d->faces[f].vertices[k];
     const size_t F = (size_t)d->num_faces;
     const int stride = d->raster_space ? 4 : 3;
     const int dims = d->raster_space ? 2 : 3;
@@ -943,12 +943,14 @@ void bin_order(const rz_mesh_desc* d, const std::vector<int32_t>& fv, std::vecto
         key[f] = ((uint64_t)morton30(q[0], q[1], q[2]) << 32) | (uint64_t)f;
     }
     std::sort(key.begin(), key.end());
-    order.resize(F);
-    bin_fv.resize(3 * F);
+    out.resize(F);
     for (size_t i = 0; i < F; ++i) {
         const size_t f = (size_t)(key[i] & 0xFFFFFFFFu);
-        order[i] = (int32_t)f;
-        for (int k = 0; k < 3; ++k) bin_fv[3 * i + k] = fv[3 * f + k];
+        const rz_triangle& t = d->faces[f];
+        BinFace& b = out[i];
+        b.q0 = make_int4(t.vertices[0], t.vertices[1], t.vertices[2], (int32_t)f);
+        b.q1 = make_int4(t.normals[0], t.normals[1], t.normals[2], t.material);
+        b.q2 = make_int4(t.uvs[0] < 0 ? -1 : t.uvs[0], t.uvs[1] < 0 ? -1 : t.uvs[1], t.uvs[2] < 0 ? -1 : t.uvs[2], 0);
     }
 }
 
@@ -1029,23 +1031,9 @@ rz_status rz_mesh_create(rz_ctx* ctx, const rz_mesh_desc* d, rz_mesh** out) {
     }
     if ((st = upload(ctx, m, &m->uvs, d->uvs, (size_t)m->nt * 8)) != RZ_OK) return cleanup(st);
     {
-        std::vector<int32_t> fv(3 * F), fn(3 * F), fuv(3 * F), fmat(F);
-        for (size_t f = 0; f < F; ++f) {
-            const rz_triangle& t = d->faces[f];
-            for (int k = 0; k < 3; ++k) {
-                fv[3 * f + k] = t.vertices[k];
-                fn[3 * f + k] = t.normals[k];
-                fuv[3 * f + k] = t.uvs[k] < 0 ? -1 : t.uvs[k];
-            }
-            fmat[f] = t.material;
-        }
-        std::vector<int32_t> order, bin_fv;
-        bin_order(d, fv, order, bin_fv);
-        if ((st = upload(ctx, m, &m->bin_order, order.data(), 4 * F)) != RZ_OK) return cleanup(st);
-        if ((st = upload(ctx, m, &m->bin_fv, bin_fv.data(), 12 * F)) != RZ_OK) return cleanup(st);
-        if ((st = upload(ctx, m, &m->fn, fn.data(), 12 * F)) != RZ_OK) return cleanup(st);
-        if ((st = upload(ctx, m, &m->fuv, fuv.data(), 12 * F)) != RZ_OK) return cleanup(st);
-        if ((st = upload(ctx, m, &m->fmat, fmat.data(), 4 * F)) != RZ_OK) return cleanup(st);
+        std::vector<BinFace> faces;
+        bin_faces(d, faces);
+        if ((st = upload(ctx, m, &m->bin_faces, faces.data(), sizeof(BinFace) * F)) != RZ_OK) return cleanup(st);
     }
     {
         // materials: RGBA8 texel pool plus one 256-entry normalisation table each

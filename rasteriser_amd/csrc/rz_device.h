@@ -39,14 +39,20 @@ constexpr uint32_t kListMeshShift = 22;
 constexpr uint32_t kListTileMask = (1u << kListMeshShift) - 1u;
 constexpr int32_t kMaxBatchMeshes = 1 << (32 - kListMeshShift);
 
-// Per-face setup record written by the bin pass (64 bytes, one per face):
+// Per-face setup record written by the bin pass (128 bytes, one per face):
 //   r0 = (x0, y0, z0, area)            raster vertex 0, area = edge(v2, v0, v1)
 //   r1 = (x1, y1, z1, bits(bbox_lo))   bbox_lo = tl.x | tl.y << 16
 //   r2 = (x2, y2, z2, bits(bbox_hi))   bbox_hi = br.x | br.y << 16
 //   r3 = (w0, w1, w2, 0)               1/clip_w per vertex
-// A culled face has bbox_lo = 0xFFFFFFFF, bbox_hi = 0 (empty).
+// and, for the shading of the pixels the face wins, its corners' camera-space normals
+// (as the vertex pass wrote them) and uvs ((0, 0) for a corner without one):
+//   r4 = (n0.x, n0.y, n0.z, u0)
+//   r5 = (n1.x, n1.y, n1.z, v0)
+//   r6 = (n2.x, n2.y, n2.z, u1)
+//   r7 = (v1, u2, v2, bits(material))
+// A culled face has bbox_lo = 0xFFFFFFFF, bbox_hi = 0 (empty) and nothing else.
 struct FaceRecord {
-    float4 r0, r1, r2, r3;
+    float4 r0, r1, r2, r3, r4, r5, r6, r7;
 };
 
 // Device copy of one material.
@@ -76,10 +82,18 @@ struct VertexParams {
     float Wf, Hf;
 };
 
+// A face's indices in the bin pass's order (48 bytes): (v0, v1, v2, face), (n0, n1, n2,
+// material), (t0, t1, t2, 0) with t = -1 for a corner without a uv.
+struct BinFace {
+    int4 q0, q1, q2;
+};
+
 struct BinParams {
-    const int32_t* fv;          // 3 vertex indices per face, in binning order
-    const int32_t* order;       // binning order -> face index
+    const BinFace* faces;       // in binning order
     const float4* raster;
+    const float4* cnormal;      // camera-space normals (flat: one per face)
+    const float2* uvs;
+    int32_t flat;
     FaceRecord* rec;
     uint32_t* tile_count;
     uint32_t* bins;
@@ -147,11 +161,6 @@ struct BatchLists {
 struct ShadeParams {
     const unsigned long long* vis;
     const FaceRecord* rec;
-    const int32_t* fn;          // 3 normal indices per face (flat: the face itself)
-    const int32_t* fuv;         // 3 uv indices per face, -1 = none
-    const int32_t* fmat;
-    const float4* cnormal;
-    const float2* uvs;
     const DevMaterial* mats;
     const uint32_t* texels;
     const float* luts;

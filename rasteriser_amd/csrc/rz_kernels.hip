@@ -123,12 +123,20 @@ __device__ __forceinline__ void bin_body(const BinParams& p, int blk) {
     // faces are taken in the mesh's binning order (centroid Morton order), so the
     // workgroup's faces are screen neighbours; f is the face's own index
     const int i = blk * 256 + threadIdx.x;
-    const int f = i < p.nf ? p.order[i] : p.nf;
+    int f = p.nf;
     bool front = false;
     uint32_t tx0 = 1, tx1 = 0, ty0 = 1, ty1 = 0;   // empty tile rectangle
     if (i < p.nf) {
-        const int i0 = p.fv[3 * i], i1 = p.fv[3 * i + 1], i2 = p.fv[3 * i + 2];
-        const float4 a = p.raster[i0], b = p.raster[i1], c = p.raster[i2];
+        const BinFace q = p.faces[i];
+        f = q.q0.w;
+        const float4 a = p.raster[q.q0.x], b = p.raster[q.q0.y], c = p.raster[q.q0.z];
+        // the shading attributes are read beside the vertices, before the cull decides
+        // whether they are kept
+        const float4 n0 = p.cnormal[p.flat ? f : q.q1.x], n1 = p.cnormal[p.flat ? f : q.q1.y];
+        const float4 n2 = p.cnormal[p.flat ? f : q.q1.z];
+        const float2 t0 = q.q2.x >= 0 ? p.uvs[q.q2.x] : make_float2(0.f, 0.f);
+        const float2 t1 = q.q2.y >= 0 ? p.uvs[q.q2.y] : make_float2(0.f, 0.f);
+        const float2 t2 = q.q2.z >= 0 ? p.uvs[q.q2.z] : make_float2(0.f, 0.f);
         const float s = signed_area(a.x, a.y, b.x, b.y, c.x, c.y);
         front = (s > 0.0f) != (p.wind_cw != 0);
         FaceRecord* r = p.rec + f;
@@ -140,6 +148,10 @@ __device__ __forceinline__ void bin_body(const BinParams& p, int blk) {
             r->r1 = make_float4(b.x, b.y, b.z, __uint_as_float(tlx | (tly << 16)));
             r->r2 = make_float4(c.x, c.y, c.z, __uint_as_float(brx | (bry << 16)));
             r->r3 = make_float4(a.w, b.w, c.w, 0.0f);
+            r->r4 = make_float4(n0.x, n0.y, n0.z, t0.x);
+            r->r5 = make_float4(n1.x, n1.y, n1.z, t0.y);
+            r->r6 = make_float4(n2.x, n2.y, n2.z, t1.x);
+            r->r7 = make_float4(t1.y, t2.x, t2.y, __int_as_float(q.q1.w));
             if (tlx <= brx && tly <= bry) {
                 tx0 = tlx >> p.ts_log2; tx1 = brx >> p.ts_log2;
                 ty0 = tly >> p.ts_log2; ty1 = bry >> p.ts_log2;
@@ -311,6 +323,16 @@ __device__ __forceinline__ void stage_luts(const ShadeParams& p, float* s_lut, i
     for (int e = tid; e < n; e += nthreads) s_lut[e] = p.luts[e];
 }
 
+// The first kLdsMats materials, staged in LDS by the whole workgroup (a barrier must
+// follow before they are read).
+constexpr int kLdsMats = 8;
+__device__ __forceinline__ void stage_mats(const ShadeParams& p, DevMaterial* s_mats, int tid, int nthreads) {
+    const int n = (p.nmat < kLdsMats ? p.nmat : kLdsMats) * (int)(sizeof(DevMaterial) / 4);
+    const uint32_t* src = reinterpret_cast<const uint32_t*>(p.mats);
+    uint32_t* dst = reinterpret_cast<uint32_t*>(s_mats);
+    for (int e = tid; e < n; e += nthreads) dst[e] = src[e];
+}
+
 // Shades one pixel from its visibility word; every lane of the wave calls it together
 // (the learned shader and sample collection work per wave).  Writes frame, depth and
 // ids; returns whether the pixel has a winning fragment.
@@ -319,7 +341,8 @@ __device__ __forceinline__ void stage_luts(const ShadeParams& p, float* s_lut, i
 // colour/255 appended to the sample buffer for training).
 // kShadeFeat: rec receives the fragment's record words 0-5 (the caller stores them).
 template <int MODE>
-__device__ __forceinline__ bool shade_pixel(const ShadeParams& p, const float* s_lut, const uint16_t* s_net,
+__device__ __forceinline__ bool shade_pixel(const ShadeParams& p, const float* s_lut, const DevMaterial* s_mats,
+                                            const uint16_t* s_net,
                                             uint32_t px, uint32_t py, bool valid, unsigned long long key, int lane,
                                             uint16_t* s_in, float* s_out, uint32_t* rec = nullptr) {
     const size_t np = (size_t)p.W * p.H;
@@ -338,6 +361,7 @@ __device__ __forceinline__ bool shade_pixel(const ShadeParams& p, const float* s
         const uint32_t f = (uint32_t)key;
         const FaceRecord& r = p.rec[f];
         const float4 a = r.r0, b = r.r1, c = r.r2, w = r.r3;
+        const float4 g0 = r.r4, g1 = r.r5, g2 = r.r6, g3 = r.r7;
         const float pxf = (float)px, pyf = (float)py;
         // barycentric and depth, exactly as the raster pass computed them
         const float e0 = edge(pxf, pyf, b.x, b.y, c.x, c.y);
@@ -348,32 +372,23 @@ __device__ __forceinline__ bool shade_pixel(const ShadeParams& p, const float* s
         // drawing.cpp:125-135: perspective-correct interpolation
         const float i0 = w.x * b0, i1 = w.y * b1, i2 = w.z * b2;
         const float d = 1.f / (i0 + i1 + i2);
-        int n0, n1, n2;
-        if (p.flat) {
-            n0 = n1 = n2 = (int)f;
-        } else {
-            n0 = p.fn[3 * f]; n1 = p.fn[3 * f + 1]; n2 = p.fn[3 * f + 2];
-        }
-        const float4 N0 = p.cnormal[n0], N1 = p.cnormal[n1], N2 = p.cnormal[n2];
-        float nx = d * (i0 * N0.x + i1 * N1.x + i2 * N2.x);
-        float ny = d * (i0 * N0.y + i1 * N1.y + i2 * N2.y);
-        float nz = d * (i0 * N0.z + i1 * N1.z + i2 * N2.z);
+        float nx = d * (i0 * g0.x + i1 * g1.x + i2 * g2.x);
+        float ny = d * (i0 * g0.y + i1 * g1.y + i2 * g2.y);
+        float nz = d * (i0 * g0.z + i1 * g1.z + i2 * g2.z);
         const float rn = 1.f / sqrtf((nx * nx + ny * ny) + nz * nz);
         nx = nx * rn; ny = ny * rn; nz = nz * rn;
         if (p.wind_cw) { nx = -nx; ny = -ny; nz = -nz; }
-        const int mi = p.fmat[f];
-        const DevMaterial& m = p.mats[mi];
+        const int mi = __float_as_int(g3.w);
+        DevMaterial m;
+        if (s_mats && mi < kLdsMats)
+            m = s_mats[mi];
+        else
+            m = p.mats[mi];
         float alb[3];
         float u = 0.f, v = 0.f;
         if (m.tex_w > 0 || m.nmap_w > 0) {
-            float2 t[3];
-#pragma unroll
-            for (int k = 0; k < 3; ++k) {
-                const int ui = p.fuv[3 * f + k];
-                t[k] = ui >= 0 ? p.uvs[ui] : make_float2(0.f, 0.f);
-            }
-            u = d * (i0 * t[0].x + i1 * t[1].x + i2 * t[2].x);
-            v = d * (i0 * t[0].y + i1 * t[1].y + i2 * t[2].y);
+            u = d * (i0 * g0.w + i1 * g2.w + i2 * g3.y);
+            v = d * (i0 * g1.w + i1 * g3.x + i2 * g3.z);
         }
         if (m.tex_w > 0) {
             if (mi < kLdsLuts)
@@ -484,10 +499,11 @@ __device__ __forceinline__ uint32_t region_append(bool hit, const uint32_t w[6],
 }
 
 // Deferred shade pass: 64 x 4 pixel blocks, each wave 64 consecutive pixels of a row.
-// s_lut / s_net: the normalisation tables and network, staged in LDS by the caller.
+// s_lut / s_mats / s_net: the normalisation tables, materials and network, staged in LDS
+// by the caller.
 template <int MODE>
-__device__ __forceinline__ void shade_body(const ShadeParams& p, const float* s_lut, const uint16_t* s_net,
-                                           int blk) {
+__device__ __forceinline__ void shade_body(const ShadeParams& p, const float* s_lut, const DevMaterial* s_mats,
+                                           const uint16_t* s_net, int blk) {
     __shared__ __align__(16) uint16_t s_in[MODE == kShadeMlp ? 4 : 1][64 * 16];
     __shared__ __align__(16) float s_out[MODE == kShadeMlp ? 4 : 1][64 * 4];
     const int tid = threadIdx.y * 64 + threadIdx.x;
@@ -497,7 +513,8 @@ __device__ __forceinline__ void shade_body(const ShadeParams& p, const float* s_
     const bool valid = px < p.W && py < p.H;
     const unsigned long long key = valid ? p.vis[(size_t)py * p.W + px] : kEmptyKey;
     uint32_t rec[6];
-    const bool hit = shade_pixel<MODE>(p, s_lut, s_net, px, py, valid, key, threadIdx.x, s_in[ws], s_out[ws], rec);
+    const bool hit =
+        shade_pixel<MODE>(p, s_lut, s_mats, s_net, px, py, valid, key, threadIdx.x, s_in[ws], s_out[ws], rec);
     if (MODE == kShadeFeat) {
         // the wave's pixels lie in up to 64 / tile-edge tiles: one slot reservation per tile
         const uint32_t tile = (py >> p.ts_log2) * (uint32_t)p.tiles_x + (px >> p.ts_log2);
@@ -528,7 +545,8 @@ __device__ __forceinline__ void shade_body(const ShadeParams& p, const float* s_
 // others itself (one workgroup per tile, visibility words from HBM).
 template <int TS, int BLOCK, int MODE>
 __device__ __forceinline__ void shade_tile_body(const RasterParams& rp, const ShadeParams& p, const float* s_lut,
-                                                const uint16_t* s_net, int tile, uint32_t& cov) {
+                                                const DevMaterial* s_mats, const uint16_t* s_net, int tile,
+                                                uint32_t& cov) {
     constexpr int NW = BLOCK / 64;
     __shared__ __align__(16) uint16_t s_in[MODE == kShadeMlp ? NW : 1][64 * 16];
     __shared__ __align__(16) float s_out[MODE == kShadeMlp ? NW : 1][64 * 4];
@@ -544,7 +562,7 @@ __device__ __forceinline__ void shade_tile_body(const RasterParams& rp, const Sh
         const bool valid = x < p.W && y < p.H;
         const unsigned long long key = valid ? p.vis[(size_t)y * p.W + x] : kEmptyKey;
         uint32_t rec[6];
-        const bool hit = shade_pixel<MODE>(p, s_lut, s_net, x, y, valid, key, lane, s_in[ws], s_out[ws], rec);
+        const bool hit = shade_pixel<MODE>(p, s_lut, s_mats, s_net, x, y, valid, key, lane, s_in[ws], s_out[ws], rec);
         cov += hit ? 1u : 0u;
         if (MODE == kShadeFeat)
             nrec = region_append<NW>(hit, rec, (uint32_t)((size_t)y * p.W + x),
@@ -601,7 +619,8 @@ __device__ __forceinline__ bool fragment_key(float pxf, float pyf, float x0, flo
 // k_shade_tiles shades them.
 template <int TS, int BLOCK, bool FUSED, int MODE>
 __device__ __forceinline__ void raster_body(const RasterParams& p, const ShadeParams* sp, const float* s_lut,
-                                            const uint16_t* s_net, int tile, uint32_t& frags, uint32_t& cov) {
+                                            const DevMaterial* s_mats, const uint16_t* s_net, int tile,
+                                            uint32_t& frags, uint32_t& cov) {
     constexpr int NPX = TS * TS;
     constexpr int PPT = NPX / BLOCK;
     constexpr int NW = BLOCK / 64;
@@ -754,7 +773,8 @@ __device__ __forceinline__ void raster_body(const RasterParams& p, const ShadePa
             const uint32_t x = ox + (l % TS), y = oy + (l / TS);
             const bool valid = x < p.W && y < p.H;
             uint32_t rec[6];
-            const bool hit = shade_pixel<MODE>(*sp, s_lut, s_net, x, y, valid, s_key[l], lane, s_in, s_out, rec);
+            const bool hit =
+                shade_pixel<MODE>(*sp, s_lut, s_mats, s_net, x, y, valid, s_key[l], lane, s_in, s_out, rec);
             cov += hit ? 1u : 0u;
             if (MODE == kShadeFeat)
                 nrec = region_append<NW>(hit, rec, y * p.W + x,
@@ -895,8 +915,8 @@ __global__ __launch_bounds__(BLOCK, TS == 32 ? 6 : 8) void k_raster(RasterParams
     const uint32_t n = p.list_count[xcd * kCountStride];
     uint32_t frags = 0, cov = 0;
     for (uint32_t k = blockIdx.x >> 3; k < n; k += step)
-        raster_body<TS, BLOCK, false, kShadePlain>(p, nullptr, nullptr, nullptr, (int)p.lists[xcd * p.ntiles + k],
-                                                   frags, cov);
+        raster_body<TS, BLOCK, false, kShadePlain>(p, nullptr, nullptr, nullptr, nullptr,
+                                                   (int)p.lists[xcd * p.ntiles + k], frags, cov);
     if (p.counters[kCntOverflowTiles] != 0ull)
         frags += overflow_faces(p, blockIdx.x * BLOCK + threadIdx.x, gridDim.x * BLOCK);
     store_block_sum(frags, p.part_frag + blockIdx.x, threadIdx.x, BLOCK);
@@ -908,13 +928,16 @@ template <int TS, int BLOCK, int MODE>
 __global__ __launch_bounds__(BLOCK, MODE == kShadeMlp ? 3 : 6) void k_raster_shade(RasterParams p, ShadeParams sp) {
     const float* s_lut = sp.luts;
     __shared__ __align__(16) uint16_t s_net[MODE == kShadeMlp ? kPackedBytes / 2 : 8];
+    __shared__ DevMaterial s_mats[kLdsMats];
     if (MODE == kShadeMlp) stage_net(sp.mlp, s_net, threadIdx.x, BLOCK);
+    stage_mats(sp, s_mats, threadIdx.x, BLOCK);
     // (raster_body's first barrier orders these stores before any read)
     const uint32_t xcd = blockIdx.x & 7u, step = gridDim.x >> 3;
     const uint32_t n = p.list_count[xcd * kCountStride];
     uint32_t frags = 0, cov = 0;
     for (uint32_t k = blockIdx.x >> 3; k < n; k += step)
-        raster_body<TS, BLOCK, true, MODE>(p, &sp, s_lut, s_net, (int)p.lists[xcd * p.ntiles + k], frags, cov);
+        raster_body<TS, BLOCK, true, MODE>(p, &sp, s_lut, s_mats, s_net, (int)p.lists[xcd * p.ntiles + k], frags,
+                                           cov);
     if (p.counters[kCntOverflowTiles] != 0ull)
         frags += overflow_faces(p, blockIdx.x * BLOCK + threadIdx.x, gridDim.x * BLOCK);
     __syncthreads();
@@ -927,7 +950,9 @@ __global__ __launch_bounds__(BLOCK) void k_shade_tiles(RasterParams p, ShadePara
     if (p.counters[kCntOverflowTiles] != 0ull) {   // else the raster pass shaded every tile
         __shared__ float s_lut[kLdsLuts * 256];
         __shared__ __align__(16) uint16_t s_net[MODE == kShadeMlp ? kPackedBytes / 2 : 8];
+        __shared__ DevMaterial s_mats[kLdsMats];
         stage_luts(sp, s_lut, threadIdx.x, BLOCK);
+        stage_mats(sp, s_mats, threadIdx.x, BLOCK);
         if (MODE == kShadeMlp) stage_net(sp.mlp, s_net, threadIdx.x, BLOCK);
         __syncthreads();
         const uint32_t xcd = blockIdx.x & 7u, step = gridDim.x >> 3;
@@ -935,7 +960,7 @@ __global__ __launch_bounds__(BLOCK) void k_shade_tiles(RasterParams p, ShadePara
         for (uint32_t k = blockIdx.x >> 3; k < n; k += step) {
             const int tile = (int)p.lists[xcd * p.ntiles + k];
             if (p.tile_count[(size_t)tile * kCountStride] > p.cap)   // the others were shaded by the raster pass
-                shade_tile_body<TS, BLOCK, MODE>(p, sp, s_lut, s_net, tile, cov);
+                shade_tile_body<TS, BLOCK, MODE>(p, sp, s_lut, s_mats, s_net, tile, cov);
         }
     }
     store_block_sum(cov, sp.part_cov + blockIdx.x, threadIdx.x, BLOCK);
@@ -944,14 +969,16 @@ template <int MODE>
 __global__ __launch_bounds__(256) void k_shade(ShadeParams p) {
     __shared__ float s_lut[kLdsLuts * 256];
     __shared__ __align__(16) uint16_t s_net[MODE == kShadeMlp ? kPackedBytes / 2 : 8];
+    __shared__ DevMaterial s_mats[kLdsMats];
     const int tid = threadIdx.y * 64 + threadIdx.x;
     stage_luts(p, s_lut, tid, 256);
+    stage_mats(p, s_mats, tid, 256);
     if (MODE == kShadeMlp) stage_net(p.mlp, s_net, tid, 256);
     __syncthreads();
     const XcdGroups g{1, 16, shade_blocks_x(p.W), shade_blocks_y(p.H)};
     for (uint32_t u = blockIdx.x, n = g.grid(); u < n; u += gridDim.x) {
         const int blk = g.unit(u);
-        if (blk >= 0) shade_body<MODE>(p, s_lut, s_net, blk);
+        if (blk >= 0) shade_body<MODE>(p, s_lut, s_mats, s_net, blk);
     }
 }
 
@@ -995,7 +1022,7 @@ __global__ __launch_bounds__(BLOCK, MODE == kShadeMlp ? 3 : 6) void k_raster_sha
             if (k >= n) break;
             const uint32_t e = bl.lists[(size_t)x * bl.stride + k];
             const uint32_t m = e >> kListMeshShift;
-            raster_body<TS, BLOCK, true, MODE>(items[m], &sitems[m], sitems[m].luts, s_net,
+            raster_body<TS, BLOCK, true, MODE>(items[m], &sitems[m], sitems[m].luts, nullptr, s_net,
                                                (int)(e & kListTileMask), frags, cov);
         }
     }
@@ -1027,7 +1054,7 @@ __global__ __launch_bounds__(BLOCK) void k_shade_tiles_batch(const RasterParams*
             const uint32_t m = e >> kListMeshShift, tile = e & kListTileMask;
             const RasterParams& p = items[m];
             if (p.tile_count[(size_t)tile * kCountStride] > p.cap)
-                shade_tile_body<TS, BLOCK, MODE>(p, sitems[m], sitems[m].luts, s_net, (int)tile, cov);
+                shade_tile_body<TS, BLOCK, MODE>(p, sitems[m], sitems[m].luts, nullptr, s_net, (int)tile, cov);
         }
     }
     store_block_sum(cov, part_cov + blockIdx.x, threadIdx.x, BLOCK);
