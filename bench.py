@@ -67,7 +67,6 @@ def algorithmic_bytes(counts, scene, args, fused, cleared=True):
     out = {
         "vertex": 28 * V + 24 * Vn,
         "bin": 12 * T + 16 * V + 4 * P16,
-        "overflow": 0,
     }
     if fused:
         # cleared buffers: the bin launch writes the cleared frame and depth (7 B/px), the
@@ -87,6 +86,8 @@ def per_pass_table(us, nbytes):
     for p, t in us.items():
         if p == "network" and t == 0:
             continue   # the learned shader's pass: not part of the plain frame
+        if p == "overflow" and t == 0:
+            continue   # single frames run the overflow loop inside the raster pass
         b = nbytes.get(p, 0)
         gbs = b / (t * 1e-6) / 1e9 if t > 0 else 0.0
         table[p] = {"avg_us": t, "alg_bytes": b, "GBps": gbs, "frac": gbs / HBM_PEAK_GBS}
