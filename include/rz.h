@@ -255,8 +255,10 @@ typedef struct rz_batch_item {
 
 /* No reference counterpart (the reference draws one mesh per call).  Draws every
  * item as rz_draw_frame would, with each pass of every item in one launch (5
- * launches per batch).  Requires RZ_FLAG_DEVICE_BUFFERS; RZ_FLAG_CLEARED and
- * RZ_FLAG_ASYNC apply to all items.  stats (may be NULL) sums over the items. */
+ * launches per batch; the raster-and-shade pass is one resident grid over the tiles
+ * of all items).  At most 1024 items.  Requires RZ_FLAG_DEVICE_BUFFERS;
+ * RZ_FLAG_CLEARED and RZ_FLAG_ASYNC apply to all items.  stats (may be NULL) sums
+ * over the items. */
 rz_status rz_draw_batch(rz_ctx* ctx, const rz_batch_item* items, int32_t num_items, uint32_t flags,
                         rz_frame_stats* stats);
 /* `frames` repetitions of the batch captured in one HIP graph that owns its working
@@ -276,8 +278,8 @@ rz_status rz_autotune(rz_ctx* ctx, const rz_batch_item* items, int32_t num_items
  * HIP graph of `reps` back-to-back launches of the pass, timed by HIP events around its
  * second launch.  us[0..5] = vertex, bin, raster, overflow, shade, network in
  * microseconds per launch (bin = vertex+bin minus vertex, since binning appends to the
- * tile lists; network = the learned shader's second pass, 0 without
- * RZ_FLAG_MLP_SHADE).  Device buffers and RZ_FLAG_CLEARED required.  Leaves the frame
+ * tile lists; overflow = 0, the overflow loop runs inside the raster pass; network =
+ * the learned shader's second pass, 0 without RZ_FLAG_MLP_SHADE).  Device buffers and RZ_FLAG_CLEARED required.  Leaves the frame
  * rendered in frame_device/depth_device. */
 rz_status rz_time_passes(rz_ctx* ctx, const rz_mesh* mesh, rz_light* lights, int32_t num_lights,
                          const rz_args* args, uint8_t* frame_device, float* depth_device, uint32_t flags,
