@@ -4,16 +4,21 @@
 //               (geometry.cpp:44-74, 85-95, 110-116, 135-142) and the normal
 //               transform (geometry.cpp:35-42, 97-108); clears the per-frame bins.
 //   k_bin       backface test (geometry.cpp:76-83, drawing.cpp:176-178), clamped
-//               bbox (drawing.cpp:77-93), per-face setup record, and appends the face
-//               to the bin of every tile its bbox meets (fixed capacity per tile).
-//   k_raster    one workgroup per tile: expands (face, bbox pixel) candidates with a
-//               wavefront prefix scan, runs the inside and depth tests of
-//               drawing.cpp:96-121 and keeps per pixel the minimum 64-bit
-//               (ordered depth << 32 | face) key in LDS; writes the visibility word.
-//   k_overflow  faces of tiles whose bin overflowed, with global 64-bit atomicMin.
+//               bbox (drawing.cpp:77-93), per-face setup and shading record, appends
+//               the face to the bin of every tile its bbox meets (fixed capacity per
+//               tile) and each tile's first face puts the tile on a per-XCD list.
+//   k_raster    resident grid over the tile lists, one tile per workgroup at a time:
+//               expands (face, bbox pixel) candidates with wavefront prefix scans,
+//               runs the inside and depth tests of drawing.cpp:96-121 and keeps per
+//               pixel the minimum 64-bit (ordered depth << 32 | face) key in LDS;
+//               writes the visibility words; then its share of the faces of tiles
+//               whose bin overflowed, with global 64-bit atomicMin.
+//   k_raster_shade  k_raster, shading each tile from LDS (the default pipeline);
+//               k_shade_tiles shades the overflowed tiles afterwards.
 //   k_shade     deferred shade of the winning fragment: depth, perspective-correct
 //               normal/uv, texture, Lambert (drawing.cpp:121-146, shading.cpp:20-34,
 //               material.cpp:11-26, CImg _linear_atXY).
+//   k_*_batch   the same passes over many meshes per launch.
 //
 // The packed key makes the winner the lexicographic minimum of (depth, face index)
 // over covered fragments with depth below the initial depth: the same fragment the
@@ -605,7 +610,7 @@ __device__ __forceinline__ bool fragment_key(float pxf, float pyf, float x0, flo
     return true;
 }
 
-// One workgroup per tile.  The tile's bin is taken BLOCK faces at a time: the faces
+// One tile, by one workgroup.  The tile's bin is taken BLOCK faces at a time: the faces
 // whose bbox meets the tile are compacted (ballot + popcount ranks, wave totals in LDS)
 // and their candidate pixel counts (bbox inside the tile) prefix-summed across the
 // workgroup.  Candidates are then handed out one per thread, up to 4096 per round: a
@@ -860,9 +865,9 @@ __device__ __forceinline__ void overflow_body(const RasterParams& p, int blk, in
 }
 
 // ---------------------------------------------------------------------------------
-// Entry kernels.  Single frame: parameters by value, one block per unit of work.
-// Batch: per-mesh parameter tables in device memory and a map from block to
-// (mesh, block within that mesh's pass), so one launch covers every mesh.
+// Entry kernels.  Single frame: parameters by value.  Batch: per-mesh parameter tables
+// in device memory, with a map from block to (mesh, block within that mesh's pass) or
+// the shared tile lists, so one launch covers every mesh.
 
 __global__ __launch_bounds__(256) void k_vertex(VertexParams p) { vertex_body(p, blockIdx.x); }
 // Fills nbytes at base with a 32-bit pattern laid out as in a 4-byte-aligned array of
