@@ -141,6 +141,7 @@ struct RasterParams {
     int32_t ts_log2;
     uint32_t cap;
     uint32_t max_blocks;        // workgroups of the list-driven launch (a multiple of 8)
+    int32_t narrow;             // 16-pixel tiles by 64-thread workgroups (frames with many tiles)
     int32_t pre_cleared;        // outputs cleared by the bin launch
     uint32_t* rec_counts;       // separate shade pipeline, learned shader: zeroed per listed tile
     const uint32_t* lists;      // per-XCD lists of tiles with faces (see BinParams)

@@ -1104,7 +1104,7 @@ hipError_t launch_bin(const BinParams& p, hipStream_t s) {
 }
 
 // Resident workgroups per CU of the list-driven raster kernel for this configuration.
-uint32_t raster_blocks_per_cu(int tile_size, bool fused, int mode) {
+uint32_t raster_blocks_per_cu(int tile_size, bool fused, int mode, bool narrow) {
     const void* f = nullptr;
     int block = 256;
 #define RZ_PICK(TS, B)                                                                              \
@@ -1121,7 +1121,13 @@ uint32_t raster_blocks_per_cu(int tile_size, bool fused, int mode) {
         f = reinterpret_cast<const void*>(&k_raster_shade<TS, B, kShadePlain>);
     switch (tile_size) {
         case 8: RZ_PICK(8, 64) break;
-        case 16: RZ_PICK(16, 256) break;
+        case 16:
+            if (narrow) {
+                RZ_PICK(16, 64)
+            } else {
+                RZ_PICK(16, 256)
+            }
+            break;
         default: RZ_PICK(32, 256) break;
     }
 #undef RZ_PICK
@@ -1134,7 +1140,12 @@ hipError_t launch_raster(const RasterParams& p, int tile_size, hipStream_t s) {
     const uint32_t grid = p.max_blocks;
     switch (tile_size) {
         case 8: hipLaunchKernelGGL((k_raster<8, 64>), dim3(grid), dim3(64), 0, s, p); break;
-        case 16: hipLaunchKernelGGL((k_raster<16, 256>), dim3(grid), dim3(256), 0, s, p); break;
+        case 16:
+            if (p.narrow)
+                hipLaunchKernelGGL((k_raster<16, 64>), dim3(grid), dim3(64), 0, s, p);
+            else
+                hipLaunchKernelGGL((k_raster<16, 256>), dim3(grid), dim3(256), 0, s, p);
+            break;
         case 32: hipLaunchKernelGGL((k_raster<32, 256>), dim3(grid), dim3(256), 0, s, p); break;
         default: return hipErrorInvalidValue;
     }
@@ -1151,7 +1162,10 @@ hipError_t launch_raster_shade_mode(const RasterParams& p, const ShadeParams& sp
         }
         case 16: {
             const uint32_t grid = p.max_blocks;
-            hipLaunchKernelGGL((k_raster_shade<16, 256, MODE>), dim3(grid), dim3(256), 0, s, p, sp);
+            if (p.narrow)
+                hipLaunchKernelGGL((k_raster_shade<16, 64, MODE>), dim3(grid), dim3(64), 0, s, p, sp);
+            else
+                hipLaunchKernelGGL((k_raster_shade<16, 256, MODE>), dim3(grid), dim3(256), 0, s, p, sp);
             break;
         }
         case 32: {
