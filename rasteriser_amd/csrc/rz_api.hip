@@ -986,15 +986,44 @@ rz_status rz_mesh_create(rz_ctx* ctx, const rz_mesh_desc* d, rz_mesh** out) {
         return s;
     };
     const size_t V = (size_t)m->nv, N = (size_t)m->nn, F = (size_t)m->nf;
+    // The faces in binning order; vertices, normals and uvs are renumbered by first use
+    // in that order, so that the bin pass reads them (and the vertex pass writes them)
+    // in address order.  Only storage moves: every value and face index is the caller's.
+    std::vector<BinFace> faces;
+    bin_faces(d, faces);
+    std::vector<int32_t> vorder, norder, torder;   // new index -> caller's index
+    {
+        std::vector<int32_t> vmap(V, -1), nmap(N, -1), tmap((size_t)m->nt, -1);
+        auto use = [](int32_t& idx, std::vector<int32_t>& map, std::vector<int32_t>& order) {
+            if (idx < 0) return;
+            if (map[(size_t)idx] < 0) {
+                map[(size_t)idx] = (int32_t)order.size();
+                order.push_back(idx);
+            }
+            idx = map[(size_t)idx];
+        };
+        for (BinFace& b : faces) {
+            use(b.q0.x, vmap, vorder); use(b.q0.y, vmap, vorder); use(b.q0.z, vmap, vorder);
+            use(b.q1.x, nmap, norder); use(b.q1.y, nmap, norder); use(b.q1.z, nmap, norder);
+            use(b.q2.x, tmap, torder); use(b.q2.y, tmap, torder); use(b.q2.z, tmap, torder);
+        }
+        for (size_t i = 0; i < V; ++i) if (vmap[i] < 0) vorder.push_back((int32_t)i);
+        for (size_t i = 0; i < N; ++i) if (nmap[i] < 0) norder.push_back((int32_t)i);
+        for (size_t i = 0; i < tmap.size(); ++i) if (tmap[i] < 0) torder.push_back((int32_t)i);
+    }
     // positions: SoA for the coalesced vertex stage
     if (m->raster_space) {
-        if ((st = upload(ctx, m, &m->rin, d->vertices, V * 16)) != RZ_OK) return cleanup(st);
+        std::vector<float> rv(4 * V);
+        for (size_t i = 0; i < V; ++i)
+            for (int c = 0; c < 4; ++c) rv[4 * i + c] = d->vertices[4 * (size_t)vorder[i] + c];
+        if ((st = upload(ctx, m, &m->rin, rv.data(), V * 16)) != RZ_OK) return cleanup(st);
     } else {
         std::vector<float> sx(V), sy(V), sz(V);
         for (size_t i = 0; i < V; ++i) {
-            sx[i] = d->vertices[3 * i];
-            sy[i] = d->vertices[3 * i + 1];
-            sz[i] = d->vertices[3 * i + 2];
+            const size_t o = (size_t)vorder[i];
+            sx[i] = d->vertices[3 * o];
+            sy[i] = d->vertices[3 * o + 1];
+            sz[i] = d->vertices[3 * o + 2];
         }
         if ((st = upload(ctx, m, &m->px, sx.data(), V * 4)) != RZ_OK) return cleanup(st);
         if ((st = upload(ctx, m, &m->py, sy.data(), V * 4)) != RZ_OK) return cleanup(st);
@@ -1032,20 +1061,24 @@ rz_status rz_mesh_create(rz_ctx* ctx, const rz_mesh_desc* d, rz_mesh** out) {
     {
         std::vector<float> sx(N), sy(N), sz(N);
         for (size_t i = 0; i < N; ++i) {
-            sx[i] = d->normals[3 * i];
-            sy[i] = d->normals[3 * i + 1];
-            sz[i] = d->normals[3 * i + 2];
+            const size_t o = (size_t)norder[i];
+            sx[i] = d->normals[3 * o];
+            sy[i] = d->normals[3 * o + 1];
+            sz[i] = d->normals[3 * o + 2];
         }
         if ((st = upload(ctx, m, &m->nx, sx.data(), N * 4)) != RZ_OK) return cleanup(st);
         if ((st = upload(ctx, m, &m->ny, sy.data(), N * 4)) != RZ_OK) return cleanup(st);
         if ((st = upload(ctx, m, &m->nz, sz.data(), N * 4)) != RZ_OK) return cleanup(st);
     }
-    if ((st = upload(ctx, m, &m->uvs, d->uvs, (size_t)m->nt * 8)) != RZ_OK) return cleanup(st);
     {
-        std::vector<BinFace> faces;
-        bin_faces(d, faces);
-        if ((st = upload(ctx, m, &m->bin_faces, faces.data(), sizeof(BinFace) * F)) != RZ_OK) return cleanup(st);
+        std::vector<float> uv(2 * (size_t)m->nt);
+        for (size_t i = 0; i < (size_t)m->nt; ++i) {
+            uv[2 * i] = d->uvs[2 * (size_t)torder[i]];
+            uv[2 * i + 1] = d->uvs[2 * (size_t)torder[i] + 1];
+        }
+        if ((st = upload(ctx, m, &m->uvs, uv.data(), (size_t)m->nt * 8)) != RZ_OK) return cleanup(st);
     }
+    if ((st = upload(ctx, m, &m->bin_faces, faces.data(), sizeof(BinFace) * F)) != RZ_OK) return cleanup(st);
     {
         // materials: RGBA8 texel pool plus one 256-entry normalisation table each
         std::vector<DevMaterial> dm(m->nm);
