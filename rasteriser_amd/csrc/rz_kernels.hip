@@ -610,6 +610,68 @@ __device__ __forceinline__ bool fragment_key(float pxf, float pyf, float x0, flo
     return true;
 }
 
+// Candidate rectangles larger than this are shrunk first (shrink_rect).
+constexpr int kShrinkPixels = 128;
+
+// Range of T ∩ {lo <= v <= hi} along the other axis (u), T the triangle (u_k, v_k):
+// the vertices inside the slab and the edges' crossings of its two lines.  Returns false
+// when the slab misses T.  The crossings use the hardware reciprocal; callers widen the
+// range by a pixel for that.
+__device__ __forceinline__ bool slab_range(float u0, float v0, float u1, float v1, float u2, float v2, float lo,
+                                           float hi, float& umin, float& umax) {
+    umin = INFINITY; umax = -INFINITY;
+    auto vert = [&](float u, float v) {
+        if (v >= lo && v <= hi) { umin = fminf(umin, u); umax = fmaxf(umax, u); }
+    };
+    auto cross = [&](float pu, float pv, float qu, float qv) {
+        if (pv == qv) return;
+        const float k = (qu - pu) * __builtin_amdgcn_rcpf(qv - pv);
+        if ((pv - lo) * (qv - lo) <= 0.0f) { const float u = pu + (lo - pv) * k; umin = fminf(umin, u); umax = fmaxf(umax, u); }
+        if ((pv - hi) * (qv - hi) <= 0.0f) { const float u = pu + (hi - pv) * k; umin = fminf(umin, u); umax = fmaxf(umax, u); }
+    };
+    vert(u0, v0); vert(u1, v1); vert(u2, v2);
+    cross(u0, v0, u1, v1); cross(u1, v1, u2, v2); cross(u2, v2, u0, v0);
+    return umin <= umax;
+}
+
+// Shrinks a large face's candidate rectangle in its tile (tile-local [rx0, rx1] x
+// [ry0, ry1], origin ox, oy) to the pixels that can pass the inside test; returns false
+// when none can.  Bound used: the computed edge function differs from the exact one by
+// at most 5.7 u L (D + 1) (u = 2^-24, L the edge's length, D the largest coordinate
+// distance between a vertex and a rectangle pixel), so a passing pixel lies within
+// m = 8.01 u (D + 1) of every edge's half-plane — when the area's sign is certain
+// (|area| > 2 m L_max) — hence within g = m (1 + 3 D / r) of the triangle (r its
+// inradius, >= |area| / 2 / perimeter): the offset triangle is the triangle scaled by
+// 1 + m / r about its incentre, and no vertex is farther than 2 sqrt(2) D from it.
+// (Lengths are bounded by |dx| + |dy|; the products' and the subtraction's roundings
+// give 4 u (|A| |py - ay| + |B| |px - ax|) <= 5.7 u L (D + 1).)  The rectangle becomes the bounding box of the
+// triangle's part within g of the rectangle, widened by g + 1 pixel (crossing rounding).
+// Faces where g would exceed 4 pixels keep their rectangle.
+__device__ __forceinline__ bool shrink_rect(const float4& a, const float4& b, const float4& c, int ox, int oy,
+                                            int& rx0, int& rx1, int& ry0, int& ry1) {
+    const float X0 = (float)(ox + rx0), X1 = (float)(ox + rx1), Y0 = (float)(oy + ry0), Y1 = (float)(oy + ry1);
+    const float vx0 = fminf(fminf(a.x, b.x), c.x), vx1 = fmaxf(fmaxf(a.x, b.x), c.x);
+    const float vy0 = fminf(fminf(a.y, b.y), c.y), vy1 = fmaxf(fmaxf(a.y, b.y), c.y);
+    const float D = fmaxf(fmaxf(vx1 - X0, X1 - vx0), fmaxf(vy1 - Y0, Y1 - vy0));
+    const float area = fabsf(a.w);
+    // (comparisons false for NaN: such faces keep their rectangle)
+    if (!(D <= 1048576.0f) || !(area >= 1.0f) || !(area <= 0x1p40f)) return true;
+    const float l0 = fabsf(b.x - a.x) + fabsf(b.y - a.y), l1 = fabsf(c.x - b.x) + fabsf(c.y - b.y);
+    const float l2 = fabsf(a.x - c.x) + fabsf(a.y - c.y);
+    const float m = 8.01f * 0x1p-24f * (D + 1.0f);
+    if (!(area > 2.0f * m * fmaxf(fmaxf(l0, l1), l2) * 1.01f)) return true;
+    const float r = 0.5f * area / ((l0 + l1 + l2) * 1.01f);
+    const float g = m * (1.0f + 3.0f * D / r) * 1.01f;
+    if (!(g <= 4.0f)) return true;
+    float xmin, xmax, ymin, ymax;
+    if (!slab_range(a.x, a.y, b.x, b.y, c.x, c.y, Y0 - g, Y1 + g, xmin, xmax)) return false;
+    if (!slab_range(a.y, a.x, b.y, b.x, c.y, c.x, X0 - g, X1 + g, ymin, ymax)) return false;
+    const float w = g + 1.0f;
+    rx0 = max(rx0, (int)floorf(xmin - w) - ox); rx1 = min(rx1, (int)ceilf(xmax + w) - ox);
+    ry0 = max(ry0, (int)floorf(ymin - w) - oy); ry1 = min(ry1, (int)ceilf(ymax + w) - oy);
+    return rx0 <= rx1 && ry0 <= ry1;
+}
+
 // One tile, by one workgroup.  The tile's bin is taken BLOCK faces at a time: the faces
 // whose bbox meets the tile are compacted (ballot + popcount ranks, wave totals in LDS)
 // and their candidate pixel counts (bbox inside the tile) prefix-summed across the
@@ -685,9 +747,13 @@ __device__ __forceinline__ void raster_body(const RasterParams& p, const ShadePa
             const FaceRecord* r = p.rec + f;
             a = r->r0; b = r->r1; c = r->r2;
             const uint32_t lo = __float_as_uint(b.w), hi = __float_as_uint(c.w);
-            const int rx0 = max((int)(lo & 0xFFFFu) - ox, 0), rx1 = min((int)(hi & 0xFFFFu) - ox, TS - 1);
-            const int ry0 = max((int)(lo >> 16) - oy, 0), ry1 = min((int)(hi >> 16) - oy, TS - 1);
-            if (rx0 <= rx1 && ry0 <= ry1) {
+            int rx0 = max((int)(lo & 0xFFFFu) - ox, 0), rx1 = min((int)(hi & 0xFFFFu) - ox, TS - 1);
+            int ry0 = max((int)(lo >> 16) - oy, 0), ry1 = min((int)(hi >> 16) - oy, TS - 1);
+            // large rectangles: only the part the triangle can reach
+            bool keep = true;
+            if (rx0 <= rx1 && ry0 <= ry1 && (rx1 - rx0 + 1) * (ry1 - ry0 + 1) > kShrinkPixels)
+                keep = shrink_rect(a, b, c, ox, oy, rx0, rx1, ry0, ry1);
+            if (keep && rx0 <= rx1 && ry0 <= ry1) {
                 const uint32_t rw = rx1 - rx0 + 1;
                 cnt = rw * (ry1 - ry0 + 1);
                 rect = (uint32_t)rx0 | ((uint32_t)ry0 << 8) | (rw << 16);

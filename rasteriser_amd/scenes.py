@@ -261,6 +261,51 @@ def icosphere_scene(seed: int = 2, width: int = 1920, height: int = 1080, flat: 
     return scene, args
 
 
+def large_triangle_scene(seed: int = 9, n: int = 3000, size: int = 1024, near: float = 0.1,
+                         far: float = 100.0):
+    """Screen-space triangles far larger than a tile: a third with edges 20-600 px, a third
+    slivers (the third vertex within 1e-3..1 px of the line through the other two), a
+    third with a vertex up to 5,000 px off screen; eye depths and winding as in
+    screen_soup_scene.  Exercises the raster pass's reduction of large candidate
+    rectangles, whose exactness rests on the rounding bounds of the edge functions."""
+    rng = SplitMix64(0x5EED0000 + seed)
+    kind = (rng.next_u64(n) % np.uint64(3)).astype(np.int64)
+    c = rng.uniform(2 * n, 0.0, float(size)).reshape(n, 2)
+    ang = rng.uniform(n, 0.0, 2.0 * math.pi)
+    edge = rng.uniform(n, 20.0, 600.0)
+    jit = rng.uniform(2 * n, -0.5, 0.5).reshape(n, 2)
+    rad = edge / math.sqrt(3.0)
+    a = np.stack([ang, ang + 2.0 * math.pi / 3.0 + jit[:, 0], ang + 4.0 * math.pi / 3.0 + jit[:, 1]], 1)
+    xs = c[:, 0:1] + rad[:, None] * np.cos(a)
+    ys = c[:, 1:2] - rad[:, None] * np.sin(a)
+    # slivers: vertex 2 near the segment (0, 1), offset along its normal by 1e-3..1 px
+    sl = kind == 1
+    t = rng.uniform(n, 0.1, 0.9)
+    off = np.exp(rng.uniform(n, math.log(1e-3), 0.0)) * np.where(rng.uniform(n) < 0.5, -1.0, 1.0)
+    dx, dy = xs[:, 1] - xs[:, 0], ys[:, 1] - ys[:, 0]
+    ln = np.maximum(np.hypot(dx, dy), 1e-9)
+    xs[sl, 2] = (xs[:, 0] + t * dx - off * dy / ln)[sl]
+    ys[sl, 2] = (ys[:, 0] + t * dy + off * dx / ln)[sl]
+    # far vertices: vertex 0 pushed up to 5,000 px away
+    fa = kind == 2
+    far_d = rng.uniform(n, 500.0, 5000.0)
+    fang = rng.uniform(n, 0.0, 2.0 * math.pi)
+    xs[fa, 0] = (c[:, 0] + far_d * np.cos(fang))[fa]
+    ys[fa, 0] = (c[:, 1] + far_d * np.sin(fang))[fa]
+    d = rng.uniform(3 * n, 0.1, 100.0).reshape(n, 3)
+    z = (far + near) / (far - near) - 2.0 * far * near / ((far - near) * d)
+    verts = np.stack([xs, ys, z, 1.0 / d], axis=2).reshape(-1, 4).astype(np.float32)
+    tri = np.arange(3 * n, dtype=np.int64).reshape(n, 3)
+    back = rng.uniform(n) < 0.5
+    tri[back, 1], tri[back, 2] = tri[back, 2].copy(), tri[back, 1].copy()
+    faces = make_faces(tri, n=np.zeros_like(tri), t=None)
+    scene = Scene(verts, np.array([[0.0, 0.0, 1.0]], np.float32), np.zeros((1, 2), np.float32), faces,
+                  [MaterialSpec((1.0, 1.0, 1.0))],
+                  make_lights([[0.0, 0.0, -1.0]], [255.0 * math.pi], [[1.0, 1.0, 1.0]]),
+                  raster_space=True)
+    return scene, make_args(size, size, fovy=60.0, z_near=near, z_far=far)
+
+
 def screen_soup_scene(seed: int = 4, n: int = 4_000_000, size: int = 4096,
                       near: float = 0.1, far: float = 100.0):
     """n screen-space triangles: centroid uniform, edge 0.5-3 px, random angle, per-vertex

@@ -31,6 +31,17 @@ def test_tuning_and_overflow(ctx, tile, cap):
         assert gpu[3]["overflow_tiles"] > 0
 
 
+@pytest.mark.parametrize("tile,cap", [(8, 256), (16, 512), (32, 2048)])
+def test_large_triangles_and_slivers(ctx, tile, cap):
+    """Triangles much larger than a tile, slivers within 1e-3 px of a line and faces with a
+    vertex thousands of pixels off screen: the raster pass shrinks large candidate
+    rectangles to the triangle's reach, and the fragment count (every (face, pixel)
+    passing the inside test) must still equal the oracle's."""
+    scene, args = scenes.large_triangle_scene(n=600)
+    gpu, ref = check(ctx, scene, args, tuning=(tile, cap), label=f"large triangles tile {tile}")
+    assert ref[3]["fragments"] > 5_000_000
+
+
 def test_not_cleared_buffers(ctx):
     """draw_frame does not clear: the depth test reads the existing depth and pixels it
     does not win keep their old colour (drawing.cpp:119, renderer.cpp:107-108)."""
