@@ -1,0 +1,72 @@
+"""The C++ host layer on the GPU: the `renderer` command line (the reference's flags and
+program flow over librz, via the C++ draw_frame) renders an OBJ it loads itself, and
+its frame.png must equal the oracle's frame of the same scene byte for byte."""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import oracle
+from rasteriser_amd import scenes
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIBDIR = os.path.join(ROOT, "rasteriser_amd", "_lib")
+RENDERER = os.path.join(LIBDIR, "renderer")
+
+pytestmark = pytest.mark.gpu
+
+
+def write_obj(path, scene):
+    """Triangles with v//vn corners, one material of Kd 1 (decimals that read back exactly)."""
+    with open(path + ".mtl", "w") as f:
+        f.write("newmtl white\nKd 1 1 1\n")
+    with open(path, "w") as f:
+        f.write(f"mtllib {os.path.basename(path)}.mtl\n")
+        for v in scene.vertices:
+            f.write("v %.9g %.9g %.9g\n" % tuple(float(x) for x in v[:3]))
+        for n in scene.normals:
+            f.write("vn %.9g %.9g %.9g\n" % tuple(float(x) for x in n[:3]))
+        f.write("usemtl white\n")
+        for t in scene.faces:
+            v, n = t[0:3] + 1, t[3:6] + 1
+            f.write(f"f {v[0]}//{n[0]} {v[1]}//{n[1]} {v[2]}//{n[2]}\n")
+
+
+def write_lights(path, lights):
+    with open(path, "w") as f:
+        for L in lights:
+            f.write(",".join("%.9g" % float(x) for x in L[:7]) + "\n")
+
+
+def load_png(path):
+    L = C.CDLL(os.path.join(LIBDIR, "librz_host.so"))
+    L.rzh_load_image.argtypes = [C.c_char_p, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int), C.c_void_p]
+    W, H, Ch = C.c_int(), C.c_int(), C.c_int()
+    assert L.rzh_load_image(path.encode(), C.byref(W), C.byref(H), C.byref(Ch), None) == 0
+    out = np.zeros((H.value, W.value, Ch.value), np.uint8)
+    assert L.rzh_load_image(path.encode(), C.byref(W), C.byref(H), C.byref(Ch), out.ctypes.data) == 0
+    return out
+
+
+@pytest.mark.parametrize("flat", [False, True])
+def test_renderer_cli_frame_matches_oracle(tmp_path, flat):
+    if not os.path.exists(RENDERER):
+        pytest.skip("renderer not built")
+    scene, args = scenes.uv_sphere_scene()
+    args.flat = 1 if flat else 0
+    obj = str(tmp_path / "sphere.obj")
+    lights = str(tmp_path / "lights.csv")
+    write_obj(obj, scene)
+    write_lights(lights, scene.lights)
+    cmd = [RENDERER, "-o", obj, "--mats-dir", str(tmp_path) + "/", "-l", lights, "-x", str(args.image_width),
+           "-y", str(args.image_height), "--fovy", "60", "--near", "0.1", "--far", "100", "--out-dir",
+           str(tmp_path)] + (["-f"] if flat else [])
+    print("running", " ".join(cmd), flush=True)
+    r = subprocess.run(cmd, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    got = load_png(str(tmp_path / "frame.png"))
+    of, od, oids, counts, _ = oracle.draw_frame(scene, args)
+    assert (oids >= 0).sum() > 50_000
+    assert np.array_equal(got[:, :, :3], np.moveaxis(of, 0, 2)), "frame.png differs from the oracle's frame"
