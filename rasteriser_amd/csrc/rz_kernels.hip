@@ -977,11 +977,12 @@ __global__ __launch_bounds__(256) void k_bin(BinParams p) {
 // the overflow loop (the tile's visibility words were cleared by the bin pass and the
 // raster loop skipped it), so the frame needs no launch of its own for it.
 // Statistics are per-workgroup partial sums.
-// (minimum waves per SIMD in the launch bounds: 8 for the raster kernel (6 for 32-pixel
-// tiles, whose LDS allows no more), 6 for the fused one, 3 for the fused learned shader:
-// the most each reaches without register spills)
+// (minimum waves per SIMD in the launch bounds: 7 for the raster kernel on 16-pixel
+// tiles (8 for 8-pixel ones, 6 for 32-pixel tiles, whose LDS allows no more), 6 for the
+// fused one, 3 for the fused learned shader: the most each reaches without register
+// spills)
 template <int TS, int BLOCK>
-__global__ __launch_bounds__(BLOCK, TS == 32 ? 6 : 8) void k_raster(RasterParams p) {
+__global__ __launch_bounds__(BLOCK, TS == 32 ? 6 : (BLOCK == 256 ? 7 : 8)) void k_raster(RasterParams p) {
     const uint32_t xcd = blockIdx.x & 7u, step = gridDim.x >> 3;
     const uint32_t n = p.list_count[xcd * kCountStride];
     uint32_t frags = 0, cov = 0;
