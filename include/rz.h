@@ -268,7 +268,7 @@ rz_status rz_batch_graph_create(rz_ctx* ctx, const rz_batch_item* items, int32_t
                                 int32_t frames, rz_graph** out_graph);
 /* Chooses the tile size (8, 16, 32) and bin capacity for this workload by measuring:
  * one probe frame per tile size gives the largest per-tile face count, the capacity
- * is the next power of two that holds it (within a 4 GiB bin budget), and the tile
+ * is the next power of two that holds it (within a 24 GiB bin budget), and the tile
  * size with the fastest timed frames wins.  The choice is stored in the context
  * (as rz_set_tuning would) and returned. */
 rz_status rz_autotune(rz_ctx* ctx, const rz_batch_item* items, int32_t num_items, uint32_t flags,

@@ -1364,7 +1364,9 @@ rz_status rz_autotune(rz_ctx* ctx, const rz_batch_item* items, int32_t n, uint32
                       int32_t* bin_capacity, float* ms_per_frame) {
     if (!ctx) return RZ_ERR_INVALID_ARG;
     const int32_t keep_ts = ctx->tile_size, keep_cap = ctx->bin_cap;
-    const size_t kBinBudget = (size_t)4 << 30;   // bytes of bins the tuner may choose
+    // bytes of bins the tuner may choose: a twelfth of the 288 GB of HBM (bins are only
+    // touched where written, so a large capacity costs address space, not traffic)
+    const size_t kBinBudget = (size_t)24 << 30;
     int best_ts = keep_ts, best_cap = keep_cap;
     float best_ms = 0.f;
     rz_status st = RZ_OK;
