@@ -71,7 +71,11 @@ struct rz_mesh {
     // the faces' indices in binning order: sorted by the Morton code of their centroid
     // (the bin pass's workgroups then cover compact screen regions whatever the input
     // order; visibility does not depend on it)
-    BinFace* bin_faces = nullptr;
+    int4* bin_face_v = nullptr;       // (v0, v1, v2, face)
+    int4* bin_face_attr = nullptr;    // (n0, n1, n2, material), (t0, t1, t2, 0)
+    // every face has the same normal, uv (-1: none) and material: shading takes them
+    // from here and the bin pass gathers nothing (uniform = 1)
+    int32_t uniform = 0, uniform_n = 0, uniform_t = -1, uniform_m = 0;
     DevMaterial* mats = nullptr;
     uint32_t* texels = nullptr;
     float* luts = nullptr;
@@ -326,11 +330,14 @@ void fill_frame(const rz_mesh* mesh, rz_light* lights, int32_t nl, const rz_args
 
     BinParams& bp = L.bp;
     std::memset(&bp, 0, sizeof bp);
-    bp.faces = mesh->bin_faces;
+    bp.face_v = mesh->bin_face_v;
+    bp.face_attr = mesh->bin_face_attr;
     bp.raster = vp.raster;
     bp.cnormal = vp.cnormal;
     bp.uvs = mesh->uvs;
     bp.flat = a.flat != 0;
+    const bool uniform = mesh->uniform && !a.flat;
+    bp.gather = uniform ? 0 : 1;
     bp.rec = (FaceRecord*)buf[kBufRec];
     bp.tile_count = vp.tile_count;
     bp.bins = (uint32_t*)buf[kBufBins];
@@ -365,6 +372,12 @@ void fill_frame(const rz_mesh* mesh, rz_light* lights, int32_t nl, const rz_args
     sp.vis = rp.vis;
     sp.rec = bp.rec;
     sp.mats = mesh->mats; sp.texels = mesh->texels; sp.luts = mesh->luts;
+    sp.uniform = uniform ? 1 : -1;
+    sp.uniform_n = mesh->uniform_n;
+    sp.uniform_t = mesh->uniform_t;
+    sp.uniform_m = mesh->uniform_m;
+    sp.cnormal = vp.cnormal;
+    sp.uvs = mesh->uvs;
     sp.ftx = mesh->ftx; sp.fty = mesh->fty; sp.ftz = mesh->ftz;
     std::memcpy(sp.normal_mat, mats.normal, 64);
     sp.part_cov = part + lay.off_cov;
@@ -1078,7 +1091,29 @@ rz_status rz_mesh_create(rz_ctx* ctx, const rz_mesh_desc* d, rz_mesh** out) {
         }
         if ((st = upload(ctx, m, &m->uvs, uv.data(), (size_t)m->nt * 8)) != RZ_OK) return cleanup(st);
     }
-    if ((st = upload(ctx, m, &m->bin_faces, faces.data(), sizeof(BinFace) * F)) != RZ_OK) return cleanup(st);
+    {
+        std::vector<int4> fv(F), fa(2 * F);
+        for (size_t i = 0; i < F; ++i) {
+            fv[i] = faces[i].q0;
+            fa[2 * i] = faces[i].q1;
+            fa[2 * i + 1] = faces[i].q2;
+        }
+        if ((st = upload(ctx, m, &m->bin_face_v, fv.data(), 16 * F)) != RZ_OK) return cleanup(st);
+        if ((st = upload(ctx, m, &m->bin_face_attr, fa.data(), 32 * F)) != RZ_OK) return cleanup(st);
+        // (renumbered indices: a mesh whose faces all share one normal, uv and material)
+        bool same = F > 0;
+        for (size_t i = 0; i < F && same; ++i) {
+            const int4 a = faces[i].q1, b = faces[i].q2, a0 = faces[0].q1, b0 = faces[0].q2;
+            same = a.x == a0.x && a.y == a0.x && a.z == a0.x && a.w == a0.w && b.x == b0.x && b.y == b0.x &&
+                   b.z == b0.x;
+        }
+        if (same) {
+            m->uniform = 1;
+            m->uniform_n = faces[0].q1.x;
+            m->uniform_t = faces[0].q2.x;
+            m->uniform_m = faces[0].q1.w;
+        }
+    }
     {
         // materials: RGBA8 texel pool plus one 256-entry normalisation table each
         std::vector<DevMaterial> dm(m->nm);

@@ -82,18 +82,20 @@ struct VertexParams {
     float Wf, Hf;
 };
 
-// A face's indices in the bin pass's order (48 bytes): (v0, v1, v2, face), (n0, n1, n2,
-// material), (t0, t1, t2, 0) with t = -1 for a corner without a uv.
+// A face's indices in the bin pass's order, in two arrays: (v0, v1, v2, face), and
+// (n0, n1, n2, material), (t0, t1, t2, 0) with t = -1 for a corner without a uv.
 struct BinFace {
     int4 q0, q1, q2;
 };
 
 struct BinParams {
-    const BinFace* faces;       // in binning order
+    const int4* face_v;         // (v0, v1, v2, face) in binning order
+    const int4* face_attr;      // (n0, n1, n2, material), (t0, t1, t2, 0) in binning order
     const float4* raster;
     const float4* cnormal;      // camera-space normals (flat: one per face)
     const float2* uvs;
     int32_t flat;
+    int32_t gather;             // 0: every face has the same normal, uv and material (ShadeParams::uniform)
     FaceRecord* rec;
     uint32_t* tile_count;
     uint32_t* bins;
@@ -163,6 +165,11 @@ struct ShadeParams {
     const unsigned long long* vis;
     const FaceRecord* rec;
     const DevMaterial* mats;
+    // uniform >= 0: every face shares normal uniform_n (camera space: cnormal), uv
+    // uniform_t (-1: none) and material uniform_m, and the records hold no copies
+    int32_t uniform, uniform_n, uniform_t, uniform_m;
+    const float4* cnormal;
+    const float2* uvs;
     const uint32_t* texels;
     const float* luts;
     const float* ftx; const float* fty; const float* ftz;   // model-space face tangents

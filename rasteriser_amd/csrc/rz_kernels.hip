@@ -132,16 +132,22 @@ __device__ __forceinline__ void bin_body(const BinParams& p, int blk) {
     bool front = false;
     uint32_t tx0 = 1, tx1 = 0, ty0 = 1, ty1 = 0;   // empty tile rectangle
     if (i < p.nf) {
-        const BinFace q = p.faces[i];
-        f = q.q0.w;
-        const float4 a = p.raster[q.q0.x], b = p.raster[q.q0.y], c = p.raster[q.q0.z];
+        const int4 q0 = p.face_v[i];
+        f = q0.w;
+        const float4 a = p.raster[q0.x], b = p.raster[q0.y], c = p.raster[q0.z];
         // the shading attributes are read beside the vertices, before the cull decides
-        // whether they are kept
-        const float4 n0 = p.cnormal[p.flat ? f : q.q1.x], n1 = p.cnormal[p.flat ? f : q.q1.y];
-        const float4 n2 = p.cnormal[p.flat ? f : q.q1.z];
-        const float2 t0 = q.q2.x >= 0 ? p.uvs[q.q2.x] : make_float2(0.f, 0.f);
-        const float2 t1 = q.q2.y >= 0 ? p.uvs[q.q2.y] : make_float2(0.f, 0.f);
-        const float2 t2 = q.q2.z >= 0 ? p.uvs[q.q2.z] : make_float2(0.f, 0.f);
+        // whether they are kept (none for a mesh whose faces all share them)
+        float4 n0 = make_float4(0.f, 0.f, 0.f, 0.f), n1 = n0, n2 = n0;
+        float2 t0 = make_float2(0.f, 0.f), t1 = t0, t2 = t0;
+        int32_t mat = 0;
+        if (p.gather) {
+            const int4 q1 = p.face_attr[2 * i], q2 = p.face_attr[2 * i + 1];
+            n0 = p.cnormal[p.flat ? f : q1.x]; n1 = p.cnormal[p.flat ? f : q1.y]; n2 = p.cnormal[p.flat ? f : q1.z];
+            if (q2.x >= 0) t0 = p.uvs[q2.x];
+            if (q2.y >= 0) t1 = p.uvs[q2.y];
+            if (q2.z >= 0) t2 = p.uvs[q2.z];
+            mat = q1.w;
+        }
         const float s = signed_area(a.x, a.y, b.x, b.y, c.x, c.y);
         front = (s > 0.0f) != (p.wind_cw != 0);
         FaceRecord* r = p.rec + f;
@@ -153,10 +159,12 @@ __device__ __forceinline__ void bin_body(const BinParams& p, int blk) {
             r->r1 = make_float4(b.x, b.y, b.z, __uint_as_float(tlx | (tly << 16)));
             r->r2 = make_float4(c.x, c.y, c.z, __uint_as_float(brx | (bry << 16)));
             r->r3 = make_float4(a.w, b.w, c.w, 0.0f);
-            r->r4 = make_float4(n0.x, n0.y, n0.z, t0.x);
-            r->r5 = make_float4(n1.x, n1.y, n1.z, t0.y);
-            r->r6 = make_float4(n2.x, n2.y, n2.z, t1.x);
-            r->r7 = make_float4(t1.y, t2.x, t2.y, __int_as_float(q.q1.w));
+            if (p.gather) {
+                r->r4 = make_float4(n0.x, n0.y, n0.z, t0.x);
+                r->r5 = make_float4(n1.x, n1.y, n1.z, t0.y);
+                r->r6 = make_float4(n2.x, n2.y, n2.z, t1.x);
+                r->r7 = make_float4(t1.y, t2.x, t2.y, __int_as_float(mat));
+            }
             if (tlx <= brx && tly <= bry) {
                 tx0 = tlx >> p.ts_log2; tx1 = brx >> p.ts_log2;
                 ty0 = tly >> p.ts_log2; ty1 = bry >> p.ts_log2;
@@ -366,7 +374,17 @@ __device__ __forceinline__ bool shade_pixel(const ShadeParams& p, const float* s
         const uint32_t f = (uint32_t)key;
         const FaceRecord& r = p.rec[f];
         const float4 a = r.r0, b = r.r1, c = r.r2, w = r.r3;
-        const float4 g0 = r.r4, g1 = r.r5, g2 = r.r6, g3 = r.r7;
+        float4 g0, g1, g2, g3;   // the record's attribute words (see FaceRecord)
+        if (p.uniform >= 0) {    // uniform over the launch
+            const float4 n = p.cnormal[p.uniform_n];
+            const float2 t = p.uniform_t >= 0 ? p.uvs[p.uniform_t] : make_float2(0.f, 0.f);
+            g0 = make_float4(n.x, n.y, n.z, t.x);
+            g1 = make_float4(n.x, n.y, n.z, t.y);
+            g2 = make_float4(n.x, n.y, n.z, t.x);
+            g3 = make_float4(t.y, t.x, t.y, __int_as_float(p.uniform_m));
+        } else {
+            g0 = r.r4; g1 = r.r5; g2 = r.r6; g3 = r.r7;
+        }
         const float pxf = (float)px, pyf = (float)py;
         // barycentric and depth, exactly as the raster pass computed them
         const float e0 = edge(pxf, pyf, b.x, b.y, c.x, c.y);
