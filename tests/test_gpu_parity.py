@@ -54,6 +54,36 @@ def test_not_cleared_buffers(ctx):
     check(ctx, scene, args, frame=frame, depth=depth, label="not cleared")
 
 
+@pytest.mark.parametrize("case", ["all_back", "off_screen", "one_pixel", "huge_coords"])
+def test_degenerate_frames(ctx, case):
+    """Frames with nothing or almost nothing to draw: (almost) every face turned to face
+    away, every face off screen, a 1x1 image, and a third of the vertices at raster
+    coordinates of 10^9 pixels."""
+    from rasteriser_amd.scene import Scene
+    scene, args = scenes.large_triangle_scene(n=300, size=256)
+    v = scene.vertices.copy()
+    faces = scene.faces.copy()
+    if case == "all_back":
+        x, y = v[faces[:, 0:3], 0].astype(np.float64), v[faces[:, 0:3], 1].astype(np.float64)
+        s2 = (x[:, 0] * y[:, 1] - x[:, 1] * y[:, 0] + x[:, 1] * y[:, 2] - x[:, 2] * y[:, 1]
+              + x[:, 2] * y[:, 0] - x[:, 0] * y[:, 2])
+        flip = s2 < 0   # front under the raster y flip: wind them the other way
+        faces[flip, 1], faces[flip, 2] = faces[flip, 2].copy(), faces[flip, 1].copy()
+    elif case == "off_screen":
+        v[:, 0] += 100_000.0
+    elif case == "one_pixel":
+        args = scenes.make_args(1, 1, fovy=60.0, z_near=0.1, z_far=100.0)
+        v[:, 0:2] *= 1.0 / 256.0
+    elif case == "huge_coords":
+        v[::3, 0:2] *= 1e7
+    scene = Scene(v, scene.normals, scene.uvs, faces, scene.materials, scene.lights, raster_space=True)
+    gpu, ref = check(ctx, scene, args, label=f"degenerate {case}")
+    if case == "off_screen":
+        assert (gpu[2] < 0).all()
+    if case == "all_back":
+        assert gpu[3]["front_triangles"] < 10
+
+
 def test_wind_clockwise(ctx):
     scene, args = scenes.uv_sphere_scene(width=256, height=256)
     args.wind_clockwise = 1
