@@ -628,8 +628,10 @@ __device__ __forceinline__ bool fragment_key(float pxf, float pyf, float x0, flo
     return true;
 }
 
-// Candidate rectangles larger than this are shrunk first (shrink_rect).
+// Candidate rectangles larger than this are shrunk first (shrink_rect): single frames,
+// and batches (whose triangle soups make large rectangles the common case).
 constexpr int kShrinkPixels = 128;
+constexpr int kShrinkPixelsBatch = 32;
 
 // Range of T ∩ {lo <= v <= hi} along the other axis (u), T the triangle (u_k, v_k):
 // the vertices inside the slab and the edges' crossings of its two lines.  Returns false
@@ -702,7 +704,7 @@ __device__ __forceinline__ bool shrink_rect(const float4& a, const float4& b, co
 // deferred shade, without the round trip of the visibility buffer through HBM).  Tiles
 // whose bin overflowed are skipped: the overflow loop fills their visibility words and
 // k_shade_tiles shades them.
-template <int TS, int BLOCK, bool FUSED, int MODE>
+template <int TS, int BLOCK, bool FUSED, int MODE, int SHRINK = kShrinkPixels>
 __device__ __forceinline__ void raster_body(const RasterParams& p, const ShadeParams* sp, const float* s_lut,
                                             const DevMaterial* s_mats, const uint16_t* s_net, int tile,
                                             uint32_t& frags, uint32_t& cov) {
@@ -769,7 +771,7 @@ __device__ __forceinline__ void raster_body(const RasterParams& p, const ShadePa
             int ry0 = max((int)(lo >> 16) - oy, 0), ry1 = min((int)(hi >> 16) - oy, TS - 1);
             // large rectangles: only the part the triangle can reach
             bool keep = true;
-            if (rx0 <= rx1 && ry0 <= ry1 && (rx1 - rx0 + 1) * (ry1 - ry0 + 1) > kShrinkPixels)
+            if (rx0 <= rx1 && ry0 <= ry1 && (rx1 - rx0 + 1) * (ry1 - ry0 + 1) > SHRINK)
                 keep = shrink_rect(a, b, c, ox, oy, rx0, rx1, ry0, ry1);
             if (keep && rx0 <= rx1 && ry0 <= ry1) {
                 const uint32_t rw = rx1 - rx0 + 1;
@@ -1112,7 +1114,7 @@ __global__ __launch_bounds__(BLOCK, MODE == kShadeMlp ? 3 : 6) void k_raster_sha
             if (k >= n) break;
             const uint32_t e = bl.lists[(size_t)x * bl.stride + k];
             const uint32_t m = e >> kListMeshShift;
-            raster_body<TS, BLOCK, true, MODE>(items[m], &sitems[m], sitems[m].luts, nullptr, s_net,
+            raster_body<TS, BLOCK, true, MODE, kShrinkPixelsBatch>(items[m], &sitems[m], sitems[m].luts, nullptr, s_net,
                                                (int)(e & kListTileMask), frags, cov);
         }
     }
