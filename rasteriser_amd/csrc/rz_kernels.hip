@@ -346,6 +346,18 @@ __device__ __forceinline__ void stage_mats(const ShadeParams& p, DevMaterial* s_
     for (int e = tid; e < n; e += nthreads) dst[e] = src[e];
 }
 
+// A pixel no fragment won: 0 / 1.0 when the buffers are known cleared (unless the bin
+// launch already wrote them), id -1; otherwise untouched (the reference never clears).
+__device__ __forceinline__ void clear_pixel(const ShadeParams& p, size_t i) {
+    if (p.pre_cleared) return;
+    if (p.cleared) {
+        const size_t np = (size_t)p.W * p.H;
+        p.frame[i] = 0; p.frame[i + np] = 0; p.frame[i + 2 * np] = 0;
+        p.depth[i] = 1.0f;
+    }
+    if (p.ids) p.ids[i] = -1;
+}
+
 // Shades one pixel from its visibility word; every lane of the wave calls it together
 // (the learned shader and sample collection work per wave).  Writes frame, depth and
 // ids; returns whether the pixel has a winning fragment.
@@ -363,13 +375,7 @@ __device__ __forceinline__ bool shade_pixel(const ShadeParams& p, const float* s
     const bool hit = valid && key != kEmptyKey;
     float feat[kMlpIn];
     uint8_t rgb[3] = {0, 0, 0};
-    if (valid && !hit && !p.pre_cleared) {
-        if (p.cleared) {
-            p.frame[i] = 0; p.frame[i + np] = 0; p.frame[i + 2 * np] = 0;
-            p.depth[i] = 1.0f;
-        }
-        if (p.ids) p.ids[i] = -1;
-    }
+    if (valid && !hit) clear_pixel(p, i);
     if (hit) {
         const uint32_t f = (uint32_t)key;
         const FaceRecord& r = p.rec[f];
@@ -854,7 +860,53 @@ __device__ __forceinline__ void raster_body(const RasterParams& p, const ShadePa
         __syncthreads();       // ranks and wave totals are rewritten by the next chunk
     }
 
-    if (FUSED) {
+    if (FUSED && PPT > 1) {
+        // Several pixels per thread: the covered pixels are compacted first, so that a
+        // thread shades one of them per pass instead of running the shading chain once
+        // per pixel it owns (a sparse tile then costs one chain, not PPT).
+        uint16_t* s_in = reinterpret_cast<uint16_t*>(s_raw) + (kNet ? wave : 0) * 64 * 16;
+        float* s_out = reinterpret_cast<float*>(s_raw + NW * 64 * 16 * 2) + (kNet ? wave : 0) * 64 * 4;
+        __shared__ uint16_t s_hits[PPT > 1 ? NPX : 1];
+        uint32_t nh = 0;   // uniform over the workgroup
+#pragma unroll
+        for (int j = 0; j < PPT; ++j) {
+            const int l = tid + j * BLOCK;
+            const uint32_t x = ox + (l % TS), y = oy + (l / TS);
+            const bool valid = x < p.W && y < p.H;
+            const bool hit = valid && s_key[l] != kEmptyKey;
+            if (valid && !hit) clear_pixel(*sp, (size_t)y * p.W + x);
+            const unsigned long long bal = __ballot(hit);
+            if (lane == 0) s_wsum[wave] = (uint32_t)__popcll(bal);
+            __syncthreads();
+            uint32_t off = nh, tot = nh;
+#pragma unroll
+            for (int w = 0; w < NW; ++w) {
+                const uint32_t v = s_wsum[w];
+                if (w < wave) off += v;
+                tot += v;
+            }
+            __syncthreads();   // s_wsum is rewritten by the next pass
+            if (hit) s_hits[off + (uint32_t)__popcll(bal & below)] = (uint16_t)l;
+            nh = tot;
+        }
+        __syncthreads();
+        uint32_t nrec = 0;
+        for (uint32_t k0 = 0; k0 < nh; k0 += BLOCK) {   // uniform over the workgroup
+            const uint32_t k = k0 + (uint32_t)tid;
+            const bool valid = k < nh;
+            const int l = valid ? (int)s_hits[k] : 0;
+            const uint32_t x = ox + (l % TS), y = oy + (l / TS);
+            uint32_t rec[6];
+            const bool hit = shade_pixel<MODE>(*sp, s_lut, s_mats, s_net, x, y, valid,
+                                               valid ? s_key[l] : kEmptyKey, lane, s_in, s_out, rec);
+            cov += hit ? 1u : 0u;
+            if (MODE == kShadeFeat)
+                nrec = region_append<NW>(hit, rec, y * p.W + x,
+                                         reinterpret_cast<uint4*>(sp->samples) + 2 * (size_t)tile * sp->rec_region,
+                                         nrec, tid, s_wfeat);
+        }
+        if (MODE == kShadeFeat && tid == 0) sp->rec_counts[(size_t)tile * kCountStride] = nrec;
+    } else if (FUSED) {
         uint16_t* s_in = reinterpret_cast<uint16_t*>(s_raw) + (kNet ? wave : 0) * 64 * 16;
         float* s_out = reinterpret_cast<float*>(s_raw + NW * 64 * 16 * 2) + (kNet ? wave : 0) * 64 * 4;
         uint32_t nrec = 0;
