@@ -127,6 +127,42 @@ def test_collect_train_and_shade(ctx):
         mesh.close()
 
 
+def test_learned_shader_with_narrow_workgroups(ctx):
+    """At 2048x2048 (16,384 tiles) the raster pass runs 64-thread workgroups with several
+    pixels per thread and compacts the covered pixels before shading: the learned
+    shader's fragment records must still be the collected inputs, fragment for fragment,
+    in both pipelines."""
+    scene, args = scenes.learned_shading_scene(width=2048, height=2048, subdivisions=4, map_size=128)
+    mesh = ctx.upload(scene)
+    cap = 2_000_000
+    samples = ctx.empty((cap, rz.RZ_MLP_SAMPLE_FLOATS), np.float32)
+    try:
+        frame_p, depth_p, ids_p, _ = ctx.draw(mesh, scene.lights_copy(), args, ids=True)
+        covered = ids_p >= 0
+        n = ctx.mlp_collect(mesh, scene.lights_copy(), args, samples, 0)
+        assert n == int(covered.sum()) > 100_000
+        feat = samples.download()[:n, :12]
+        ctx.mlp_init(3)
+        ctx.mlp_train(samples, n, steps=50, batch=4096, lr=3e-3, seed=4)
+        y = ctx.mlp_infer(feat)
+        want = np.sort(_colour_bytes(y).astype(np.int64).view([("", np.int64)] * 3).ravel())
+        for fused in (True, False):
+            ctx.set_fused_shade(fused)
+            try:
+                frame_m, depth_m, ids_m, _ = ctx.draw(mesh, scene.lights_copy(), args, ids=True,
+                                                      flags=rz.RZ_FLAG_MLP_SHADE)
+            finally:
+                ctx.set_fused_shade(True)
+            assert np.array_equal(ids_m, ids_p)
+            assert np.array_equal(depth_m.view(np.uint32), depth_p.view(np.uint32))
+            assert (frame_m[:, ~covered] == 0).all()
+            got = np.sort(frame_m[:, covered].T.astype(np.int64).copy().view([("", np.int64)] * 3).ravel())
+            assert np.array_equal(got, want), f"fused={fused}"
+    finally:
+        samples.close()
+        mesh.close()
+
+
 def test_mlp_flag_without_weights_is_an_error():
     c = rz.Context(0)
     try:
