@@ -42,11 +42,12 @@ def test_large_triangles_and_slivers(ctx, tile, cap):
     assert ref[3]["fragments"] > 5_000_000
 
 
-def test_not_cleared_buffers(ctx):
+@pytest.mark.parametrize("W,H", [(300, 200), (2048, 2048)])
+def test_not_cleared_buffers(ctx, W, H):
     """draw_frame does not clear: the depth test reads the existing depth and pixels it
-    does not win keep their old colour (drawing.cpp:119, renderer.cpp:107-108)."""
-    scene, args = scenes.uv_sphere_scene(width=300, height=200)
-    W, H = 300, 200
+    does not win keep their old colour (drawing.cpp:119, renderer.cpp:107-108).  2048x2048
+    has 16,384 tiles: 64-thread workgroups that compact the covered pixels."""
+    scene, args = scenes.uv_sphere_scene(width=W, height=H)
     rng = np.random.default_rng(7)
     frame = rng.integers(0, 256, (3, H, W), dtype=np.uint8)
     depth = rng.uniform(0.85, 1.0, (H, W)).astype(np.float32)
