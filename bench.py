@@ -65,7 +65,8 @@ def algorithmic_bytes(counts, scene, args, fused, cleared=True):
     raster_in = 48 * Tf + 4 * P16 + depth_in
     shade_out = 7 * W * H + 108 * Tvis + 4 * tex + 40 * L   # frame + depth, winners' data, texture, lights
     out = {
-        "vertex": 28 * V + 24 * Vn,
+        # (screen-space input bypasses the position transform: the bin pass reads it as given)
+        "vertex": (0 if getattr(scene, "raster_space", False) else 28 * V) + 24 * Vn,
         "bin": 12 * T + 16 * V + 4 * P16,
     }
     if fused:

@@ -253,7 +253,7 @@ FrameLayout layout_frame(const rz_mesh* mesh, const rz_args& a, int ts, int cap,
     L.n_part = L.off_cov + (batch ? 0
                                   : L.fused ? (size_t)raster_grid + tiles_grid
                                             : (size_t)shade_blocks_x(L.W) * shade_blocks_y(L.H));
-    L.bytes[kBufRaster] = (size_t)mesh->nv * 16;
+    L.bytes[kBufRaster] = mesh->raster_space ? 0 : (size_t)mesh->nv * 16;   // (raster space: read as given)
     L.bytes[kBufCnormal] = (size_t)L.nfn * 16;
     L.bytes[kBufRec] = (size_t)mesh->nf * sizeof(FaceRecord);
     L.bytes[kBufTiles] = ((size_t)L.ntiles + 8) * 4 * kCountStride;   // + the 8 list counters
@@ -317,7 +317,6 @@ void fill_frame(const rz_mesh* mesh, rz_light* lights, int32_t nl, const rz_args
     std::memcpy(vp.camera, mats.camera, 64);
     std::memcpy(vp.normal, mats.normal, 64);
     vp.px = mesh->px; vp.py = mesh->py; vp.pz = mesh->pz;
-    vp.raster_in = mesh->raster_space ? mesh->rin : nullptr;
     vp.nx = a.flat ? mesh->fnx : mesh->nx;
     vp.ny = a.flat ? mesh->fny : mesh->ny;
     vp.nz = a.flat ? mesh->fnz : mesh->nz;
@@ -326,6 +325,11 @@ void fill_frame(const rz_mesh* mesh, rz_light* lights, int32_t nl, const rz_args
     vp.tile_count = (uint32_t*)buf[kBufTiles];
     vp.counters = counters;
     vp.nv = mesh->nv; vp.nn = lay.nfn; vp.ntiles = lay.ntiles + 8;   // clears the list counters too
+    // raster-space vertices need no vertex stage: the bin pass reads them where they are
+    if (mesh->raster_space) {
+        vp.nv = 0;
+        vp.raster = mesh->rin;
+    }
     vp.Wf = (float)lay.W; vp.Hf = (float)lay.H;
 
     BinParams& bp = L.bp;

@@ -72,7 +72,6 @@ struct VertexParams {
     float camera[16];
     float normal[16];
     const float* px; const float* py; const float* pz;   // model positions (SoA)
-    const float4* raster_in;                             // raster-space input, or null
     const float* nx; const float* ny; const float* nz;   // normals to transform (SoA)
     float4* raster;                                      // out: (x_px, y_px, ndc_z, 1/w)
     float4* cnormal;                                     // out: camera-space normal (xyz)

@@ -53,24 +53,20 @@ __device__ __forceinline__ void store_block_sum(uint32_t v, uint32_t* out, int t
 // ---------------------------------------------------------------------------------
 __device__ __forceinline__ void vertex_body(const VertexParams& p, int blk) {
     const int i = blk * 256 + threadIdx.x;
-    if (i < p.nv) {
+    if (i < p.nv) {   // (raster-space meshes have nv = 0: the bin pass reads their vertices)
+        const float x = p.px[i], y = p.py[i], z = p.pz[i];
+        const float* m = p.camera;
+        // GLM mat4*vec4 (pairwise), w = 1
+        const float cx = (m[0] * x + m[4] * y) + (m[8] * z + m[12] * 1.0f);
+        const float cy = (m[1] * x + m[5] * y) + (m[9] * z + m[13] * 1.0f);
+        const float cz = (m[2] * x + m[6] * y) + (m[10] * z + m[14] * 1.0f);
+        const float cw = (m[3] * x + m[7] * y) + (m[11] * z + m[15] * 1.0f);
+        const float nx = cx / cw, ny = cy / cw, nz = cz / cw, nw = 1.f / cw;
         float4 r;
-        if (p.raster_in) {
-            r = p.raster_in[i];
-        } else {
-            const float x = p.px[i], y = p.py[i], z = p.pz[i];
-            const float* m = p.camera;
-            // GLM mat4*vec4 (pairwise), w = 1
-            const float cx = (m[0] * x + m[4] * y) + (m[8] * z + m[12] * 1.0f);
-            const float cy = (m[1] * x + m[5] * y) + (m[9] * z + m[13] * 1.0f);
-            const float cz = (m[2] * x + m[6] * y) + (m[10] * z + m[14] * 1.0f);
-            const float cw = (m[3] * x + m[7] * y) + (m[11] * z + m[15] * 1.0f);
-            const float nx = cx / cw, ny = cy / cw, nz = cz / cw, nw = 1.f / cw;
-            r.x = 0.5f * (nx + 1.0f) * p.Wf;
-            r.y = 0.5f * (-ny + 1.0f) * p.Hf;
-            r.z = nz;
-            r.w = nw;
-        }
+        r.x = 0.5f * (nx + 1.0f) * p.Wf;
+        r.y = 0.5f * (-ny + 1.0f) * p.Hf;
+        r.z = nz;
+        r.w = nw;
         p.raster[i] = r;
     }
     if (i < p.nn) {
