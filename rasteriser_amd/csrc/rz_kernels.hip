@@ -128,16 +128,21 @@ __device__ __forceinline__ void bin_body(const BinParams& p, int blk) {
     bool front = false;
     uint32_t tx0 = 1, tx1 = 0, ty0 = 1, ty1 = 0;   // empty tile rectangle
     if (i < p.nf) {
+        // the face's attribute indices are read beside its vertex indices, and the shading
+        // attributes beside the vertices, before the cull decides whether they are kept
+        // (none for a mesh whose faces all share them): two dependent round trips, not three
         const int4 q0 = p.face_v[i];
+        int4 q1 = make_int4(0, 0, 0, 0), q2 = q1;
+        if (p.gather) {
+            q1 = p.face_attr[2 * i];
+            q2 = p.face_attr[2 * i + 1];
+        }
         f = q0.w;
         const float4 a = p.raster[q0.x], b = p.raster[q0.y], c = p.raster[q0.z];
-        // the shading attributes are read beside the vertices, before the cull decides
-        // whether they are kept (none for a mesh whose faces all share them)
         float4 n0 = make_float4(0.f, 0.f, 0.f, 0.f), n1 = n0, n2 = n0;
         float2 t0 = make_float2(0.f, 0.f), t1 = t0, t2 = t0;
         int32_t mat = 0;
         if (p.gather) {
-            const int4 q1 = p.face_attr[2 * i], q2 = p.face_attr[2 * i + 1];
             n0 = p.cnormal[p.flat ? f : q1.x]; n1 = p.cnormal[p.flat ? f : q1.y]; n2 = p.cnormal[p.flat ? f : q1.z];
             if (q2.x >= 0) t0 = p.uvs[q2.x];
             if (q2.y >= 0) t1 = p.uvs[q2.y];
@@ -341,6 +346,33 @@ __device__ __forceinline__ void stage_mats(const ShadeParams& p, DevMaterial* s_
     uint32_t* dst = reinterpret_cast<uint32_t*>(s_mats);
     for (int e = tid; e < n; e += nthreads) dst[e] = src[e];
 }
+
+// stage_mats in two halves, so that other reads can be issued between the loads and the
+// LDS stores (which wait for them): each thread holds up to kPasses words meanwhile.
+template <int BLOCK>
+struct MatStage {
+    static constexpr int kWords = kLdsMats * (int)(sizeof(DevMaterial) / 4);
+    static constexpr int kPasses = (kWords + BLOCK - 1) / BLOCK;
+    int n;
+    uint32_t v[kPasses];
+    __device__ __forceinline__ MatStage(const ShadeParams& p, int tid) {
+        n = (p.nmat < kLdsMats ? p.nmat : kLdsMats) * (int)(sizeof(DevMaterial) / 4);
+        const uint32_t* src = reinterpret_cast<const uint32_t*>(p.mats);
+#pragma unroll
+        for (int j = 0; j < kPasses; ++j) {
+            const int e = tid + j * BLOCK;
+            v[j] = e < n ? src[e] : 0u;
+        }
+    }
+    __device__ __forceinline__ void store(DevMaterial* s_mats, int tid) const {
+        uint32_t* dst = reinterpret_cast<uint32_t*>(s_mats);
+#pragma unroll
+        for (int j = 0; j < kPasses; ++j) {
+            const int e = tid + j * BLOCK;
+            if (e < n) dst[e] = v[j];
+        }
+    }
+};
 
 // A pixel no fragment won: 0 / 1.0 when the buffers are known cleared (unless the bin
 // launch already wrote them), id -1; otherwise untouched (the reference never clears).
@@ -1049,15 +1081,44 @@ __global__ __launch_bounds__(256) void k_bin(BinParams p) {
 // tiles (8 for 8-pixel ones, 6 for 32-pixel tiles, whose LDS allows no more), 6 for the
 // fused one, 3 for the fused learned shader: the most each reaches without register
 // spills)
+// A resident workgroup's walk over its XCD's tile list: list entries k0, k0 + step, ...
+// below the list's count.  The first entry is read beside the count rather than after
+// it, one dependent memory round trip less before the first tile starts (an entry past
+// the count is never used; k0 < ntiles keeps the read inside the list).
+// A read the compiler issues where it is written, ahead of the caller's staging loads,
+// instead of sinking it to its first use (a relaxed atomic load: no ordering, no wait).
+template <typename T>
+__device__ __forceinline__ T issue_now(const T* q) {
+    return __hip_atomic_load(const_cast<T*>(q), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+// Whether the bin pass overflowed a tile (the count's low word: fewer than 2^32 tiles),
+// read early like the list, held as a uniform flag.
+__device__ __forceinline__ bool overflowed_tiles(const RasterParams& p) {
+    const uint32_t lo = issue_now(reinterpret_cast<const uint32_t*>(p.counters + kCntOverflowTiles));
+    return __builtin_amdgcn_readfirstlane(lo) != 0u;
+}
+struct ListWalk {
+    const uint32_t* list;
+    uint32_t k0, step, n, first;
+    __device__ __forceinline__ explicit ListWalk(const RasterParams& p) {
+        const uint32_t xcd = blockIdx.x & 7u;
+        list = p.lists + (size_t)xcd * p.ntiles;
+        k0 = blockIdx.x >> 3;
+        step = gridDim.x >> 3;
+        first = issue_now(list + (k0 < (uint32_t)p.ntiles ? k0 : 0u));
+        n = issue_now(p.list_count + xcd * kCountStride);
+    }
+    __device__ __forceinline__ int tile(uint32_t k) const { return (int)(k == k0 ? first : list[k]); }
+};
+
 template <int TS, int BLOCK>
 __global__ __launch_bounds__(BLOCK, TS == 32 ? 6 : (BLOCK == 256 ? 7 : 8)) void k_raster(RasterParams p) {
-    const uint32_t xcd = blockIdx.x & 7u, step = gridDim.x >> 3;
-    const uint32_t n = p.list_count[xcd * kCountStride];
+    const ListWalk w(p);
+    const bool overflow = overflowed_tiles(p);
     uint32_t frags = 0, cov = 0;
-    for (uint32_t k = blockIdx.x >> 3; k < n; k += step)
-        raster_body<TS, BLOCK, false, kShadePlain>(p, nullptr, nullptr, nullptr, nullptr,
-                                                   (int)p.lists[xcd * p.ntiles + k], frags, cov);
-    if (p.counters[kCntOverflowTiles] != 0ull)
+    for (uint32_t k = w.k0; k < w.n; k += w.step)
+        raster_body<TS, BLOCK, false, kShadePlain>(p, nullptr, nullptr, nullptr, nullptr, w.tile(k), frags, cov);
+    if (overflow)
         frags += overflow_faces(p, blockIdx.x * BLOCK + threadIdx.x, gridDim.x * BLOCK);
     store_block_sum(frags, p.part_frag + blockIdx.x, threadIdx.x, BLOCK);
 }
@@ -1069,16 +1130,18 @@ __global__ __launch_bounds__(BLOCK, MODE == kShadeMlp ? 3 : 6) void k_raster_sha
     const float* s_lut = sp.luts;
     __shared__ __align__(16) uint16_t s_net[MODE == kShadeMlp ? kPackedBytes / 2 : 8];
     __shared__ DevMaterial s_mats[kLdsMats];
+    // the materials' staging reads, the list reads and the overflow count go out together:
+    // one memory round trip before the first tile instead of three
+    const MatStage<BLOCK> mats(sp, threadIdx.x);
+    const ListWalk w(p);
+    const bool overflow = overflowed_tiles(p);
     if (MODE == kShadeMlp) stage_net(sp.mlp, s_net, threadIdx.x, BLOCK);
-    stage_mats(sp, s_mats, threadIdx.x, BLOCK);
+    mats.store(s_mats, threadIdx.x);
     // (raster_body's first barrier orders these stores before any read)
-    const uint32_t xcd = blockIdx.x & 7u, step = gridDim.x >> 3;
-    const uint32_t n = p.list_count[xcd * kCountStride];
     uint32_t frags = 0, cov = 0;
-    for (uint32_t k = blockIdx.x >> 3; k < n; k += step)
-        raster_body<TS, BLOCK, true, MODE>(p, &sp, s_lut, s_mats, s_net, (int)p.lists[xcd * p.ntiles + k], frags,
-                                           cov);
-    if (p.counters[kCntOverflowTiles] != 0ull)
+    for (uint32_t k = w.k0; k < w.n; k += w.step)
+        raster_body<TS, BLOCK, true, MODE>(p, &sp, s_lut, s_mats, s_net, w.tile(k), frags, cov);
+    if (overflow)
         frags += overflow_faces(p, blockIdx.x * BLOCK + threadIdx.x, gridDim.x * BLOCK);
     __syncthreads();
     store_block_sum(frags, p.part_frag + blockIdx.x, threadIdx.x, BLOCK);
