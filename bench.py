@@ -3,8 +3,9 @@
 
 Workload: the 81,920-face displaced, textured icosphere (6 subdivisions, smooth
 normals, 1024x1024 RGBA8 texture, three lights) at 1920x1080 with a seeded YXZ
-rotation — one step is one whole frame (vertex, bin, raster/depth, overflow, shade)
-with the mesh resident in HBM and the frame/depth buffers in device memory.
+rotation — one step is one whole frame (bin with the vertex transform, raster/depth,
+overflow, shade) with the mesh resident in HBM and the frame/depth buffers in device
+memory.
 
 Timed region: `steps` frames captured in one HIP graph, launched once between two
 device synchronisations.  `value` = faces / frame time in Mtri/s.
@@ -64,11 +65,11 @@ def algorithmic_bytes(counts, scene, args, fused, cleared=True):
     depth_in = 0 if cleared else 4 * W * H
     raster_in = 48 * Tf + 4 * P16 + depth_in
     shade_out = 7 * W * H + 108 * Tvis + 4 * tex + 40 * L   # frame + depth, winners' data, texture, lights
-    out = {
-        # (screen-space input bypasses the position transform: the bin pass reads it as given)
-        "vertex": (0 if getattr(scene, "raster_space", False) else 28 * V) + 24 * Vn,
-        "bin": 12 * T + 16 * V + 4 * P16,
-    }
+    # the bin pass transforms each face's corners itself (no vertex pass): it reads the
+    # face indices, the positions (screen-space input: 16 B raster vertices, as given) and
+    # the normals, and writes the bin entries
+    raster_space = getattr(scene, "raster_space", False)
+    out = {"bin": 12 * T + (16 if raster_space else 12) * V + 12 * Vn + 4 * P16}
     if fused:
         # cleared buffers: the bin launch writes the cleared frame and depth (7 B/px), the
         # raster pass shades the covered pixels only
@@ -87,8 +88,9 @@ def per_pass_table(us, nbytes):
     for p, t in us.items():
         if p == "network" and t == 0:
             continue   # the learned shader's pass: not part of the plain frame
-        if p == "overflow" and t == 0:
-            continue   # single frames run the overflow loop inside the raster pass
+        if p in ("overflow", "vertex") and t == 0:
+            continue   # single frames: the overflow loop runs inside the raster pass, the
+                       # vertex transform inside the bin pass
         b = nbytes.get(p, 0)
         gbs = b / (t * 1e-6) / 1e9 if t > 0 else 0.0
         table[p] = {"avg_us": t, "alg_bytes": b, "GBps": gbs, "frac": gbs / HBM_PEAK_GBS}
@@ -409,7 +411,7 @@ def main():
     log("per pass:", json.dumps(per_pass))
     log("per pass, other pipeline:", json.dumps(per_pass_other))
     dom = max(per_pass, key=lambda p: per_pass[p]["avg_us"])
-    kernel = {"vertex": "k_vertex", "bin": "k_bin", "overflow": "k_overflow",
+    kernel = {"bin": "k_bin",
               "raster": "k_raster_shade" if fused else "k_raster",
               "shade": "k_shade_tiles" if fused else "k_shade"}[dom]
     roofline = {"bound": "hbm", "achieved": per_pass[dom]["GBps"], "peak": HBM_PEAK_GBS, "unit": "GB/s",

@@ -146,7 +146,8 @@ typedef struct rz_frame_stats {
     uint64_t fragments;           /* (pixel, front face) pairs passing the inside test (drawing.cpp:111) */
     uint64_t covered_pixels;      /* pixels holding a winning fragment                       */
     float ms_vertex;              /* GPU time per pass, HIP events on the context stream;    */
-    float ms_bin;                 /* zero unless rz_set_profiling(ctx, 1)                    */
+    float ms_bin;                 /* zero unless rz_set_profiling(ctx, 1) (ms_vertex: batches */
+                                  /* only; a single frame's bin pass does the transform)    */
     float ms_raster;
     float ms_overflow;
     float ms_shade;
@@ -277,10 +278,10 @@ rz_status rz_autotune(rz_ctx* ctx, const rz_batch_item* items, int32_t num_items
 /* No reference counterpart.  Times each pass of one frame on the context's stream: a
  * HIP graph of `reps` back-to-back launches of the pass, timed by HIP events around its
  * second launch.  us[0..5] = vertex, bin, raster, overflow, shade, network in
- * microseconds per launch (bin = vertex+bin minus vertex, since binning appends to the
- * tile lists; overflow = 0, the overflow loop runs inside the raster pass; network =
- * the learned shader's second pass, 0 without RZ_FLAG_MLP_SHADE).  Device buffers and RZ_FLAG_CLEARED required.  Leaves the frame
- * rendered in frame_device/depth_device. */
+ * microseconds per launch (vertex = 0: the bin pass transforms each face's corners
+ * itself; overflow = 0: the overflow loop runs inside the raster pass; network = the
+ * learned shader's second pass, 0 without RZ_FLAG_MLP_SHADE).  Device buffers and
+ * RZ_FLAG_CLEARED required.  Leaves the frame rendered in frame_device/depth_device. */
 rz_status rz_time_passes(rz_ctx* ctx, const rz_mesh* mesh, rz_light* lights, int32_t num_lights,
                          const rz_args* args, uint8_t* frame_device, float* depth_device, uint32_t flags,
                          int32_t reps, float us[6]);

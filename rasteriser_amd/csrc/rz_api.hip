@@ -16,7 +16,6 @@
 #include "rz_mlp.h"
 
 namespace rz {
-hipError_t launch_vertex(const VertexParams& p, hipStream_t s);
 hipError_t launch_bin(const BinParams& p, hipStream_t s);
 hipError_t launch_raster(const RasterParams& p, int tile_size, hipStream_t s);
 hipError_t launch_raster_shade(const RasterParams& p, const ShadeParams& sp, int tile_size, hipStream_t s);
@@ -64,6 +63,9 @@ struct rz_mesh {
     bool raster_space = false;
     float *px = nullptr, *py = nullptr, *pz = nullptr;
     float4* rin = nullptr;
+    // (x, y, z, 0) copies for the single-frame bin pass, which transforms each face's
+    // corners itself: positions, vertex normals, face normals
+    float4 *pos4 = nullptr, *nrm4 = nullptr, *fnrm4 = nullptr;
     float *nx = nullptr, *ny = nullptr, *nz = nullptr;      // vertex normals
     float *fnx = nullptr, *fny = nullptr, *fnz = nullptr;   // face normals (flat)
     float *ftx = nullptr, *fty = nullptr, *ftz = nullptr;   // face tangents (normal maps)
@@ -76,6 +78,7 @@ struct rz_mesh {
     // every face has the same normal, uv (-1: none) and material: shading takes them
     // from here and the bin pass gathers nothing (uniform = 1)
     int32_t uniform = 0, uniform_n = 0, uniform_t = -1, uniform_m = 0;
+    float uniform_nrm[3] = {0.f, 0.f, 0.f};   // model-space normal uniform_n
     DevMaterial* mats = nullptr;
     uint32_t* texels = nullptr;
     float* luts = nullptr;
@@ -87,6 +90,7 @@ struct rz_graph {
     hipGraphExec_t exec = nullptr;
     int32_t frames = 0;
     uint64_t buffer_gen = 0;   // ctx->buffer_gen when captured (graphs over the context's buffers)
+    int32_t ntiles = 0;        // frame graphs: tiles of the captured frames (counter-line layout)
     bool owns_buffers = false; // batch graphs carry their own working buffers and tables
     DevBuf arena, tables;
 };
@@ -113,6 +117,11 @@ struct rz_ctx {
     std::vector<uint32_t> host_partials;
     unsigned long long* host_counters = nullptr;   // pinned
     uint64_t buffer_gen = 0;                       // bumped whenever a per-frame buffer is reallocated
+    // Single frames' counter lines: generation count_gen is all zero for a layout of
+    // count_ntiles tiles in the tile_count buffer at count_ptr (anything else: unknown)
+    const void* count_ptr = nullptr;
+    int32_t count_ntiles = -1;
+    int count_gen = 0;
 };
 
 namespace {
@@ -148,6 +157,7 @@ rz_status ensure(rz_ctx* ctx, DevBuf& b, size_t bytes) {
     RZ_HIP(ctx, hipMalloc(&b.ptr, bytes));
     b.bytes = bytes;
     ++ctx->buffer_gen;
+    if (&b == &ctx->tile_count) ctx->count_ntiles = -1;   // new memory: contents unknown
     return RZ_OK;
 }
 
@@ -253,14 +263,19 @@ FrameLayout layout_frame(const rz_mesh* mesh, const rz_args& a, int ts, int cap,
     L.n_part = L.off_cov + (batch ? 0
                                   : L.fused ? (size_t)raster_grid + tiles_grid
                                             : (size_t)shade_blocks_x(L.W) * shade_blocks_y(L.H));
-    L.bytes[kBufRaster] = mesh->raster_space ? 0 : (size_t)mesh->nv * 16;   // (raster space: read as given)
-    L.bytes[kBufCnormal] = (size_t)L.nfn * 16;
+    // batches: the vertex pass's outputs (raster space: read as given), one generation
+    // of tile counts + the 8 list counters, and the global counters apart.  Single frames
+    // transform in the bin pass and keep two generations of counter lines (count_lines):
+    // each frame's bin pass zeroes the other one for the next frame.
+    L.bytes[kBufRaster] = batch && !mesh->raster_space ? (size_t)mesh->nv * 16 : 0;
+    L.bytes[kBufCnormal] = batch ? (size_t)L.nfn * 16 : 0;
     L.bytes[kBufRec] = (size_t)mesh->nf * sizeof(FaceRecord);
-    L.bytes[kBufTiles] = ((size_t)L.ntiles + 8) * 4 * kCountStride;   // + the 8 list counters
+    L.bytes[kBufTiles] = batch ? ((size_t)L.ntiles + 8) * 4 * kCountStride
+                               : (size_t)2 * count_lines(L.ntiles) * 4 * kCountStride;
     L.bytes[kBufLists] = batch ? 0 : (size_t)8 * L.ntiles * 4;
     L.bytes[kBufBins] = (size_t)L.ntiles * L.cap * 4;
     L.bytes[kBufVis] = L.np * 8;
-    L.bytes[kBufCounters] = kNumCounters * 8;
+    L.bytes[kBufCounters] = batch ? kNumCounters * 8 : 0;
     L.bytes[kBufPart] = L.n_part * 4;
     return L;
 }
@@ -276,6 +291,7 @@ struct FrameLaunch {
     uint32_t* part = nullptr;
     unsigned long long* counters = nullptr;
     uint32_t mlp_blocks = 0;   // learned shader: workgroups of k_mlp_shade (0: not used)
+    int gen = 0;               // single frames: the generation of counter lines used
 };
 
 rz_status check_frame(rz_ctx* ctx, const rz_mesh* mesh, const rz_light* lights, int32_t nl, const rz_args* args) {
@@ -293,8 +309,9 @@ rz_status check_frame(rz_ctx* ctx, const rz_mesh* mesh, const rz_light* lights, 
 
 // Builds matrices and light directions (written back to `lights`, as drawing.cpp:238
 // does) and fills every pass's parameters over the working buffers `buf`.
+// Single frames use generation `gen` of the counter lines (and zero the other one).
 void fill_frame(const rz_mesh* mesh, rz_light* lights, int32_t nl, const rz_args& a, uint32_t flags,
-                const FrameLayout& lay, void* const buf[kNumBufs], FrameLaunch& L) {
+                const FrameLayout& lay, void* const buf[kNumBufs], FrameLaunch& L, int gen = 0) {
     const bool cleared = (flags & RZ_FLAG_CLEARED) != 0;
     rz_matrices mats;
     build_matrices(a, mats);
@@ -307,10 +324,14 @@ void fill_frame(const rz_mesh* mesh, rz_light* lights, int32_t nl, const rz_args
             L.sp.light_nd[l][c] = -lights[l].trans_dir[c];
         }
     }
-    auto* counters = (unsigned long long*)buf[kBufCounters];
+    const size_t gen_words = (size_t)count_lines(lay.ntiles) * kCountStride;
+    uint32_t* tile_count = (uint32_t*)buf[kBufTiles] + (lay.batch ? 0 : gen * gen_words);
+    auto* counters = lay.batch ? (unsigned long long*)buf[kBufCounters]
+                               : (unsigned long long*)(tile_count + ((size_t)lay.ntiles + 8) * kCountStride);
     auto* part = (uint32_t*)buf[kBufPart];
     L.part = part;
     L.counters = counters;
+    L.gen = gen;
 
     VertexParams& vp = L.vp;
     std::memset(&vp, 0, sizeof vp);
@@ -322,7 +343,7 @@ void fill_frame(const rz_mesh* mesh, rz_light* lights, int32_t nl, const rz_args
     vp.nz = a.flat ? mesh->fnz : mesh->nz;
     vp.raster = (float4*)buf[kBufRaster];
     vp.cnormal = (float4*)buf[kBufCnormal];
-    vp.tile_count = (uint32_t*)buf[kBufTiles];
+    vp.tile_count = tile_count;
     vp.counters = counters;
     vp.nv = mesh->nv; vp.nn = lay.nfn; vp.ntiles = lay.ntiles + 8;   // clears the list counters too
     // raster-space vertices need no vertex stage: the bin pass reads them where they are
@@ -355,6 +376,19 @@ void fill_frame(const rz_mesh* mesh, rz_light* lights, int32_t nl, const rz_args
     bp.lists = (uint32_t*)buf[kBufLists];   // (batches: replaced by the batch's lists)
     bp.list_count = bp.tile_count + (size_t)lay.ntiles * kCountStride;
     bp.ovf_vis = (unsigned long long*)buf[kBufVis];
+    if (!lay.batch) {
+        // no vertex pass: the bin pass transforms each face's corners
+        bp.pos = mesh->raster_space ? nullptr : mesh->pos4;
+        bp.raster = mesh->raster_space ? mesh->rin : nullptr;
+        bp.nrm = a.flat ? mesh->fnrm4 : mesh->nrm4;
+        bp.cnormal = nullptr;
+        std::memcpy(bp.camera, mats.camera, 64);
+        std::memcpy(bp.normal, mats.normal, 64);
+        bp.Wf = (float)lay.W; bp.Hf = (float)lay.H;
+        bp.zero_lines = (uint32_t*)buf[kBufTiles] + (1 - gen) * gen_words;
+        bp.zero_count = count_lines(lay.ntiles);
+        bp.zero_blocks = (bp.zero_count + 255) / 256;
+    }
 
     RasterParams& rp = L.rp;
     std::memset(&rp, 0, sizeof rp);
@@ -380,7 +414,14 @@ void fill_frame(const rz_mesh* mesh, rz_light* lights, int32_t nl, const rz_args
     sp.uniform_n = mesh->uniform_n;
     sp.uniform_t = mesh->uniform_t;
     sp.uniform_m = mesh->uniform_m;
-    sp.cnormal = vp.cnormal;
+    {
+        // the uniform normal as the vertex transform gives it (same operations, in order)
+        const float* m = mats.normal;
+        const float x = mesh->uniform_nrm[0], y = mesh->uniform_nrm[1], z = mesh->uniform_nrm[2], w = 0.0f;
+        sp.uniform_cn = make_float4((m[0] * x + m[4] * y) + (m[8] * z + m[12] * w),
+                                    (m[1] * x + m[5] * y) + (m[9] * z + m[13] * w),
+                                    (m[2] * x + m[6] * y) + (m[10] * z + m[14] * w), 0.0f);
+    }
     sp.uvs = mesh->uvs;
     sp.ftx = mesh->ftx; sp.fty = mesh->fty; sp.ftz = mesh->ftz;
     std::memcpy(sp.normal_mat, mats.normal, 64);
@@ -402,7 +443,7 @@ constexpr uint32_t kNarrowTiles = 16384;
 
 // Single frame: validates, sizes the context's working buffers and fills L.
 rz_status prepare_frame(rz_ctx* ctx, const rz_mesh* mesh, rz_light* lights, int32_t nl, const rz_args* args,
-                        uint32_t flags, FrameLaunch& L) {
+                        uint32_t flags, FrameLaunch& L, int gen) {
     rz_status st = check_frame(ctx, mesh, lights, nl, args);
     if (st != RZ_OK) return st;
     RZ_HIP(ctx, hipSetDevice(ctx->device));
@@ -429,7 +470,7 @@ rz_status prepare_frame(rz_ctx* ctx, const rz_mesh* mesh, rz_light* lights, int3
     for (int i = 0; i < kNumBufs; ++i) ptrs[i] = bufs[i]->ptr;
     if ((flags & RZ_FLAG_MLP_SHADE) && !ctx->mlp_ready)
         return fail(ctx, RZ_ERR_INVALID_ARG, "RZ_FLAG_MLP_SHADE without network weights (rz_mlp_init / rz_mlp_set_weights)");
-    fill_frame(mesh, lights, nl, *args, flags, lay, ptrs, L);
+    fill_frame(mesh, lights, nl, *args, flags, lay, ptrs, L, gen);
     L.rp.narrow = narrow ? 1 : 0;
     L.sp.mlp = (const uint16_t*)ctx->mlp_packed.ptr;
     L.sp.max_blocks = lay.tiles_grid;
@@ -501,7 +542,7 @@ constexpr uint64_t kHoldTicks = 20000;
 rz_status enqueue_frame(rz_ctx* ctx, const FrameLaunch& L, hipStream_t s, bool prof) {
     if (prof) RZ_HIP(ctx, launch_hold(kHoldTicks, s));
     if (prof) RZ_HIP(ctx, hipEventRecord(ctx->ev[0], s));
-    RZ_HIP(ctx, launch_vertex(L.vp, s));
+    // (no vertex pass: the bin pass transforms each face's corners)
     if (prof) RZ_HIP(ctx, hipEventRecord(ctx->ev[1], s));
     RZ_HIP(ctx, launch_bin(L.bp, s));
     if (prof) RZ_HIP(ctx, hipEventRecord(ctx->ev[2], s));
@@ -518,6 +559,34 @@ rz_status enqueue_frame(rz_ctx* ctx, const FrameLaunch& L, hipStream_t s, bool p
         RZ_HIP(ctx, launch_shade(L.sp, s));
     RZ_HIP(ctx, launch_mlp_phase(L, s));
     if (prof) RZ_HIP(ctx, hipEventRecord(ctx->ev[5], s));
+    return RZ_OK;
+}
+
+// Makes generation `gen` of the single-frame counter lines all zero for `ntiles` tiles,
+// unless the context knows it is (a zeroing launch on the context's stream otherwise).
+rz_status counts_ready(rz_ctx* ctx, int32_t ntiles, int gen) {
+    if (ctx->count_ptr == ctx->tile_count.ptr && ctx->count_ntiles == ntiles && ctx->count_gen == gen) return RZ_OK;
+    BinParams z;
+    std::memset(&z, 0, sizeof z);
+    z.zero_count = count_lines(ntiles);
+    z.zero_lines = (uint32_t*)ctx->tile_count.ptr + (size_t)gen * z.zero_count * kCountStride;
+    z.zero_blocks = (z.zero_count + 255) / 256;
+    RZ_HIP(ctx, launch_bin(z, ctx->stream));
+    ctx->count_ptr = ctx->tile_count.ptr;
+    ctx->count_ntiles = ntiles;
+    ctx->count_gen = gen;
+    return RZ_OK;
+}
+
+// A single frame on the context's stream, prepared with generation ctx->count_gen: the
+// counter lines are made ready first, and the other generation is the next frame's.
+rz_status run_frame(rz_ctx* ctx, const FrameLaunch& L, bool prof) {
+    rz_status st = counts_ready(ctx, L.lay.ntiles, L.gen);
+    if (st != RZ_OK) return st;
+    ctx->count_ntiles = -1;   // unknown until the frame is enqueued whole
+    if ((st = enqueue_frame(ctx, L, ctx->stream, prof)) != RZ_OK) return st;
+    ctx->count_ntiles = L.lay.ntiles;
+    ctx->count_gen = L.gen ^ 1;
     return RZ_OK;
 }
 
@@ -1045,6 +1114,9 @@ rz_status rz_mesh_create(rz_ctx* ctx, const rz_mesh_desc* d, rz_mesh** out) {
         if ((st = upload(ctx, m, &m->px, sx.data(), V * 4)) != RZ_OK) return cleanup(st);
         if ((st = upload(ctx, m, &m->py, sy.data(), V * 4)) != RZ_OK) return cleanup(st);
         if ((st = upload(ctx, m, &m->pz, sz.data(), V * 4)) != RZ_OK) return cleanup(st);
+        std::vector<float4> p4(V);
+        for (size_t i = 0; i < V; ++i) p4[i] = make_float4(sx[i], sy[i], sz[i], 0.f);
+        if ((st = upload(ctx, m, &m->pos4, p4.data(), V * 16)) != RZ_OK) return cleanup(st);
         // face normals for flat shading, model space
         std::vector<float> fx(F), fy(F), fz(F);
         for (size_t f = 0; f < F; ++f) {
@@ -1057,6 +1129,9 @@ rz_status rz_mesh_create(rz_ctx* ctx, const rz_mesh_desc* d, rz_mesh** out) {
         if ((st = upload(ctx, m, &m->fnx, fx.data(), F * 4)) != RZ_OK) return cleanup(st);
         if ((st = upload(ctx, m, &m->fny, fy.data(), F * 4)) != RZ_OK) return cleanup(st);
         if ((st = upload(ctx, m, &m->fnz, fz.data(), F * 4)) != RZ_OK) return cleanup(st);
+        std::vector<float4> f4(F);
+        for (size_t f = 0; f < F; ++f) f4[f] = make_float4(fx[f], fy[f], fz[f], 0.f);
+        if ((st = upload(ctx, m, &m->fnrm4, f4.data(), F * 16)) != RZ_OK) return cleanup(st);
         // face tangents for normal maps, model space
         for (size_t f = 0; f < F; ++f) {
             const rz_triangle& t = d->faces[f];
@@ -1086,6 +1161,9 @@ rz_status rz_mesh_create(rz_ctx* ctx, const rz_mesh_desc* d, rz_mesh** out) {
         if ((st = upload(ctx, m, &m->nx, sx.data(), N * 4)) != RZ_OK) return cleanup(st);
         if ((st = upload(ctx, m, &m->ny, sy.data(), N * 4)) != RZ_OK) return cleanup(st);
         if ((st = upload(ctx, m, &m->nz, sz.data(), N * 4)) != RZ_OK) return cleanup(st);
+        std::vector<float4> n4(N);
+        for (size_t i = 0; i < N; ++i) n4[i] = make_float4(sx[i], sy[i], sz[i], 0.f);
+        if ((st = upload(ctx, m, &m->nrm4, n4.data(), N * 16)) != RZ_OK) return cleanup(st);
     }
     {
         std::vector<float> uv(2 * (size_t)m->nt);
@@ -1116,6 +1194,8 @@ rz_status rz_mesh_create(rz_ctx* ctx, const rz_mesh_desc* d, rz_mesh** out) {
             m->uniform_n = faces[0].q1.x;
             m->uniform_t = faces[0].q2.x;
             m->uniform_m = faces[0].q1.w;
+            if (m->uniform_n >= 0 && (size_t)m->uniform_n < N)
+                for (int c = 0; c < 3; ++c) m->uniform_nrm[c] = d->normals[3 * (size_t)norder[m->uniform_n] + c];
         }
     }
     {
@@ -1188,73 +1268,86 @@ rz_status rz_time_passes(rz_ctx* ctx, const rz_mesh* mesh, rz_light* lights, int
         return fail(ctx, RZ_ERR_INVALID_ARG, "rz_time_passes: device buffers, reps >= 1 and RZ_FLAG_CLEARED required");
     if ((flags & RZ_FLAG_MLP_SHADE) && !ctx->mlp_ready)
         return fail(ctx, RZ_ERR_INVALID_ARG, "RZ_FLAG_MLP_SHADE without network weights");
-    FrameLaunch L;
-    rz_status st = prepare_frame(ctx, mesh, lights, nl, args, flags, L);
-    if (st != RZ_OK) return st;
-    bind_outputs(L, true, frame_device, depth_device, nullptr);
+    // the frame with each generation of counter lines (the bin launches alternate)
+    FrameLaunch L2[2];
+    rz_status st = RZ_OK;
+    for (int g = 0; g < 2; ++g) {
+        if ((st = prepare_frame(ctx, mesh, lights, nl, args, flags, L2[g], g)) != RZ_OK) return st;
+        bind_outputs(L2[g], true, frame_device, depth_device, nullptr);
+    }
+    const int32_t ntiles = L2[0].lay.ntiles;
     hipStream_t s = ctx->stream;
-    // one graph of `reps` launches of the pass(es) `enqueue_one` issues; ms of the second launch
-    auto timed = [&](auto enqueue_one, float& ms) -> rz_status {
+    // one graph of `reps` launches of the pass(es) `enqueue_one(k)` issues; ms of the
+    // second launch.  `ready` runs before each launch (outside the timed region).
+    auto timed = [&](auto enqueue_one, auto ready, float& ms) -> rz_status {
         hipGraphExec_t exec = nullptr;
-        rz_status r = capture(ctx, reps, [&](int32_t, hipStream_t cs) { return enqueue_one(cs); }, &exec);
+        rz_status r = capture(ctx, reps, [&](int32_t k, hipStream_t cs) { return enqueue_one(k, cs); }, &exec);
         if (r != RZ_OK) return r;
-        hipError_t e = hipGraphLaunch(exec, s);
-        if (e == hipSuccess) e = hipEventRecord(ctx->ev[0], s);
-        if (e == hipSuccess) e = hipGraphLaunch(exec, s);
-        if (e == hipSuccess) e = hipEventRecord(ctx->ev[5], s);
-        if (e == hipSuccess) e = hipStreamSynchronize(s);
-        if (e == hipSuccess) e = hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[5]);
+        hipError_t e = hipSuccess;
+        r = ready();
+        if (r == RZ_OK) e = hipGraphLaunch(exec, s);
+        if (r == RZ_OK && e == hipSuccess) r = ready();
+        if (r == RZ_OK && e == hipSuccess) e = hipEventRecord(ctx->ev[0], s);
+        if (r == RZ_OK && e == hipSuccess) e = hipGraphLaunch(exec, s);
+        if (r == RZ_OK && e == hipSuccess) e = hipEventRecord(ctx->ev[5], s);
+        if (r == RZ_OK && e == hipSuccess) e = hipStreamSynchronize(s);
+        if (r == RZ_OK && e == hipSuccess) e = hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[5]);
         (void)hipGraphExecDestroy(exec);
+        if (r != RZ_OK) return r;
         if (e != hipSuccess) return fail(ctx, RZ_ERR_HIP, "rz_time_passes: %s", hipGetErrorString(e));
         return RZ_OK;
     };
-    const bool feat = L.sp.mode == kShadeFeat;
-    auto vertex = [&](hipStream_t cs) -> rz_status { RZ_HIP(ctx, launch_vertex(L.vp, cs)); return RZ_OK; };
-    auto vertex_bin = [&](hipStream_t cs) -> rz_status {
-        RZ_HIP(ctx, launch_vertex(L.vp, cs));
-        RZ_HIP(ctx, launch_bin(L.bp, cs));
+    auto nothing = [&]() -> rz_status { return RZ_OK; };
+    // bin launches k = 0, 1, ... use generation k % 2, from a ready generation 0; after
+    // `reps` of them generation reps % 2 is the ready one
+    auto bin = [&](int32_t k, hipStream_t cs) -> rz_status {
+        RZ_HIP(ctx, launch_bin(L2[k & 1].bp, cs));
         return RZ_OK;
     };
-    auto raster = [&](hipStream_t cs) -> rz_status {
+    auto bin_ready = [&]() -> rz_status {
+        rz_status r = counts_ready(ctx, ntiles, 0);
+        ctx->count_gen = reps & 1;   // (as it stands after the launch that follows)
+        return r;
+    };
+    float ms_b = 0, ms_r = 0, ms_s = 0, ms_n = 0;
+    if ((st = timed(bin, bin_ready, ms_b)) != RZ_OK) return st;
+    // one whole frame: raster, overflow and shade are idempotent on its binned state
+    const FrameLaunch& L = L2[ctx->count_gen];
+    if ((st = run_frame(ctx, L, false)) != RZ_OK) return st;
+    const bool feat = L.sp.mode == kShadeFeat;
+    auto raster = [&](int32_t, hipStream_t cs) -> rz_status {
         if (L.lay.fused)
             RZ_HIP(ctx, launch_raster_shade(L.rp, L.sp, L.lay.ts, cs));
         else
             RZ_HIP(ctx, launch_raster(L.rp, L.lay.ts, cs));
         return RZ_OK;
     };
-    auto shade = [&](hipStream_t cs) -> rz_status {
+    auto shade = [&](int32_t, hipStream_t cs) -> rz_status {
         if (L.lay.fused)
             RZ_HIP(ctx, launch_shade_tiles(L.rp, tiles_params(L), L.lay.ts, cs));
         else
             RZ_HIP(ctx, launch_shade(L.sp, cs));
         return RZ_OK;
     };
-    auto network = [&](hipStream_t cs) -> rz_status { RZ_HIP(ctx, launch_mlp_phase(L, cs)); return RZ_OK; };
+    auto network = [&](int32_t, hipStream_t cs) -> rz_status { RZ_HIP(ctx, launch_mlp_phase(L, cs)); return RZ_OK; };
     // the separate shade pass of the learned shader accumulates into per-tile record counts
     // that its raster pass zeroes: timed as raster + shade minus raster
-    auto raster_shade = [&](hipStream_t cs) -> rz_status {
-        rz_status r = raster(cs);
-        return r != RZ_OK ? r : shade(cs);
+    auto raster_shade = [&](int32_t k, hipStream_t cs) -> rz_status {
+        rz_status r = raster(k, cs);
+        return r != RZ_OK ? r : shade(k, cs);
     };
     const bool with_raster = feat && !L.lay.fused;
-    float ms_v = 0, ms_vb = 0, ms_r = 0, ms_s = 0, ms_n = 0;
-    // vertex alone, then vertex+bin (each repetition leaves a whole binned frame behind);
-    // raster, overflow and shade are idempotent on that state; the network pass last,
-    // on the records of one whole frame
-    if ((st = enqueue_frame(ctx, L, s, false)) != RZ_OK) return st;
-    if ((st = timed(vertex, ms_v)) != RZ_OK) return st;
-    if ((st = timed(vertex_bin, ms_vb)) != RZ_OK) return st;
-    if ((st = timed(raster, ms_r)) != RZ_OK) return st;
+    if ((st = timed(raster, nothing, ms_r)) != RZ_OK) return st;
     if (with_raster) {
-        if ((st = timed(raster_shade, ms_s)) != RZ_OK) return st;
+        if ((st = timed(raster_shade, nothing, ms_s)) != RZ_OK) return st;
         ms_s -= ms_r;
-    } else if ((st = timed(shade, ms_s)) != RZ_OK) {
+    } else if ((st = timed(shade, nothing, ms_s)) != RZ_OK) {
         return st;
     }
-    if (feat && (st = timed(network, ms_n)) != RZ_OK) return st;
+    if (feat && (st = timed(network, nothing, ms_n)) != RZ_OK) return st;
     const float k = 1000.0f / (float)reps;
-    us[0] = ms_v * k;
-    us[1] = (ms_vb - ms_v) * k;
+    us[0] = 0.0f;   // no vertex pass: the bin pass transforms each face's corners
+    us[1] = ms_b * k;
     us[2] = ms_r * k;
     us[3] = 0.0f;   // the overflow loop runs inside the raster pass
     us[4] = ms_s * k;
@@ -1269,7 +1362,7 @@ rz_status rz_draw_frame(rz_ctx* ctx, const rz_mesh* mesh, rz_light* lights, int3
     const bool cleared = (flags & RZ_FLAG_CLEARED) != 0;
     const bool on_device = (flags & RZ_FLAG_DEVICE_BUFFERS) != 0;
     FrameLaunch L;
-    rz_status st = prepare_frame(ctx, mesh, lights, nl, args, flags, L);
+    rz_status st = prepare_frame(ctx, mesh, lights, nl, args, flags, L, ctx->count_gen);
     if (st != RZ_OK) return st;
     if (!frame_buffer || !depth_buffer) return fail(ctx, RZ_ERR_INVALID_ARG, "null frame or depth buffer");
     const size_t np = L.lay.np;
@@ -1292,7 +1385,7 @@ rz_status rz_draw_frame(rz_ctx* ctx, const rz_mesh* mesh, rz_light* lights, int3
         }
     }
     bind_outputs(L, cleared, frame_d, depth_d, ids_d);
-    if ((st = enqueue_frame(ctx, L, s, ctx->profiling)) != RZ_OK) return st;
+    if ((st = run_frame(ctx, L, ctx->profiling)) != RZ_OK) return st;
     if (!on_device) {
         RZ_HIP(ctx, hipMemcpyAsync(frame_buffer, frame_d, np * 3, hipMemcpyDeviceToHost, s));
         RZ_HIP(ctx, hipMemcpyAsync(depth_buffer, depth_d, np * 4, hipMemcpyDeviceToHost, s));
@@ -1323,8 +1416,9 @@ rz_status rz_graph_create(rz_ctx* ctx, const rz_mesh* mesh, rz_light* lights, in
     if (!frame_d || !depth_d) return fail(ctx, RZ_ERR_INVALID_ARG, "null frame or depth buffer");
     std::vector<FrameLaunch> launches(frames);
     rz_args a = args ? *args : rz_args{};
+    // frame k uses generation k % 2 of the counter lines (rz_graph_launch readies 0)
     for (int32_t k = 0; k < frames; ++k) {
-        rz_status st = prepare_frame(ctx, mesh, lights, nl, args ? &a : nullptr, flags, launches[k]);
+        rz_status st = prepare_frame(ctx, mesh, lights, nl, args ? &a : nullptr, flags, launches[k], k & 1);
         if (st != RZ_OK) return st;
         bind_outputs(launches[k], (flags & RZ_FLAG_CLEARED) != 0, frame_d, depth_d, ids_d);
         a.tait_bryan_angles[1] += ry_step;   // the spin loop's rotation about y (renderer.cpp:124)
@@ -1338,6 +1432,7 @@ rz_status rz_graph_create(rz_ctx* ctx, const rz_mesh* mesh, rz_light* lights, in
     gr->exec = exec;
     gr->frames = frames;
     gr->buffer_gen = ctx->buffer_gen;
+    gr->ntiles = launches[0].lay.ntiles;
     *out = gr;
     return RZ_OK;
 }
@@ -1348,7 +1443,16 @@ rz_status rz_graph_launch(rz_graph* g) {
     if (!g->owns_buffers && g->buffer_gen != ctx->buffer_gen)
         return fail(ctx, RZ_ERR_INVALID_ARG, "graph is stale: a later frame reallocated the context's buffers");
     RZ_HIP(ctx, hipSetDevice(ctx->device));
+    if (!g->owns_buffers) {   // a frame graph: its first frame uses generation 0
+        rz_status st = counts_ready(ctx, g->ntiles, 0);
+        if (st != RZ_OK) return st;
+        ctx->count_ntiles = -1;
+    }
     RZ_HIP(ctx, hipGraphLaunch(g->exec, ctx->stream));
+    if (!g->owns_buffers) {
+        ctx->count_ntiles = g->ntiles;
+        ctx->count_gen = g->frames & 1;
+    }
     return RZ_OK;
 }
 
@@ -1529,7 +1633,7 @@ rz_status rz_mlp_collect(rz_ctx* ctx, const rz_mesh* mesh, rz_light* lights, int
                          float* samples_device, int64_t capacity, int64_t* count) {
     if (!ctx || !samples_device || capacity < 0 || !count) return RZ_ERR_INVALID_ARG;
     FrameLaunch L;
-    rz_status st = prepare_frame(ctx, mesh, lights, nl, args, RZ_FLAG_CLEARED, L);
+    rz_status st = prepare_frame(ctx, mesh, lights, nl, args, RZ_FLAG_CLEARED, L, ctx->count_gen);
     if (st != RZ_OK) return st;
     const size_t np = L.lay.np;
     if ((st = ensure(ctx, ctx->frame, np * 3)) != RZ_OK) return st;
@@ -1542,7 +1646,7 @@ rz_status rz_mlp_collect(rz_ctx* ctx, const rz_mesh* mesh, rz_light* lights, int
     L.sp.samples = samples_device;
     L.sp.sample_count = (unsigned long long*)ctx->mlp_count.ptr;
     L.sp.sample_cap = (unsigned long long)capacity;
-    if ((st = enqueue_frame(ctx, L, ctx->stream, false)) != RZ_OK) return st;
+    if ((st = run_frame(ctx, L, false)) != RZ_OK) return st;
     unsigned long long end = 0;
     RZ_HIP(ctx, hipMemcpyAsync(&end, ctx->mlp_count.ptr, 8, hipMemcpyDeviceToHost, ctx->stream));
     RZ_HIP(ctx, hipStreamSynchronize(ctx->stream));

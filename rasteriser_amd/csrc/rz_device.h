@@ -20,7 +20,7 @@ constexpr int kBigTriangleTiles = 16;           // above this many tiles a face 
 // so packing 32 counters per line would queue every face of 32 tiles behind each other.
 constexpr uint32_t kCountStride = 32;
 
-// Global counter slots (uint64) cleared by the vertex pass each frame.  Only the
+// Global counter slots (uint64), zero when a frame's bin pass starts.  Only the
 // overflow-tile count is a device-wide atomic (it is rare and doubles as the flag
 // the overflow pass tests); every other statistic is a per-workgroup partial sum
 // written with a plain store and reduced only when a caller asks for statistics
@@ -92,6 +92,14 @@ struct BinParams {
     const int4* face_attr;      // (n0, n1, n2, material), (t0, t1, t2, 0) in binning order
     const float4* raster;
     const float4* cnormal;      // camera-space normals (flat: one per face)
+    // Single frames have no vertex pass: each face transforms its own corners.  pos
+    // (x, y, z, -): model positions (null: `raster` holds raster-space positions as
+    // given); nrm (x, y, z, -): model normals (flat: one per face; null: `cnormal`).
+    const float4* pos;
+    const float4* nrm;
+    float camera[16];
+    float normal[16];
+    float Wf, Hf;
     const float2* uvs;
     int32_t flat;
     int32_t gather;             // 0: every face has the same normal, uv and material (ShadeParams::uniform)
@@ -126,6 +134,12 @@ struct BinParams {
     unsigned long long* clear_vis;
     size_t clear_np;
     uint32_t clear_blocks;
+    // single frames: the other generation of the counter lines (tile counts, list counters,
+    // global counters; see count_lines), zeroed by zero_blocks more workgroups for the
+    // next frame while this one fills its own
+    uint32_t* zero_lines;
+    uint32_t zero_count;        // lines
+    uint32_t zero_blocks;
 };
 
 struct RasterParams {
@@ -164,10 +178,11 @@ struct ShadeParams {
     const unsigned long long* vis;
     const FaceRecord* rec;
     const DevMaterial* mats;
-    // uniform >= 0: every face shares normal uniform_n (camera space: cnormal), uv
-    // uniform_t (-1: none) and material uniform_m, and the records hold no copies
+    // uniform >= 0: every face shares normal uniform_n (camera space: uniform_cn, as the
+    // vertex transform gives it), uv uniform_t (-1: none) and material uniform_m, and the
+    // records hold no copies
     int32_t uniform, uniform_n, uniform_t, uniform_m;
-    const float4* cnormal;
+    float4 uniform_cn;
     const float2* uvs;
     const uint32_t* texels;
     const float* luts;
@@ -210,6 +225,9 @@ __host__ __device__ inline uint32_t vertex_blocks(int32_t nv, int32_t nn, int32_
     return (uint32_t)((n + 255) / 256);
 }
 __host__ __device__ inline uint32_t bin_blocks(int32_t nf) { return (uint32_t)((nf + 255) / 256); }
+// Counter lines of one generation (single frames): tile counts, then the 8 list counters,
+// then one line holding the kNumCounters global counters.
+__host__ __device__ inline uint32_t count_lines(int32_t ntiles) { return (uint32_t)ntiles + 9u; }
 __host__ __device__ inline uint32_t overflow_blocks(int32_t nf) {
     const uint32_t b = bin_blocks(nf);
     return b > 1024u ? 1024u : b;

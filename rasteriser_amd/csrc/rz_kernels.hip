@@ -1,12 +1,15 @@
 // rz_kernels.hip — gfx950 kernels of the frame path.
 //
-//   k_vertex    SoA vertex stage: clip = camera*(p,1), z-divide, NDC->raster
-//               (geometry.cpp:44-74, 85-95, 110-116, 135-142) and the normal
-//               transform (geometry.cpp:35-42, 97-108); clears the per-frame bins.
-//   k_bin       backface test (geometry.cpp:76-83, drawing.cpp:176-178), clamped
-//               bbox (drawing.cpp:77-93), per-face setup and shading record, appends
-//               the face to the bin of every tile its bbox meets (fixed capacity per
-//               tile) and each tile's first face puts the tile on a per-XCD list.
+//   k_bin       per face: the vertex transform of its corners, clip = camera*(p,1),
+//               z-divide, NDC->raster (geometry.cpp:44-74, 85-95, 110-116, 135-142),
+//               and of its normals (geometry.cpp:35-42, 97-108); backface test
+//               (geometry.cpp:76-83, drawing.cpp:176-178), clamped bbox
+//               (drawing.cpp:77-93), per-face setup and shading record, appends the
+//               face to the bin of every tile its bbox meets (fixed capacity per tile)
+//               and each tile's first face puts the tile on a per-XCD list; more
+//               workgroups zero the next frame's generation of counter lines.
+//               (Batches run the transform as a vertex pass of their own,
+//               k_vertex_batch, which also clears the meshes' counters.)
 //   k_raster    resident grid over the tile lists, one tile per workgroup at a time:
 //               expands (face, bbox pixel) candidates with wavefront prefix scans,
 //               runs the inside and depth tests of drawing.cpp:96-121 and keeps per
@@ -51,35 +54,39 @@ __device__ __forceinline__ void store_block_sum(uint32_t v, uint32_t* out, int t
 }
 
 // ---------------------------------------------------------------------------------
+// A model position to (x_px, y_px, ndc_z, 1/w): GLM mat4*vec4 (pairwise) with w = 1,
+// perspective divide, viewport (geometry.cpp, drawing.cpp:20-40).
+__device__ __forceinline__ float4 raster_vertex(const float* m, float x, float y, float z, float Wf, float Hf) {
+    const float cx = (m[0] * x + m[4] * y) + (m[8] * z + m[12] * 1.0f);
+    const float cy = (m[1] * x + m[5] * y) + (m[9] * z + m[13] * 1.0f);
+    const float cz = (m[2] * x + m[6] * y) + (m[10] * z + m[14] * 1.0f);
+    const float cw = (m[3] * x + m[7] * y) + (m[11] * z + m[15] * 1.0f);
+    const float nx = cx / cw, ny = cy / cw, nz = cz / cw, nw = 1.f / cw;
+    float4 r;
+    r.x = 0.5f * (nx + 1.0f) * Wf;
+    r.y = 0.5f * (-ny + 1.0f) * Hf;
+    r.z = nz;
+    r.w = nw;
+    return r;
+}
+
+// A model normal to camera space: the normal matrix times (x, y, z, 0), pairwise.
+__device__ __forceinline__ float4 camera_normal(const float* m, float x, float y, float z) {
+    const float w = 0.0f;
+    float4 n;
+    n.x = (m[0] * x + m[4] * y) + (m[8] * z + m[12] * w);
+    n.y = (m[1] * x + m[5] * y) + (m[9] * z + m[13] * w);
+    n.z = (m[2] * x + m[6] * y) + (m[10] * z + m[14] * w);
+    n.w = 0.0f;
+    return n;
+}
+
+// Batches: the vertex pass of every mesh (single frames transform in the bin pass).
 __device__ __forceinline__ void vertex_body(const VertexParams& p, int blk) {
     const int i = blk * 256 + threadIdx.x;
-    if (i < p.nv) {   // (raster-space meshes have nv = 0: the bin pass reads their vertices)
-        const float x = p.px[i], y = p.py[i], z = p.pz[i];
-        const float* m = p.camera;
-        // GLM mat4*vec4 (pairwise), w = 1
-        const float cx = (m[0] * x + m[4] * y) + (m[8] * z + m[12] * 1.0f);
-        const float cy = (m[1] * x + m[5] * y) + (m[9] * z + m[13] * 1.0f);
-        const float cz = (m[2] * x + m[6] * y) + (m[10] * z + m[14] * 1.0f);
-        const float cw = (m[3] * x + m[7] * y) + (m[11] * z + m[15] * 1.0f);
-        const float nx = cx / cw, ny = cy / cw, nz = cz / cw, nw = 1.f / cw;
-        float4 r;
-        r.x = 0.5f * (nx + 1.0f) * p.Wf;
-        r.y = 0.5f * (-ny + 1.0f) * p.Hf;
-        r.z = nz;
-        r.w = nw;
-        p.raster[i] = r;
-    }
-    if (i < p.nn) {
-        const float x = p.nx[i], y = p.ny[i], z = p.nz[i];
-        const float* m = p.normal;
-        const float w = 0.0f;
-        float4 n;
-        n.x = (m[0] * x + m[4] * y) + (m[8] * z + m[12] * w);
-        n.y = (m[1] * x + m[5] * y) + (m[9] * z + m[13] * w);
-        n.z = (m[2] * x + m[6] * y) + (m[10] * z + m[14] * w);
-        n.w = 0.0f;
-        p.cnormal[i] = n;
-    }
+    if (i < p.nv)   // (raster-space meshes have nv = 0: the bin pass reads their vertices)
+        p.raster[i] = raster_vertex(p.camera, p.px[i], p.py[i], p.pz[i], p.Wf, p.Hf);
+    if (i < p.nn) p.cnormal[i] = camera_normal(p.normal, p.nx[i], p.ny[i], p.nz[i]);
     if (i < p.ntiles) p.tile_count[(size_t)i * kCountStride] = 0u;
     // (the last 8 entries are the tile lists' counters; their second word is a batch's
     // work cursor)
@@ -138,16 +145,29 @@ __device__ __forceinline__ void bin_body(const BinParams& p, int blk) {
             q2 = p.face_attr[2 * i + 1];
         }
         f = q0.w;
-        const float4 a = p.raster[q0.x], b = p.raster[q0.y], c = p.raster[q0.z];
+        // every read is issued before any transform waits on one
+        const float4* vsrc = p.pos ? p.pos : p.raster;
+        float4 a = vsrc[q0.x], b = vsrc[q0.y], c = vsrc[q0.z];
         float4 n0 = make_float4(0.f, 0.f, 0.f, 0.f), n1 = n0, n2 = n0;
         float2 t0 = make_float2(0.f, 0.f), t1 = t0, t2 = t0;
         int32_t mat = 0;
         if (p.gather) {
-            n0 = p.cnormal[p.flat ? f : q1.x]; n1 = p.cnormal[p.flat ? f : q1.y]; n2 = p.cnormal[p.flat ? f : q1.z];
+            const float4* nsrc = p.nrm ? p.nrm : p.cnormal;
+            n0 = nsrc[p.flat ? f : q1.x]; n1 = nsrc[p.flat ? f : q1.y]; n2 = nsrc[p.flat ? f : q1.z];
             if (q2.x >= 0) t0 = p.uvs[q2.x];
             if (q2.y >= 0) t1 = p.uvs[q2.y];
             if (q2.z >= 0) t2 = p.uvs[q2.z];
             mat = q1.w;
+        }
+        if (p.pos) {
+            a = raster_vertex(p.camera, a.x, a.y, a.z, p.Wf, p.Hf);
+            b = raster_vertex(p.camera, b.x, b.y, b.z, p.Wf, p.Hf);
+            c = raster_vertex(p.camera, c.x, c.y, c.z, p.Wf, p.Hf);
+        }
+        if (p.gather && p.nrm) {
+            n0 = camera_normal(p.normal, n0.x, n0.y, n0.z);
+            n1 = p.flat ? n0 : camera_normal(p.normal, n1.x, n1.y, n1.z);
+            n2 = p.flat ? n0 : camera_normal(p.normal, n2.x, n2.y, n2.z);
         }
         const float s = signed_area(a.x, a.y, b.x, b.y, c.x, c.y);
         front = (s > 0.0f) != (p.wind_cw != 0);
@@ -410,7 +430,7 @@ __device__ __forceinline__ bool shade_pixel(const ShadeParams& p, const float* s
         const float4 a = r.r0, b = r.r1, c = r.r2, w = r.r3;
         float4 g0, g1, g2, g3;   // the record's attribute words (see FaceRecord)
         if (p.uniform >= 0) {    // uniform over the launch
-            const float4 n = p.cnormal[p.uniform_n];
+            const float4 n = p.uniform_cn;
             const float2 t = p.uniform_t >= 0 ? p.uvs[p.uniform_t] : make_float2(0.f, 0.f);
             g0 = make_float4(n.x, n.y, n.z, t.x);
             g1 = make_float4(n.x, n.y, n.z, t.y);
@@ -1035,7 +1055,6 @@ __device__ __forceinline__ void overflow_body(const RasterParams& p, int blk, in
 // in device memory, with a map from block to (mesh, block within that mesh's pass) or
 // the shared tile lists, so one launch covers every mesh.
 
-__global__ __launch_bounds__(256) void k_vertex(VertexParams p) { vertex_body(p, blockIdx.x); }
 // Fills nbytes at base with a 32-bit pattern laid out as in a 4-byte-aligned array of
 // it (bytes at a ragged head and tail, 16-byte stores between); thread t of nthreads.
 __device__ __forceinline__ void fill_pattern(uint8_t* base, size_t nbytes, uint32_t pat, size_t t, size_t nthreads) {
@@ -1059,14 +1078,31 @@ __device__ __forceinline__ void clear_body(const BinParams& p, uint32_t blk, uin
     if (p.clear_vis) fill_pattern(reinterpret_cast<uint8_t*>(p.clear_vis), 8 * p.clear_np, 0xFFFFFFFFu, t, nt);
 }
 
-// Binning workgroups first, then (fused pipeline, cleared buffers) the clearing ones,
-// which run on the CUs the latency-bound binning leaves idle.
+// The next frame's generation of counter lines: words 0-1 of each line (a count, and a
+// list counter's second word), all of the last line (the global counters).
+__device__ __forceinline__ void zero_lines_body(const BinParams& p, uint32_t blk) {
+    const uint32_t i = blk * 256 + threadIdx.x;
+    if (i >= p.zero_count) return;
+    uint32_t* line = p.zero_lines + (size_t)i * kCountStride;
+    if (i + 1 < p.zero_count) {
+        *reinterpret_cast<uint2*>(line) = make_uint2(0u, 0u);
+    } else {
+#pragma unroll
+        for (int k = 0; k < 2 * kNumCounters; k += 4) *reinterpret_cast<uint4*>(line + k) = make_uint4(0u, 0u, 0u, 0u);
+    }
+}
+
+// Binning workgroups first, then the ones zeroing the next frame's counter lines, then
+// (fused pipeline, cleared buffers) the clearing ones, which run on the CUs the
+// latency-bound binning leaves idle.
 __global__ __launch_bounds__(256) void k_bin(BinParams p) {
     const uint32_t nb = bin_blocks(p.nf);
     if (blockIdx.x < nb)
         bin_body(p, blockIdx.x);
+    else if (blockIdx.x < nb + p.zero_blocks)
+        zero_lines_body(p, blockIdx.x - nb);
     else
-        clear_body(p, blockIdx.x - nb, p.clear_blocks);
+        clear_body(p, blockIdx.x - nb - p.zero_blocks, p.clear_blocks);
 }
 // The raster and overflow-shade passes are driven by the per-XCD lists of tiles that
 // received a face (built by the bin pass): a resident grid of workgroups, a multiple of
@@ -1290,13 +1326,8 @@ hipError_t launch_hold(uint64_t ticks, hipStream_t s) {
     return hipGetLastError();
 }
 
-hipError_t launch_vertex(const VertexParams& p, hipStream_t s) {
-    hipLaunchKernelGGL(k_vertex, dim3(vertex_blocks(p.nv, p.nn, p.ntiles)), dim3(256), 0, s, p);
-    return hipGetLastError();
-}
-
 hipError_t launch_bin(const BinParams& p, hipStream_t s) {
-    const uint32_t blocks = bin_blocks(p.nf) + (p.clear_np ? p.clear_blocks : 0u);
+    const uint32_t blocks = bin_blocks(p.nf) + p.zero_blocks + (p.clear_np ? p.clear_blocks : 0u);
     if (blocks) hipLaunchKernelGGL(k_bin, dim3(blocks), dim3(256), 0, s, p);
     return hipGetLastError();
 }

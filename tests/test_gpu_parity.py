@@ -288,3 +288,72 @@ def test_normal_map_and_64_lights(ctx):
     scene, args = scenes.learned_shading_scene(width=640, height=360, subdivisions=5, map_size=256)
     gpu, ref = check(ctx, scene, args, label="normal map 64 lights")
     assert (gpu[2] >= 0).sum() > 20000
+
+
+def test_counter_generations_across_calls():
+    """Single frames keep two generations of tile counters: each frame's bin pass zeroes
+    the one the next frame uses.  Eager frames, frame graphs with odd and even frame
+    counts, a change of image size, the overflow path and pass timing interleaved on one
+    context must each still match the oracle."""
+    import oracle
+    import rasteriser_amd as rz
+    from tests.parity import assert_same_frame
+    big, small = (640, 480), (320, 200)
+    with rz.Context(0) as c:
+        scenes_by_size = {wh: scenes.icosphere_scene(subdivisions=4, width=wh[0], height=wh[1]) for wh in (big, small)}
+        meshes = {wh: c.upload(sa[0]) for wh, sa in scenes_by_size.items()}
+        bufs = {wh: (c.empty((3, wh[1], wh[0]), np.uint8), c.empty((wh[1], wh[0]), np.float32),
+                     c.empty((wh[1], wh[0]), np.int32)) for wh in (big, small)}
+        refs = {wh: oracle.draw_frame(*scenes_by_size[wh]) for wh in (big, small)}
+
+        def compare(wh, label, stats=None):
+            f, d, i = bufs[wh]
+            ref = refs[wh]
+            st = stats or {"fragments": ref[3]["fragments"], "front_triangles": ref[3]["front_triangles"],
+                           "covered_pixels": int((ref[2] >= 0).sum())}
+            assert_same_frame((f.download(), d.download(), i.download(), st, ref[4]), ref, label)
+
+        def eager(wh, label):
+            scene, args = scenes_by_size[wh]
+            f, d, i = bufs[wh]
+            print("eager", label, flush=True)
+            st = c.draw_device(meshes[wh], scene.lights_copy(), args, f.ptr, d.ptr, i.ptr,
+                               rz.RZ_FLAG_CLEARED, stats=True)
+            compare(wh, label, st)
+
+        def graph(wh, frames, label, launches=1):
+            scene, args = scenes_by_size[wh]
+            f, d, i = bufs[wh]
+            g = c.graph(meshes[wh], scene.lights_copy(), args, f.ptr, d.ptr, frames, ids_ptr=i.ptr)
+            try:
+                for k in range(launches):
+                    print("graph", label, frames, "frames, launch", k, flush=True)
+                    i.upload(np.full((wh[1], wh[0]), 5, np.int32))
+                    g.launch()
+                    c.sync()
+                    compare(wh, f"{label} launch {k}")
+                    eager(wh, f"{label} eager after launch {k}")
+            finally:
+                g.close()
+
+        eager(big, "first")
+        graph(big, 3, "odd graph", launches=2)
+        eager(small, "smaller image")
+        graph(big, 2, "even graph", launches=2)
+        eager(big, "after even graph")
+        c.set_tuning(16, 4)   # overflowing bins: the overflow count lives in the counter lines
+        eager(big, "overflow 1")
+        eager(small, "overflow smaller")
+        graph(big, 3, "overflow odd graph", launches=2)
+        c.set_tuning(16, 512)
+        scene, args = scenes_by_size[big]
+        f, d, i = bufs[big]
+        for reps in (3, 4):
+            print("time_passes reps", reps, flush=True)
+            c.time_passes(meshes[big], scene.lights_copy(), args, f, d, reps=reps)
+            eager(big, f"after time_passes reps {reps}")
+        for m in meshes.values():
+            m.close()
+        for b in bufs.values():
+            for x in b:
+                x.close()
