@@ -281,7 +281,7 @@ def batched(ctx, rz, scenes, np, frames):
             d.close()
         for m in meshes:
             m.close()
-        ctx.set_tuning(16, 512)
+        ctx.set_tuning(16, 0)
 
 
 def stress(ctx, rz, scenes, np, frames):
@@ -329,7 +329,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--flat", action="store_true", help="flat instead of smooth normals")
     ap.add_argument("--tile", type=int, default=16)
-    ap.add_argument("--cap", type=int, default=512)
+    ap.add_argument("--cap", type=int, default=0, help="bin capacity (0: librz's automatic choice)")
     ap.add_argument("--no-mlp", action="store_true", help="skip the learned-shading measurement")
     ap.add_argument("--stress", action="store_true",
                     help="also measure the 4M-triangle 4096x4096 depth stress frame")
@@ -455,7 +455,7 @@ def main():
 
     mlp = None
     if not a.no_mlp:
-        ctx.set_tuning(16, 512)
+        ctx.set_tuning(16, 0)
         mlp = learned_shading(ctx, rz, scenes, np, a.mlp_steps, a.mlp_batch, max(10, min(a.steps, 100)))
         log(f"learned shading: {mlp}")
     ctx.close()

@@ -99,7 +99,7 @@ struct rz_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
     int tile_size = 16;
-    int bin_cap = 512;
+    int bin_cap = 0;   // 0: automatic (frame_bin_cap)
     bool fused = true;
     bool profiling = false;
     uint32_t persistent_blocks = 0;   // workgroups of the overflow shade launch (a few per CU)
@@ -441,6 +441,23 @@ void fill_frame(const rz_mesh* mesh, rz_light* lights, int32_t nl, const rz_args
 // Frames with at least this many 16-pixel tiles raster them with 64-thread workgroups.
 constexpr uint32_t kNarrowTiles = 16384;
 
+// Bin capacity of the automatic setting when a bin per tile for every face would not fit
+// kAutoBinBytes (and of batches, which the autotuner sizes).
+constexpr int kDefaultBinCap = 512;
+constexpr size_t kAutoBinBytes = (size_t)4 << 30;
+
+// A single frame's per-tile bin capacity: the context's, or (automatic) room in every
+// tile for every face of the mesh when that fits kAutoBinBytes — bins are written only
+// where faces land, so their size costs address space, not traffic.  With a capacity of
+// at least the face count no bin can overflow and the frame needs no overflow-shade
+// launch.
+int frame_bin_cap(const rz_ctx* ctx, const rz_mesh* mesh, uint32_t ntiles) {
+    if (ctx->bin_cap > 0) return ctx->bin_cap;
+    const size_t nf = mesh->nf > 0 ? (size_t)mesh->nf : 1;
+    if (nf <= ((size_t)1 << 20) && (size_t)ntiles * nf * 4 <= kAutoBinBytes) return (int)nf;
+    return kDefaultBinCap;
+}
+
 // Single frame: validates, sizes the context's working buffers and fills L.
 rz_status prepare_frame(rz_ctx* ctx, const rz_mesh* mesh, rz_light* lights, int32_t nl, const rz_args* args,
                         uint32_t flags, FrameLaunch& L, int gen) {
@@ -459,8 +476,10 @@ rz_status prepare_frame(rz_ctx* ctx, const rz_mesh* mesh, rz_light* lights, int3
     const uint32_t per_cu = raster_blocks_per_cu(ctx->tile_size, ctx->fused, mode, narrow);
     uint32_t grid = (uint32_t)ctx->cus * per_cu / 8u * 8u;
     if (grid < 8) grid = 8;
-    const FrameLayout lay =
-        layout_frame(mesh, *args, ctx->tile_size, ctx->bin_cap, ctx->fused, grid, ctx->persistent_blocks, false);
+    const int cap = frame_bin_cap(ctx, mesh, ntiles);
+    const bool can_overflow = (uint32_t)cap < (uint32_t)mesh->nf;
+    const FrameLayout lay = layout_frame(mesh, *args, ctx->tile_size, cap, ctx->fused, grid,
+                                         can_overflow ? ctx->persistent_blocks : 0u, false);
     DevBuf* bufs[kNumBufs] = {&ctx->raster, &ctx->cnormal, &ctx->rec,   &ctx->tile_count, &ctx->bins,
                               &ctx->vis,    &ctx->lists,   &ctx->counters, &ctx->partials};
     void* ptrs[kNumBufs];
@@ -553,10 +572,10 @@ rz_status enqueue_frame(rz_ctx* ctx, const FrameLaunch& L, hipStream_t s, bool p
     if (prof) RZ_HIP(ctx, hipEventRecord(ctx->ev[3], s));
     // (the overflow loop runs inside the raster pass)
     if (prof) RZ_HIP(ctx, hipEventRecord(ctx->ev[4], s));
-    if (L.lay.fused)
-        RZ_HIP(ctx, launch_shade_tiles(L.rp, tiles_params(L), L.lay.ts, s));
-    else
+    if (!L.lay.fused)
         RZ_HIP(ctx, launch_shade(L.sp, s));
+    else if (L.lay.tiles_grid > 0)   // (0: no bin can overflow)
+        RZ_HIP(ctx, launch_shade_tiles(L.rp, tiles_params(L), L.lay.ts, s));
     RZ_HIP(ctx, launch_mlp_phase(L, s));
     if (prof) RZ_HIP(ctx, hipEventRecord(ctx->ev[5], s));
     return RZ_OK;
@@ -657,7 +676,8 @@ rz_status prepare_batch(rz_ctx* ctx, const rz_batch_item* items, int32_t n, uint
         rz_status st = check_frame(ctx, it.mesh, it.lights, it.num_lights, &it.args);
         if (st != RZ_OK) return st;
         if (!it.frame || !it.depth) return fail(ctx, RZ_ERR_INVALID_ARG, "batch item %d: null frame or depth", i);
-        lays[i] = layout_frame(it.mesh, it.args, ctx->tile_size, ctx->bin_cap, true, 0, 0, true);
+        lays[i] = layout_frame(it.mesh, it.args, ctx->tile_size, ctx->bin_cap > 0 ? ctx->bin_cap : kDefaultBinCap,
+                               true, 0, 0, true);
         if ((uint32_t)lays[i].ntiles > kListTileMask)
             return fail(ctx, RZ_ERR_INVALID_ARG, "batch item %d: %d tiles, at most %u", i, lays[i].ntiles, kListTileMask);
         stat_bytes += align256(lays[i].bytes[kBufCounters]) + align256(lays[i].bytes[kBufPart]);
@@ -908,8 +928,8 @@ rz_status rz_set_tuning(rz_ctx* ctx, int32_t tile_size, int32_t bin_capacity) {
     if (!ctx) return RZ_ERR_INVALID_ARG;
     if (tile_size != 8 && tile_size != 16 && tile_size != 32)
         return fail(ctx, RZ_ERR_INVALID_ARG, "tile size %d not in {8,16,32}", tile_size);
-    if (bin_capacity < 1 || bin_capacity > (1 << 20))
-        return fail(ctx, RZ_ERR_INVALID_ARG, "bin capacity %d not in [1, 2^20]", bin_capacity);
+    if (bin_capacity < 0 || bin_capacity > (1 << 20))
+        return fail(ctx, RZ_ERR_INVALID_ARG, "bin capacity %d not in [0, 2^20] (0: automatic)", bin_capacity);
     ctx->tile_size = tile_size;
     ctx->bin_cap = bin_capacity;
     return RZ_OK;
@@ -1341,6 +1361,8 @@ rz_status rz_time_passes(rz_ctx* ctx, const rz_mesh* mesh, rz_light* lights, int
     if (with_raster) {
         if ((st = timed(raster_shade, nothing, ms_s)) != RZ_OK) return st;
         ms_s -= ms_r;
+    } else if (L.lay.fused && L.lay.tiles_grid == 0) {
+        ms_s = 0.0f;   // no bin can overflow: the frame has no overflow-shade launch
     } else if ((st = timed(shade, nothing, ms_s)) != RZ_OK) {
         return st;
     }

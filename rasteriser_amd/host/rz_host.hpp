@@ -77,7 +77,7 @@ struct Args {
     float fovy_degrees = 45.f, z_near = 0.1f, z_far = 6.f;
     int device = 0;
     int tile_size = 16;
-    int bin_capacity = 512;
+    int bin_capacity = 0;   // 0: librz's automatic capacity
     int frames = 120;            // spin mode without a display: frames to render
     std::string out_dir = ".";
     std::string dump_mesh;       // write the loaded mesh as raw arrays (loader checks)

@@ -22,7 +22,7 @@ def render_gpu(ctx, scene, args, frame=None, depth=None, tuning=None):
     finally:
         mesh.close()
         if tuning is not None:
-            ctx.set_tuning(16, 512)
+            ctx.set_tuning(16, 0)
     return out + (lights,)
 
 

@@ -258,7 +258,7 @@ def test_batch_graph_and_autotune(ctx):
             assert np.array_equal(f.download(), fb) and np.array_equal(i.download(), ib)
             assert np.array_equal(d.download().view(np.uint32), db.view(np.uint32))
     g.close()
-    ctx.set_tuning(16, 512)
+    ctx.set_tuning(16, 0)
     for m in meshes:
         m.close()
 
@@ -279,7 +279,7 @@ def test_batch_overflow_path(ctx, tile, cap):
             assert np.array_equal(i.download(), oi) and np.array_equal(f.download(), of)
             assert np.array_equal(d.download().view(np.uint32), od.view(np.uint32))
     finally:
-        ctx.set_tuning(16, 512)
+        ctx.set_tuning(16, 0)
 
 
 def test_normal_map_and_64_lights(ctx):
@@ -345,7 +345,7 @@ def test_counter_generations_across_calls():
         eager(big, "overflow 1")
         eager(small, "overflow smaller")
         graph(big, 3, "overflow odd graph", launches=2)
-        c.set_tuning(16, 512)
+        c.set_tuning(16, 0)
         scene, args = scenes_by_size[big]
         f, d, i = bufs[big]
         for reps in (3, 4):
