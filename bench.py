@@ -88,9 +88,10 @@ def per_pass_table(us, nbytes):
     for p, t in us.items():
         if p == "network" and t == 0:
             continue   # the learned shader's pass: not part of the plain frame
-        if p in ("overflow", "vertex") and t == 0:
+        if p in ("overflow", "vertex", "shade") and t == 0:
             continue   # single frames: the overflow loop runs inside the raster pass, the
-                       # vertex transform inside the bin pass
+                       # vertex transform inside the bin pass; no overflow-shade launch when
+                       # no bin can overflow
         b = nbytes.get(p, 0)
         gbs = b / (t * 1e-6) / 1e9 if t > 0 else 0.0
         table[p] = {"avg_us": t, "alg_bytes": b, "GBps": gbs, "frac": gbs / HBM_PEAK_GBS}
