@@ -70,7 +70,8 @@ const char* kUsage =
     "         [-s] [-f] [--wind-clockwise] [--rx <rad>] [--ry <rad>] [--rz <rad>] [--scale <f>]\n"
     "         [--dx <d>] [--dy <d>] [--dz <d>]\n"
     "         [--fovy <deg>] [--near <d>] [--far <d>] [--device <n>] [--tile 8|16|32] [--cap <n>]\n"
-    "         [--frames <n>] [--out-dir <dir>] [--dump-mesh <prefix>]\n";
+    "         [--frames <n>] [--out-dir <dir>] [--dump-mesh <prefix>]\n"
+    "(--cap 0, the default: librz sizes the bins itself)\n";
 
 }  // namespace
 
