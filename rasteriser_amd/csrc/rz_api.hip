@@ -345,6 +345,7 @@ void fill_frame(const rz_mesh* mesh, rz_light* lights, int32_t nl, const rz_args
     vp.cnormal = (float4*)buf[kBufCnormal];
     vp.tile_count = tile_count;
     vp.counters = counters;
+    // (batches only: single frames transform in the bin pass)
     vp.nv = mesh->nv; vp.nn = lay.nfn; vp.ntiles = lay.ntiles + 8;   // clears the list counters too
     // raster-space vertices need no vertex stage: the bin pass reads them where they are
     if (mesh->raster_space) {
@@ -1093,8 +1094,8 @@ rz_status rz_mesh_create(rz_ctx* ctx, const rz_mesh_desc* d, rz_mesh** out) {
     };
     const size_t V = (size_t)m->nv, N = (size_t)m->nn, F = (size_t)m->nf;
     // The faces in binning order; vertices, normals and uvs are renumbered by first use
-    // in that order, so that the bin pass reads them (and the vertex pass writes them)
-    // in address order.  Only storage moves: every value and face index is the caller's.
+    // in that order, so that the bin pass reads them (and a batch's vertex pass writes
+    // them) in address order.  Only storage moves: every value and face index is the caller's.
     std::vector<BinFace> faces;
     bin_faces(d, faces);
     std::vector<int32_t> vorder, norder, torder;   // new index -> caller's index
@@ -1117,7 +1118,8 @@ rz_status rz_mesh_create(rz_ctx* ctx, const rz_mesh_desc* d, rz_mesh** out) {
         for (size_t i = 0; i < N; ++i) if (nmap[i] < 0) norder.push_back((int32_t)i);
         for (size_t i = 0; i < tmap.size(); ++i) if (tmap[i] < 0) torder.push_back((int32_t)i);
     }
-    // positions: SoA for the coalesced vertex stage
+    // positions: (x, y, z, 0) per vertex for the single-frame bin pass, SoA for a batch's
+    // coalesced vertex pass
     if (m->raster_space) {
         std::vector<float> rv(4 * V);
         for (size_t i = 0; i < V; ++i)

@@ -45,7 +45,7 @@ constexpr int32_t kMaxBatchMeshes = 1 << (32 - kListMeshShift);
 //   r2 = (x2, y2, z2, bits(bbox_hi))   bbox_hi = br.x | br.y << 16
 //   r3 = (w0, w1, w2, 0)               1/clip_w per vertex
 // and, for the shading of the pixels the face wins, its corners' camera-space normals
-// (as the vertex pass wrote them) and uvs ((0, 0) for a corner without one):
+// (as the vertex transform gives them) and uvs ((0, 0) for a corner without one):
 //   r4 = (n0.x, n0.y, n0.z, u0)
 //   r5 = (n1.x, n1.y, n1.z, v0)
 //   r6 = (n2.x, n2.y, n2.z, u1)
