@@ -812,6 +812,8 @@ __device__ __forceinline__ void raster_body(const RasterParams& p, const ShadePa
     __syncthreads();
 
     const unsigned long long below = (1ull << lane) - 1ull;
+    const unsigned long long upto = (below << 1) | 1ull;   // bits 0..lane
+    static_assert(BLOCK % 64 == 0, "candidates are dealt in whole waves");
     for (uint32_t base = 0; base < n; base += BLOCK) {
         const uint32_t i = base + tid;
         uint32_t cnt = 0, rect = 0, f = 0;
@@ -884,8 +886,9 @@ __device__ __forceinline__ void raster_body(const RasterParams& p, const ShadePa
             __syncthreads();
             const uint32_t r1 = min(total, r0 + kWords * 64u);
             for (uint32_t k = r0 + tid; k < r1; k += BLOCK) {
-                const uint32_t q = k - r0, wi = q >> 6, bit = q & 63u;
-                const unsigned long long upto = bit == 63u ? ~0ull : (2ull << bit) - 1ull;
+                // (r0 and BLOCK are multiples of 64: candidate k's bit in its mask word is
+                // the lane, so the mask of bits up to it is the loop-invariant upto)
+                const uint32_t wi = (k - r0) >> 6;
                 const int j = (int)(before + s_wpre[wi] + (uint32_t)__popcll(s_mask[wi] & upto)) - 1;
                 const uint32_t rc = s_rect[j];
                 const uint32_t rw = rc >> 16;
