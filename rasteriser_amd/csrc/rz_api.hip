@@ -256,8 +256,9 @@ FrameLayout layout_frame(const rz_mesh* mesh, const rz_args& a, int ts, int cap,
     L.np = (size_t)L.W * L.H;
     L.nfn = a.flat ? mesh->nf : mesh->nn;
     L.off_front = 0;
-    L.off_pairs = L.off_front + bin_blocks(mesh->nf);
-    L.off_frag = L.off_pairs + bin_blocks(mesh->nf);
+    const size_t nbin = batch ? bin_blocks(mesh->nf) : frame_bin_blocks(mesh->nf);   // bin workgroups
+    L.off_pairs = L.off_front + nbin;
+    L.off_frag = L.off_pairs + nbin;
     L.off_ovf = L.off_frag + (batch ? 0 : (size_t)raster_grid);
     L.off_cov = L.off_ovf + (batch ? overflow_blocks(mesh->nf) : 0);   // single frames: in the raster pass
     L.n_part = L.off_cov + (batch ? 0
@@ -388,7 +389,7 @@ void fill_frame(const rz_mesh* mesh, rz_light* lights, int32_t nl, const rz_args
         bp.Wf = (float)lay.W; bp.Hf = (float)lay.H;
         bp.zero_lines = (uint32_t*)buf[kBufTiles] + (1 - gen) * gen_words;
         bp.zero_count = count_lines(lay.ntiles);
-        bp.zero_blocks = (bp.zero_count + 255) / 256;
+        bp.zero_blocks = (bp.zero_count + kFrameBinBlock - 1) / kFrameBinBlock;
     }
 
     RasterParams& rp = L.rp;
@@ -550,7 +551,10 @@ void bind_outputs(FrameLaunch& L, bool cleared, uint8_t* frame_d, float* depth_d
     L.bp.clear_vis = vis ? L.rp.vis : nullptr;
     L.bp.clear_np = (pre || ids || vis) ? L.lay.np : 0;
     const size_t per = (L.lay.np + kClearPixelsPerBlock - 1) / kClearPixelsPerBlock;
-    const uint32_t nclear = L.lay.batch ? (uint32_t)(per < kClearBlocks ? per : kClearBlocks) : kClearBlocks;
+    // (kClearBlocks workgroups of 256 threads' worth: single frames' bin launch has
+    // kFrameBinBlock-thread workgroups)
+    const uint32_t nclear = L.lay.batch ? (uint32_t)(per < kClearBlocks ? per : kClearBlocks)
+                                        : kClearBlocks * (256u / (uint32_t)kFrameBinBlock);
     L.bp.clear_blocks = (pre || ids || vis) ? nclear : 0u;
     L.rp.pre_cleared = L.orp.pre_cleared = pre ? 1 : 0;
     L.sp.pre_cleared = pre ? 1 : 0;
@@ -590,7 +594,7 @@ rz_status counts_ready(rz_ctx* ctx, int32_t ntiles, int gen) {
     std::memset(&z, 0, sizeof z);
     z.zero_count = count_lines(ntiles);
     z.zero_lines = (uint32_t*)ctx->tile_count.ptr + (size_t)gen * z.zero_count * kCountStride;
-    z.zero_blocks = (z.zero_count + 255) / 256;
+    z.zero_blocks = (z.zero_count + kFrameBinBlock - 1) / kFrameBinBlock;
     RZ_HIP(ctx, launch_bin(z, ctx->stream));
     ctx->count_ptr = ctx->tile_count.ptr;
     ctx->count_ntiles = ntiles;

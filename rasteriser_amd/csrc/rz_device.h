@@ -225,6 +225,11 @@ __host__ __device__ inline uint32_t vertex_blocks(int32_t nv, int32_t nn, int32_
     return (uint32_t)((n + 255) / 256);
 }
 __host__ __device__ inline uint32_t bin_blocks(int32_t nf) { return (uint32_t)((nf + 255) / 256); }
+// A single frame's bin launch: workgroups of kFrameBinBlock threads (batches: 256).
+constexpr int kFrameBinBlock = 128;
+__host__ __device__ inline uint32_t frame_bin_blocks(int32_t nf) {
+    return (uint32_t)((nf + kFrameBinBlock - 1) / kFrameBinBlock);
+}
 // Counter lines of one generation (single frames): tile counts, then the 8 list counters,
 // then one line holding the kNumCounters global counters.
 __host__ __device__ inline uint32_t count_lines(int32_t ntiles) { return (uint32_t)ntiles + 9u; }

@@ -127,10 +127,11 @@ __device__ __forceinline__ void bin_push(const BinParams& p, uint32_t face, uint
         tile_overflowed(p, tile);
 }
 
+template <int BLOCK>
 __device__ __forceinline__ void bin_body(const BinParams& p, int blk) {
     // faces are taken in the mesh's binning order (centroid Morton order), so the
     // workgroup's faces are screen neighbours; f is the face's own index
-    const int i = blk * 256 + threadIdx.x;
+    const int i = blk * BLOCK + threadIdx.x;
     int f = p.nf;
     bool front = false;
     uint32_t tx0 = 1, tx1 = 0, ty0 = 1, ty1 = 0;   // empty tile rectangle
@@ -196,8 +197,8 @@ __device__ __forceinline__ void bin_body(const BinParams& p, int blk) {
         }
     }
     const uint32_t nt = (tx0 <= tx1 && ty0 <= ty1) ? (tx1 - tx0 + 1) * (ty1 - ty0 + 1) : 0u;
-    store_block_sum(front ? 1u : 0u, p.part_front + blk, threadIdx.x, 256);
-    store_block_sum(nt, p.part_pairs + blk, threadIdx.x, 256);
+    store_block_sum(front ? 1u : 0u, p.part_front + blk, threadIdx.x, BLOCK);
+    store_block_sum(nt, p.part_pairs + blk, threadIdx.x, BLOCK);
     const bool big = nt > (uint32_t)kBigTriangleTiles;
     const bool small = nt > 0 && !big;
 
@@ -206,11 +207,11 @@ __device__ __forceinline__ void bin_body(const BinParams& p, int blk) {
     // Faces of one workgroup are neighbours in the mesh, so their tiles usually fall in
     // a small rectangle; when the union of their tile rectangles is larger than
     // kAggTiles the pairs go to the tile counters one by one.
-    constexpr int kAggTiles = 1024;
+    constexpr int kAggTiles = 4 * BLOCK;
     __shared__ int s_rect[4];                       // union tile rect: x0 y0 x1 y1
     __shared__ uint32_t s_cnt[kAggTiles];
     __shared__ uint32_t s_base[kAggTiles];
-    __shared__ uint8_t s_slot[256 * kBigTriangleTiles];
+    __shared__ uint8_t s_slot[BLOCK * kBigTriangleTiles];
     const int tid = threadIdx.x;
     if (tid == 0) {
         s_rect[0] = s_rect[1] = 0x7FFFFFFF;
@@ -237,7 +238,7 @@ __device__ __forceinline__ void bin_body(const BinParams& p, int blk) {
     const bool agg = s_rect[2] >= 0 && uw * uh <= kAggTiles;   // uniform over the workgroup
     if (agg) {
         const int area = uw * uh;
-        for (int i = tid; i < area; i += 256) s_cnt[i] = 0u;
+        for (int i = tid; i < area; i += BLOCK) s_cnt[i] = 0u;
         __syncthreads();
         if (small) {
             int k = 0;
@@ -247,7 +248,7 @@ __device__ __forceinline__ void bin_body(const BinParams& p, int blk) {
                         (uint8_t)atomicAdd(&s_cnt[((int)ty - uy0) * uw + ((int)tx - ux0)], 1u);
         }
         __syncthreads();
-        for (int i = tid; i < area; i += 256) {
+        for (int i = tid; i < area; i += BLOCK) {
             const uint32_t c = s_cnt[i];
             if (!c) continue;
             const uint32_t tile = (uint32_t)(uy0 + i / uw) * p.tiles_x + (uint32_t)(ux0 + i % uw);
@@ -1072,8 +1073,9 @@ __device__ __forceinline__ void fill_pattern(uint8_t* base, size_t nbytes, uint3
     for (size_t i = head + 16 * n16 + t; i < nbytes; i += nthreads) base[i] = (uint8_t)(pat >> (8u * ((a + i) & 3u)));
 }
 
+template <int BLOCK>
 __device__ __forceinline__ void clear_body(const BinParams& p, uint32_t blk, uint32_t nblk) {
-    const size_t t = (size_t)blk * 256 + threadIdx.x, nt = (size_t)nblk * 256;
+    const size_t t = (size_t)blk * BLOCK + threadIdx.x, nt = (size_t)nblk * BLOCK;
     if (p.clear_frame) fill_pattern(p.clear_frame, 3 * p.clear_np, 0u, t, nt);
     if (p.clear_depth)
         fill_pattern(reinterpret_cast<uint8_t*>(p.clear_depth), 4 * p.clear_np, 0x3F800000u, t, nt);   // 1.0f
@@ -1083,8 +1085,9 @@ __device__ __forceinline__ void clear_body(const BinParams& p, uint32_t blk, uin
 
 // The next frame's generation of counter lines: words 0-1 of each line (a count, and a
 // list counter's second word), all of the last line (the global counters).
+template <int BLOCK>
 __device__ __forceinline__ void zero_lines_body(const BinParams& p, uint32_t blk) {
-    const uint32_t i = blk * 256 + threadIdx.x;
+    const uint32_t i = blk * BLOCK + threadIdx.x;
     if (i >= p.zero_count) return;
     uint32_t* line = p.zero_lines + (size_t)i * kCountStride;
     if (i + 1 < p.zero_count) {
@@ -1098,14 +1101,15 @@ __device__ __forceinline__ void zero_lines_body(const BinParams& p, uint32_t blk
 // Binning workgroups first, then the ones zeroing the next frame's counter lines, then
 // (fused pipeline, cleared buffers) the clearing ones, which run on the CUs the
 // latency-bound binning leaves idle.
-__global__ __launch_bounds__(256) void k_bin(BinParams p) {
-    const uint32_t nb = bin_blocks(p.nf);
+// (single frames: kFrameBinBlock-thread workgroups)
+__global__ __launch_bounds__(kFrameBinBlock) void k_bin(BinParams p) {
+    const uint32_t nb = frame_bin_blocks(p.nf);
     if (blockIdx.x < nb)
-        bin_body(p, blockIdx.x);
+        bin_body<kFrameBinBlock>(p, blockIdx.x);
     else if (blockIdx.x < nb + p.zero_blocks)
-        zero_lines_body(p, blockIdx.x - nb);
+        zero_lines_body<kFrameBinBlock>(p, blockIdx.x - nb);
     else
-        clear_body(p, blockIdx.x - nb - p.zero_blocks, p.clear_blocks);
+        clear_body<kFrameBinBlock>(p, blockIdx.x - nb - p.zero_blocks, p.clear_blocks);
 }
 // The raster and overflow-shade passes are driven by the per-XCD lists of tiles that
 // received a face (built by the bin pass): a resident grid of workgroups, a multiple of
@@ -1235,9 +1239,9 @@ __global__ __launch_bounds__(256) void k_bin_batch(const BinParams* __restrict__
     const BinParams& p = items[m.x];
     const uint32_t nb = bin_blocks(p.nf);
     if (m.y < nb)
-        bin_body(p, (int)m.y);
+        bin_body<256>(p, (int)m.y);
     else
-        clear_body(p, m.y - nb, p.clear_blocks);
+        clear_body<256>(p, m.y - nb, p.clear_blocks);
 }
 // A batch's raster pass: one resident grid over the tile lists the meshes share, entry
 // (mesh << kListMeshShift) | tile, raster and shade of the tile with its mesh's
@@ -1330,8 +1334,8 @@ hipError_t launch_hold(uint64_t ticks, hipStream_t s) {
 }
 
 hipError_t launch_bin(const BinParams& p, hipStream_t s) {
-    const uint32_t blocks = bin_blocks(p.nf) + p.zero_blocks + (p.clear_np ? p.clear_blocks : 0u);
-    if (blocks) hipLaunchKernelGGL(k_bin, dim3(blocks), dim3(256), 0, s, p);
+    const uint32_t blocks = frame_bin_blocks(p.nf) + p.zero_blocks + (p.clear_np ? p.clear_blocks : 0u);
+    if (blocks) hipLaunchKernelGGL(k_bin, dim3(blocks), dim3(kFrameBinBlock), 0, s, p);
     return hipGetLastError();
 }
 
