@@ -300,6 +300,7 @@ def test_counter_generations_across_calls():
     from tests.parity import assert_same_frame
     big, small = (640, 480), (320, 200)
     with rz.Context(0) as c:
+        assert c.tuning() == (16, 0)   # default: automatic bin capacity
         scenes_by_size = {wh: scenes.icosphere_scene(subdivisions=4, width=wh[0], height=wh[1]) for wh in (big, small)}
         meshes = {wh: c.upload(sa[0]) for wh, sa in scenes_by_size.items()}
         bufs = {wh: (c.empty((3, wh[1], wh[0]), np.uint8), c.empty((wh[1], wh[0]), np.float32),
