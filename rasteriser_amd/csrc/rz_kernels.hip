@@ -54,6 +54,29 @@ __device__ __forceinline__ void store_block_sum(uint32_t v, uint32_t* out, int t
 }
 
 // ---------------------------------------------------------------------------------
+// store_block_sum of two values at once (one exchange through LDS instead of two).
+__device__ __forceinline__ void store_block_sums(uint32_t v0, uint32_t* out0, uint32_t v1, uint32_t* out1, int tid,
+                                                 int nthreads) {
+    __shared__ uint32_t s_part2[2][16];
+    v0 = wave_sum(v0);
+    v1 = wave_sum(v1);
+    if ((tid & 63) == 0) {
+        s_part2[0][tid >> 6] = v0;
+        s_part2[1][tid >> 6] = v1;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        uint32_t t0 = 0, t1 = 0;
+        for (int w = 0; w < (nthreads >> 6); ++w) {
+            t0 += s_part2[0][w];
+            t1 += s_part2[1][w];
+        }
+        *out0 = t0;
+        *out1 = t1;
+    }
+    __syncthreads();
+}
+
 // A model position to (x_px, y_px, ndc_z, 1/w): GLM mat4*vec4 (pairwise) with w = 1,
 // perspective divide, viewport (geometry.cpp, drawing.cpp:20-40).
 __device__ __forceinline__ float4 raster_vertex(const float* m, float x, float y, float z, float Wf, float Hf) {
@@ -197,8 +220,7 @@ __device__ __forceinline__ void bin_body(const BinParams& p, int blk) {
         }
     }
     const uint32_t nt = (tx0 <= tx1 && ty0 <= ty1) ? (tx1 - tx0 + 1) * (ty1 - ty0 + 1) : 0u;
-    store_block_sum(front ? 1u : 0u, p.part_front + blk, threadIdx.x, BLOCK);
-    store_block_sum(nt, p.part_pairs + blk, threadIdx.x, BLOCK);
+    store_block_sums(front ? 1u : 0u, p.part_front + blk, nt, p.part_pairs + blk, threadIdx.x, BLOCK);
     const bool big = nt > (uint32_t)kBigTriangleTiles;
     const bool small = nt > 0 && !big;
 
