@@ -19,19 +19,40 @@ pytestmark = pytest.mark.gpu
 
 
 def write_obj(path, scene):
-    """Triangles with v//vn corners, one material of Kd 1 (decimals that read back exactly)."""
+    """The scene's one material (Kd, and map_Kd when it has a texture, saved as PNG
+    beside the OBJ), its vertices, normals and uvs (%.9g: decimals that read back
+    exactly), and faces v/vt/vn (v//vn for a face without uvs)."""
+    mat = scene.materials[0]
     with open(path + ".mtl", "w") as f:
-        f.write("newmtl white\nKd 1 1 1\n")
+        f.write("newmtl m0\nKd %.9g %.9g %.9g\n" % tuple(float(c) for c in mat.diffuse))
+        if mat.texture is not None:
+            tex = np.ascontiguousarray(mat.texture)
+            H, W, C = tex.shape
+            L = C_lib()
+            assert L.rzh_save_png(os.path.join(os.path.dirname(path), "tex.png").encode(), tex.ctypes.data,
+                                  W, H, C) == 0
+            f.write("map_Kd tex.png\n")
     with open(path, "w") as f:
         f.write(f"mtllib {os.path.basename(path)}.mtl\n")
         for v in scene.vertices:
             f.write("v %.9g %.9g %.9g\n" % tuple(float(x) for x in v[:3]))
         for n in scene.normals:
             f.write("vn %.9g %.9g %.9g\n" % tuple(float(x) for x in n[:3]))
-        f.write("usemtl white\n")
+        for t in scene.uvs:
+            f.write("vt %.9g %.9g\n" % tuple(float(x) for x in t[:2]))
+        f.write("usemtl m0\n")
         for t in scene.faces:
-            v, n = t[0:3] + 1, t[3:6] + 1
-            f.write(f"f {v[0]}//{n[0]} {v[1]}//{n[1]} {v[2]}//{n[2]}\n")
+            v, n, u = t[0:3] + 1, t[3:6] + 1, t[6:9] + 1
+            if (t[6:9] >= 0).all():
+                f.write(f"f {v[0]}/{u[0]}/{n[0]} {v[1]}/{u[1]}/{n[1]} {v[2]}/{u[2]}/{n[2]}\n")
+            else:
+                f.write(f"f {v[0]}//{n[0]} {v[1]}//{n[1]} {v[2]}//{n[2]}\n")
+
+
+def C_lib():
+    L = C.CDLL(os.path.join(LIBDIR, "librz_host.so"))
+    L.rzh_save_png.argtypes = [C.c_char_p, C.c_void_p, C.c_int, C.c_int, C.c_int]
+    return L
 
 
 def write_lights(path, lights):
@@ -50,23 +71,32 @@ def load_png(path):
     return out
 
 
-@pytest.mark.parametrize("flat", [False, True])
-def test_renderer_cli_frame_matches_oracle(tmp_path, flat):
+@pytest.mark.parametrize("case", ["smooth", "flat", "textured"])
+def test_renderer_cli_frame_matches_oracle(tmp_path, case):
+    """UV sphere (smooth, flat), and a textured icosphere whose RGBA texture the renderer
+    loads as a PNG named by map_Kd."""
     if not os.path.exists(RENDERER):
         pytest.skip("renderer not built")
-    scene, args = scenes.uv_sphere_scene()
+    flat = case == "flat"
+    if case == "textured":
+        scene, args = scenes.icosphere_scene(subdivisions=4, width=480, height=360, texture_size=128)
+    else:
+        scene, args = scenes.uv_sphere_scene()
     args.flat = 1 if flat else 0
     obj = str(tmp_path / "sphere.obj")
     lights = str(tmp_path / "lights.csv")
     write_obj(obj, scene)
     write_lights(lights, scene.lights)
+    rot = []
+    for flag, angle in zip(("--rx", "--ry", "--rz"), args.tait_bryan_angles):
+        rot += [flag, "%.9g" % float(angle)]
     cmd = [RENDERER, "-o", obj, "--mats-dir", str(tmp_path) + "/", "-l", lights, "-x", str(args.image_width),
            "-y", str(args.image_height), "--fovy", "60", "--near", "0.1", "--far", "100", "--out-dir",
-           str(tmp_path)] + (["-f"] if flat else [])
+           str(tmp_path)] + rot + (["-f"] if flat else [])
     print("running", " ".join(cmd), flush=True)
     r = subprocess.run(cmd, capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stderr
     got = load_png(str(tmp_path / "frame.png"))
     of, od, oids, counts, _ = oracle.draw_frame(scene, args)
-    assert (oids >= 0).sum() > 50_000
+    assert (oids >= 0).sum() > (30_000 if case == "textured" else 50_000)
     assert np.array_equal(got[:, :, :3], np.moveaxis(of, 0, 2)), "frame.png differs from the oracle's frame"
