@@ -71,15 +71,18 @@ def load_png(path):
     return out
 
 
-@pytest.mark.parametrize("case", ["smooth", "flat", "textured"])
+@pytest.mark.parametrize("case", ["smooth", "flat", "textured", "textured_rgb"])
 def test_renderer_cli_frame_matches_oracle(tmp_path, case):
-    """UV sphere (smooth, flat), and a textured icosphere whose RGBA texture the renderer
-    loads as a PNG named by map_Kd."""
+    """UV sphere (smooth, flat), and a textured icosphere whose RGBA (or RGB) texture the
+    renderer loads as a PNG named by map_Kd."""
     if not os.path.exists(RENDERER):
         pytest.skip("renderer not built")
     flat = case == "flat"
-    if case == "textured":
+    if case.startswith("textured"):
         scene, args = scenes.icosphere_scene(subdivisions=4, width=480, height=360, texture_size=128)
+        if case == "textured_rgb":   # 3 channels: the normalisation spans R, G and B only
+            m = scene.materials[0]
+            scene.materials[0] = type(m)(m.diffuse, np.ascontiguousarray(m.texture[:, :, :3]), m.normal_map)
     else:
         scene, args = scenes.uv_sphere_scene()
     args.flat = 1 if flat else 0
@@ -98,5 +101,5 @@ def test_renderer_cli_frame_matches_oracle(tmp_path, case):
     assert r.returncode == 0, r.stderr
     got = load_png(str(tmp_path / "frame.png"))
     of, od, oids, counts, _ = oracle.draw_frame(scene, args)
-    assert (oids >= 0).sum() > (30_000 if case == "textured" else 50_000)
+    assert (oids >= 0).sum() > (30_000 if case.startswith("textured") else 50_000)
     assert np.array_equal(got[:, :, :3], np.moveaxis(of, 0, 2)), "frame.png differs from the oracle's frame"
