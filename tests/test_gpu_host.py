@@ -71,10 +71,12 @@ def load_png(path):
     return out
 
 
-@pytest.mark.parametrize("case", ["smooth", "flat", "textured", "textured_rgb"])
+@pytest.mark.parametrize("case", ["smooth", "flat", "textured", "textured_rgb", "spin_one_frame"])
 def test_renderer_cli_frame_matches_oracle(tmp_path, case):
-    """UV sphere (smooth, flat), and a textured icosphere whose RGBA (or RGB) texture the
-    renderer loads as a PNG named by map_Kd."""
+    """UV sphere (smooth, flat), a textured icosphere whose RGBA (or RGB) texture the
+    renderer loads as a PNG named by map_Kd, and the spin loop (-s: the mesh uploaded
+    once, frames drawn from the cached mesh) for one frame, drawn before its first
+    rotation step."""
     if not os.path.exists(RENDERER):
         pytest.skip("renderer not built")
     flat = case == "flat"
@@ -95,7 +97,7 @@ def test_renderer_cli_frame_matches_oracle(tmp_path, case):
         rot += [flag, "%.9g" % float(angle)]
     cmd = [RENDERER, "-o", obj, "--mats-dir", str(tmp_path) + "/", "-l", lights, "-x", str(args.image_width),
            "-y", str(args.image_height), "--fovy", "60", "--near", "0.1", "--far", "100", "--out-dir",
-           str(tmp_path)] + rot + (["-f"] if flat else [])
+           str(tmp_path)] + rot + (["-f"] if flat else []) + (["-s", "--frames", "1"] if case == "spin_one_frame" else [])
     print("running", " ".join(cmd), flush=True)
     r = subprocess.run(cmd, capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stderr
