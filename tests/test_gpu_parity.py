@@ -22,6 +22,15 @@ def test_icosphere_1080p(ctx, flat):
     assert (gpu[2] >= 0).sum() > 300000
 
 
+@pytest.mark.parametrize("seed", [3, 4, 5])
+def test_icosphere_1080p_more_rotations(ctx, seed):
+    """The 1080p workload at further seeded rotations, lights and textures (smooth and,
+    for one seed, flat): every face id, depth bit and colour byte as the oracle's."""
+    scene, args = scenes.icosphere_scene(seed=seed, flat=(seed == 4))
+    gpu, ref = check(ctx, scene, args, label=f"icosphere 1080p seed {seed}")
+    assert (gpu[2] >= 0).sum() > 300000
+
+
 @pytest.mark.parametrize("tile,cap", [(8, 64), (16, 512), (32, 2048), (16, 1), (16, 4)])
 def test_tuning_and_overflow(ctx, tile, cap):
     """Every tile size, and tiny bin capacities that force the overflow path."""
