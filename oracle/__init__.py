@@ -74,6 +74,7 @@ def ref_lib():
         P = C.c_void_p
         R.cimg_ref_normalize.argtypes = [P, C.c_int, C.c_int, C.c_int, P]
         R.cimg_ref_linear_atXY.argtypes = [P, C.c_int, C.c_int, C.c_int, P, P, C.c_int, C.c_int, P]
+        R.cimg_ref_save_depth_png.argtypes = [C.c_char_p, P, C.c_int, C.c_int]
         _REF = R
     return _REF
 

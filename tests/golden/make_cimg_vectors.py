@@ -5,7 +5,11 @@ tree's vendor/cimg/CImg.h) and records, for seeded inputs:
   * CImg<float>::normalize(0,1) of small 8-bit images (CImg.h:26786-26794), including
     a constant image, one already spanning [0,1] and a 3-channel one;
   * CImg<float>::_linear_atXY of channels 0..2 (CImg.h:13475-13491) at sample points
-    in range, on texel corners, on the far edges, outside the image and at NaN.
+    in range, on texel corners, on the far edges, outside the image and at NaN;
+  * the bytes of depth.png as the reference writes it, depth_buffer.normalize(0,255)
+    .save("depth.png") (renderer.cpp:93; CImg's libpng writer), for depth buffers with
+    the cleared value 1.0 around drawn depths, a constant buffer, a buffer holding
+    -0.0 and denormals, and one whose range is a few ulps wide.
 The vectors are data (inputs and outputs); they let the oracle be checked where the
 reference tree is absent.  Run from the repository root:
     python tests/golden/make_cimg_vectors.py
@@ -17,6 +21,44 @@ import numpy as np
 
 ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
 sys.path.insert(0, ROOT)
+
+
+def depth_buffers():
+    """Seeded depth buffers (name -> [H][W] float32) for the depth.png vectors."""
+    rng = np.random.default_rng(20261203)
+    out = {}
+    d = np.ones((23, 31), np.float32)
+    d[3:19, 4:27] = rng.uniform(0.93, 0.999, (16, 23)).astype(np.float32)
+    out["depth_frame"] = d
+    out["depth_const"] = np.full((5, 7), 0.25, np.float32)
+    d = rng.uniform(-1.0, 1.0, (6, 9)).astype(np.float32)
+    d[0, 0], d[0, 1], d[1, 0] = -0.0, np.float32(1e-40), np.float32(-3e-39)
+    out["depth_signed"] = d
+    base = np.float32(0.7)
+    d = np.array([[np.nextafter(base, np.float32(2), dtype=np.float32) if (x + y) % 3 == 0 else base
+                   for x in range(8)] for y in range(4)], np.float32)
+    d[2, 5] = np.float32(np.nextafter(np.nextafter(base, np.float32(2), dtype=np.float32), np.float32(2),
+                                      dtype=np.float32))
+    out["depth_narrow"] = d
+    return out
+
+
+def depth_png_vectors(ref):
+    """depth.png written by CImg (renderer.cpp:93), decoded to bytes."""
+    import tempfile
+
+    from PIL import Image
+    out = {}
+    with tempfile.TemporaryDirectory() as tmp:
+        for name, d in depth_buffers().items():
+            H, W = d.shape
+            path = os.path.join(tmp, name + ".png")
+            d = np.ascontiguousarray(d)
+            if ref.cimg_ref_save_depth_png(path.encode(), d.ctypes.data, W, H) != 0:
+                raise SystemExit(f"CImg could not write {path}")
+            out[name + "_in"] = d
+            out[name + "_png"] = np.asarray(Image.open(path)).copy()
+    return out
 
 
 def main():
@@ -53,6 +95,7 @@ def main():
         out[f"{name}_fx"] = fx
         out[f"{name}_fy"] = fy
         out[f"{name}_samples"] = vals
+    out.update(depth_png_vectors(ref))
     path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "cimg_vectors.npz")
     np.savez_compressed(path, **out)
     print("wrote", path, sorted(out))

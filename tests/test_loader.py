@@ -170,12 +170,40 @@ def test_png_codec_against_pillow(tmp_path):
         assert np.array_equal(load(L, p), np.asarray(Image.open(p)))
 
 
-def test_depth_png_is_truncated_normalisation(tmp_path):
+def test_depth_png_matches_cimg_vectors(tmp_path):
+    """depth.png = depth_buffer.normalize(0,255).save(...) (renderer.cpp:93): the bytes the
+    reference's own CImg wrote for the same depth buffers (tests/golden/cimg_vectors.npz:
+    cleared 1.0 around drawn depths, a constant buffer, -0.0 and denormals, a range a few
+    ulps wide)."""
     Image = pytest.importorskip("PIL.Image")
     L = host_lib()
-    d = np.array([[0.5, 0.75], [1.0, 0.6]], np.float32)
-    p = str(tmp_path / "d.png")
-    assert L.rzh_save_depth_png(p.encode(), d.ctypes.data, 2, 2) == 0
-    got = np.asarray(Image.open(p))
-    want = np.floor((d - np.float32(0.5)) / np.float32(0.5) * np.float32(255)).astype(np.uint8)
-    assert np.array_equal(got, want)
+    g = np.load(os.path.join(ROOT, "tests", "golden", "cimg_vectors.npz"))
+    names = [k[:-3] for k in g.files if k.startswith("depth_") and k.endswith("_in")]
+    assert len(names) == 4
+    for name in names:
+        d = np.ascontiguousarray(g[name + "_in"])
+        H, W = d.shape
+        p = str(tmp_path / f"{name}.png")
+        assert L.rzh_save_depth_png(p.encode(), d.ctypes.data, W, H) == 0
+        assert np.array_equal(np.asarray(Image.open(p)), g[name + "_png"]), name
+
+
+def test_depth_png_matches_live_cimg(tmp_path):
+    """The same against the reference's CImg writer itself, on a rendered depth buffer of
+    the oracle (oracle/_ref, built where the reference tree is present)."""
+    Image = pytest.importorskip("PIL.Image")
+    import oracle
+    from rasteriser_amd import scenes
+    ref = oracle.ref_lib()
+    if ref is None:
+        pytest.skip("oracle/_ref not built (reference tree absent)")
+    L = host_lib()
+    scene, args = scenes.uv_sphere_scene(width=96, height=64)
+    _, d, _, _, _ = oracle.draw_frame(scene, args)
+    for k, depth in enumerate([d, np.where(d < 1, d * np.float32(-3), d).astype(np.float32)]):
+        depth = np.ascontiguousarray(depth)
+        ours, theirs = str(tmp_path / f"ours{k}.png"), str(tmp_path / f"cimg{k}.png")
+        assert L.rzh_save_depth_png(ours.encode(), depth.ctypes.data, 96, 64) == 0
+        assert ref.cimg_ref_save_depth_png(theirs.encode(), depth.ctypes.data, 96, 64) == 0
+        a, b = np.asarray(Image.open(ours)), np.asarray(Image.open(theirs))
+        assert a.shape == (64, 96) and np.array_equal(a, b), k

@@ -510,3 +510,36 @@ def test_normal_map_tilts_toward_the_tangent():
     fb, _, ib, _, _ = oracle.draw_frame(scene, args)
     assert np.array_equal(ia, ib)
     assert np.abs(fa.astype(int) - fb.astype(int))[:, ia >= 0].mean() > 5
+
+
+def test_interpolated_normal_is_renormalised():
+    """drawing.cpp:131 normalises the interpolated normal: vertex normals of lengths 3,
+    0.5 and 7 along +z shade the square exactly like unit ones (Lambert with one light
+    along the normal is I*c*a/pi, a constant); without the normalisation the colour would
+    follow the interpolated length across each face."""
+    scene = scenes.square_scene()
+    scene.normals = np.array([[0, 0, 3.0], [0, 0, 0.5], [0, 0, 7.0]], np.float32)
+    scene.faces = make_faces([[0, 1, 2], [2, 1, 3]], n=[[0, 1, 2], [2, 1, 0]], t=[[0, 1, 2], [2, 1, 3]])
+    I = 60.5 * math.pi
+    scene.lights = make_lights([[0, 0, -1]], [I], [[1, 1, 1]])
+    f, d, ids, _, _ = oracle.draw_frame(scene, make_args(96, 64))
+    cov = ids >= 0
+    assert cov.sum() > 500
+    assert (f[:, cov] == 60).all()          # trunc(60.5 * dot), dot within an ulp or two of 1
+
+
+def test_light_direction_is_renormalised():
+    """geometry.cpp:124-133: trans_dir = normalize(view * (direction, 0)).  Directions of
+    length 4 and 15 come back as unit vectors (the view matrix is a translation, so the
+    direction is only rescaled), and shading uses the unit vector: a light at 2/3 of the
+    way to head-on gives trunc(I*c*a/pi * 2/3)."""
+    scene = scenes.square_scene()
+    I = 150.75 * math.pi
+    for direction, cos in (([0, 0, -4.0], 1.0), ([5.0, 10.0, -10.0], 2.0 / 3.0)):
+        scene.lights = make_lights([direction], [I], [[1, 1, 1]])
+        f, d, ids, _, lights = oracle.draw_frame(scene, make_args(96, 64))
+        t = lights[0, 7:10].astype(np.float64)
+        want = np.array(direction) / np.linalg.norm(direction)
+        assert np.abs(t - want).max() <= 2 * np.spacing(np.float32(1)), (t, want)
+        cov = ids >= 0
+        assert cov.sum() > 500 and (f[:, cov] == math.floor(150.75 * cos)).all(), direction
