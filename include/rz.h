@@ -30,7 +30,7 @@
 extern "C" {
 #endif
 
-#define RZ_ABI_VERSION 1
+#define RZ_ABI_VERSION 2
 
 typedef enum rz_status {
     RZ_OK = 0,
@@ -152,6 +152,8 @@ typedef struct rz_frame_stats {
     float ms_overflow;
     float ms_shade;
     float ms_frame;
+    uint64_t raster_tiles;        /* tiles that received a face: the ones the raster pass visits */
+    uint64_t raster_pixels;       /* their pixels inside the image                           */
 } rz_frame_stats;
 
 typedef struct rz_ctx rz_ctx;
@@ -205,7 +207,8 @@ void rz_mesh_destroy(rz_ctx* ctx, rz_mesh* mesh);
 
 /* Replaces draw_frame (headers/drawing.h:16-18, drawing.cpp:205-258) for a cached
  * mesh.  lights[i].trans_dir is written back (drawing.cpp:238 mutates the caller's
- * lights).  frame_buffer: uint8 [3][H][W]; depth_buffer: float [H][W]; both are read
+ * lights); any number of lights up to 2^20 (the first 64 travel in the kernel
+ * arguments, the rest in a device array).  Image sides up to 65,535 pixels.  frame_buffer: uint8 [3][H][W]; depth_buffer: float [H][W]; both are read
  * (unless RZ_FLAG_CLEARED) and written in place like the reference, which does not
  * clear them.  id_buffer (may be NULL): int32 [H][W], the index of the face whose
  * fragment the pixel finally holds, or -1; a pixel the frame does not touch keeps
@@ -242,6 +245,34 @@ rz_status rz_graph_create(rz_ctx* ctx, const rz_mesh* mesh, rz_light* lights, in
 rz_status rz_graph_launch(rz_graph* graph);
 void rz_graph_destroy(rz_graph* graph);
 
+/* ---- the spin loop ---------------------------------------------------------------- */
+
+typedef struct rz_spin rz_spin;
+
+/* No reference counterpart as an object: the reference's spin loop (renderer.cpp:94-126)
+ * calls draw_frame once per iteration with tait_bryan_angles.y advanced by the elapsed
+ * time.  rz_spin_create captures that frame once (device buffers, RZ_FLAG_DEVICE_BUFFERS
+ * required; light loop only); each rz_spin_frame sets the frame's rotation in the captured
+ * kernels' arguments (the matrices built on the host exactly as rz_draw_frame builds them)
+ * and replays it on the context stream, asynchronously, with no re-capture.  The results
+ * are rz_draw_frame's for those angles, bit for bit.  The mesh and buffers must outlive
+ * the spin object; a later frame that grows the context's buffers makes rz_spin_frame
+ * refuse it. */
+rz_status rz_spin_create(rz_ctx* ctx, const rz_mesh* mesh, rz_light* lights, int32_t num_lights,
+                         const rz_args* args, uint8_t* frame_device, float* depth_device, int32_t* id_device,
+                         uint32_t flags, rz_spin** out_spin);
+rz_status rz_spin_frame(rz_spin* spin, const float tait_bryan_angles[3]);
+void rz_spin_destroy(rz_spin* spin);
+
+/* Reference: renderer.cpp:114, frame_buffer.draw_text(5, 5, text, white, black), the spin
+ * loop's frame rate.  Draws up to 32 characters of text into a planar device frame at
+ * (x, y) on the context stream (asynchronous): 12x16-pixel cells, foreground where the
+ * glyph is set and background over the rest of each cell, clipped to the frame.  Glyphs
+ * are librz's own 5x7 bitmaps drawn at twice their size, for the characters a frame rate
+ * prints (digits, '.', '-', "inf", "nan"; others draw as background), not CImg's font. */
+rz_status rz_draw_text(rz_ctx* ctx, uint8_t* frame_device, uint32_t width, uint32_t height, int32_t x,
+                       int32_t y, const char* text, const uint8_t foreground[3], const uint8_t background[3]);
+
 /* ---- batches ----------------------------------------------------------------- */
 
 /* One mesh of a batch: its own image size, transform and lights (args), and its own
@@ -258,13 +289,21 @@ typedef struct rz_batch_item {
 } rz_batch_item;
 
 /* No reference counterpart (the reference draws one mesh per call).  Draws every
- * item as rz_draw_frame would, with each pass of every item in one launch (5
- * launches per batch; the raster-and-shade pass is one resident grid over the tiles
- * of all items).  At most 1024 items.  Requires RZ_FLAG_DEVICE_BUFFERS;
- * RZ_FLAG_CLEARED and RZ_FLAG_ASYNC apply to all items.  stats (may be NULL) sums
- * over the items. */
+ * item as rz_draw_frame would (bit for bit), each pass of every item in one launch (see
+ * rz_set_batch_persistent: all passes in one persistent launch, a work queue of every
+ * item's vertex transform, binning and buffer clears, then the raster-and-shade of the
+ * tiles of all items, the overflow loop of tiles whose bins overflowed and their shade,
+ * each waiting only on the work it reads).  At most 1024 items.  Requires RZ_FLAG_DEVICE_BUFFERS; RZ_FLAG_CLEARED and RZ_FLAG_ASYNC apply to
+ * all items.  stats (may be NULL) sums over the items; with rz_set_profiling its ms_*
+ * are the spans of each phase's work inside the launch (they overlap) and ms_frame the
+ * launch. */
 rz_status rz_draw_batch(rz_ctx* ctx, const rz_batch_item* items, int32_t num_items, uint32_t flags,
                         rz_frame_stats* stats);
+/* 1: a batch frame is the one persistent launch above; 0 (default): five launches, one
+ * per pass (vertex, bin, raster-and-shade, overflow, overflow shade).  Same bits.  Read
+ * when a batch is prepared (rz_draw_batch, rz_batch_graph_create, rz_autotune). */
+rz_status rz_set_batch_persistent(rz_ctx* ctx, int32_t enable);
+int32_t rz_get_batch_persistent(const rz_ctx* ctx);
 /* `frames` repetitions of the batch captured in one HIP graph that owns its working
  * buffers and parameter tables (independent of later calls on the context); the
  * meshes and output buffers must outlive it. */
@@ -280,11 +319,15 @@ rz_status rz_autotune(rz_ctx* ctx, const rz_batch_item* items, int32_t num_items
 
 /* No reference counterpart.  Times each pass of one frame on the context's stream: a
  * HIP graph of `reps` back-to-back launches of the pass, timed by HIP events around its
- * second launch.  us[0..5] = vertex, bin, raster, overflow, shade, network in
+ * second launch.  us[0..5] = vertex, bin, raster, overflow, shade, shade chain in
  * microseconds per launch (vertex = 0: the bin pass transforms each face's corners
- * itself; overflow = 0: the overflow loop runs inside the raster pass; network = the
- * learned shader's second pass, 0 without RZ_FLAG_MLP_SHADE).  Device buffers and
- * RZ_FLAG_CLEARED required.  Leaves the frame rendered in frame_device/depth_device. */
+ * itself; overflow = 0: the overflow loop runs inside the raster pass; shade = the
+ * separate shade pass, or with the fused pipeline the overflow shade, 0 when no bin can
+ * overflow).  "shade chain" is the pass that shades (raster+shade fused, else the separate
+ * shade pass) run with neither the light loop nor the network: the per-pixel work every
+ * shading mode shares, so the pass's time minus it is the light loop's (or with
+ * RZ_FLAG_MLP_SHADE the network's) share.  Device buffers and RZ_FLAG_CLEARED required.
+ * Leaves the frame rendered in frame_device/depth_device. */
 rz_status rz_time_passes(rz_ctx* ctx, const rz_mesh* mesh, rz_light* lights, int32_t num_lights,
                          const rz_args* args, uint8_t* frame_device, float* depth_device, uint32_t flags,
                          int32_t reps, float us[6]);

@@ -112,6 +112,31 @@ class FrameGraph:
             pass
 
 
+class SpinLoop:
+    """The spin loop's frame captured once (rz_spin_create); frame(angles) replays it
+    with that rotation."""
+
+    def __init__(self, ctx: "Context", handle):
+        self.ctx, self.handle = ctx, handle
+        ctx._children.add(self)
+
+    def frame(self, angles):
+        a = (C.c_float * 3)(*[float(x) for x in angles])
+        _check(self.ctx.lib.rz_spin_frame(self.handle, a), self.ctx.handle)
+
+    def close(self):
+        if self.handle:
+            if self.ctx.handle:
+                self.ctx.lib.rz_spin_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 class DeviceArray:
     """A device allocation of the context's device holding an array of `shape`/`dtype`."""
 
@@ -203,6 +228,14 @@ class Context:
     def fused_shade(self) -> bool:
         return bool(self.lib.rz_get_fused_shade(self.handle))
 
+    def set_batch_persistent(self, on: bool):
+        """rz_set_batch_persistent: a batch frame as one persistent launch (default) or five."""
+        _check(self.lib.rz_set_batch_persistent(self.handle, int(bool(on))), self.handle)
+
+    @property
+    def batch_persistent(self) -> bool:
+        return bool(self.lib.rz_get_batch_persistent(self.handle))
+
     def set_profiling(self, on: bool):
         _check(self.lib.rz_set_profiling(self.handle, int(bool(on))), self.handle)
 
@@ -267,6 +300,25 @@ class Context:
                                         C.byref(h)), self.handle)
         return FrameGraph(self, h, frames)
 
+    def spin(self, mesh: Mesh, lights: np.ndarray, args: Args, frame_ptr: int, depth_ptr: int,
+             ids_ptr: int = 0, flags: int = RZ_FLAG_CLEARED) -> SpinLoop:
+        """rz_spin_create over device buffers."""
+        lights = _lights(lights)
+        h = C.c_void_p()
+        _check(self.lib.rz_spin_create(self.handle, mesh.handle, lights.ctypes.data, lights.shape[0],
+                                       C.byref(args), C.c_void_p(frame_ptr), C.c_void_p(depth_ptr),
+                                       C.c_void_p(ids_ptr) if ids_ptr else None,
+                                       flags | RZ_FLAG_DEVICE_BUFFERS, C.byref(h)), self.handle)
+        return SpinLoop(self, h)
+
+    def draw_text(self, frame_ptr: int, width: int, height: int, x: int, y: int, text: str,
+                  fg=(255, 255, 255), bg=(0, 0, 0)):
+        """rz_draw_text into a planar device frame (asynchronous on the context stream)."""
+        f = (C.c_uint8 * 3)(*fg)
+        b = (C.c_uint8 * 3)(*bg)
+        _check(self.lib.rz_draw_text(self.handle, C.c_void_p(frame_ptr), int(width), int(height), int(x), int(y),
+                                     text.encode(), f, b), self.handle)
+
     def _batch_items(self, batch):
         """batch: list of (Mesh, lights (L,10) float32, Args, frame DeviceArray, depth DeviceArray,
         ids DeviceArray or None)."""
@@ -311,14 +363,14 @@ class Context:
     def time_passes(self, mesh: Mesh, lights: np.ndarray, args: Args, frame: "DeviceArray",
                     depth: "DeviceArray", reps: int = 200, flags: int = RZ_FLAG_CLEARED) -> dict:
         """rz_time_passes: microseconds per launch of each pass (graph of `reps` launches);
-        "network" is the learned shader's second pass (0 without RZ_FLAG_MLP_SHADE)."""
+        "chain" is the shading pass run with neither light loop nor network."""
         lights = _lights(lights)
         us = (C.c_float * 6)()
         _check(self.lib.rz_time_passes(self.handle, mesh.handle, lights.ctypes.data, lights.shape[0],
                                        C.byref(args), C.c_void_p(frame.ptr), C.c_void_p(depth.ptr),
                                        flags | RZ_FLAG_CLEARED | RZ_FLAG_DEVICE_BUFFERS, int(reps), us),
                self.handle)
-        return dict(zip(("vertex", "bin", "raster", "overflow", "shade", "network"), [float(u) for u in us]))
+        return dict(zip(("vertex", "bin", "raster", "overflow", "shade", "chain"), [float(u) for u in us]))
 
     # ---- learned shader -------------------------------------------------------------
     def mlp_init(self, seed: int = 0):

@@ -77,7 +77,8 @@ class FrameStats(C.Structure):
                 ("bin_pairs", C.c_uint64), ("overflow_tiles", C.c_uint64),
                 ("fragments", C.c_uint64), ("covered_pixels", C.c_uint64),
                 ("ms_vertex", C.c_float), ("ms_bin", C.c_float), ("ms_raster", C.c_float),
-                ("ms_overflow", C.c_float), ("ms_shade", C.c_float), ("ms_frame", C.c_float)]
+                ("ms_overflow", C.c_float), ("ms_shade", C.c_float), ("ms_frame", C.c_float),
+                ("raster_tiles", C.c_uint64), ("raster_pixels", C.c_uint64)]
 
     def as_dict(self):
         return {name: getattr(self, name) for name, _ in self._fields_}
@@ -130,7 +131,14 @@ def declare(lib):
                                       C.c_int32, C.c_float, C.POINTER(P)]),
         "rz_graph_launch": (C.c_int, [P]),
         "rz_graph_destroy": (None, [P]),
+        "rz_spin_create": (C.c_int, [P, P, P, C.c_int32, C.POINTER(Args), P, P, P, C.c_uint32, C.POINTER(P)]),
+        "rz_spin_frame": (C.c_int, [P, C.POINTER(C.c_float)]),
+        "rz_spin_destroy": (None, [P]),
+        "rz_draw_text": (C.c_int, [P, P, C.c_uint32, C.c_uint32, C.c_int32, C.c_int32, C.c_char_p,
+                                   C.POINTER(C.c_uint8), C.POINTER(C.c_uint8)]),
         "rz_draw_batch": (C.c_int, [P, P, C.c_int32, C.c_uint32, C.POINTER(FrameStats)]),
+        "rz_set_batch_persistent": (C.c_int, [P, C.c_int32]),
+        "rz_get_batch_persistent": (C.c_int32, [P]),
         "rz_batch_graph_create": (C.c_int, [P, P, C.c_int32, C.c_uint32, C.c_int32, C.POINTER(P)]),
         "rz_autotune": (C.c_int, [P, P, C.c_int32, C.c_uint32, C.POINTER(C.c_int32), C.POINTER(C.c_int32),
                                   C.POINTER(C.c_float)]),
@@ -161,8 +169,9 @@ def declare(lib):
 EXPORTED = ("rz_abi_version", "rz_status_string", "rz_last_error", "rz_create", "rz_destroy",
             "rz_set_tuning", "rz_get_tuning", "rz_set_fused_shade", "rz_get_fused_shade", "rz_set_profiling", "rz_stream", "rz_sync",
             "rz_mesh_create", "rz_mesh_destroy", "rz_draw_frame", "rz_draw_frame_desc",
-            "rz_graph_create", "rz_graph_launch", "rz_graph_destroy",
-            "rz_draw_batch", "rz_batch_graph_create", "rz_autotune",
+            "rz_graph_create", "rz_graph_launch", "rz_graph_destroy", "rz_spin_create", "rz_spin_frame",
+            "rz_spin_destroy", "rz_draw_text",
+            "rz_draw_batch", "rz_set_batch_persistent", "rz_get_batch_persistent", "rz_batch_graph_create", "rz_autotune",
             "rz_time_passes", "rz_mlp_init", "rz_mlp_set_weights", "rz_mlp_get_weights", "rz_mlp_collect",
             "rz_mlp_train", "rz_mlp_infer",
             "rz_device_alloc", "rz_device_free", "rz_copy_to_device", "rz_copy_to_host", "rz_copy_device",

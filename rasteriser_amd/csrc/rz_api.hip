@@ -25,13 +25,16 @@ hipError_t launch_vertex_batch(const VertexParams* items, const uint2* map, uint
 hipError_t launch_bin_batch(const BinParams* items, const uint2* map, uint32_t blocks, hipStream_t s);
 hipError_t launch_overflow_batch(const RasterParams* items, const uint2* map, uint32_t blocks, hipStream_t s);
 uint32_t raster_batch_blocks_per_cu(int tile_size, int mode);
+uint32_t batch_frame_blocks_per_cu(int tile_size, int mode);
+hipError_t launch_batch_frame(const VertexParams* v, const BinParams* b, const RasterParams* r,
+                              const RasterParams* o, const ShadeParams* sh, const BatchLists& bl,
+                              const BatchQueue& q, int tile_size, int mode, hipStream_t s);
 hipError_t launch_batch_tiles(const RasterParams* items, const ShadeParams* sitems, const BatchLists& bl,
                               uint32_t raster_grid, uint32_t tiles_grid, int tile_size, int mode, hipStream_t s,
                               bool raster);
 hipError_t launch_hold(uint64_t ticks, hipStream_t s);
-hipError_t launch_mlp_shade(const void* records, const uint32_t* counts, uint32_t region, const uint32_t* lists,
-                            const uint32_t* list_count, uint32_t ntiles, const uint16_t* packed, uint8_t* frame,
-                            size_t np, uint32_t blocks, hipStream_t s);
+hipError_t launch_text(uint8_t* frame, uint32_t W, uint32_t H, int32_t x0, int32_t y0, const char* text,
+                       const uint8_t fg[3], const uint8_t bg[3], hipStream_t s);
 uint32_t raster_blocks_per_cu(int tile_size, bool fused, int mode, bool narrow);
 hipError_t launch_copy(const void* src, void* dst, size_t n16, hipStream_t s);
 hipError_t launch_mlp_pack(const float* W, uint16_t* packed, hipStream_t s);
@@ -45,7 +48,11 @@ using namespace rz;
 
 namespace {
 
-constexpr uint32_t kMaxImageSide = 32767;   // bbox corners are packed in 16 bits
+// bbox corners are packed in 16 bits each (largest corner W - 1, H - 1)
+constexpr uint32_t kMaxImageSide = 65535;
+// Lights per frame: the first kMaxLights travel in the kernel arguments, the rest in a
+// device array (the reference takes any number, shading.cpp:24-29).
+constexpr int32_t kMaxFrameLights = 1 << 20;
 // Workgroups per CU of the overflow shade launch (persistent: it has work only when a
 // bin overflowed).
 constexpr int kPersistentPerCU = 4;
@@ -93,6 +100,7 @@ struct rz_graph {
     int32_t ntiles = 0;        // frame graphs: tiles of the captured frames (counter-line layout)
     bool owns_buffers = false; // batch graphs carry their own working buffers and tables
     DevBuf arena, tables;
+    DevBuf lights_ext;         // frame graphs with more lights than the kernel arguments hold
 };
 
 struct rz_ctx {
@@ -101,6 +109,7 @@ struct rz_ctx {
     int tile_size = 16;
     int bin_cap = 0;   // 0: automatic (frame_bin_cap)
     bool fused = true;
+    bool batch_persistent = false;  // a batch frame is one launch (k_batch_frame), else five
     bool profiling = false;
     uint32_t persistent_blocks = 0;   // workgroups of the overflow shade launch (a few per CU)
     int cus = 0;
@@ -110,8 +119,7 @@ struct rz_ctx {
     DevBuf batch_arena, batch_tables;
     // learned shader: FP32 parameters, Adam moments, packed BF16 network, training scratch
     DevBuf mlp_w, mlp_m, mlp_v, mlp_packed, mlp_gpart, mlp_lpart, mlp_loss, mlp_count;
-    DevBuf frag_records;   // the learned shader's fragment records (kFragRecordBytes per pixel)
-    DevBuf frag_counts;    // records per region
+    DevBuf lights_ext;     // eager frames' lights beyond the kernel arguments
     bool mlp_ready = false;
     uint32_t mlp_steps = 0;
     std::vector<uint32_t> host_partials;
@@ -145,7 +153,9 @@ rz_status fail(rz_ctx* ctx, rz_status st, const char* fmt, ...) {
                         #call, hipGetErrorString(e_), __FILE__, __LINE__);                         \
     } while (0)
 
-rz_status ensure(rz_ctx* ctx, DevBuf& b, size_t bytes) {
+// Grows a working buffer to `bytes`.  stales_graphs: frame graphs captured over the old
+// allocation refuse to launch afterwards (ctx->buffer_gen).
+rz_status ensure(rz_ctx* ctx, DevBuf& b, size_t bytes, bool stales_graphs = true) {
     if (bytes == 0) bytes = 256;
     if (b.bytes >= bytes) return RZ_OK;
     if (b.ptr) {
@@ -156,7 +166,7 @@ rz_status ensure(rz_ctx* ctx, DevBuf& b, size_t bytes) {
     }
     RZ_HIP(ctx, hipMalloc(&b.ptr, bytes));
     b.bytes = bytes;
-    ++ctx->buffer_gen;
+    if (stales_graphs) ++ctx->buffer_gen;
     if (&b == &ctx->tile_count) ctx->count_ntiles = -1;   // new memory: contents unknown
     return RZ_OK;
 }
@@ -239,7 +249,7 @@ enum { kBufRaster, kBufCnormal, kBufRec, kBufTiles, kBufBins, kBufVis, kBufLists
 // the tile lists and the statistics of those two launches, and each mesh keeps the
 // partial sums of its overflow loop (k_overflow_batch).
 FrameLayout layout_frame(const rz_mesh* mesh, const rz_args& a, int ts, int cap, bool fused, uint32_t raster_grid,
-                         uint32_t tiles_grid, bool batch) {
+                         uint32_t tiles_grid, bool batch, int bblock = 256) {
     FrameLayout L;
     L.ts = ts;
     L.fused = fused || batch;
@@ -256,11 +266,11 @@ FrameLayout layout_frame(const rz_mesh* mesh, const rz_args& a, int ts, int cap,
     L.np = (size_t)L.W * L.H;
     L.nfn = a.flat ? mesh->nf : mesh->nn;
     L.off_front = 0;
-    const size_t nbin = batch ? bin_blocks(mesh->nf) : frame_bin_blocks(mesh->nf);   // bin workgroups
+    const size_t nbin = batch ? bin_blocks(mesh->nf, bblock) : frame_bin_blocks(mesh->nf);   // bin workgroups
     L.off_pairs = L.off_front + nbin;
     L.off_frag = L.off_pairs + nbin;
     L.off_ovf = L.off_frag + (batch ? 0 : (size_t)raster_grid);
-    L.off_cov = L.off_ovf + (batch ? overflow_blocks(mesh->nf) : 0);   // single frames: in the raster pass
+    L.off_cov = L.off_ovf + (batch ? overflow_blocks(mesh->nf, bblock) : 0);   // single frames: in the raster pass
     L.n_part = L.off_cov + (batch ? 0
                                   : L.fused ? (size_t)raster_grid + tiles_grid
                                             : (size_t)shade_blocks_x(L.W) * shade_blocks_y(L.H));
@@ -291,14 +301,16 @@ struct FrameLaunch {
     FrameLayout lay;
     uint32_t* part = nullptr;
     unsigned long long* counters = nullptr;
-    uint32_t mlp_blocks = 0;   // learned shader: workgroups of k_mlp_shade (0: not used)
     int gen = 0;               // single frames: the generation of counter lines used
+    // lights beyond the kernel arguments (six floats each, ShadeParams::lights_ext), for
+    // the caller to place in device memory
+    std::vector<float> ext_lights;
 };
 
 rz_status check_frame(rz_ctx* ctx, const rz_mesh* mesh, const rz_light* lights, int32_t nl, const rz_args* args) {
     if (!mesh || !args) return fail(ctx, RZ_ERR_INVALID_ARG, "null mesh or args");
-    if (nl < 0 || nl > kMaxLights || (nl > 0 && !lights))
-        return fail(ctx, RZ_ERR_INVALID_ARG, "light count %d not in [0,%d]", nl, kMaxLights);
+    if (nl < 0 || nl > kMaxFrameLights || (nl > 0 && !lights))
+        return fail(ctx, RZ_ERR_INVALID_ARG, "light count %d not in [0,%d]", nl, kMaxFrameLights);
     const uint32_t W = args->image_width, H = args->image_height;
     if (W == 0 || H == 0 || W > kMaxImageSide || H > kMaxImageSide)
         return fail(ctx, RZ_ERR_INVALID_ARG, "image %ux%u not in [1,%u]^2", W, H, kMaxImageSide);
@@ -318,11 +330,14 @@ void fill_frame(const rz_mesh* mesh, rz_light* lights, int32_t nl, const rz_args
     build_matrices(a, mats);
     L.lay = lay;
     std::memset(&L.sp, 0, sizeof L.sp);
+    L.ext_lights.assign(nl > kMaxLights ? 6 * (size_t)(nl - kMaxLights) : 0, 0.0f);
     for (int32_t l = 0; l < nl; ++l) {
         light_direction(mats.view, lights[l].direction, lights[l].trans_dir);   // drawing.cpp:238
+        float* ic = l < kMaxLights ? L.sp.light_ic[l] : &L.ext_lights[6 * (size_t)(l - kMaxLights)];
+        float* nd = l < kMaxLights ? L.sp.light_nd[l] : ic + 3;
         for (int c = 0; c < 3; ++c) {
-            L.sp.light_ic[l][c] = lights[l].intensity * lights[l].colour[c];
-            L.sp.light_nd[l][c] = -lights[l].trans_dir[c];
+            ic[c] = lights[l].intensity * lights[l].colour[c];
+            nd[c] = -lights[l].trans_dir[c];
         }
     }
     const size_t gen_words = (size_t)count_lines(lay.ntiles) * kCountStride;
@@ -460,24 +475,46 @@ int frame_bin_cap(const rz_ctx* ctx, const rz_mesh* mesh, uint32_t ntiles) {
     return kDefaultBinCap;
 }
 
-// Single frame: validates, sizes the context's working buffers and fills L.
+// Launch shape of a single frame's raster pass on this context: 64-thread workgroups on
+// 16-pixel tiles for frames of many tiles, and the resident grid (a multiple of 8).
+// 16-pixel tiles: 256-thread workgroups finish one tile soonest, and at 1080p every
+// covered tile is in flight at once (1,490 tiles, 1,536 workgroups); with many more
+// tiles than resident workgroups, 64-thread workgroups keep four times as many tiles
+// in flight (1080p: 23 against 63 us; 4M triangles at 4096x4096: 326 against 254 us)
+void frame_shape(const rz_ctx* ctx, const rz_args& a, int mode, uint32_t& ntiles, bool& narrow, uint32_t& grid) {
+    ntiles = ((a.image_width + ctx->tile_size - 1) / ctx->tile_size) *
+             ((a.image_height + ctx->tile_size - 1) / ctx->tile_size);
+    narrow = ctx->tile_size == 16 && ntiles >= kNarrowTiles;
+    const uint32_t per_cu = raster_blocks_per_cu(ctx->tile_size, ctx->fused, mode, narrow);
+    grid = (uint32_t)ctx->cus * per_cu / 8u * 8u;
+    if (grid < 8) grid = 8;
+}
+
+// The lights beyond the kernel arguments into `buf` (grown as needed; the stream is
+// drained first, since queued frames may still read the old contents).
+rz_status stage_ext_lights(rz_ctx* ctx, DevBuf& buf, FrameLaunch& L, bool stales_graphs) {
+    L.sp.lights_ext = nullptr;
+    if (L.ext_lights.empty()) return RZ_OK;
+    const size_t bytes = L.ext_lights.size() * sizeof(float);
+    rz_status st = ensure(ctx, buf, bytes, stales_graphs);
+    if (st != RZ_OK) return st;
+    RZ_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    RZ_HIP(ctx, hipMemcpy(buf.ptr, L.ext_lights.data(), bytes, hipMemcpyHostToDevice));
+    L.sp.lights_ext = (const float*)buf.ptr;
+    return RZ_OK;
+}
+
+// Single frame: validates, sizes the context's working buffers and fills L.  mode: the
+// shading mode (-1: the one flags select).
 rz_status prepare_frame(rz_ctx* ctx, const rz_mesh* mesh, rz_light* lights, int32_t nl, const rz_args* args,
-                        uint32_t flags, FrameLaunch& L, int gen) {
+                        uint32_t flags, FrameLaunch& L, int gen, int mode = -1) {
     rz_status st = check_frame(ctx, mesh, lights, nl, args);
     if (st != RZ_OK) return st;
     RZ_HIP(ctx, hipSetDevice(ctx->device));
-    // single frames run the learned shader in two phases: fragment records, then k_mlp_shade
-    const int mode = (flags & RZ_FLAG_MLP_SHADE) ? kShadeFeat : kShadePlain;
-    // 16-pixel tiles: 256-thread workgroups finish one tile soonest, and at 1080p every
-    // covered tile is in flight at once (1,490 tiles, 1,536 workgroups); with many more
-    // tiles than resident workgroups, 64-thread workgroups keep four times as many tiles
-    // in flight (1080p: 23 against 63 us; 4M triangles at 4096x4096: 326 against 254 us)
-    const uint32_t ntiles = ((args->image_width + ctx->tile_size - 1) / ctx->tile_size) *
-                            ((args->image_height + ctx->tile_size - 1) / ctx->tile_size);
-    const bool narrow = ctx->tile_size == 16 && ntiles >= kNarrowTiles;
-    const uint32_t per_cu = raster_blocks_per_cu(ctx->tile_size, ctx->fused, mode, narrow);
-    uint32_t grid = (uint32_t)ctx->cus * per_cu / 8u * 8u;
-    if (grid < 8) grid = 8;
+    if (mode < 0) mode = (flags & RZ_FLAG_MLP_SHADE) ? kShadeMlp : kShadePlain;
+    uint32_t ntiles = 0, grid = 0;
+    bool narrow = false;
+    frame_shape(ctx, *args, mode, ntiles, narrow, grid);
     const int cap = frame_bin_cap(ctx, mesh, ntiles);
     const bool can_overflow = (uint32_t)cap < (uint32_t)mesh->nf;
     const FrameLayout lay = layout_frame(mesh, *args, ctx->tile_size, cap, ctx->fused, grid,
@@ -492,31 +529,12 @@ rz_status prepare_frame(rz_ctx* ctx, const rz_mesh* mesh, rz_light* lights, int3
     if ((flags & RZ_FLAG_MLP_SHADE) && !ctx->mlp_ready)
         return fail(ctx, RZ_ERR_INVALID_ARG, "RZ_FLAG_MLP_SHADE without network weights (rz_mlp_init / rz_mlp_set_weights)");
     fill_frame(mesh, lights, nl, *args, flags, lay, ptrs, L, gen);
+    if ((st = stage_ext_lights(ctx, ctx->lights_ext, L, false)) != RZ_OK) return st;
     L.rp.narrow = narrow ? 1 : 0;
+    L.sp.mode = mode;
     L.sp.mlp = (const uint16_t*)ctx->mlp_packed.ptr;
     L.sp.max_blocks = lay.tiles_grid;
-    if (mode == kShadeFeat) {
-        // one region of records per tile
-        const uint32_t region = (uint32_t)(lay.ts * lay.ts);
-        if ((st = ensure(ctx, ctx->frag_records, (size_t)lay.ntiles * region * kFragRecordBytes)) != RZ_OK) return st;
-        if ((st = ensure(ctx, ctx->frag_counts, (size_t)lay.ntiles * 4 * kCountStride)) != RZ_OK) return st;
-        L.sp.mode = kShadeFeat;
-        L.sp.samples = (float*)ctx->frag_records.ptr;
-        L.sp.rec_counts = (uint32_t*)ctx->frag_counts.ptr;
-        L.sp.rec_region = region;
-        L.sp.tiles_x = lay.tiles_x;
-        L.sp.ts_log2 = lay.tsl;
-        if (!lay.fused) L.rp.rec_counts = L.sp.rec_counts;   // zeroed per listed tile by k_raster
-        L.mlp_blocks = (uint32_t)ctx->cus * 8u;
-    }
     return RZ_OK;
-}
-
-// The learned shader's second phase, after the pass that appended the fragment records.
-hipError_t launch_mlp_phase(const FrameLaunch& L, hipStream_t s) {
-    if (L.sp.mode != kShadeFeat) return hipSuccess;
-    return launch_mlp_shade(L.sp.samples, L.sp.rec_counts, L.sp.rec_region, L.rp.lists, L.rp.list_count,
-                            (uint32_t)L.lay.ntiles, L.sp.mlp, L.sp.frame, L.lay.np, L.mlp_blocks, s);
 }
 
 // The overflow shade's parameters: its coverage partials follow the raster pass's.
@@ -581,7 +599,6 @@ rz_status enqueue_frame(rz_ctx* ctx, const FrameLaunch& L, hipStream_t s, bool p
         RZ_HIP(ctx, launch_shade(L.sp, s));
     else if (L.lay.tiles_grid > 0)   // (0: no bin can overflow)
         RZ_HIP(ctx, launch_shade_tiles(L.rp, tiles_params(L), L.lay.ts, s));
-    RZ_HIP(ctx, launch_mlp_phase(L, s));
     if (prof) RZ_HIP(ctx, hipEventRecord(ctx->ev[5], s));
     return RZ_OK;
 }
@@ -631,6 +648,28 @@ void reduce_stats(const FrameLaunch& L, const unsigned long long* counters, cons
     st->covered_pixels += sum(y.off_cov, y.n_part);
 }
 
+// Tiles the raster pass visits (those on the tile lists) and their pixels inside the
+// image, from host copies of the 8 list counters (kCountStride apart) and the lists
+// (`stride` entries apart); entries are (mesh << kListMeshShift) | tile when `lays`
+// holds one layout per mesh of a batch, plain tiles otherwise.
+void count_listed(const uint32_t* counts, const uint32_t* lists, size_t stride, const FrameLayout* lays,
+                  size_t nlays, bool tagged, uint64_t* tiles_out, uint64_t* pixels_out) {
+    for (int x = 0; x < 8; ++x) {
+        const uint32_t n = counts[(size_t)x * kCountStride];
+        for (uint32_t k = 0; k < n && k < stride; ++k) {
+            const uint32_t e = lists[(size_t)x * stride + k];
+            const uint32_t m = tagged ? e >> kListMeshShift : 0u, t = tagged ? e & kListTileMask : e;
+            if (m >= nlays) continue;
+            const FrameLayout& y = lays[m];
+            const uint32_t tx = t % (uint32_t)y.tiles_x, ty = t / (uint32_t)y.tiles_x;
+            const uint64_t w = std::min<uint64_t>((uint64_t)y.ts, (uint64_t)y.W - (uint64_t)tx * y.ts);
+            const uint64_t h = std::min<uint64_t>((uint64_t)y.ts, (uint64_t)y.H - (uint64_t)ty * y.ts);
+            tiles_out[tagged ? m : 0] += 1;
+            pixels_out[tagged ? m : 0] += w * h;
+        }
+    }
+}
+
 // ---------------------------------------------------------------------------------
 // Batches: every mesh's working buffers carved from one arena, the per-mesh parameters
 // and the block maps of the vertex, bin and overflow passes in one table buffer.  The
@@ -655,7 +694,15 @@ struct BatchLaunch {
     BatchLists bl = {};
     uint32_t raster_grid = 0, tiles_grid = 0;
     size_t arena_part_bytes = 0;   // statistics: first bytes of the arena (the batch's, then every frame's)
+    bool persistent = false;       // one k_batch_frame launch (queue q), else five launches
+    BatchQueue q = {};
+    size_t part_off = 0;           // the batch's statistics, after the queue block
 };
+
+// The persistent launch's queue block at the start of the arena: ticket, bin and overflow
+// counts, error flag, the phase clock, then one vertex count per mesh (all 8 bytes).
+constexpr size_t kQueueFixed = 1 + 2 + 1 + 10;
+size_t queue_bytes(int32_t n) { return (kQueueFixed + (size_t)n) * 8; }
 
 rz_status prepare_batch(rz_ctx* ctx, const rz_batch_item* items, int32_t n, uint32_t flags, DevBuf& arena,
                         DevBuf& tables, BatchLaunch& B) {
@@ -669,12 +716,18 @@ rz_status prepare_batch(rz_ctx* ctx, const rz_batch_item* items, int32_t n, uint
         return fail(ctx, RZ_ERR_INVALID_ARG, "RZ_FLAG_MLP_SHADE without network weights (rz_mlp_init / rz_mlp_set_weights)");
     B.ts = ctx->tile_size;
     B.mode = (flags & RZ_FLAG_MLP_SHADE) ? kShadeMlp : kShadePlain;
-    uint32_t grid = (uint32_t)ctx->cus * raster_batch_blocks_per_cu(B.ts, B.mode) / 8u * 8u;
+    B.persistent = ctx->batch_persistent;
+    // the persistent launch's workgroups are its raster and overflow-shade slots
+    const int bblock = B.persistent ? batch_block(B.ts) : 256;
+    const uint32_t per_cu = B.persistent ? batch_frame_blocks_per_cu(B.ts, B.mode)
+                                         : raster_batch_blocks_per_cu(B.ts, B.mode);
+    uint32_t grid = (uint32_t)ctx->cus * per_cu / 8u * 8u;
     B.raster_grid = grid < 8 ? 8 : grid;
-    B.tiles_grid = (uint32_t)ctx->persistent_blocks;
+    B.tiles_grid = B.persistent ? B.raster_grid : (uint32_t)ctx->persistent_blocks;
     std::vector<FrameLayout> lays(n);
-    // arena: [batch statistics][statistics of every frame][tile lists][the other buffers of every frame]
-    const size_t batch_stat_bytes = align256(((size_t)B.raster_grid * 2 + B.tiles_grid) * 4);
+    // arena: [queue][batch statistics][statistics of every frame][tile lists][the other buffers of every frame]
+    const size_t qbytes = align256(queue_bytes(n));
+    const size_t batch_stat_bytes = qbytes + align256(((size_t)B.raster_grid * 2 + B.tiles_grid) * 4);
     size_t stat_bytes = batch_stat_bytes, work_bytes = 0, stride = 0;
     for (int32_t i = 0; i < n; ++i) {
         const rz_batch_item& it = items[i];
@@ -682,7 +735,7 @@ rz_status prepare_batch(rz_ctx* ctx, const rz_batch_item* items, int32_t n, uint
         if (st != RZ_OK) return st;
         if (!it.frame || !it.depth) return fail(ctx, RZ_ERR_INVALID_ARG, "batch item %d: null frame or depth", i);
         lays[i] = layout_frame(it.mesh, it.args, ctx->tile_size, ctx->bin_cap > 0 ? ctx->bin_cap : kDefaultBinCap,
-                               true, 0, 0, true);
+                               true, 0, 0, true, bblock);
         if ((uint32_t)lays[i].ntiles > kListTileMask)
             return fail(ctx, RZ_ERR_INVALID_ARG, "batch item %d: %d tiles, at most %u", i, lays[i].ntiles, kListTileMask);
         stat_bytes += align256(lays[i].bytes[kBufCounters]) + align256(lays[i].bytes[kBufPart]);
@@ -695,8 +748,10 @@ rz_status prepare_batch(rz_ctx* ctx, const rz_batch_item* items, int32_t n, uint
     B.frames.assign(n, FrameLaunch());
     B.faces.assign(n, 0);
     B.arena_part_bytes = stat_bytes;
+    B.part_off = qbytes;
     char* base = (char*)arena.ptr;
-    uint32_t* batch_part = (uint32_t*)base;
+    unsigned long long* queue = (unsigned long long*)base;
+    uint32_t* batch_part = (uint32_t*)(base + qbytes);
     uint32_t* lists = (uint32_t*)(base + stat_bytes);
     size_t so = batch_stat_bytes, wo = stat_bytes + list_bytes;
     std::vector<uint2> map[3];
@@ -725,8 +780,8 @@ rz_status prepare_batch(rz_ctx* ctx, const rz_batch_item* items, int32_t n, uint
         L.rp.lists = L.orp.lists = lists;
         L.rp.list_count = L.orp.list_count = L0.bp.list_count;
         B.faces[i] = (uint64_t)it.mesh->nf;
-        const uint32_t nb[3] = {vertex_blocks(L.vp.nv, L.vp.nn, L.vp.ntiles), bin_blocks(L.bp.nf) + L.bp.clear_blocks,
-                                overflow_blocks(L.rp.nf)};
+        const uint32_t nb[3] = {vertex_blocks(L.vp.nv, L.vp.nn, L.vp.ntiles, bblock),
+                                bin_blocks(L.bp.nf, bblock) + L.bp.clear_blocks, overflow_blocks(L.rp.nf, bblock)};
         for (int p = 0; p < 3; ++p)
             for (uint32_t b = 0; b < nb[p]; ++b) map[p].push_back(make_uint2((uint32_t)i, b));
     }
@@ -750,8 +805,29 @@ rz_status prepare_batch(rz_ctx* ctx, const rz_batch_item* items, int32_t n, uint
         total = align256(total + map[p].size() * sizeof(uint2));
         B.blocks[p] = (uint32_t)map[p].size();
     }
+    // per mesh: vertex items of one frame (the persistent launch's bin items wait for them)
+    const size_t off_nvb = total;
+    std::vector<uint32_t> nvb(n, 0);
+    for (const uint2& e : map[0]) ++nvb[e.x];
+    total = align256(total + (size_t)n * 4);
+    // the items' lights beyond the kernel arguments, one after the other
+    const size_t off_ext = total;
+    std::vector<size_t> ext_at(n, 0);
+    for (int32_t i = 0; i < n; ++i) {
+        ext_at[i] = total;
+        total += B.frames[i].ext_lights.size() * sizeof(float);
+    }
+    total = align256(total);
     if ((st = ensure(ctx, tables, total)) != RZ_OK) return st;
+    for (int32_t i = 0; i < n; ++i)
+        B.frames[i].sp.lights_ext =
+            B.frames[i].ext_lights.empty() ? nullptr : (const float*)((char*)tables.ptr + ext_at[i]);
     std::vector<char> host(total, 0);
+    for (int32_t i = 0; i < n; ++i)
+        if (!B.frames[i].ext_lights.empty())
+            std::memcpy(host.data() + ext_at[i], B.frames[i].ext_lights.data(),
+                        B.frames[i].ext_lights.size() * sizeof(float));
+    (void)off_ext;
     for (int32_t i = 0; i < n; ++i) {
         std::memcpy(host.data() + off[0] + i * sz[0], &B.frames[i].vp, sz[0]);
         std::memcpy(host.data() + off[1] + i * sz[1], &B.frames[i].bp, sz[1]);
@@ -761,6 +837,7 @@ rz_status prepare_batch(rz_ctx* ctx, const rz_batch_item* items, int32_t n, uint
     }
     for (int p = 0; p < 3; ++p)
         if (!map[p].empty()) std::memcpy(host.data() + off[4 + p], map[p].data(), map[p].size() * sizeof(uint2));
+    std::memcpy(host.data() + off_nvb, nvb.data(), (size_t)n * 4);
     // the table buffer may still be read by queued work of an earlier batch
     RZ_HIP(ctx, hipStreamSynchronize(ctx->stream));
     RZ_HIP(ctx, hipMemcpy(tables.ptr, host.data(), total, hipMemcpyHostToDevice));
@@ -771,10 +848,34 @@ rz_status prepare_batch(rz_ctx* ctx, const rz_batch_item* items, int32_t n, uint
     B.d_sp = (const ShadeParams*)(t + off[3]);
     B.d_orp = (const RasterParams*)(t + off_orp);
     for (int p = 0; p < 3; ++p) B.d_map[p] = (const uint2*)(t + off[4 + p]);
+    // the queue's counters start at zero (they only grow over the frames of this batch)
+    const std::vector<unsigned long long> zeros(queue_bytes(n) / 8, 0ull);
+    RZ_HIP(ctx, hipMemcpy(queue, zeros.data(), queue_bytes(n), hipMemcpyHostToDevice));
+    BatchQueue& q = B.q;
+    q.ticket = queue;
+    q.done = queue + 1;
+    q.error = (uint32_t*)(queue + 3);
+    q.clock = nullptr;   // (queue + 4 when rz_set_profiling: set per frame)
+    q.vdone = queue + kQueueFixed;
+    q.nvb = (const uint32_t*)(t + off_nvb);
+    q.vmap = B.d_map[0];
+    q.bmap = B.d_map[1];
+    q.omap = B.d_map[2];
+    q.nV = B.blocks[0];
+    q.nB = B.blocks[1];
+    q.nO = B.blocks[2];
+    q.G = B.raster_grid;
     return RZ_OK;
 }
 
 rz_status enqueue_batch(rz_ctx* ctx, const BatchLaunch& B, hipStream_t s, bool prof) {
+    if (B.persistent) {
+        if (prof) RZ_HIP(ctx, launch_hold(kHoldTicks, s));
+        if (prof) RZ_HIP(ctx, hipEventRecord(ctx->ev[0], s));
+        RZ_HIP(ctx, launch_batch_frame(B.d_vp, B.d_bp, B.d_rp, B.d_orp, B.d_sp, B.bl, B.q, B.ts, B.mode, s));
+        if (prof) RZ_HIP(ctx, hipEventRecord(ctx->ev[5], s));
+        return RZ_OK;
+    }
     if (prof) RZ_HIP(ctx, launch_hold(kHoldTicks, s));
     if (prof) RZ_HIP(ctx, hipEventRecord(ctx->ev[0], s));
     RZ_HIP(ctx, launch_vertex_batch(B.d_vp, B.d_map[0], B.blocks[0], s));
@@ -828,9 +929,15 @@ rz_status capture(rz_ctx* ctx, int32_t frames, F enqueue, hipGraphExec_t* out) {
 rz_status batch_stats(rz_ctx* ctx, const BatchLaunch& B, const DevBuf& arena, rz_frame_stats* stats) {
     std::vector<char> host(B.arena_part_bytes);
     RZ_HIP(ctx, hipMemcpyAsync(host.data(), arena.ptr, host.size(), hipMemcpyDeviceToHost, ctx->stream));
+    std::vector<uint32_t> list_counts(8 * kCountStride), lists((size_t)8 * B.bl.stride);
+    RZ_HIP(ctx, hipMemcpyAsync(list_counts.data(), B.bl.list_count, list_counts.size() * 4, hipMemcpyDeviceToHost,
+                               ctx->stream));
+    RZ_HIP(ctx, hipMemcpyAsync(lists.data(), B.bl.lists, lists.size() * 4, hipMemcpyDeviceToHost, ctx->stream));
     RZ_HIP(ctx, hipStreamSynchronize(ctx->stream));
     std::memset(stats, 0, sizeof *stats);
-    const uint32_t* bp = (const uint32_t*)host.data();   // the batch's raster and overflow-shade partials
+    if (B.persistent && ((const uint32_t*)host.data())[6] != 0u)   // BatchQueue::error (queue word 3)
+        return fail(ctx, RZ_ERR_HIP, "batch frame: a work-queue wait gave up");
+    const uint32_t* bp = (const uint32_t*)(host.data() + B.part_off);   // the batch's raster and overflow-shade partials
     for (uint32_t i = 0; i < B.raster_grid; ++i) stats->fragments += bp[i];
     for (uint32_t i = 0; i < B.raster_grid + B.tiles_grid; ++i) stats->covered_pixels += bp[B.raster_grid + i];
     for (size_t i = 0; i < B.frames.size(); ++i) {
@@ -839,6 +946,15 @@ rz_status batch_stats(rz_ctx* ctx, const BatchLaunch& B, const DevBuf& arena, rz
         const size_t op = (size_t)((char*)L.part - (char*)arena.ptr);
         reduce_stats(L, (const unsigned long long*)(host.data() + oc), (const uint32_t*)(host.data() + op),
                      B.faces[i], stats);
+    }
+    std::vector<FrameLayout> lays;
+    for (const FrameLaunch& L : B.frames) lays.push_back(L.lay);
+    std::vector<uint64_t> tiles(lays.size(), 0), pixels(lays.size(), 0);
+    count_listed(list_counts.data(), lists.data(), B.bl.stride, lays.data(), lays.size(), true, tiles.data(),
+                 pixels.data());
+    for (size_t i = 0; i < lays.size(); ++i) {
+        stats->raster_tiles += tiles[i];
+        stats->raster_pixels += pixels[i];
     }
     return RZ_OK;
 }
@@ -919,7 +1035,7 @@ void rz_destroy(rz_ctx* ctx) {
                       &ctx->vis,    &ctx->lists,   &ctx->frame,   &ctx->depth, &ctx->ids,        &ctx->counters,
                       &ctx->partials, &ctx->batch_arena, &ctx->batch_tables, &ctx->mlp_w, &ctx->mlp_m,
                       &ctx->mlp_v, &ctx->mlp_packed, &ctx->mlp_gpart, &ctx->mlp_lpart, &ctx->mlp_loss,
-                      &ctx->mlp_count, &ctx->frag_records, &ctx->frag_counts};
+                      &ctx->mlp_count, &ctx->lights_ext};
     for (DevBuf* b : bufs)
         if (b->ptr) (void)hipFree(b->ptr);
     for (hipEvent_t e : ctx->ev)
@@ -1335,12 +1451,16 @@ rz_status rz_time_passes(rz_ctx* ctx, const rz_mesh* mesh, rz_light* lights, int
         ctx->count_gen = reps & 1;   // (as it stands after the launch that follows)
         return r;
     };
-    float ms_b = 0, ms_r = 0, ms_s = 0, ms_n = 0;
+    float ms_b = 0, ms_r = 0, ms_s = 0, ms_c = 0;
     if ((st = timed(bin, bin_ready, ms_b)) != RZ_OK) return st;
     // one whole frame: raster, overflow and shade are idempotent on its binned state
-    const FrameLaunch& L = L2[ctx->count_gen];
+    const int gen = ctx->count_gen;
+    const FrameLaunch& L = L2[gen];
     if ((st = run_frame(ctx, L, false)) != RZ_OK) return st;
-    const bool feat = L.sp.mode == kShadeFeat;
+    // the same frame's shading with neither light loop nor network (kShadeChain)
+    FrameLaunch Lc;
+    if ((st = prepare_frame(ctx, mesh, lights, nl, args, flags, Lc, gen, kShadeChain)) != RZ_OK) return st;
+    bind_outputs(Lc, true, frame_device, depth_device, nullptr);
     auto raster = [&](int32_t, hipStream_t cs) -> rz_status {
         if (L.lay.fused)
             RZ_HIP(ctx, launch_raster_shade(L.rp, L.sp, L.lay.ts, cs));
@@ -1355,31 +1475,29 @@ rz_status rz_time_passes(rz_ctx* ctx, const rz_mesh* mesh, rz_light* lights, int
             RZ_HIP(ctx, launch_shade(L.sp, cs));
         return RZ_OK;
     };
-    auto network = [&](int32_t, hipStream_t cs) -> rz_status { RZ_HIP(ctx, launch_mlp_phase(L, cs)); return RZ_OK; };
-    // the separate shade pass of the learned shader accumulates into per-tile record counts
-    // that its raster pass zeroes: timed as raster + shade minus raster
-    auto raster_shade = [&](int32_t k, hipStream_t cs) -> rz_status {
-        rz_status r = raster(k, cs);
-        return r != RZ_OK ? r : shade(k, cs);
+    auto chain = [&](int32_t, hipStream_t cs) -> rz_status {
+        if (Lc.lay.fused)
+            RZ_HIP(ctx, launch_raster_shade(Lc.rp, Lc.sp, Lc.lay.ts, cs));
+        else
+            RZ_HIP(ctx, launch_shade(Lc.sp, cs));
+        return RZ_OK;
     };
-    const bool with_raster = feat && !L.lay.fused;
+    // (the chain first: the passes timed after it leave the frame as the caller's
+    // shading draws it)
+    if ((st = timed(chain, nothing, ms_c)) != RZ_OK) return st;
     if ((st = timed(raster, nothing, ms_r)) != RZ_OK) return st;
-    if (with_raster) {
-        if ((st = timed(raster_shade, nothing, ms_s)) != RZ_OK) return st;
-        ms_s -= ms_r;
-    } else if (L.lay.fused && L.lay.tiles_grid == 0) {
+    if (L.lay.fused && L.lay.tiles_grid == 0) {
         ms_s = 0.0f;   // no bin can overflow: the frame has no overflow-shade launch
     } else if ((st = timed(shade, nothing, ms_s)) != RZ_OK) {
         return st;
     }
-    if (feat && (st = timed(network, nothing, ms_n)) != RZ_OK) return st;
     const float k = 1000.0f / (float)reps;
     us[0] = 0.0f;   // no vertex pass: the bin pass transforms each face's corners
     us[1] = ms_b * k;
     us[2] = ms_r * k;
     us[3] = 0.0f;   // the overflow loop runs inside the raster pass
     us[4] = ms_s * k;
-    us[5] = ms_n * k;
+    us[5] = ms_c * k;
     return RZ_OK;
 }
 
@@ -1419,15 +1537,22 @@ rz_status rz_draw_frame(rz_ctx* ctx, const rz_mesh* mesh, rz_light* lights, int3
         RZ_HIP(ctx, hipMemcpyAsync(depth_buffer, depth_d, np * 4, hipMemcpyDeviceToHost, s));
         if (id_buffer) RZ_HIP(ctx, hipMemcpyAsync(id_buffer, ids_d, np * 4, hipMemcpyDeviceToHost, s));
     }
+    std::vector<uint32_t> list_counts, lists;
     if (stats) {
         RZ_HIP(ctx, hipMemcpyAsync(ctx->host_counters, L.counters, kNumCounters * 8, hipMemcpyDeviceToHost, s));
         ctx->host_partials.resize(L.lay.n_part);
         RZ_HIP(ctx, hipMemcpyAsync(ctx->host_partials.data(), L.part, L.lay.n_part * 4, hipMemcpyDeviceToHost, s));
+        list_counts.resize(8 * kCountStride);
+        lists.resize((size_t)8 * L.lay.ntiles);
+        RZ_HIP(ctx, hipMemcpyAsync(list_counts.data(), L.bp.list_count, list_counts.size() * 4, hipMemcpyDeviceToHost, s));
+        RZ_HIP(ctx, hipMemcpyAsync(lists.data(), L.bp.lists, lists.size() * 4, hipMemcpyDeviceToHost, s));
     }
     if (stats || !on_device || !(flags & RZ_FLAG_ASYNC)) RZ_HIP(ctx, hipStreamSynchronize(s));
     if (stats) {
         std::memset(stats, 0, sizeof *stats);
         reduce_stats(L, ctx->host_counters, ctx->host_partials.data(), (uint64_t)mesh->nf, stats);
+        count_listed(list_counts.data(), lists.data(), (size_t)L.lay.ntiles, &L.lay, 1, false, &stats->raster_tiles,
+                     &stats->raster_pixels);
         if (ctx->profiling && (st = event_times(ctx, stats)) != RZ_OK) return st;
     }
     return RZ_OK;
@@ -1451,17 +1576,210 @@ rz_status rz_graph_create(rz_ctx* ctx, const rz_mesh* mesh, rz_light* lights, in
         bind_outputs(launches[k], (flags & RZ_FLAG_CLEARED) != 0, frame_d, depth_d, ids_d);
         a.tait_bryan_angles[1] += ry_step;   // the spin loop's rotation about y (renderer.cpp:124)
     }
-    hipGraphExec_t exec = nullptr;
-    rz_status st = capture(ctx, frames, [&](int32_t k, hipStream_t s) { return enqueue_frame(ctx, launches[k], s, false); },
-                           &exec);
-    if (st != RZ_OK) return st;
     rz_graph* gr = new rz_graph();
     gr->ctx = ctx;
-    gr->exec = exec;
     gr->frames = frames;
-    gr->buffer_gen = ctx->buffer_gen;
     gr->ntiles = launches[0].lay.ntiles;
+    // lights beyond the kernel arguments: the graph's own copy (a later frame may rewrite
+    // the context's); every frame has the same ones (the view matrix is fixed)
+    rz_status st = stage_ext_lights(ctx, gr->lights_ext, launches[0], false);
+    for (int32_t k = 1; k < frames && st == RZ_OK; ++k) launches[k].sp.lights_ext = launches[0].sp.lights_ext;
+    if (st == RZ_OK)
+        st = capture(ctx, frames, [&](int32_t k, hipStream_t s) { return enqueue_frame(ctx, launches[k], s, false); },
+                     &gr->exec);
+    if (st != RZ_OK) {
+        rz_graph_destroy(gr);
+        return st;
+    }
+    gr->buffer_gen = ctx->buffer_gen;
     *out = gr;
+    return RZ_OK;
+}
+
+// ---- the spin loop as a replayed graph ---------------------------------------------
+
+}  // extern "C"
+
+namespace {
+
+// The fields of a single frame's launch that depend on the model rotation: the camera and
+// normal matrices of the bin pass's vertex transform, and the shade pass's normal matrix
+// (tangents) and uniform camera-space normal.  fill_frame sets them the same way.
+void set_frame_rotation(FrameLaunch& L, const rz_mesh* mesh, const rz_args& a) {
+    rz_matrices mats;
+    build_matrices(a, mats);
+    std::memcpy(L.bp.camera, mats.camera, 64);
+    std::memcpy(L.bp.normal, mats.normal, 64);
+    std::memcpy(L.sp.normal_mat, mats.normal, 64);
+    const float* m = mats.normal;
+    const float x = mesh->uniform_nrm[0], y = mesh->uniform_nrm[1], z = mesh->uniform_nrm[2], w = 0.0f;
+    L.sp.uniform_cn = make_float4((m[0] * x + m[4] * y) + (m[8] * z + m[12] * w),
+                                  (m[1] * x + m[5] * y) + (m[9] * z + m[13] * w),
+                                  (m[2] * x + m[6] * y) + (m[10] * z + m[14] * w), 0.0f);
+}
+
+}  // namespace
+
+struct rz_spin {
+    rz_ctx* ctx = nullptr;
+    const rz_mesh* mesh = nullptr;
+    rz_args args{};
+    FrameLaunch L[2];                 // the frame with each generation of counter lines
+    hipGraph_t graph[2] = {};
+    hipGraphExec_t exec[2] = {};
+    // captured kernel nodes whose arguments carry the rotation: the bin pass, the pass
+    // that shades (raster+shade, or the separate shade pass), the overflow shade
+    hipGraphNode_t bin[2] = {}, shade[2] = {}, tiles[2] = {};
+    hipKernelNodeParams kbin[2] = {}, kshade[2] = {}, ktiles[2] = {};
+    int next = 0;                     // generation of the next frame
+    uint64_t buffer_gen = 0;
+    DevBuf lights_ext;
+};
+
+namespace {
+
+// The node the last launch on a capturing stream added.
+rz_status captured_node(rz_ctx* ctx, hipStream_t s, hipGraphNode_t* node, hipKernelNodeParams* kp) {
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    const hipGraphNode_t* deps = nullptr;
+    size_t nd = 0;
+    RZ_HIP(ctx, hipStreamGetCaptureInfo_v2(s, &cs, nullptr, nullptr, &deps, &nd));
+    if (cs != hipStreamCaptureStatusActive || nd != 1 || !deps)
+        return fail(ctx, RZ_ERR_HIP, "spin graph: unexpected capture state (%d, %zu dependencies)", (int)cs, nd);
+    *node = deps[0];
+    RZ_HIP(ctx, hipGraphKernelNodeGetParams(*node, kp));
+    return RZ_OK;
+}
+
+// One frame's launches on a capturing stream, noting the nodes set_frame_rotation
+// concerns.
+rz_status capture_spin_frame(rz_ctx* ctx, rz_spin* sp, int g, hipStream_t s) {
+    const FrameLaunch& L = sp->L[g];
+    RZ_HIP(ctx, launch_bin(L.bp, s));
+    rz_status st = captured_node(ctx, s, &sp->bin[g], &sp->kbin[g]);
+    if (st != RZ_OK) return st;
+    if (L.lay.fused) {
+        RZ_HIP(ctx, launch_raster_shade(L.rp, L.sp, L.lay.ts, s));
+    } else {
+        RZ_HIP(ctx, launch_raster(L.rp, L.lay.ts, s));
+        RZ_HIP(ctx, launch_shade(L.sp, s));
+    }
+    if ((st = captured_node(ctx, s, &sp->shade[g], &sp->kshade[g])) != RZ_OK) return st;
+    if (L.lay.fused && L.lay.tiles_grid > 0) {
+        RZ_HIP(ctx, launch_shade_tiles(L.rp, tiles_params(L), L.lay.ts, s));
+        if ((st = captured_node(ctx, s, &sp->tiles[g], &sp->ktiles[g])) != RZ_OK) return st;
+    }
+    return RZ_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+rz_status rz_spin_create(rz_ctx* ctx, const rz_mesh* mesh, rz_light* lights, int32_t nl, const rz_args* args,
+                         uint8_t* frame_d, float* depth_d, int32_t* ids_d, uint32_t flags, rz_spin** out) {
+    if (!ctx || !out) return RZ_ERR_INVALID_ARG;
+    *out = nullptr;
+    if (!(flags & RZ_FLAG_DEVICE_BUFFERS))
+        return fail(ctx, RZ_ERR_INVALID_ARG, "a spin graph needs device buffers (RZ_FLAG_DEVICE_BUFFERS)");
+    if (flags & RZ_FLAG_MLP_SHADE) return fail(ctx, RZ_ERR_INVALID_ARG, "a spin graph shades with the light loop");
+    if (!frame_d || !depth_d || !args) return fail(ctx, RZ_ERR_INVALID_ARG, "null frame, depth or args");
+    rz_spin* sp = new rz_spin();
+    sp->ctx = ctx;
+    sp->mesh = mesh;
+    sp->args = *args;
+    rz_status st = RZ_OK;
+    for (int g = 0; g < 2 && st == RZ_OK; ++g) {
+        st = prepare_frame(ctx, mesh, lights, nl, args, flags, sp->L[g], g);
+        if (st == RZ_OK) bind_outputs(sp->L[g], (flags & RZ_FLAG_CLEARED) != 0, frame_d, depth_d, ids_d);
+    }
+    // lights beyond the kernel arguments: the spin graph's own copy
+    if (st == RZ_OK) st = stage_ext_lights(ctx, sp->lights_ext, sp->L[0], false);
+    if (st == RZ_OK) sp->L[1].sp.lights_ext = sp->L[0].sp.lights_ext;
+    for (int g = 0; g < 2 && st == RZ_OK; ++g) {
+        hipStream_t s = ctx->stream;
+        hipError_t e = hipStreamSynchronize(s);
+        if (e == hipSuccess) e = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal);
+        if (e != hipSuccess) {
+            st = fail(ctx, RZ_ERR_HIP, "spin capture: %s", hipGetErrorString(e));
+            break;
+        }
+        st = capture_spin_frame(ctx, sp, g, s);
+        hipGraph_t graph = nullptr;
+        e = hipStreamEndCapture(s, &graph);
+        sp->graph[g] = graph;
+        if (st == RZ_OK && e != hipSuccess) st = fail(ctx, RZ_ERR_HIP, "spin capture: %s", hipGetErrorString(e));
+        if (st == RZ_OK) {
+            e = hipGraphInstantiate(&sp->exec[g], graph, nullptr, nullptr, 0);
+            if (e != hipSuccess) st = fail(ctx, RZ_ERR_HIP, "hipGraphInstantiate: %s", hipGetErrorString(e));
+        }
+    }
+    if (st != RZ_OK) {
+        rz_spin_destroy(sp);
+        return st;
+    }
+    sp->buffer_gen = ctx->buffer_gen;
+    *out = sp;
+    return RZ_OK;
+}
+
+rz_status rz_spin_frame(rz_spin* sp, const float tait_bryan_angles[3]) {
+    if (!sp || !tait_bryan_angles) return RZ_ERR_INVALID_ARG;
+    rz_ctx* ctx = sp->ctx;
+    if (sp->buffer_gen != ctx->buffer_gen)
+        return fail(ctx, RZ_ERR_INVALID_ARG, "spin graph is stale: a later frame reallocated the context's buffers");
+    RZ_HIP(ctx, hipSetDevice(ctx->device));
+    const int g = sp->next;
+    for (int k = 0; k < 3; ++k) sp->args.tait_bryan_angles[k] = tait_bryan_angles[k];
+    FrameLaunch& L = sp->L[g];
+    set_frame_rotation(L, sp->mesh, sp->args);
+    // the captured kernels' arguments with the new rotation
+    void* bin_args[] = {&L.bp};
+    hipKernelNodeParams kp = sp->kbin[g];
+    kp.kernelParams = bin_args;
+    kp.extra = nullptr;
+    RZ_HIP(ctx, hipGraphExecKernelNodeSetParams(sp->exec[g], sp->bin[g], &kp));
+    void* fused_args[] = {&L.rp, &L.sp};
+    void* shade_args[] = {&L.sp};
+    kp = sp->kshade[g];
+    kp.kernelParams = L.lay.fused ? fused_args : shade_args;
+    kp.extra = nullptr;
+    RZ_HIP(ctx, hipGraphExecKernelNodeSetParams(sp->exec[g], sp->shade[g], &kp));
+    if (sp->tiles[g]) {
+        ShadeParams tp = tiles_params(L);
+        void* tiles_args[] = {&L.rp, &tp};
+        kp = sp->ktiles[g];
+        kp.kernelParams = tiles_args;
+        kp.extra = nullptr;
+        RZ_HIP(ctx, hipGraphExecKernelNodeSetParams(sp->exec[g], sp->tiles[g], &kp));
+    }
+    rz_status st = counts_ready(ctx, L.lay.ntiles, g);
+    if (st != RZ_OK) return st;
+    ctx->count_ntiles = -1;
+    RZ_HIP(ctx, hipGraphLaunch(sp->exec[g], ctx->stream));
+    ctx->count_ntiles = L.lay.ntiles;
+    ctx->count_gen = g ^ 1;
+    sp->next = g ^ 1;
+    return RZ_OK;
+}
+
+void rz_spin_destroy(rz_spin* sp) {
+    if (!sp) return;
+    (void)hipSetDevice(sp->ctx->device);
+    (void)hipStreamSynchronize(sp->ctx->stream);
+    for (int g = 0; g < 2; ++g) {
+        if (sp->exec[g]) (void)hipGraphExecDestroy(sp->exec[g]);
+        if (sp->graph[g]) (void)hipGraphDestroy(sp->graph[g]);
+    }
+    if (sp->lights_ext.ptr) (void)hipFree(sp->lights_ext.ptr);
+    delete sp;
+}
+
+rz_status rz_draw_text(rz_ctx* ctx, uint8_t* frame_device, uint32_t width, uint32_t height, int32_t x, int32_t y,
+                       const char* text, const uint8_t foreground[3], const uint8_t background[3]) {
+    if (!ctx || !frame_device || !text || !foreground || !background) return RZ_ERR_INVALID_ARG;
+    RZ_HIP(ctx, hipSetDevice(ctx->device));
+    RZ_HIP(ctx, launch_text(frame_device, width, height, x, y, text, foreground, background, ctx->stream));
     return RZ_OK;
 }
 
@@ -1488,9 +1806,10 @@ void rz_graph_destroy(rz_graph* g) {
     if (!g) return;
     (void)hipSetDevice(g->ctx->device);
     (void)hipStreamSynchronize(g->ctx->stream);
-    (void)hipGraphExecDestroy(g->exec);
+    if (g->exec) (void)hipGraphExecDestroy(g->exec);
     if (g->arena.ptr) (void)hipFree(g->arena.ptr);
     if (g->tables.ptr) (void)hipFree(g->tables.ptr);
+    if (g->lights_ext.ptr) (void)hipFree(g->lights_ext.ptr);
     delete g;
 }
 
@@ -1499,15 +1818,52 @@ rz_status rz_draw_batch(rz_ctx* ctx, const rz_batch_item* items, int32_t n, uint
     BatchLaunch B;
     rz_status st = prepare_batch(ctx, items, n, flags, ctx->batch_arena, ctx->batch_tables, B);
     if (st != RZ_OK) return st;
+    const bool phase_clock = ctx->profiling && B.persistent;
+    if (phase_clock) {
+        // per phase: first start (min, from all ones) and last end (max, from zero)
+        unsigned long long init[10];
+        for (int k = 0; k < 5; ++k) {
+            init[2 * k] = ~0ull;
+            init[2 * k + 1] = 0ull;
+        }
+        B.q.clock = B.q.ticket + 4;
+        RZ_HIP(ctx, hipMemcpy(B.q.clock, init, sizeof init, hipMemcpyHostToDevice));
+    }
     if ((st = enqueue_batch(ctx, B, ctx->stream, ctx->profiling)) != RZ_OK) return st;
     if (stats) {
         if ((st = batch_stats(ctx, B, ctx->batch_arena, stats)) != RZ_OK) return st;
-        if (ctx->profiling && (st = event_times(ctx, stats)) != RZ_OK) return st;
+        if (ctx->profiling) {
+            if (phase_clock) {
+                // the phases overlap: each is reported as the span of its items
+                unsigned long long clk[10];
+                RZ_HIP(ctx, hipMemcpy(clk, B.q.clock, sizeof clk, hipMemcpyDeviceToHost));
+                float* ms[5] = {&stats->ms_vertex, &stats->ms_bin, &stats->ms_raster, &stats->ms_overflow,
+                                &stats->ms_shade};
+                for (int k = 0; k < 5; ++k)
+                    *ms[k] = clk[2 * k + 1] > clk[2 * k] ? (float)(clk[2 * k + 1] - clk[2 * k]) * 1e-5f : 0.0f;
+                RZ_HIP(ctx, hipEventElapsedTime(&stats->ms_frame, ctx->ev[0], ctx->ev[5]));
+            } else if ((st = event_times(ctx, stats)) != RZ_OK) {
+                return st;
+            }
+        }
     } else if (!(flags & RZ_FLAG_ASYNC)) {
         RZ_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        if (B.persistent) {
+            uint32_t err = 0;
+            RZ_HIP(ctx, hipMemcpy(&err, B.q.error, 4, hipMemcpyDeviceToHost));
+            if (err) return fail(ctx, RZ_ERR_HIP, "batch frame: a work-queue wait gave up");
+        }
     }
     return RZ_OK;
 }
+
+rz_status rz_set_batch_persistent(rz_ctx* ctx, int32_t enable) {
+    if (!ctx) return RZ_ERR_INVALID_ARG;
+    ctx->batch_persistent = enable != 0;
+    return RZ_OK;
+}
+
+int32_t rz_get_batch_persistent(const rz_ctx* ctx) { return ctx && ctx->batch_persistent ? 1 : 0; }
 
 rz_status rz_batch_graph_create(rz_ctx* ctx, const rz_batch_item* items, int32_t n, uint32_t flags, int32_t frames,
                                 rz_graph** out) {

@@ -12,6 +12,8 @@
 
 namespace rz {
 
+// Lights carried in the shade pass's kernel arguments; a frame with more lights reads
+// the rest from a device array (ShadeParams::lights_ext).
 constexpr int kMaxLights = 64;
 constexpr uint64_t kEmptyKey = ~0ull;           // visibility word of a pixel with no fragment
 constexpr uint32_t kNoFace = 0xFFFFFFFFu;
@@ -25,8 +27,13 @@ constexpr uint32_t kCountStride = 32;
 // the overflow pass tests); every other statistic is a per-workgroup partial sum
 // written with a plain store and reduced only when a caller asks for statistics
 // (same-address device-scope atomics serialise across the eight XCDs).
-// kShadeFeat: the learned shader's first phase (fragment records for k_mlp_shade)
-enum ShadeMode : int { kShadePlain = 0, kShadeMlp = 1, kShadeCollect = 2, kShadeFeat = 3 };
+// Shading modes: the reference's light loop, the learned surrogate on the matrix cores,
+// the light loop with every shaded fragment's network inputs and colour appended to a
+// training buffer, and kShadeChain: the per-pixel work every mode shares (visibility
+// word, face record, interpolation, texture and normal-map samples, the stores) with
+// neither light loop nor network, colour 0 -- the timing reference that splits a shade
+// pass's time into that chain and the light loop or the network.
+enum ShadeMode : int { kShadePlain = 0, kShadeMlp = 1, kShadeCollect = 2, kShadeChain = 3 };
 
 enum Counter : int {
     kCntOverflowTiles = 0,
@@ -158,7 +165,6 @@ struct RasterParams {
     uint32_t max_blocks;        // workgroups of the list-driven launch (a multiple of 8)
     int32_t narrow;             // 16-pixel tiles by 64-thread workgroups (frames with many tiles)
     int32_t pre_cleared;        // outputs cleared by the bin launch
-    uint32_t* rec_counts;       // separate shade pipeline, learned shader: zeroed per listed tile
     const uint32_t* lists;      // per-XCD lists of tiles with faces (see BinParams)
     const uint32_t* list_count;
 };
@@ -199,13 +205,7 @@ struct ShadeParams {
     int32_t nmat;
     int32_t mode;                    // ShadeMode
     const uint16_t* mlp;             // packed network (kShadeMlp)
-    float* samples;                  // training samples (kShadeCollect) or fragment records (kShadeFeat) out
-    // kShadeFeat: the records of a tile's covered pixels are compacted into the tile's
-    // region of rec_region (= tile pixels) records, their number in
-    // rec_counts[tile * kCountStride]
-    uint32_t* rec_counts;
-    uint32_t rec_region;
-    int32_t tiles_x, ts_log2;
+    float* samples;                  // training samples out (kShadeCollect)
     unsigned long long* sample_count;
     unsigned long long sample_cap;
     float inv_p00, inv_p11;          // 1 / projection[0][0], 1 / projection[1][1] (view vector)
@@ -214,17 +214,26 @@ struct ShadeParams {
     float normal_mat[16];            // inverse-transpose of the model-view (tangents)
     float light_ic[kMaxLights][3];   // intensity * colour per light
     float light_nd[kMaxLights][3];   // -trans_dir per light
+    // lights kMaxLights .. nl-1 (nl > kMaxLights only): (intensity * colour, -trans_dir),
+    // six floats per light, in device memory
+    const float* lights_ext;
 };
 
 // Launch shapes (shared by the launchers, the batch block maps and the host's
-// statistics reduction).
-__host__ __device__ inline uint32_t vertex_blocks(int32_t nv, int32_t nn, int32_t ntiles) {
+// statistics reduction).  `block`: workgroup size of a batch's vertex, bin and overflow
+// work (256, or the persistent batch launch's batch_block).
+__host__ __device__ inline uint32_t vertex_blocks(int32_t nv, int32_t nn, int32_t ntiles, int block = 256) {
     int32_t n = nv > nn ? nv : nn;
     if (ntiles > n) n = ntiles;
     if (kNumCounters > n) n = kNumCounters;
-    return (uint32_t)((n + 255) / 256);
+    return (uint32_t)((n + block - 1) / block);
 }
-__host__ __device__ inline uint32_t bin_blocks(int32_t nf) { return (uint32_t)((nf + 255) / 256); }
+__host__ __device__ inline uint32_t bin_blocks(int32_t nf, int block = 256) {
+    return (uint32_t)((nf + block - 1) / block);
+}
+// Workgroup size of a batch's persistent launch (k_batch_frame): one thread per pixel of
+// an 8-pixel tile, 256 on larger tiles.
+__host__ __device__ inline int batch_block(int tile_size) { return tile_size == 8 ? 64 : 256; }
 // A single frame's bin launch: workgroups of kFrameBinBlock threads (batches: 256).
 constexpr int kFrameBinBlock = 128;
 __host__ __device__ inline uint32_t frame_bin_blocks(int32_t nf) {
@@ -233,10 +242,36 @@ __host__ __device__ inline uint32_t frame_bin_blocks(int32_t nf) {
 // Counter lines of one generation (single frames): tile counts, then the 8 list counters,
 // then one line holding the kNumCounters global counters.
 __host__ __device__ inline uint32_t count_lines(int32_t ntiles) { return (uint32_t)ntiles + 9u; }
-__host__ __device__ inline uint32_t overflow_blocks(int32_t nf) {
-    const uint32_t b = bin_blocks(nf);
-    return b > 1024u ? 1024u : b;
+__host__ __device__ inline uint32_t overflow_blocks(int32_t nf, int block = 256) {
+    const uint32_t b = bin_blocks(nf, block), most = 1024u * 256u / (uint32_t)block;
+    return b > most ? most : b;
 }
+
+// The persistent batch launch's work queue.  Every workgroup takes tickets in order until
+// it draws an exit ticket; a frame's tickets are, in order: the vertex items, the bin items
+// (binning and clearing workgroups of every mesh), G raster slots (each runs the raster
+// pass over the shared tile lists until they are exhausted), the overflow items, G
+// overflow-shade slots, and G exit tickets (G = workgroups of the launch), so each frame
+// takes exactly `per_frame` tickets and frame f's tickets are [f, f + 1) * per_frame.  An
+// item waits only on items of lower tickets (a bin item on its mesh's vertex items and on
+// the first mesh's, which clear the shared list counters; raster slots and overflow items
+// on every bin item; overflow-shade slots on every overflow item), which the running
+// workgroups that drew them finish without waiting on later ones: no wait can deadlock,
+// whatever the number of resident workgroups.  The counters only grow (a frame waits for
+// (f + 1) times its per-frame counts) and start at zero when the batch is prepared.
+struct BatchQueue {
+    unsigned long long* ticket;          // tickets drawn
+    unsigned long long* vdone;           // per mesh: vertex items finished
+    unsigned long long* done;            // [0] bin items, [1] overflow items finished
+    uint32_t* error;                     // nonzero: a wait gave up (a bug guard; never set by a correct queue)
+    const uint32_t* nvb;                 // per mesh: vertex items of one frame
+    const uint2* vmap;                   // vertex item -> (mesh, workgroup of the mesh's pass)
+    const uint2* bmap;                   // bin item -> (mesh, workgroup)
+    const uint2* omap;                   // overflow item -> (mesh, workgroup)
+    uint32_t nV, nB, nO, G;
+    unsigned long long* clock;           // null, or per phase (vertex, bin, raster, overflow, shade) the
+                                         // first item's start (min) and the last item's end (max), 100 MHz
+};
 __host__ __device__ inline uint32_t shade_blocks_x(uint32_t W) { return (W + 63) / 64; }
 __host__ __device__ inline uint32_t shade_blocks_y(uint32_t H) { return (H + 3) / 4; }
 

@@ -105,8 +105,9 @@ __device__ __forceinline__ float4 camera_normal(const float* m, float x, float y
 }
 
 // Batches: the vertex pass of every mesh (single frames transform in the bin pass).
+template <int BLOCK = 256>
 __device__ __forceinline__ void vertex_body(const VertexParams& p, int blk) {
-    const int i = blk * 256 + threadIdx.x;
+    const int i = blk * BLOCK + threadIdx.x;
     if (i < p.nv)   // (raster-space meshes have nv = 0: the bin pass reads their vertices)
         p.raster[i] = raster_vertex(p.camera, p.px[i], p.py[i], p.pz[i], p.Wf, p.Hf);
     if (i < p.nn) p.cnormal[i] = camera_normal(p.normal, p.nx[i], p.ny[i], p.nz[i]);
@@ -150,8 +151,16 @@ __device__ __forceinline__ void bin_push(const BinParams& p, uint32_t face, uint
         tile_overflowed(p, tile);
 }
 
+// LDS bin_body needs (carved by the caller: kernels that run other phases share it):
+// per-tile counts and bases of the union rectangle, each face's slots, the rectangle.
 template <int BLOCK>
-__device__ __forceinline__ void bin_body(const BinParams& p, int blk) {
+struct BinLds {
+    static constexpr int kAggTiles = 4 * BLOCK;
+    static constexpr int kBytes = 2 * kAggTiles * 4 + BLOCK * kBigTriangleTiles + 16;
+};
+
+template <int BLOCK>
+__device__ __forceinline__ void bin_body(const BinParams& p, int blk, unsigned char* lds) {
     // faces are taken in the mesh's binning order (centroid Morton order), so the
     // workgroup's faces are screen neighbours; f is the face's own index
     const int i = blk * BLOCK + threadIdx.x;
@@ -229,11 +238,11 @@ __device__ __forceinline__ void bin_body(const BinParams& p, int blk) {
     // Faces of one workgroup are neighbours in the mesh, so their tiles usually fall in
     // a small rectangle; when the union of their tile rectangles is larger than
     // kAggTiles the pairs go to the tile counters one by one.
-    constexpr int kAggTiles = 4 * BLOCK;
-    __shared__ int s_rect[4];                       // union tile rect: x0 y0 x1 y1
-    __shared__ uint32_t s_cnt[kAggTiles];
-    __shared__ uint32_t s_base[kAggTiles];
-    __shared__ uint8_t s_slot[BLOCK * kBigTriangleTiles];
+    constexpr int kAggTiles = BinLds<BLOCK>::kAggTiles;
+    uint32_t* s_cnt = reinterpret_cast<uint32_t*>(lds);
+    uint32_t* s_base = s_cnt + kAggTiles;
+    uint8_t* s_slot = reinterpret_cast<uint8_t*>(s_base + kAggTiles);
+    int* s_rect = reinterpret_cast<int*>(s_slot + BLOCK * kBigTriangleTiles);   // union tile rect: x0 y0 x1 y1
     const int tid = threadIdx.x;
     if (tid == 0) {
         s_rect[0] = s_rect[1] = 0x7FFFFFFF;
@@ -434,13 +443,12 @@ __device__ __forceinline__ void clear_pixel(const ShadeParams& p, size_t i) {
 // ids; returns whether the pixel has a winning fragment.
 // MODE: kShadePlain (the reference's light loop), kShadeMlp (the learned surrogate on
 // matrix cores), kShadeCollect (light loop, and each shaded fragment's MLP inputs and
-// colour/255 appended to the sample buffer for training).
-// kShadeFeat: rec receives the fragment's record words 0-5 (the caller stores them).
+// colour/255 appended to the sample buffer for training), kShadeChain (neither: colour 0).
 template <int MODE>
 __device__ __forceinline__ bool shade_pixel(const ShadeParams& p, const float* s_lut, const DevMaterial* s_mats,
                                             const uint16_t* s_net,
                                             uint32_t px, uint32_t py, bool valid, unsigned long long key, int lane,
-                                            uint16_t* s_in, float* s_out, uint32_t* rec = nullptr) {
+                                            uint16_t* s_in, float* s_out) {
     const size_t np = (size_t)p.W * p.H;
     const size_t i = (size_t)py * p.W + px;
     const bool hit = valid && key != kEmptyKey;
@@ -502,7 +510,7 @@ __device__ __forceinline__ bool shade_pixel(const ShadeParams& p, const float* s
         feat[0] = nx; feat[1] = ny; feat[2] = nz;
         if (m.nmap_w > 0) {
             normal_map_sample(p, m, u, v, ts);
-            if (MODE != kShadeMlp && MODE != kShadeFeat)   // the network takes the sample itself
+            if (MODE == kShadePlain || MODE == kShadeCollect)   // the network takes the sample itself
                 bend_normal(p, f, ts, nx, ny, nz);
         }
         p.depth[i] = z;
@@ -511,18 +519,33 @@ __device__ __forceinline__ bool shade_pixel(const ShadeParams& p, const float* s
             // shading.cpp:20-34
             constexpr float kOneOverPi = 0.318309886183790671537767526745028724f;
             float acc0 = 0.f, acc1 = 0.f, acc2 = 0.f;
-            for (int l = 0; l < p.nl; ++l) {
+            const int nk = p.nl < kMaxLights ? p.nl : kMaxLights;
+            for (int l = 0; l < nk; ++l) {
                 const float dd = (nx * p.light_nd[l][0] + ny * p.light_nd[l][1]) + nz * p.light_nd[l][2];
                 const float mm = gmax(0.f, dd);
                 acc0 = acc0 + p.light_ic[l][0] * alb[0] * mm * kOneOverPi;
                 acc1 = acc1 + p.light_ic[l][1] * alb[1] * mm * kOneOverPi;
                 acc2 = acc2 + p.light_ic[l][2] * alb[2] * mm * kOneOverPi;
             }
+            // the lights beyond the kernel arguments, in order (uniform: scalar loads)
+            for (int l = kMaxLights; l < p.nl; ++l) {
+                const float* e = p.lights_ext + 6 * (l - kMaxLights);
+                const float dd = (nx * e[3] + ny * e[4]) + nz * e[5];
+                const float mm = gmax(0.f, dd);
+                acc0 = acc0 + e[0] * alb[0] * mm * kOneOverPi;
+                acc1 = acc1 + e[1] * alb[1] * mm * kOneOverPi;
+                acc2 = acc2 + e[2] * alb[2] * mm * kOneOverPi;
+            }
             rgb[0] = (uint8_t)cast_u32(gmin(acc0, 255.f));
             rgb[1] = (uint8_t)cast_u32(gmin(acc1, 255.f));
             rgb[2] = (uint8_t)cast_u32(gmin(acc2, 255.f));
         }
-        if (MODE != kShadePlain) {
+        if (MODE == kShadeChain) {
+            // the samples every mode takes stay live (the compiler would drop their loads)
+            asm volatile("" ::"v"(alb[0]), "v"(alb[1]), "v"(alb[2]), "v"(ts[0]), "v"(ts[1]), "v"(ts[2]), "v"(nx),
+                         "v"(ny), "v"(nz));
+        }
+        if (MODE == kShadeMlp || MODE == kShadeCollect) {
             // view vector: from the surface point (eye space, depth d) to the eye
             const float xn = 2.0f * pxf / (float)p.W - 1.0f, yn = 1.0f - 2.0f * pyf / (float)p.H;
             float vx = -(xn * d) * p.inv_p00, vy = -(yn * d) * p.inv_p11, vz = d;
@@ -545,10 +568,6 @@ __device__ __forceinline__ bool shade_pixel(const ShadeParams& p, const float* s
             for (int c = 0; c < 3; ++c) o[kMlpIn + c] = (float)rgb[c] / 255.0f;
         }
     }
-    if (MODE == kShadeFeat && hit) {
-#pragma unroll
-        for (int k = 0; k < 6; ++k) rec[k] = bf16x2(feat[2 * k], feat[2 * k + 1]);
-    }
     if (MODE == kShadeMlp) {
 #pragma unroll
         for (int k = 0; k < 16; ++k) s_in[lane * 16 + k] = to_bf16((hit && k < kMlpIn) ? feat[k] : 0.f);
@@ -564,38 +583,12 @@ __device__ __forceinline__ bool shade_pixel(const ShadeParams& p, const float* s
             }
         }
     }
-    if (hit && MODE != kShadeFeat) {   // (kShadeFeat: k_mlp_shade writes the colour)
+    if (hit) {
         p.frame[i] = rgb[0];
         p.frame[i + np] = rgb[1];
         p.frame[i + 2 * np] = rgb[2];
     }
     return hit;
-}
-
-// kShadeFeat: stores the records of the workgroup's covered pixels compactly in its
-// region, after `base` earlier ones (ballot ranks within a wave, wave totals through LDS;
-// no global atomics); returns the new count.  Every thread of the workgroup calls it.
-template <int NW>
-__device__ __forceinline__ uint32_t region_append(bool hit, const uint32_t w[6], uint32_t pixel, uint4* region,
-                                                  uint32_t base, int tid, uint32_t* s_w) {
-    const int lane = tid & 63, wave = tid >> 6;
-    const unsigned long long bal = __ballot(hit);
-    if (lane == 0) s_w[wave] = (uint32_t)__popcll(bal);
-    __syncthreads();
-    uint32_t off = base, total = base;
-#pragma unroll
-    for (int k = 0; k < NW; ++k) {
-        const uint32_t v = s_w[k];
-        if (k < wave) off += v;
-        total += v;
-    }
-    __syncthreads();   // s_w is rewritten by the next call
-    if (hit) {
-        const uint32_t slot = off + (uint32_t)__popcll(bal & ((1ull << lane) - 1ull));
-        region[2 * slot] = make_uint4(w[0], w[1], w[2], w[3]);
-        region[2 * slot + 1] = make_uint4(w[4], w[5], pixel, 0u);
-    }
-    return total;
 }
 
 // Deferred shade pass: 64 x 4 pixel blocks, each wave 64 consecutive pixels of a row.
@@ -612,32 +605,7 @@ __device__ __forceinline__ void shade_body(const ShadeParams& p, const float* s_
     const uint32_t px = (blk % nbx) * 64 + threadIdx.x, py = (blk / nbx) * 4 + threadIdx.y;
     const bool valid = px < p.W && py < p.H;
     const unsigned long long key = valid ? p.vis[(size_t)py * p.W + px] : kEmptyKey;
-    uint32_t rec[6];
-    const bool hit =
-        shade_pixel<MODE>(p, s_lut, s_mats, s_net, px, py, valid, key, threadIdx.x, s_in[ws], s_out[ws], rec);
-    if (MODE == kShadeFeat) {
-        // the wave's pixels lie in up to 64 / tile-edge tiles: one slot reservation per tile
-        const uint32_t tile = (py >> p.ts_log2) * (uint32_t)p.tiles_x + (px >> p.ts_log2);
-        const int lane = threadIdx.x;
-        unsigned long long todo = __ballot(hit);
-        while (todo) {   // uniform over the wave
-            const int leader = __ffsll((long long)todo) - 1;
-            const uint32_t t = __shfl(tile, leader, 64);
-            const unsigned long long same = __ballot(hit && tile == t);
-            uint32_t base = 0;
-            if (lane == leader) base = atomicAdd(&p.rec_counts[(size_t)t * kCountStride], (uint32_t)__popcll(same));
-            base = __shfl(base, leader, 64);
-            const uint32_t slot = base + (uint32_t)__popcll(same & ((1ull << lane) - 1ull));
-            // (a frame's records never exceed the tile's pixels; the bound keeps any
-            // launch sequence that skips the raster pass's reset inside the region)
-            if (((same >> lane) & 1ull) && slot < p.rec_region) {
-                uint4* o = reinterpret_cast<uint4*>(p.samples) + 2 * ((size_t)t * p.rec_region + slot);
-                o[0] = make_uint4(rec[0], rec[1], rec[2], rec[3]);
-                o[1] = make_uint4(rec[4], rec[5], (uint32_t)((size_t)py * p.W + px), 0u);
-            }
-            todo &= ~same;
-        }
-    }
+    const bool hit = shade_pixel<MODE>(p, s_lut, s_mats, s_net, px, py, valid, key, threadIdx.x, s_in[ws], s_out[ws]);
     store_block_sum(hit ? 1u : 0u, p.part_cov + blk, tid, 256);
 }
 
@@ -653,23 +621,15 @@ __device__ __forceinline__ void shade_tile_body(const RasterParams& rp, const Sh
     const int tid = threadIdx.x, lane = tid & 63;
     const int ws = MODE == kShadeMlp ? tid >> 6 : 0;
     const int ox = (tile % rp.tiles_x) * TS, oy = (tile / rp.tiles_x) * TS;
-    __shared__ uint32_t s_w[NW];
-    uint32_t nrec = 0;
 #pragma unroll
     for (int l0 = 0; l0 < TS * TS; l0 += BLOCK) {
         const int l = l0 + tid;
         const uint32_t x = ox + (l % TS), y = oy + (l / TS);
         const bool valid = x < p.W && y < p.H;
         const unsigned long long key = valid ? p.vis[(size_t)y * p.W + x] : kEmptyKey;
-        uint32_t rec[6];
-        const bool hit = shade_pixel<MODE>(p, s_lut, s_mats, s_net, x, y, valid, key, lane, s_in[ws], s_out[ws], rec);
+        const bool hit = shade_pixel<MODE>(p, s_lut, s_mats, s_net, x, y, valid, key, lane, s_in[ws], s_out[ws]);
         cov += hit ? 1u : 0u;
-        if (MODE == kShadeFeat)
-            nrec = region_append<NW>(hit, rec, (uint32_t)((size_t)y * p.W + x),
-                                     reinterpret_cast<uint4*>(p.samples) + 2 * (size_t)tile * p.rec_region, nrec,
-                                     tid, s_w);
     }
-    if (MODE == kShadeFeat && tid == 0) p.rec_counts[(size_t)tile * kCountStride] = nrec;
     __syncthreads();   // the staging buffers are reused by the next tile
 }
 
@@ -781,10 +741,20 @@ __device__ __forceinline__ bool shrink_rect(const float4& a, const float4& b, co
 // deferred shade, without the round trip of the visibility buffer through HBM).  Tiles
 // whose bin overflowed are skipped: the overflow loop fills their visibility words and
 // k_shade_tiles shades them.
-template <int TS, int BLOCK, bool FUSED, int MODE, int SHRINK = kShrinkPixels>
+// Bytes of raster_body's scratch LDS: face data by rank, reused by the learned shader's
+// staging after the raster loop (and, in the batch-frame kernel, by its bin phase).
+template <int BLOCK, int MODE>
+struct RasterLds {
+    static constexpr int kRaster = 13 * BLOCK * 4;
+    static constexpr int kMlp = MODE == kShadeMlp ? (BLOCK / 64) * (64 * 16 * 2 + 64 * 4 * 4) : 0;
+    static constexpr int kBytes = kRaster > kMlp ? kRaster : kMlp;
+};
+
+// EXT_RAW: the scratch LDS (RasterLds bytes) is the caller's `raw`, else the function's own.
+template <int TS, int BLOCK, bool FUSED, int MODE, int SHRINK = kShrinkPixels, bool EXT_RAW = false>
 __device__ __forceinline__ void raster_body(const RasterParams& p, const ShadeParams* sp, const float* s_lut,
                                             const DevMaterial* s_mats, const uint16_t* s_net, int tile,
-                                            uint32_t& frags, uint32_t& cov) {
+                                            uint32_t& frags, uint32_t& cov, unsigned char* raw = nullptr) {
     constexpr int NPX = TS * TS;
     constexpr int PPT = NPX / BLOCK;
     constexpr int NW = BLOCK / 64;
@@ -793,16 +763,16 @@ __device__ __forceinline__ void raster_body(const RasterParams& p, const ShadePa
     constexpr bool kNet = FUSED && MODE == kShadeMlp;
     __shared__ unsigned long long s_key[NPX];
     __shared__ float s_d0[NPX];
-    // raster scratch (face data by rank), reused by the learned shader's staging after
-    // the raster loop
-    constexpr int kRasterScratch = 13 * BLOCK * 4;
-    constexpr int kMlpScratch = kNet ? NW * (64 * 16 * 2 + 64 * 4 * 4) : 0;
-    __shared__ __align__(16) unsigned char s_raw[kRasterScratch > kMlpScratch ? kRasterScratch : kMlpScratch];
+    unsigned char* s_raw = raw;
+    if constexpr (!EXT_RAW) {
+        __shared__ __align__(16) unsigned char s_own[RasterLds<BLOCK, MODE>::kBytes];
+        s_raw = s_own;
+    }
     float (*s_v)[BLOCK] = reinterpret_cast<float (*)[BLOCK]>(s_raw);   // x0 y0 z0 x1 y1 z1 x2 y2 z2 area
     uint32_t* s_rect = reinterpret_cast<uint32_t*>(s_raw + 10 * BLOCK * 4);   // rx0 | ry0 << 8 | rw << 16
     uint32_t* s_face = s_rect + BLOCK;
     uint32_t* s_start = s_face + BLOCK;
-    __shared__ uint32_t s_wsum[NW], s_wlive[NW], s_wfeat[NW];
+    __shared__ uint32_t s_wsum[NW], s_wlive[NW];
     __shared__ unsigned long long s_mask[kWords];
     __shared__ uint32_t s_wpre[kWords];
 
@@ -819,7 +789,6 @@ __device__ __forceinline__ void raster_body(const RasterParams& p, const ShadePa
     const uint32_t f_first = p.bins[(size_t)tile * p.cap + ((uint32_t)tid < p.cap ? (uint32_t)tid : 0u)];
     const uint32_t count = p.tile_count[(size_t)tile * kCountStride];
     const uint32_t n = count;
-    if (!FUSED && p.rec_counts && tid == 0) p.rec_counts[(size_t)tile * kCountStride] = 0u;   // filled by k_shade
     // the bin pass cleared an overflowed tile's visibility words and the overflow loop
     // fills them (uniform over the workgroup)
     if (count > p.cap) return;
@@ -964,41 +933,26 @@ __device__ __forceinline__ void raster_body(const RasterParams& p, const ShadePa
             nh = tot;
         }
         __syncthreads();
-        uint32_t nrec = 0;
         for (uint32_t k0 = 0; k0 < nh; k0 += BLOCK) {   // uniform over the workgroup
             const uint32_t k = k0 + (uint32_t)tid;
             const bool valid = k < nh;
             const int l = valid ? (int)s_hits[k] : 0;
             const uint32_t x = ox + (l % TS), y = oy + (l / TS);
-            uint32_t rec[6];
             const bool hit = shade_pixel<MODE>(*sp, s_lut, s_mats, s_net, x, y, valid,
-                                               valid ? s_key[l] : kEmptyKey, lane, s_in, s_out, rec);
+                                               valid ? s_key[l] : kEmptyKey, lane, s_in, s_out);
             cov += hit ? 1u : 0u;
-            if (MODE == kShadeFeat)
-                nrec = region_append<NW>(hit, rec, y * p.W + x,
-                                         reinterpret_cast<uint4*>(sp->samples) + 2 * (size_t)tile * sp->rec_region,
-                                         nrec, tid, s_wfeat);
         }
-        if (MODE == kShadeFeat && tid == 0) sp->rec_counts[(size_t)tile * kCountStride] = nrec;
     } else if (FUSED) {
         uint16_t* s_in = reinterpret_cast<uint16_t*>(s_raw) + (kNet ? wave : 0) * 64 * 16;
         float* s_out = reinterpret_cast<float*>(s_raw + NW * 64 * 16 * 2) + (kNet ? wave : 0) * 64 * 4;
-        uint32_t nrec = 0;
 #pragma unroll
         for (int j = 0; j < PPT; ++j) {
             const int l = tid + j * BLOCK;
             const uint32_t x = ox + (l % TS), y = oy + (l / TS);
             const bool valid = x < p.W && y < p.H;
-            uint32_t rec[6];
-            const bool hit =
-                shade_pixel<MODE>(*sp, s_lut, s_mats, s_net, x, y, valid, s_key[l], lane, s_in, s_out, rec);
+            const bool hit = shade_pixel<MODE>(*sp, s_lut, s_mats, s_net, x, y, valid, s_key[l], lane, s_in, s_out);
             cov += hit ? 1u : 0u;
-            if (MODE == kShadeFeat)
-                nrec = region_append<NW>(hit, rec, y * p.W + x,
-                                         reinterpret_cast<uint4*>(sp->samples) + 2 * (size_t)tile * sp->rec_region,
-                                         nrec, tid, s_wfeat);
         }
-        if (MODE == kShadeFeat && tid == 0) sp->rec_counts[(size_t)tile * kCountStride] = nrec;
     } else {
 #pragma unroll
         for (int j = 0; j < PPT; ++j) {
@@ -1069,11 +1023,12 @@ __device__ __forceinline__ uint32_t overflow_faces(const RasterParams& p, uint32
     return frags;
 }
 
+template <int BLOCK = 256>
 __device__ __forceinline__ void overflow_body(const RasterParams& p, int blk, int nblk) {
     const uint32_t frags = p.counters[kCntOverflowTiles] == 0ull
                                ? 0u
-                               : overflow_faces(p, (uint32_t)blk * 256u + threadIdx.x, (uint32_t)nblk * 256u);
-    store_block_sum(frags, p.part_frag + blk, threadIdx.x, 256);
+                               : overflow_faces(p, (uint32_t)blk * BLOCK + threadIdx.x, (uint32_t)nblk * BLOCK);
+    store_block_sum(frags, p.part_frag + blk, threadIdx.x, BLOCK);
 }
 
 // ---------------------------------------------------------------------------------
@@ -1125,9 +1080,10 @@ __device__ __forceinline__ void zero_lines_body(const BinParams& p, uint32_t blk
 // latency-bound binning leaves idle.
 // (single frames: kFrameBinBlock-thread workgroups)
 __global__ __launch_bounds__(kFrameBinBlock) void k_bin(BinParams p) {
+    __shared__ __align__(16) unsigned char s_lds[BinLds<kFrameBinBlock>::kBytes];
     const uint32_t nb = frame_bin_blocks(p.nf);
     if (blockIdx.x < nb)
-        bin_body<kFrameBinBlock>(p, blockIdx.x);
+        bin_body<kFrameBinBlock>(p, blockIdx.x, s_lds);
     else if (blockIdx.x < nb + p.zero_blocks)
         zero_lines_body<kFrameBinBlock>(p, blockIdx.x - nb);
     else
@@ -1257,11 +1213,12 @@ __global__ __launch_bounds__(256) void k_vertex_batch(const VertexParams* __rest
 }
 // Batch bin launch: each mesh's binning workgroups, then its clearing ones.
 __global__ __launch_bounds__(256) void k_bin_batch(const BinParams* __restrict__ items, const uint2* __restrict__ map) {
+    __shared__ __align__(16) unsigned char s_lds[BinLds<256>::kBytes];
     const uint2 m = map[blockIdx.x];
     const BinParams& p = items[m.x];
     const uint32_t nb = bin_blocks(p.nf);
     if (m.y < nb)
-        bin_body<256>(p, (int)m.y);
+        bin_body<256>(p, (int)m.y, s_lds);
     else
         clear_body<256>(p, m.y - nb, p.clear_blocks);
 }
@@ -1328,6 +1285,201 @@ __global__ __launch_bounds__(BLOCK) void k_shade_tiles_batch(const RasterParams*
     store_block_sum(cov, part_cov + blockIdx.x, threadIdx.x, BLOCK);
 }
 
+// ---------------------------------------------------------------------------------
+// A batch frame as one persistent launch (work queue: BatchQueue in rz_device.h).
+
+// Polls (thread 0) until *c reaches target; the workgroup waits at the barrier and then
+// acquires what the counted items wrote.  Gives up after kMaxSpins polls (about a second:
+// far beyond any batch frame) and flags q.error, after which every wait of the launch
+// gives up at once (a bug guard: the queue's order makes every wait end); returns
+// whether the count was reached.
+constexpr uint32_t kMaxSpins = 1u << 20;
+__device__ __forceinline__ bool wait_count(const unsigned long long* c, unsigned long long target, uint32_t* error) {
+    __shared__ int s_ok;
+    if (threadIdx.x == 0) {
+        int ok = 1;
+        uint32_t spins = 0;
+        while (__hip_atomic_load(const_cast<unsigned long long*>(c), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) <
+               target) {
+            if (++spins > kMaxSpins ||
+                __hip_atomic_load(error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u) {
+                atomicOr(error, 1u);
+                ok = 0;
+                break;
+            }
+            __builtin_amdgcn_s_sleep(2);
+        }
+        s_ok = ok;
+    }
+    __syncthreads();
+    const bool ok = s_ok != 0;
+    __threadfence();   // acquire: the finished items' writes, from any XCD
+    __syncthreads();   // (s_ok is rewritten by the next wait)
+    return ok;
+}
+
+// Releases this workgroup's writes at device scope and counts the item finished.
+__device__ __forceinline__ void finish_item(unsigned long long* c) {
+    __threadfence();
+    __syncthreads();
+    if (threadIdx.x == 0) atomicAdd(c, 1ull);
+}
+
+__device__ __forceinline__ void phase_clock(const BatchQueue& q, int phase, bool start) {
+    if (q.clock && threadIdx.x == 0) {
+        const unsigned long long now = wall_clock64();
+        if (start)
+            atomicMin(&q.clock[2 * phase], now);
+        else
+            atomicMax(&q.clock[2 * phase + 1], now);
+    }
+}
+
+template <int BLOCK, int MODE>
+struct BatchFrameLds {
+    static constexpr int kBytes =
+        BinLds<BLOCK>::kBytes > RasterLds<BLOCK, MODE>::kBytes ? BinLds<BLOCK>::kBytes : RasterLds<BLOCK, MODE>::kBytes;
+};
+
+template <int TS, int BLOCK, int MODE>
+__global__ __launch_bounds__(BLOCK, MODE == kShadeMlp ? 3 : 4) void k_batch_frame(
+    const VertexParams* __restrict__ vitems, const BinParams* __restrict__ bitems,
+    const RasterParams* __restrict__ ritems, const RasterParams* __restrict__ oitems,
+    const ShadeParams* __restrict__ sitems, BatchLists bl, BatchQueue q) {
+    // one LDS region for the bin phase's tile aggregation and the raster phase's face data
+    __shared__ __align__(16) unsigned char s_lds[BatchFrameLds<BLOCK, MODE>::kBytes];
+    __shared__ __align__(16) uint16_t s_net[MODE == kShadeMlp ? kPackedBytes / 2 : 8];
+    __shared__ unsigned long long s_ticket;
+    __shared__ uint32_t s_take;
+    if (MODE == kShadeMlp) stage_net(sitems[0].mlp, s_net, threadIdx.x, BLOCK);   // one network per batch
+    const unsigned long long per_frame =
+        (unsigned long long)q.nV + q.nB + q.G + q.nO + q.G + q.G;
+    uint32_t frags = 0, cov = 0, cov_ovf = 0;
+    for (;;) {
+        if (threadIdx.x == 0) s_ticket = atomicAdd(q.ticket, 1ull);
+        __syncthreads();
+        const unsigned long long t = s_ticket;
+        __syncthreads();   // (s_ticket is rewritten by the next draw)
+        const unsigned long long f1 = t / per_frame + 1;   // frames up to and including this one
+        uint32_t u = (uint32_t)(t % per_frame);
+        if (u < q.nV) {
+            phase_clock(q, 0, true);
+            const uint2 m = q.vmap[u];
+            vertex_body<BLOCK>(vitems[m.x], (int)m.y);
+            finish_item(q.vdone + m.x);
+            phase_clock(q, 0, false);
+        } else if ((u -= q.nV) < q.nB) {
+            const uint2 m = q.bmap[u];
+            // this mesh's vertices and counters, and the shared list counters (first mesh)
+            bool ok = wait_count(q.vdone + m.x, f1 * q.nvb[m.x], q.error);
+            ok = wait_count(q.vdone, f1 * q.nvb[0], q.error) && ok;
+            phase_clock(q, 1, true);
+            const BinParams& p = bitems[m.x];
+            const uint32_t nb = bin_blocks(p.nf, BLOCK);
+            if (ok) {
+                if (m.y < nb)
+                    bin_body<BLOCK>(p, (int)m.y, s_lds);
+                else
+                    clear_body<BLOCK>(p, m.y - nb, p.clear_blocks);
+            }
+            finish_item(q.done);
+            phase_clock(q, 1, false);
+        } else if ((u -= q.nB) < q.G) {
+            // a raster slot: tiles of the shared lists, one at a time from per-list cursors
+            // (first list u % 8, then the others), once every mesh is binned
+            if (wait_count(q.done, f1 * q.nB, q.error)) {
+                phase_clock(q, 2, true);
+                for (uint32_t j = 0; j < 8; ++j) {
+                    const uint32_t x = (u + j) & 7u;
+                    const uint32_t n = bl.list_count[x * kCountStride];
+                    uint32_t* cursor = bl.list_count + x * kCountStride + 1;
+                    for (;;) {
+                        if (threadIdx.x == 0) s_take = atomicAdd(cursor, 1u);
+                        __syncthreads();
+                        const uint32_t k = s_take;
+                        __syncthreads();   // (s_take is rewritten by the next take)
+                        if (k >= n) break;
+                        const uint32_t e = bl.lists[(size_t)x * bl.stride + k];
+                        const uint32_t m = e >> kListMeshShift;
+                        raster_body<TS, BLOCK, true, MODE, kShrinkPixelsBatch, true>(
+                            ritems[m], &sitems[m], sitems[m].luts, nullptr, s_net, (int)(e & kListTileMask), frags,
+                            cov, s_lds);
+                    }
+                }
+                phase_clock(q, 2, false);
+            }
+        } else if ((u -= q.G) < q.nO) {
+            const uint2 m = q.omap[u];
+            const bool ok = wait_count(q.done, f1 * q.nB, q.error);
+            phase_clock(q, 3, true);
+            if (ok) {
+                const RasterParams& p = oitems[m.x];
+                overflow_body<BLOCK>(p, (int)m.y, (int)overflow_blocks(p.nf, BLOCK));
+            }
+            finish_item(q.done + 1);
+            phase_clock(q, 3, false);
+        } else if ((u -= q.nO) < q.G) {
+            // an overflow-shade slot: the overflowed tiles of list u % 8, stride G / 8
+            if (wait_count(q.done + 1, f1 * q.nO, q.error) && *bl.overflow != 0ull) {
+                phase_clock(q, 4, true);
+                const uint32_t x = u & 7u, step = q.G >> 3;
+                const uint32_t n = bl.list_count[x * kCountStride];
+                for (uint32_t k = u >> 3; k < n; k += step) {
+                    const uint32_t e = bl.lists[(size_t)x * bl.stride + k];
+                    const uint32_t m = e >> kListMeshShift, tile = e & kListTileMask;
+                    const RasterParams& p = ritems[m];
+                    if (p.tile_count[(size_t)tile * kCountStride] > p.cap)
+                        shade_tile_body<TS, BLOCK, MODE>(p, sitems[m], sitems[m].luts, nullptr, s_net, (int)tile,
+                                                         cov_ovf);
+                }
+                phase_clock(q, 4, false);
+            }
+        } else {
+            break;   // this workgroup's exit ticket
+        }
+    }
+    __syncthreads();
+    store_block_sum(frags, bl.part_frag + blockIdx.x, threadIdx.x, BLOCK);
+    store_block_sum(cov, bl.part_cov + blockIdx.x, threadIdx.x, BLOCK);
+    store_block_sum(cov_ovf, bl.part_cov + q.G + blockIdx.x, threadIdx.x, BLOCK);
+}
+
+// Resident workgroups per CU of the persistent batch launch.
+uint32_t batch_frame_blocks_per_cu(int tile_size, int mode) {
+    const void* f = nullptr;
+    int block = 256;
+#define RZ_PICK(TS, B)                                                                               \
+    block = B;                                                                                       \
+    f = mode == kShadeMlp ? reinterpret_cast<const void*>(&k_batch_frame<TS, B, kShadeMlp>)           \
+                          : reinterpret_cast<const void*>(&k_batch_frame<TS, B, kShadePlain>);
+    switch (tile_size) {
+        case 8: RZ_PICK(8, 64) break;
+        case 16: RZ_PICK(16, 256) break;
+        default: RZ_PICK(32, 256) break;
+    }
+#undef RZ_PICK
+    int n = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, f, block, 0) != hipSuccess || n < 1) n = 1;
+    return (uint32_t)n;
+}
+
+hipError_t launch_batch_frame(const VertexParams* v, const BinParams* b, const RasterParams* r,
+                              const RasterParams* o, const ShadeParams* sh, const BatchLists& bl,
+                              const BatchQueue& q, int tile_size, int mode, hipStream_t s) {
+#define RZ_LAUNCH(TS, B, M) \
+    hipLaunchKernelGGL((k_batch_frame<TS, B, M>), dim3(q.G), dim3(B), 0, s, v, b, r, o, sh, bl, q)
+    const bool net = mode == kShadeMlp;
+    if (mode != kShadePlain && mode != kShadeMlp) return hipErrorInvalidValue;
+    switch (tile_size) {
+        case 8: if (net) RZ_LAUNCH(8, 64, kShadeMlp); else RZ_LAUNCH(8, 64, kShadePlain); break;
+        case 16: if (net) RZ_LAUNCH(16, 256, kShadeMlp); else RZ_LAUNCH(16, 256, kShadePlain); break;
+        case 32: if (net) RZ_LAUNCH(32, 256, kShadeMlp); else RZ_LAUNCH(32, 256, kShadePlain); break;
+        default: return hipErrorInvalidValue;
+    }
+#undef RZ_LAUNCH
+    return hipGetLastError();
+}
+
 __global__ __launch_bounds__(64) void k_hold(uint64_t ticks) {
     const uint64_t t0 = wall_clock64();
     while (wall_clock64() - t0 < ticks) __builtin_amdgcn_s_sleep(32);
@@ -1347,6 +1499,83 @@ hipError_t launch_copy(const void* src, void* dst, size_t n16, hipStream_t s) {
     const size_t blocks = (n16 + 255) / 256;
     hipLaunchKernelGGL(k_copy, dim3((unsigned)(blocks < 8192 ? blocks : 8192)), dim3(256), 0, s,
                        (const uint4*)src, (uint4*)dst, n16);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------------
+// Text into a planar frame: the spin loop's frame-rate line (renderer.cpp:114,
+// CImg::draw_text(5, 5, text, white, black)).  Glyphs are this library's own 5x7 bitmaps
+// (digits, '.', '-', ' ', and the letters of "inf" and "nan"), drawn at twice their size
+// in 12x16 cells: foreground where a glyph bit is set, background over the rest of the
+// cell; characters without a glyph draw as background.
+constexpr int kTextMax = 32;
+struct TextParams {
+    uint8_t* frame;          // planar [3][H][W]
+    uint32_t W, H;
+    int32_t x0, y0;
+    int32_t len;
+    uint8_t fg[3], bg[3];
+    char text[kTextMax];
+};
+
+__device__ __forceinline__ uint8_t glyph_row(char c, int row) {
+    // rows top to bottom, bit 4 = leftmost column
+    static constexpr uint8_t kDigits[10][7] = {
+        {0x0E, 0x11, 0x13, 0x15, 0x19, 0x11, 0x0E}, {0x04, 0x0C, 0x04, 0x04, 0x04, 0x04, 0x0E},
+        {0x0E, 0x11, 0x01, 0x02, 0x04, 0x08, 0x1F}, {0x1F, 0x02, 0x04, 0x02, 0x01, 0x11, 0x0E},
+        {0x02, 0x06, 0x0A, 0x12, 0x1F, 0x02, 0x02}, {0x1F, 0x10, 0x1E, 0x01, 0x01, 0x11, 0x0E},
+        {0x06, 0x08, 0x10, 0x1E, 0x11, 0x11, 0x0E}, {0x1F, 0x01, 0x02, 0x04, 0x08, 0x08, 0x08},
+        {0x0E, 0x11, 0x11, 0x0E, 0x11, 0x11, 0x0E}, {0x0E, 0x11, 0x11, 0x0F, 0x01, 0x02, 0x0C}};
+    static constexpr uint8_t kDot[7] = {0, 0, 0, 0, 0, 0x0C, 0x0C};
+    static constexpr uint8_t kMinus[7] = {0, 0, 0, 0x1F, 0, 0, 0};
+    static constexpr uint8_t kI[7] = {0x04, 0, 0x0C, 0x04, 0x04, 0x04, 0x0E};
+    static constexpr uint8_t kN[7] = {0, 0, 0x16, 0x19, 0x11, 0x11, 0x11};
+    static constexpr uint8_t kF[7] = {0x06, 0x09, 0x08, 0x1C, 0x08, 0x08, 0x08};
+    static constexpr uint8_t kA[7] = {0, 0, 0x0E, 0x01, 0x0F, 0x11, 0x0F};
+    if (c >= '0' && c <= '9') return kDigits[c - '0'][row];
+    switch (c) {
+        case '.': return kDot[row];
+        case '-': return kMinus[row];
+        case 'i': return kI[row];
+        case 'n': return kN[row];
+        case 'f': return kF[row];
+        case 'a': return kA[row];
+        default: return 0;
+    }
+}
+
+// One thread per pixel of the text's cells (12 * len by 16), clipped to the frame.
+__global__ __launch_bounds__(256) void k_text(TextParams p) {
+    const int i = (int)(blockIdx.x * 256 + threadIdx.x);
+    const int bw = 12 * p.len;
+    if (i >= bw * 16) return;
+    const int dx = i % bw, dy = i / bw;
+    const int64_t x = (int64_t)p.x0 + dx, y = (int64_t)p.y0 + dy;
+    if (x < 0 || y < 0 || x >= (int64_t)p.W || y >= (int64_t)p.H) return;
+    const int col = (dx % 12) >> 1, row = (dy >> 1) - 1;   // glyph rows 1..7 of the cell's 8
+    const bool on = col < 5 && row >= 0 && row < 7 && ((glyph_row(p.text[dx / 12], row) >> (4 - col)) & 1);
+    const size_t np = (size_t)p.W * p.H, q = (size_t)y * p.W + (size_t)x;
+    const uint8_t* c = on ? p.fg : p.bg;
+    p.frame[q] = c[0];
+    p.frame[q + np] = c[1];
+    p.frame[q + 2 * np] = c[2];
+}
+
+hipError_t launch_text(uint8_t* frame, uint32_t W, uint32_t H, int32_t x0, int32_t y0, const char* text,
+                       const uint8_t fg[3], const uint8_t bg[3], hipStream_t s) {
+    TextParams p{};
+    p.frame = frame;
+    p.W = W; p.H = H; p.x0 = x0; p.y0 = y0;
+    int n = 0;
+    while (text[n] && n < kTextMax) {
+        p.text[n] = text[n];
+        ++n;
+    }
+    p.len = n;
+    for (int c = 0; c < 3; ++c) { p.fg[c] = fg[c]; p.bg[c] = bg[c]; }
+    if (n == 0) return hipSuccess;
+    const int threads = 12 * n * 16;
+    hipLaunchKernelGGL(k_text, dim3((threads + 255) / 256), dim3(256), 0, s, p);
     return hipGetLastError();
 }
 
@@ -1373,8 +1602,8 @@ uint32_t raster_blocks_per_cu(int tile_size, bool fused, int mode, bool narrow) 
         f = reinterpret_cast<const void*>(&k_raster_shade<TS, B, kShadeMlp>);                        \
     else if (mode == kShadeCollect)                                                                 \
         f = reinterpret_cast<const void*>(&k_raster_shade<TS, B, kShadeCollect>);                    \
-    else if (mode == kShadeFeat)                                                                    \
-        f = reinterpret_cast<const void*>(&k_raster_shade<TS, B, kShadeFeat>);                       \
+    else if (mode == kShadeChain)                                                                   \
+        f = reinterpret_cast<const void*>(&k_raster_shade<TS, B, kShadeChain>);                      \
     else                                                                                            \
         f = reinterpret_cast<const void*>(&k_raster_shade<TS, B, kShadePlain>);
     switch (tile_size) {
@@ -1441,7 +1670,7 @@ hipError_t launch_raster_shade(const RasterParams& p, const ShadeParams& sp, int
         case kShadePlain: return launch_raster_shade_mode<kShadePlain>(p, sp, tile_size, s);
         case kShadeMlp: return launch_raster_shade_mode<kShadeMlp>(p, sp, tile_size, s);
         case kShadeCollect: return launch_raster_shade_mode<kShadeCollect>(p, sp, tile_size, s);
-        case kShadeFeat: return launch_raster_shade_mode<kShadeFeat>(p, sp, tile_size, s);
+        case kShadeChain: return launch_raster_shade_mode<kShadeChain>(p, sp, tile_size, s);
         default: return hipErrorInvalidValue;
     }
 }
@@ -1463,7 +1692,7 @@ hipError_t launch_shade_tiles(const RasterParams& p, const ShadeParams& sp, int 
         case kShadePlain: return launch_shade_tiles_mode<kShadePlain>(p, sp, tile_size, s);
         case kShadeMlp: return launch_shade_tiles_mode<kShadeMlp>(p, sp, tile_size, s);
         case kShadeCollect: return launch_shade_tiles_mode<kShadeCollect>(p, sp, tile_size, s);
-        case kShadeFeat: return launch_shade_tiles_mode<kShadeFeat>(p, sp, tile_size, s);
+        case kShadeChain: return launch_shade_tiles_mode<kShadeChain>(p, sp, tile_size, s);
         default: return hipErrorInvalidValue;
     }
 }
@@ -1474,7 +1703,7 @@ hipError_t launch_shade(const ShadeParams& p, hipStream_t s) {
         case kShadePlain: hipLaunchKernelGGL(k_shade<kShadePlain>, grid, block, 0, s, p); break;
         case kShadeMlp: hipLaunchKernelGGL(k_shade<kShadeMlp>, grid, block, 0, s, p); break;
         case kShadeCollect: hipLaunchKernelGGL(k_shade<kShadeCollect>, grid, block, 0, s, p); break;
-        case kShadeFeat: hipLaunchKernelGGL(k_shade<kShadeFeat>, grid, block, 0, s, p); break;
+        case kShadeChain: hipLaunchKernelGGL(k_shade<kShadeChain>, grid, block, 0, s, p); break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();

@@ -39,9 +39,6 @@ constexpr int kMlpSampleFloats = kMlpIn + kMlpOut;   // training sample: inputs,
 constexpr int kFragL1 = 0, kFragL2 = kFragL1 + 4 * 64 * 4, kFragL3 = kFragL2 + 16 * 64 * 4,
               kFragEnd = kFragL3 + 4 * 64 * 4;   // uint16 count: 6144
 constexpr int kPackedBytes = kFragEnd * 2 + (64 + 64 + 16) * 4;
-// A fragment record of the two-phase learned shader: the 12 inputs as bf16 (words 0-5,
-// input 2k in the low half of word k), the pixel index (word 6), word 7 zero.
-constexpr int kFragRecordBytes = 32;
 
 typedef short mlp_s4 __attribute__((ext_vector_type(4)));
 typedef short mlp_s8 __attribute__((ext_vector_type(8)));

@@ -55,66 +55,6 @@ __global__ __launch_bounds__(256) void k_mlp_infer(const float* __restrict__ x, 
         for (int c = 0; c < kMlpOut; ++c) y[i * kMlpOut + c] = s_out[wave][lane * 4 + c];
 }
 
-// The learned shader's second phase.  A kShadeFeat pass left, per tile,
-// counts[tile * kCountStride] compact fragment records (12 bf16 inputs, pixel index) at
-// records + tile * region.  A workgroup takes tiles from the per-XCD tile lists (a tile
-// without a face holds no records) and its four waves take 64 records each: the network
-// on the matrix cores, then the colour bytes of those pixels.
-constexpr int kMlpShadeWaves = 4;
-
-__global__ __launch_bounds__(64 * kMlpShadeWaves) void k_mlp_shade(
-    const uint4* __restrict__ records, const uint32_t* __restrict__ counts, uint32_t region,
-    const uint32_t* __restrict__ lists, const uint32_t* __restrict__ list_count, uint32_t ntiles,
-    const uint16_t* __restrict__ packed, uint8_t* __restrict__ frame, size_t np) {
-    __shared__ __align__(16) uint16_t s_net[kPackedBytes / 2];
-    __shared__ __align__(16) uint16_t s_in[kMlpShadeWaves][64 * 16];
-    __shared__ __align__(16) float s_out[kMlpShadeWaves][64 * 4];
-    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
-    stage_net(packed, s_net, tid, 64 * kMlpShadeWaves);
-    __syncthreads();
-    // tiles of this workgroup: list b % 8 with stride gridDim.x / 8
-    const uint32_t xcd = blockIdx.x & 7u, n = list_count[xcd * kCountStride], step = gridDim.x >> 3;
-    for (uint32_t k = blockIdx.x >> 3; k < n; k += step) {
-        const uint32_t r = lists[(size_t)xcd * ntiles + k];
-        const uint32_t c0 = counts[(size_t)r * kCountStride];
-        const uint32_t cnt = c0 < region ? c0 : region;   // a tile holds at most its pixels' records
-        const uint4* rec = records + 2 * (size_t)r * region;
-        for (uint32_t c = (uint32_t)wave * 64; c < cnt; c += 64 * kMlpShadeWaves) {   // uniform over the wave
-            const uint32_t q = c + (uint32_t)lane;
-            const bool ok = q < cnt;
-            uint4 a = make_uint4(0u, 0u, 0u, 0u), b = a;
-            if (ok) {
-                a = rec[2 * q];
-                b = rec[2 * q + 1];
-            }
-            uint4* in = reinterpret_cast<uint4*>(s_in[wave] + lane * 16);
-            in[0] = a;
-            in[1] = make_uint4(b.x, b.y, 0u, 0u);
-            __builtin_amdgcn_wave_barrier();
-            mlp_wave(s_net, s_in[wave], s_out[wave], lane);
-            __builtin_amdgcn_wave_barrier();
-            if (ok) {
-                const size_t pix = b.z;
-#pragma unroll
-                for (int ch = 0; ch < 3; ++ch) {
-                    const float y = s_out[wave][lane * 4 + ch] * 255.0f;
-                    const float yc = y > 0.f ? (y < 255.f ? y : 255.f) : 0.f;
-                    frame[pix + (size_t)ch * np] = (uint8_t)(uint32_t)(yc + 0.5f);
-                }
-            }
-            __builtin_amdgcn_wave_barrier();
-        }
-    }
-}
-
-hipError_t launch_mlp_shade(const void* records, const uint32_t* counts, uint32_t region, const uint32_t* lists,
-                            const uint32_t* list_count, uint32_t ntiles, const uint16_t* packed, uint8_t* frame,
-                            size_t np, uint32_t blocks, hipStream_t s) {
-    hipLaunchKernelGGL(k_mlp_shade, dim3(blocks), dim3(64 * kMlpShadeWaves), 0, s, (const uint4*)records, counts,
-                       region, lists, list_count, ntiles, packed, frame, np);
-    return hipGetLastError();
-}
-
 // ---------------------------------------------------------------------------------
 __device__ __forceinline__ uint64_t mix64(uint64_t z) {
     z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;

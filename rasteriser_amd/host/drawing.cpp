@@ -141,14 +141,38 @@ Args::Args(int argc, char** argv) {
 
 namespace {
 
+// A 64-bit hash of n bytes, continuing from h: FNV-1a over 8-byte words (tail bytes one
+// at a time), a word per multiply, about 1 ms for an 82k-face mesh and its texture.
+uint64_t fnv1a(const void* data, size_t n, uint64_t h) {
+    const unsigned char* p = static_cast<const unsigned char*>(data);
+    size_t i = 0;
+    for (; i + 8 <= n; i += 8) {
+        uint64_t w;
+        std::memcpy(&w, p + i, 8);
+        h = (h ^ w) * 0x100000001B3ull;
+        h ^= h >> 29;
+    }
+    for (; i < n; ++i) h = (h ^ p[i]) * 0x100000001B3ull;
+    return h;
+}
+
+// One librz context per host thread, and the mesh it last uploaded with the hash of
+// everything that mesh was made from: draw_frame re-uploads only when the arrays,
+// materials or textures it is given differ from the last call's (the spin loop draws
+// the same ones every frame, renderer.cpp:111).
 struct ThreadContext {
     rz_ctx* ctx = nullptr;
     int device = -1;
+    rz_mesh* mesh = nullptr;
+    uint64_t mesh_hash = 0;
     ~ThreadContext() {
+        if (mesh) rz_mesh_destroy(ctx, mesh);
         if (ctx) rz_destroy(ctx);
     }
     rz_ctx* get(int dev) {
         if (ctx && device != dev) {
+            if (mesh) rz_mesh_destroy(ctx, mesh);
+            mesh = nullptr;
             rz_destroy(ctx);
             ctx = nullptr;
         }
@@ -196,11 +220,57 @@ void draw_frame(const std::vector<Vec3>& model_vertices, const std::vector<Trian
     check(rz_set_tuning(ctx, arguments.tile_size, arguments.bin_capacity), ctx, "rz_set_tuning");
     std::vector<rz_material> mats;
     for (const Material& m : materials) mats.push_back(m.as_c());
-    const rz_mesh_desc d = describe(model_vertices, faces, model_vertnormals, vertuvs, mats);
+    // the mesh on the device: uploaded again only when its contents changed
+    uint64_t h = 0xCBF29CE484222325ull;
+    const size_t sizes[5] = {model_vertices.size(), faces.size(), model_vertnormals.size(), vertuvs.size(),
+                             materials.size()};
+    h = fnv1a(sizes, sizeof sizes, h);
+    h = fnv1a(model_vertices.data(), model_vertices.size() * sizeof(Vec3), h);
+    h = fnv1a(faces.data(), faces.size() * sizeof(Triangle), h);
+    h = fnv1a(model_vertnormals.data(), model_vertnormals.size() * sizeof(Vec3), h);
+    h = fnv1a(vertuvs.data(), vertuvs.size() * sizeof(Vec2), h);
+    for (const rz_material& m : mats) {
+        h = fnv1a(m.diffuse_colour, sizeof m.diffuse_colour, h);
+        const int dims[3] = {m.tex_width, m.tex_height, m.tex_channels};
+        h = fnv1a(dims, sizeof dims, h);
+        if (m.tex_channels) h = fnv1a(m.texels, (size_t)m.tex_width * m.tex_height * m.tex_channels, h);
+    }
+    if (!t_ctx.mesh || t_ctx.mesh_hash != h) {
+        if (t_ctx.mesh) rz_mesh_destroy(ctx, t_ctx.mesh);
+        t_ctx.mesh = nullptr;
+        const rz_mesh_desc d = describe(model_vertices, faces, model_vertnormals, vertuvs, mats);
+        check(rz_mesh_create(ctx, &d, &t_ctx.mesh), ctx, "rz_mesh_create");
+        t_ctx.mesh_hash = h;
+    }
     const rz_args a = arguments.as_c();
-    check(rz_draw_frame_desc(ctx, &d, lights.data(), (int32_t)lights.size(), &a, frame_buffer->data.data(),
-                             depth_buffer->data.data(), 0u),
+    check(rz_draw_frame(ctx, t_ctx.mesh, lights.data(), (int32_t)lights.size(), &a, frame_buffer->data.data(),
+                        depth_buffer->data.data(), nullptr, 0u, nullptr),
           ctx, "draw_frame");
+}
+
+DeviceFrame::DeviceFrame(rz_ctx* c, int w, int h) : ctx(c), width(w), height(h) {
+    void* p = nullptr;
+    check(rz_device_alloc(ctx, (size_t)w * h * 3, &p), ctx, "rz_device_alloc");
+    frame = static_cast<uint8_t*>(p);
+    p = nullptr;
+    const rz_status st = rz_device_alloc(ctx, (size_t)w * h * 4, &p);
+    if (st != RZ_OK) {
+        rz_device_free(ctx, frame);
+        check(st, ctx, "rz_device_alloc");
+    }
+    depth = static_cast<float*>(p);
+}
+
+DeviceFrame::~DeviceFrame() {
+    rz_device_free(ctx, frame);
+    rz_device_free(ctx, depth);
+}
+
+void DeviceFrame::download(FrameBuffer& f, DepthBuffer& d) const {
+    if (f.width != width || f.height != height || d.width != width || d.height != height)
+        throw std::runtime_error("DeviceFrame::download: buffer sizes differ");
+    check(rz_copy_to_host(ctx, f.data.data(), frame, f.data.size()), ctx, "rz_copy_to_host");
+    check(rz_copy_to_host(ctx, d.data.data(), depth, d.data.size() * sizeof(float)), ctx, "rz_copy_to_host");
 }
 
 Renderer::Renderer(int device) { check(rz_create(device, &ctx_), nullptr, "rz_create"); }

@@ -3,8 +3,10 @@
 // the model (or the default square, renderer.cpp:32-50), draw one frame into cleared
 // buffers and save frame.png and depth.png, or with -s run the spin loop (rotation
 // about y by the elapsed time, renderer.cpp:94-126).  There is no window on a GPU
-// server: the spin loop renders --frames frames, prints the frame rate of each (the
-// reference draws it into the window) and saves the last frame.
+// server: the spin loop renders --frames frames into device buffers, replaying one
+// captured frame graph with each frame's rotation (rz_spin_frame), draws the previous
+// frame's rate into the frame at (5, 5) like the reference (rz_draw_text), prints it,
+// and saves the last frame (with its rate) and depth; it prints the last frame's angles.
 #include <chrono>
 #include <cstdio>
 #include <fstream>
@@ -74,17 +76,41 @@ int main(int argc, char** argv) {
             Renderer r(arguments.device);
             r.set_tuning(arguments.tile_size, arguments.bin_capacity);
             r.upload(vertices, faces, vertnormals, vertuvs, materials);
+            rz_ctx* ctx = r.context();
+            DeviceFrame dev(ctx, W, H);
+            const rz_args a0 = arguments.as_c();
+            rz_spin* spin = nullptr;
+            // every iteration clears the buffers first (renderer.cpp:107-108): RZ_FLAG_CLEARED
+            check(rz_spin_create(ctx, r.mesh(), lights.data(), (int32_t)lights.size(), &a0, dev.frame, dev.depth,
+                                 nullptr, RZ_FLAG_CLEARED | RZ_FLAG_DEVICE_BUFFERS, &spin),
+                  ctx, "rz_spin_create");
+            const uint8_t white[3] = {255, 255, 255}, black[3] = {0, 0, 0};
+            std::string frame_rate;   // empty before the first frame is timed (renderer.cpp:99)
+            float angles[3] = {a0.tait_bryan_angles[0], a0.tait_bryan_angles[1], a0.tait_bryan_angles[2]};
+            float last_angles[3] = {angles[0], angles[1], angles[2]};
             auto last = std::chrono::steady_clock::now();
-            for (int k = 0; k < arguments.frames; ++k) {
-                frame_buffer.fill(0);
-                depth_buffer.fill(1.f);
-                r.draw(lights, arguments, &frame_buffer, &depth_buffer, true);
-                const auto now = std::chrono::steady_clock::now();
-                const float dt = std::chrono::duration<float>(now - last).count();
-                last = now;
-                std::printf("frame %d: %.1f fps\n", k, 1.f / dt);
-                arguments.tait_bryan_angles.y += dt;   // renderer.cpp:124
+            try {
+                for (int k = 0; k < arguments.frames; ++k) {
+                    check(rz_spin_frame(spin, angles), ctx, "rz_spin_frame");
+                    check(rz_draw_text(ctx, dev.frame, (uint32_t)W, (uint32_t)H, 5, 5, frame_rate.c_str(), white,
+                                       black),
+                          ctx, "rz_draw_text");
+                    check(rz_sync(ctx), ctx, "rz_sync");
+                    const auto now = std::chrono::steady_clock::now();
+                    const float dt = std::chrono::duration<float>(now - last).count();
+                    last = now;
+                    frame_rate = std::to_string(1.f / dt);   // renderer.cpp:118-121
+                    std::printf("frame %d: %s fps\n", k, frame_rate.c_str());
+                    for (int c = 0; c < 3; ++c) last_angles[c] = angles[c];
+                    angles[1] += dt;   // renderer.cpp:124
+                }
+            } catch (...) {
+                rz_spin_destroy(spin);
+                throw;
             }
+            rz_spin_destroy(spin);
+            dev.download(frame_buffer, depth_buffer);
+            std::printf("last frame angles: %.9g %.9g %.9g\n", last_angles[0], last_angles[1], last_angles[2]);
         }
         save_frame_png(dir + "/frame.png", frame_buffer.data.data(), W, H);
         save_depth_png(dir + "/depth.png", depth_buffer.data.data(), W, H);

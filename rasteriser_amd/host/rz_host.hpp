@@ -115,6 +115,20 @@ void draw_frame(const std::vector<Vec3>& model_vertices, const std::vector<Trian
                 std::vector<Light>& lights, const std::vector<Material>& materials, const Args& arguments,
                 FrameBuffer* frame_buffer, DepthBuffer* depth_buffer);
 
+// Planar frame and depth buffers in device memory (rz_device_alloc), for frames that
+// stay on the device (the spin loop); download() copies them into host buffers.
+struct DeviceFrame {
+    rz_ctx* ctx = nullptr;
+    int width = 0, height = 0;
+    uint8_t* frame = nullptr;
+    float* depth = nullptr;
+    DeviceFrame(rz_ctx* c, int w, int h);
+    ~DeviceFrame();
+    DeviceFrame(const DeviceFrame&) = delete;
+    DeviceFrame& operator=(const DeviceFrame&) = delete;
+    void download(FrameBuffer& f, DepthBuffer& d) const;
+};
+
 // A mesh kept resident on the device between frames (the spin loop's case).
 class Renderer {
 public:

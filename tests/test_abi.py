@@ -28,7 +28,7 @@ def test_library_exports_every_declared_symbol():
     for n in names:
         assert hasattr(lib, n), f"librz.so does not export {n}"
     assert set(names) == set(abi.EXPORTED)
-    assert lib.rz_abi_version() == 1
+    assert lib.rz_abi_version() == 2
 
 
 def test_record_layouts_match_the_header(tmp_path):
@@ -42,7 +42,7 @@ def test_record_layouts_match_the_header(tmp_path):
 int main(void) {
   P(rz_triangle) P(rz_light) P(rz_material) P(rz_args) P(rz_mesh_desc) P(rz_matrices) P(rz_frame_stats)
   O(rz_triangle, material) O(rz_light, trans_dir) O(rz_material, texels) O(rz_args, z_far)
-  O(rz_mesh_desc, raster_space) O(rz_frame_stats, ms_frame)
+  O(rz_mesh_desc, raster_space) O(rz_frame_stats, ms_frame) O(rz_frame_stats, raster_pixels)
   return 0;
 }
 """)
@@ -61,7 +61,8 @@ int main(void) {
                         "rz_material.texels": (abi.Material, "texels"),
                         "rz_args.z_far": (abi.Args, "z_far"),
                         "rz_mesh_desc.raster_space": (abi.MeshDesc, "raster_space"),
-                        "rz_frame_stats.ms_frame": (abi.FrameStats, "ms_frame")}.items():
+                        "rz_frame_stats.ms_frame": (abi.FrameStats, "ms_frame"),
+                        "rz_frame_stats.raster_pixels": (abi.FrameStats, "raster_pixels")}.items():
         assert int(out[key]) == getattr(T, f).offset, key
 
 

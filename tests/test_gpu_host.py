@@ -71,12 +71,14 @@ def load_png(path):
     return out
 
 
-@pytest.mark.parametrize("case", ["smooth", "flat", "textured", "textured_rgb", "spin_one_frame"])
+@pytest.mark.parametrize("case", ["smooth", "flat", "textured", "textured_rgb", "spin_one_frame", "spin_frames"])
 def test_renderer_cli_frame_matches_oracle(tmp_path, case):
     """UV sphere (smooth, flat), a textured icosphere whose RGBA (or RGB) texture the
     renderer loads as a PNG named by map_Kd, and the spin loop (-s: the mesh uploaded
-    once, frames drawn from the cached mesh) for one frame, drawn before its first
-    rotation step."""
+    once, one captured frame replayed with each frame's rotation, ry advanced by the
+    elapsed time) for one frame, drawn before its first rotation step, and for four
+    frames: the last one at the angles the renderer prints, its frame rate drawn at
+    (5, 5) like the reference's (renderer.cpp:114), the rest of it the oracle's."""
     if not os.path.exists(RENDERER):
         pytest.skip("renderer not built")
     flat = case == "flat"
@@ -97,11 +99,27 @@ def test_renderer_cli_frame_matches_oracle(tmp_path, case):
         rot += [flag, "%.9g" % float(angle)]
     cmd = [RENDERER, "-o", obj, "--mats-dir", str(tmp_path) + "/", "-l", lights, "-x", str(args.image_width),
            "-y", str(args.image_height), "--fovy", "60", "--near", "0.1", "--far", "100", "--out-dir",
-           str(tmp_path)] + rot + (["-f"] if flat else []) + (["-s", "--frames", "1"] if case == "spin_one_frame" else [])
+           str(tmp_path)] + rot + (["-f"] if flat else [])
+    if case.startswith("spin"):
+        cmd += ["-s", "--frames", "1" if case == "spin_one_frame" else "4"]
     print("running", " ".join(cmd), flush=True)
     r = subprocess.run(cmd, capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stderr
-    got = load_png(str(tmp_path / "frame.png"))
+    got = load_png(str(tmp_path / "frame.png"))[:, :, :3]
+    if case.startswith("spin"):
+        line = [l for l in r.stdout.splitlines() if l.startswith("last frame angles:")][0]
+        args.tait_bryan_angles[:] = [float(np.float32(x)) for x in line.split(":")[1].split()]
     of, od, oids, counts, _ = oracle.draw_frame(scene, args)
+    want = np.moveaxis(of, 0, 2)
     assert (oids >= 0).sum() > (30_000 if case.startswith("textured") else 50_000)
-    assert np.array_equal(got[:, :, :3], np.moveaxis(of, 0, 2)), "frame.png differs from the oracle's frame"
+    if case == "spin_frames":
+        rates = [l.split(":")[1].split()[0] for l in r.stdout.splitlines() if l.startswith("frame ")]
+        assert len(rates) == 4
+        box = (slice(5, 21), slice(5, 5 + 12 * len(rates[2])))   # the last frame shows frame 2's rate
+        inside = got[box]
+        assert ((inside == 255).all(axis=2) | (inside == 0).all(axis=2)).all()
+        assert (inside == 255).all(axis=2).sum() > 20 * len(rates[2])
+        got, want = got.copy(), want.copy()
+        got[box] = 0
+        want[box] = 0
+    assert np.array_equal(got, want), "frame.png differs from the oracle's frame"

@@ -367,3 +367,201 @@ def test_counter_generations_across_calls():
         for b in bufs.values():
             for x in b:
                 x.close()
+
+
+def test_many_lights(ctx):
+    """The reference shades any number of lights (shading.cpp:24-29): 200 lights (64 in
+    the kernel arguments, 136 from device memory) against the oracle, eager, in a frame
+    graph (its own copy of the extra lights, kept across a later frame with other lights)
+    and as batch items with 65 and 300 lights."""
+    import oracle
+    from tests.parity import assert_same_frame
+    import math
+    scene, args = scenes.icosphere_scene(subdivisions=4, width=640, height=360, texture_size=256, lights=200)
+    # dim lights, so that no pixel saturates and the 136 beyond the kernel arguments show
+    scene.lights = scenes.random_lights(scenes.SplitMix64(0x5EED7000), 200, scale=255 * math.pi / 60)
+    gpu, ref = check(ctx, scene, args, label="200 lights")
+    assert (gpu[0][:, gpu[2] >= 0] == 255).mean() < 0.01
+    W, H = 640, 360
+    mesh = ctx.upload(scene)
+    f, d, i = _device_outputs(ctx, args)
+    g = ctx.graph(mesh, scene.lights_copy(), args, f.ptr, d.ptr, 2, ids_ptr=i.ptr)
+    other, _ = scenes.icosphere_scene(seed=9, subdivisions=4, width=640, height=360, texture_size=256,
+                                      lights=150)
+    ctx.draw(mesh, other.lights_copy(), args)   # rewrites the context's extra lights
+    g.launch()
+    ctx.sync()
+    ref = oracle.draw_frame(scene, args)
+    st = {"fragments": ref[3]["fragments"], "front_triangles": ref[3]["front_triangles"],
+          "covered_pixels": int((ref[2] >= 0).sum())}
+    assert_same_frame((f.download(), d.download(), i.download(), st, ref[4]), ref, "200-light graph")
+    g.close()
+    work = [scenes.icosphere_scene(seed=s, subdivisions=3, width=256, height=200, texture_size=64, lights=n)
+            for s, n in ((11, 65), (12, 300), (13, 3))]
+    meshes = [ctx.upload(s) for s, a in work]
+    outs = [_device_outputs(ctx, a) for s, a in work]
+    batch = [(m, s.lights_copy(), a, bf, bd, bi) for m, (s, a), (bf, bd, bi) in zip(meshes, work, outs)]
+    ctx.draw_batch(batch)
+    for (s, a), (bf, bd, bi), item in zip(work, outs, batch):
+        ref = oracle.draw_frame(s, a)
+        st = {"fragments": ref[3]["fragments"], "front_triangles": ref[3]["front_triangles"],
+              "covered_pixels": int((ref[2] >= 0).sum())}
+        assert_same_frame((bf.download(), bd.download(), bi.download(), st, item[1]), ref,
+                          f"batch item with {s.lights.shape[0]} lights")
+    for m in meshes + [mesh]:
+        m.close()
+
+
+@pytest.mark.parametrize("W,H", [(40000, 48), (64, 65535)])
+def test_image_sides_above_32767(ctx, W, H):
+    """Image sides up to 65,535 pixels (bbox corners packed in 16 bits each): a 40,000-pixel
+    wide and a 65,535-pixel tall frame against the oracle."""
+    scene, args = scenes.uv_sphere_scene(width=W, height=H)
+    gpu, ref = check(ctx, scene, args, label=f"{W}x{H}")
+    assert (gpu[2] >= 0).sum() > 500
+
+
+def test_spin_loop_replays(ctx):
+    """rz_spin: one frame captured once, replayed with the rotation the spin loop reaches
+    (renderer.cpp:117-124) — each replay equals the oracle at its angles, with eager
+    frames of another size on the same context in between (they move the tile-counter
+    generations the replays rely on)."""
+    import oracle
+    from tests.parity import assert_same_frame
+    scene, args = scenes.icosphere_scene(subdivisions=4, width=640, height=480, texture_size=128)
+    other, oargs = scenes.uv_sphere_scene(width=320, height=200)
+    mesh, omesh = ctx.upload(scene), ctx.upload(other)
+    f, d, i = _device_outputs(ctx, args)
+    spin = ctx.spin(mesh, scene.lights_copy(), args, f.ptr, d.ptr, i.ptr)
+    a0 = [float(x) for x in args.tait_bryan_angles]
+    for k, dy in enumerate([0.0, 0.37, 1.25, 2.5, -0.6]):
+        angles = [a0[0], float(np.float32(np.float32(a0[1]) + np.float32(dy))), a0[2]]
+        print("spin frame", k, angles, flush=True)
+        i.upload(np.full(i.shape, 3, np.int32))
+        spin.frame(angles)
+        ctx.sync()
+        a = type(args).from_buffer_copy(args)
+        a.tait_bryan_angles[:] = angles
+        ref = oracle.draw_frame(scene, a)
+        st = {"fragments": ref[3]["fragments"], "front_triangles": ref[3]["front_triangles"],
+              "covered_pixels": int((ref[2] >= 0).sum())}
+        assert_same_frame((f.download(), d.download(), i.download(), st, ref[4]), ref, f"spin frame {k}")
+        if k in (1, 2):
+            ctx.draw(omesh, other.lights_copy(), oargs)
+    spin.close()
+    for m in (mesh, omesh):
+        m.close()
+
+
+def test_draw_text(ctx):
+    """rz_draw_text: 12x16 cells of white glyph pixels on black at (x, y), nothing else
+    touched, clipped at the frame's edges."""
+    W, H = 120, 40
+    f = ctx.empty((3, H, W), np.uint8)
+    f.upload(np.full((3, H, W), 7, np.uint8))
+    ctx.draw_text(f.ptr, W, H, 5, 5, "12.5")
+    ctx.draw_text(f.ptr, W, H, W - 20, H - 8, "88")   # clipped on two sides
+    ctx.sync()
+    img = f.download()
+    box = np.zeros((H, W), bool)
+    box[5:21, 5:53] = True
+    box[H - 8:, W - 20:] = True
+    assert (img[:, ~box] == 7).all()
+    cells = img[:, box]
+    assert ((cells == 255).all(axis=0) | (cells == 0).all(axis=0)).all()
+    lit = (img[0, 5:21, 5:53] == 255)
+    assert 40 < lit.sum() < 48 * 16 / 2
+    # the four cells differ from each other (distinct glyphs) and the '.' is the smallest
+    per = [lit[:, 12 * c:12 * c + 12].sum() for c in range(4)]
+    assert per[2] == min(per) and len(set(per)) >= 3
+    f.close()
+
+
+def test_batch_256_meshes_full_size(ctx):
+    """The batched workload at its stated size: 256 meshes (log-uniform 500-200k faces,
+    8.5M in all, viewports 256^2-1024^2) autotuned and drawn in one batch graph.  Every
+    mesh equals its own single-mesh frame at the tuned tile size and capacity, bit for bit;
+    16 seeded meshes, and every mesh whose bins overflow at that capacity, equal the
+    oracle (drawing.cpp:205-258 per mesh)."""
+    import oracle
+    from tests.parity import assert_same_frame
+    work = scenes.batch_scenes()
+    assert len(work) == 256 and sum(s.num_faces for s, a in work) > 8_000_000
+    print("uploading 256 meshes", flush=True)
+    meshes = [ctx.upload(s) for s, a in work]
+    outs = [_device_outputs(ctx, a) for s, a in work]
+    lights = [s.lights_copy() for s, a in work]
+    batch = [(m, l, a, f, d, i) for m, l, (s, a), (f, d, i) in zip(meshes, lights, work, outs)]
+    try:
+        ts, cap, ms = ctx.autotune(batch)
+        print("autotuned", ts, cap, ms, flush=True)
+        g = ctx.batch_graph(batch, frames=2)
+        for f, d, i in outs:
+            i.upload(np.full(i.shape, 7, np.int32))
+        g.launch()
+        ctx.sync()
+        g.close()
+        rng = np.random.default_rng(2026)
+        sample = set(rng.choice(256, 16, replace=False).tolist())
+        overflowed = 0
+        for k, ((scene, args), (f, d, i), m) in enumerate(zip(work, outs, meshes)):
+            fb, db, ib = f.download(), d.download(), i.download()
+            single = ctx.draw(m, scene.lights_copy(), args, ids=True, stats=True)
+            assert np.array_equal(ib, single[2]) and np.array_equal(fb, single[0]), k
+            assert np.array_equal(db.view(np.uint32), single[1].view(np.uint32)), k
+            if single[3]["overflow_tiles"] > 0:
+                overflowed += 1
+                sample.add(k)
+            if k in sample:
+                print("oracle check of mesh", k, scene.num_faces, "faces", flush=True)
+                ref = oracle.draw_frame(scene, args)
+                assert_same_frame((fb, db, ib, single[3], lights[k]), ref, f"batch mesh {k}")
+        print("meshes with overflowing bins:", overflowed, flush=True)
+    finally:
+        ctx.set_tuning(16, 0)
+        for f, d, i in outs:
+            f.close(); d.close(); i.close()
+        for m in meshes:
+            m.close()
+
+
+@pytest.mark.parametrize("tile,cap", [(16, 512), (32, 2048), (8, 64), (16, 2)])
+def test_batch_one_launch_equals_five(ctx, tile, cap):
+    """A batch frame as one persistent launch and as five launches, one per pass (the
+    default): the same bits and statistics, at every tile size and with overflowing bins;
+    a batch graph of three frames replays the one-launch queue frame after frame."""
+    work = scenes.batch_scenes(n=9, seed=41, max_faces=25_000)
+    meshes = [ctx.upload(s) for s, a in work]
+    outs = [_device_outputs(ctx, a) for s, a in work]
+    batch = [(m, s.lights_copy(), a, f, d, i) for m, (s, a), (f, d, i) in zip(meshes, work, outs)]
+    ctx.set_tuning(tile, cap)
+    try:
+        results = {}
+        for persistent in (False, True):
+            ctx.set_batch_persistent(persistent)
+            print("batch", tile, cap, "persistent", persistent, flush=True)
+            st = ctx.draw_batch(batch, stats=True)
+            results[persistent] = (st, [(f.download(), d.download(), i.download()) for f, d, i in outs])
+        (s5, r5), (s1, r1) = results[False], results[True]
+        for key in ("front_triangles", "bin_pairs", "fragments", "covered_pixels", "overflow_tiles",
+                    "raster_tiles", "raster_pixels"):
+            assert s1[key] == s5[key], (key, s1[key], s5[key])
+        if cap <= 2:
+            assert s1["overflow_tiles"] > 0
+        for k, (a, b) in enumerate(zip(r1, r5)):
+            assert np.array_equal(a[0], b[0]) and np.array_equal(a[2], b[2]), k
+            assert np.array_equal(a[1].view(np.uint32), b[1].view(np.uint32)), k
+        g = ctx.batch_graph(batch, frames=3)
+        for launch in range(2):
+            for f, d, i in outs:
+                i.upload(np.full(i.shape, 5, np.int32))
+            g.launch()
+            ctx.sync()
+            for (f, d, i), b in zip(outs, r5):
+                assert np.array_equal(i.download(), b[2]) and np.array_equal(f.download(), b[0]), launch
+        g.close()
+    finally:
+        ctx.set_tuning(16, 0)
+        ctx.set_batch_persistent(False)
+        for m in meshes:
+            m.close()
