@@ -219,6 +219,28 @@ rz_status rz_draw_frame(rz_ctx* ctx, const rz_mesh* mesh,
                         uint8_t* frame_buffer, float* depth_buffer, int32_t* id_buffer,
                         uint32_t flags, rz_frame_stats* stats);
 
+/* No reference counterpart.  For callers that cannot allocate device memory: code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc.  Draws one
+ * frame as rz_draw_frame does on device buffers (same kernels, same results), with every
+ * working buffer of the frame (face records, tile counters and lists, bins, visibility
+ * words, statistics) carved from a fixed device array of RZ_SCRATCH_BYTES that librz
+ * holds on each GPU from the first rz_create on it until the process ends; nothing is
+ * allocated or freed.  frame_device / depth_device / id_device (may be NULL) are device
+ * buffers of the context's device (RZ_FLAG_DEVICE_BUFFERS is implied); RZ_FLAG_CLEARED
+ * and RZ_FLAG_ASYNC apply.  With the context's automatic bin capacity the capacity is the
+ * largest that fits the array (tiles beyond it take the overflow path, same pixels).
+ * Limits: a frame whose working set does not fit RZ_SCRATCH_BYTES is rejected with
+ * RZ_ERR_INVALID_ARG (the 81,920-face icosphere at 1920x1080 fits, with bins of about 1,150
+ * faces; 128 bytes per face plus 8 bytes per pixel and the bins are the bulk), as are more
+ * than 64 lights and RZ_FLAG_MLP_SHADE.  Every call on one GPU uses the same array and leaves
+ * its contents behind (each call clears what must start at zero): calls that overlap on
+ * one GPU (two contexts' streams, two host threads) race and must be serialised by the
+ * caller. */
+#define RZ_SCRATCH_BYTES (64u << 20)
+rz_status rz_draw_frame_noalloc(rz_ctx* ctx, const rz_mesh* mesh, rz_light* lights, int32_t num_lights,
+                                const rz_args* args, uint8_t* frame_device, float* depth_device,
+                                int32_t* id_device, uint32_t flags);
+
 /* draw_frame with the reference's by-value semantics: uploads desc, draws one frame,
  * frees the upload (drawing.h:16-18 signature, host buffers). */
 rz_status rz_draw_frame_desc(rz_ctx* ctx, const rz_mesh_desc* desc,

@@ -288,6 +288,16 @@ class Context:
                                       C.byref(st) if stats else None), self.handle)
         return st.as_dict() if stats else None
 
+    def draw_noalloc(self, mesh: Mesh, lights: np.ndarray, args: Args, frame_ptr: int, depth_ptr: int,
+                     ids_ptr: int = 0, flags: int = RZ_FLAG_CLEARED):
+        """rz_draw_frame_noalloc: draw_frame into device buffers with librz's fixed
+        per-GPU working array (no device allocation)."""
+        lights = _lights(lights)
+        _check(self.lib.rz_draw_frame_noalloc(self.handle, mesh.handle, lights.ctypes.data, lights.shape[0],
+                                              C.byref(args), C.c_void_p(frame_ptr), C.c_void_p(depth_ptr),
+                                              C.c_void_p(ids_ptr) if ids_ptr else None,
+                                              flags | RZ_FLAG_DEVICE_BUFFERS), self.handle)
+
     def graph(self, mesh: Mesh, lights: np.ndarray, args: Args, frame_ptr: int, depth_ptr: int,
               frames: int, ids_ptr: int = 0, flags: int = RZ_FLAG_CLEARED, ry_step: float = 0.0):
         """rz_graph_create: `frames` consecutive frames captured in one HIP graph."""

@@ -127,6 +127,7 @@ def declare(lib):
                                     C.POINTER(FrameStats)]),
         "rz_draw_frame_desc": (C.c_int, [P, C.POINTER(MeshDesc), P, C.c_int32, C.POINTER(Args),
                                          P, P, C.c_uint32]),
+        "rz_draw_frame_noalloc": (C.c_int, [P, P, P, C.c_int32, C.POINTER(Args), P, P, P, C.c_uint32]),
         "rz_graph_create": (C.c_int, [P, P, P, C.c_int32, C.POINTER(Args), P, P, P, C.c_uint32,
                                       C.c_int32, C.c_float, C.POINTER(P)]),
         "rz_graph_launch": (C.c_int, [P]),
@@ -168,7 +169,7 @@ def declare(lib):
 
 EXPORTED = ("rz_abi_version", "rz_status_string", "rz_last_error", "rz_create", "rz_destroy",
             "rz_set_tuning", "rz_get_tuning", "rz_set_fused_shade", "rz_get_fused_shade", "rz_set_profiling", "rz_stream", "rz_sync",
-            "rz_mesh_create", "rz_mesh_destroy", "rz_draw_frame", "rz_draw_frame_desc",
+            "rz_mesh_create", "rz_mesh_destroy", "rz_draw_frame", "rz_draw_frame_desc", "rz_draw_frame_noalloc",
             "rz_graph_create", "rz_graph_launch", "rz_graph_destroy", "rz_spin_create", "rz_spin_frame",
             "rz_spin_destroy", "rz_draw_text",
             "rz_draw_batch", "rz_set_batch_persistent", "rz_get_batch_persistent", "rz_batch_graph_create", "rz_autotune",

@@ -7,6 +7,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -130,6 +131,8 @@ struct rz_ctx {
     const void* count_ptr = nullptr;
     int32_t count_ntiles = -1;
     int count_gen = 0;
+    // this device's copy of `scratch` (rz_draw_frame_noalloc's working memory)
+    unsigned char* scratch = nullptr;
 };
 
 namespace {
@@ -168,6 +171,32 @@ rz_status ensure(rz_ctx* ctx, DevBuf& b, size_t bytes, bool stales_graphs = true
     b.bytes = bytes;
     if (stales_graphs) ++ctx->buffer_gen;
     if (&b == &ctx->tile_count) ctx->count_ntiles = -1;   // new memory: contents unknown
+    return RZ_OK;
+}
+
+// rz_draw_frame_noalloc carves every working buffer of its frame from this array: one
+// copy per GPU, held from the library's first context on that GPU until the process
+// ends.  Host code never uses the name itself (on the host it is only a stand-in for
+// the symbol): each device's address comes from hipGetSymbolAddress once, at the first
+// rz_create on that device (scratch_address), and only that address reaches kernels.
+static __device__ __align__(256) unsigned char scratch[RZ_SCRATCH_BYTES];
+
+std::mutex g_scratch_mutex;
+std::vector<unsigned char*> g_scratch_ptr;   // per device, nullptr until looked up
+
+// This device's address of `scratch` (the device must be current).
+rz_status scratch_address(rz_ctx* ctx) {
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if ((int)g_scratch_ptr.size() <= ctx->device) g_scratch_ptr.resize((size_t)ctx->device + 1, nullptr);
+    if (!g_scratch_ptr[ctx->device]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return fail(ctx, RZ_ERR_HIP, "hipGetSymbolAddress(scratch) on device %d: %s", ctx->device,
+                        hipGetErrorString(e));
+        g_scratch_ptr[ctx->device] = static_cast<unsigned char*>(p);
+    }
+    ctx->scratch = g_scratch_ptr[ctx->device];
     return RZ_OK;
 }
 
@@ -1018,6 +1047,7 @@ rz_status rz_create(int32_t device, rz_ctx** out) {
         if (e == hipSuccess) ctx->persistent_blocks = (uint32_t)(cus * kPersistentPerCU) / 8u * 8u;
         ctx->cus = cus;
         if (e != hipSuccess) st = fail(ctx, RZ_ERR_HIP, "context setup: %s", hipGetErrorString(e));
+        if (st == RZ_OK) st = scratch_address(ctx);
     } while (0);
     if (st != RZ_OK) {
         rz_destroy(ctx);
@@ -2074,6 +2104,79 @@ rz_status rz_mlp_infer(rz_ctx* ctx, const float* features_device, int64_t n, flo
     RZ_HIP(ctx, hipSetDevice(ctx->device));
     RZ_HIP(ctx, launch_mlp_infer(features_device, n, (const uint16_t*)ctx->mlp_packed.ptr, out_device, ctx->stream));
     RZ_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return RZ_OK;
+}
+
+rz_status rz_draw_frame_noalloc(rz_ctx* ctx, const rz_mesh* mesh, rz_light* lights, int32_t nl,
+                                const rz_args* args, uint8_t* frame_device, float* depth_device, int32_t* id_device,
+                                uint32_t flags) {
+    if (!ctx) return RZ_ERR_INVALID_ARG;
+    if (!ctx->scratch) return fail(ctx, RZ_ERR_HIP, "rz_draw_frame_noalloc: no scratch address for device %d", ctx->device);
+    if (!frame_device || !depth_device) return fail(ctx, RZ_ERR_INVALID_ARG, "null frame or depth buffer");
+    if (flags & RZ_FLAG_MLP_SHADE)
+        return fail(ctx, RZ_ERR_INVALID_ARG, "rz_draw_frame_noalloc: RZ_FLAG_MLP_SHADE is not supported");
+    if (nl > kMaxLights)
+        return fail(ctx, RZ_ERR_INVALID_ARG,
+                    "rz_draw_frame_noalloc: %d lights, at most %d (more need device memory of their own)", nl,
+                    kMaxLights);
+    rz_status st = check_frame(ctx, mesh, lights, nl, args);
+    if (st != RZ_OK) return st;
+    RZ_HIP(ctx, hipSetDevice(ctx->device));
+    uint32_t ntiles = 0, grid = 0;
+    bool narrow = false;
+    frame_shape(ctx, *args, kShadePlain, ntiles, narrow, grid);
+    // Working set in `scratch`: the layout of rz_draw_frame with a bin capacity that fits.
+    // With the fused pipeline and room for every face in every bin no tile can overflow
+    // and the visibility buffer is never touched, so it takes no space.
+    const uint32_t nf = (uint32_t)(mesh->nf > 0 ? mesh->nf : 1);
+    auto total_bytes = [&](uint32_t cap, FrameLayout& lay) {
+        const bool can_overflow = cap < (uint32_t)mesh->nf;
+        lay = layout_frame(mesh, *args, ctx->tile_size, (int)cap, ctx->fused, grid,
+                           can_overflow ? ctx->persistent_blocks : 0u, false);
+        if (ctx->fused && !can_overflow) lay.bytes[kBufVis] = 0;
+        size_t t = 0;
+        for (int b = 0; b < kNumBufs; ++b) t += align256(lay.bytes[b]);
+        return t;
+    };
+    FrameLayout lay;
+    uint32_t cap = ctx->bin_cap > 0 ? (uint32_t)ctx->bin_cap : nf;
+    size_t need = total_bytes(cap, lay);
+    if (need > (size_t)RZ_SCRATCH_BYTES && ctx->bin_cap == 0) {
+        // automatic capacity: the largest that fits beside the other buffers
+        const size_t rest = total_bytes(1, lay) - align256(lay.bytes[kBufBins]);
+        const size_t per_entry = (size_t)lay.ntiles * 4;
+        cap = rest + 256 < (size_t)RZ_SCRATCH_BYTES ? (uint32_t)std::min<size_t>(
+                                                           ((size_t)RZ_SCRATCH_BYTES - rest - 256) / per_entry, nf)
+                                                     : 0u;
+        if (cap >= 1) need = total_bytes(cap, lay);
+    }
+    if (cap < 1 || need > (size_t)RZ_SCRATCH_BYTES)
+        return fail(ctx, RZ_ERR_INVALID_ARG,
+                    "rz_draw_frame_noalloc: a %ux%u frame of %d faces needs %zu bytes of working memory, more "
+                    "than the %zu of the no-allocation path",
+                    args->image_width, args->image_height, mesh->nf, need, (size_t)RZ_SCRATCH_BYTES);
+    void* ptrs[kNumBufs];
+    size_t at = 0;
+    for (int b = 0; b < kNumBufs; ++b) {
+        ptrs[b] = ctx->scratch + at;
+        at += align256(lay.bytes[b]);
+    }
+    FrameLaunch L;
+    fill_frame(mesh, lights, nl, *args, flags, lay, ptrs, L, 0);
+    L.rp.narrow = narrow ? 1 : 0;
+    L.sp.max_blocks = lay.tiles_grid;
+    bind_outputs(L, (flags & RZ_FLAG_CLEARED) != 0, frame_device, depth_device, id_device);
+    hipStream_t s = ctx->stream;
+    // the array keeps the last call's contents: the frame's counter lines (generation 0)
+    // start at zero, by a fill kernel
+    BinParams z;
+    std::memset(&z, 0, sizeof z);
+    z.zero_count = count_lines(lay.ntiles);
+    z.zero_lines = (uint32_t*)ptrs[kBufTiles];
+    z.zero_blocks = (z.zero_count + kFrameBinBlock - 1) / kFrameBinBlock;
+    RZ_HIP(ctx, launch_bin(z, s));
+    if ((st = enqueue_frame(ctx, L, s, false)) != RZ_OK) return st;
+    if (!(flags & RZ_FLAG_ASYNC)) RZ_HIP(ctx, hipStreamSynchronize(s));
     return RZ_OK;
 }
 
