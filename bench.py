@@ -219,6 +219,20 @@ def stress(ctx, rz, scenes, np, frames):
         mesh.close()
 
 
+def batch_graph_ms(ctx, batch, frames):
+    """ms per batch frame of a graph of `frames` batch frames, after one warm-up launch."""
+    g = ctx.batch_graph(batch, frames)
+    try:
+        g.launch()
+        ctx.sync()
+        t0 = time.perf_counter()
+        g.launch()
+        ctx.sync()
+        return (time.perf_counter() - t0) * 1000.0 / frames
+    finally:
+        g.close()
+
+
 def batched(ctx, rz, scenes, np, frames):
     """The 256-mesh batch: autotuned, `frames` batch frames in one HIP graph."""
     t0 = time.perf_counter()
@@ -237,14 +251,14 @@ def batched(ctx, rz, scenes, np, frames):
         t0 = time.perf_counter()
         ts, cap, tuned_ms = ctx.autotune(batch)
         t_tune = time.perf_counter() - t0
-        g = ctx.batch_graph(batch, frames)
-        g.launch()
-        ctx.sync()
-        t0 = time.perf_counter()
-        g.launch()
-        ctx.sync()
-        ms = (time.perf_counter() - t0) * 1000.0 / frames
-        g.close()
+        # the context's batch launch shape (one persistent launch or five), and the other
+        default_one = ctx.batch_persistent
+        launch_ms = {}
+        for one in (default_one, not default_one):
+            ctx.set_batch_persistent(one)
+            launch_ms["one_launch" if one else "five_launches"] = batch_graph_ms(ctx, batch, frames)
+        ctx.set_batch_persistent(default_one)
+        ms = launch_ms["one_launch" if default_one else "five_launches"]
         # counts: the tuned frame's statistics and ids; P16 at 16-pixel tiles
         st = ctx.draw_batch(with_ids, stats=True)
         ctx.set_tuning(16, 512)   # (pairs are counted from the faces' tile rectangles, overflow or not)
@@ -254,12 +268,18 @@ def batched(ctx, rz, scenes, np, frames):
         for f, d, i in outs:
             idh = i.download()
             vis += int(np.unique(idh[idh >= 0]).size)
-        # per-pass times: HIP events around each launch of one batch frame (median of 5)
+        # per-pass times: HIP events around each launch of one batch frame of five launches
+        # (median of 5); the one-launch frame's phases as spans of the workgroups' clocks
         ctx.set_profiling(True)
-        runs = [ctx.draw_batch(batch, stats=True) for _ in range(5)]
+        phases = {}
+        for one in (False, True):
+            ctx.set_batch_persistent(one)
+            runs = [ctx.draw_batch(batch, stats=True) for _ in range(5)]
+            phases[one] = {k: float(np.median([r["ms_" + k] for r in runs])) * 1000.0
+                           for k in ("vertex", "bin", "raster", "overflow", "shade", "frame")}
+        ctx.set_batch_persistent(default_one)
         ctx.set_profiling(False)
-        med = {k: float(np.median([r["ms_" + k] for r in runs])) * 1000.0
-               for k in ("vertex", "bin", "raster", "overflow", "shade")}
+        med = phases[False]
         V = sum(sc.vertices.shape[0] for sc, _ in work)
         Vn = sum(sc.normals.shape[0] for sc, _ in work)
         T = sum(sc.num_faces for sc, _ in work)
@@ -275,6 +295,9 @@ def batched(ctx, rz, scenes, np, frames):
         return {"workload": "256 meshes, log-uniform 500-200k faces, viewports 256^2-1024^2",
                 "meshes": len(work), "faces": T, "pixels": pixels, "tile": ts, "bin_capacity": cap,
                 "autotune_ms_per_frame": tuned_ms, "ms_per_batch_frame": ms,
+                "launch": "one persistent launch" if default_one else "five launches",
+                "ms_per_batch_frame_by_launch": launch_ms,
+                "one_launch_phase_spans_us": {k: phases[True][k] for k in ("bin", "raster", "overflow", "shade")},
                 "mtri_per_s": T / (ms * 1e-3) / 1e6, "gpix_per_s": pixels / (ms * 1e-3) / 1e9,
                 "counts": {"front_triangles": st["front_triangles"], "bin_pairs16": p16, "visible_faces": vis,
                            "fragments": st["fragments"], "covered_pixels": st["covered_pixels"],

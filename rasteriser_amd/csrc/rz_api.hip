@@ -27,9 +27,9 @@ hipError_t launch_bin_batch(const BinParams* items, const uint2* map, uint32_t b
 hipError_t launch_overflow_batch(const RasterParams* items, const uint2* map, uint32_t blocks, hipStream_t s);
 uint32_t raster_batch_blocks_per_cu(int tile_size, int mode);
 uint32_t batch_frame_blocks_per_cu(int tile_size, int mode);
-hipError_t launch_batch_frame(const VertexParams* v, const BinParams* b, const RasterParams* r,
-                              const RasterParams* o, const ShadeParams* sh, const BatchLists& bl,
-                              const BatchQueue& q, int tile_size, int mode, hipStream_t s);
+hipError_t launch_batch_frame(const BinParams* b, const RasterParams* r, const RasterParams* o,
+                              const ShadeParams* sh, const BatchLists& bl0, const BatchLists& bl1,
+                              const BatchSync& q, int tile_size, int mode, hipStream_t s);
 hipError_t launch_batch_tiles(const RasterParams* items, const ShadeParams* sitems, const BatchLists& bl,
                               uint32_t raster_grid, uint32_t tiles_grid, int tile_size, int mode, hipStream_t s,
                               bool raster);
@@ -262,6 +262,10 @@ struct FrameLayout {
     int ts = 16, tsl = 4, tiles_x = 0, tiles_y = 0, ntiles = 0;
     bool fused = false;     // raster pass shades non-overflowed tiles
     bool batch = false;     // one mesh of a batch (tile lists and raster statistics shared)
+    // two generations of counter lines and no vertex pass (single frames, and the meshes
+    // of a batch's persistent launch): the bin pass transforms each face's corners and
+    // zeroes the other generation
+    bool twogen = true;
     uint32_t raster_grid = 0, tiles_grid = 0;   // workgroups of the list-driven launches
     uint32_t W = 0, H = 0, cap = 0;
     size_t np = 0;
@@ -283,6 +287,7 @@ FrameLayout layout_frame(const rz_mesh* mesh, const rz_args& a, int ts, int cap,
     L.ts = ts;
     L.fused = fused || batch;
     L.batch = batch;
+    L.twogen = !batch;
     L.raster_grid = raster_grid;
     L.tiles_grid = tiles_grid;
     L.tsl = log2_exact(ts);
@@ -316,6 +321,25 @@ FrameLayout layout_frame(const rz_mesh* mesh, const rz_args& a, int ts, int cap,
     L.bytes[kBufBins] = (size_t)L.ntiles * L.cap * 4;
     L.bytes[kBufVis] = L.np * 8;
     L.bytes[kBufCounters] = batch ? kNumCounters * 8 : 0;
+    L.bytes[kBufPart] = L.n_part * 4;
+    return L;
+}
+
+// One mesh of a batch's persistent launch: a single frame's buffers (two generations of
+// counter lines, the bin pass transforming each face's corners) with the batch's shared
+// tile lists and raster statistics; its bin and overflow work in workgroups of bblock.
+FrameLayout layout_batch_sync(const rz_mesh* mesh, const rz_args& a, int ts, int cap, int bblock) {
+    FrameLayout L = layout_frame(mesh, a, ts, cap, true, 0, 0, false);
+    L.batch = true;
+    L.twogen = true;
+    const size_t nbin = bin_blocks(mesh->nf, bblock);
+    L.off_front = 0;
+    L.off_pairs = nbin;
+    L.off_frag = 2 * nbin;
+    L.off_ovf = L.off_frag;   // (raster statistics: the batch's, per workgroup)
+    L.off_cov = L.off_ovf + overflow_blocks(mesh->nf, bblock);
+    L.n_part = L.off_cov;
+    L.bytes[kBufLists] = 0;   // the batch's lists
     L.bytes[kBufPart] = L.n_part * 4;
     return L;
 }
@@ -370,8 +394,8 @@ void fill_frame(const rz_mesh* mesh, rz_light* lights, int32_t nl, const rz_args
         }
     }
     const size_t gen_words = (size_t)count_lines(lay.ntiles) * kCountStride;
-    uint32_t* tile_count = (uint32_t*)buf[kBufTiles] + (lay.batch ? 0 : gen * gen_words);
-    auto* counters = lay.batch ? (unsigned long long*)buf[kBufCounters]
+    uint32_t* tile_count = (uint32_t*)buf[kBufTiles] + (lay.twogen ? gen * gen_words : 0);
+    auto* counters = !lay.twogen ? (unsigned long long*)buf[kBufCounters]
                                : (unsigned long long*)(tile_count + ((size_t)lay.ntiles + 8) * kCountStride);
     auto* part = (uint32_t*)buf[kBufPart];
     L.part = part;
@@ -422,7 +446,7 @@ void fill_frame(const rz_mesh* mesh, rz_light* lights, int32_t nl, const rz_args
     bp.lists = (uint32_t*)buf[kBufLists];   // (batches: replaced by the batch's lists)
     bp.list_count = bp.tile_count + (size_t)lay.ntiles * kCountStride;
     bp.ovf_vis = (unsigned long long*)buf[kBufVis];
-    if (!lay.batch) {
+    if (lay.twogen) {
         // no vertex pass: the bin pass transforms each face's corners
         bp.pos = mesh->raster_space ? nullptr : mesh->pos4;
         bp.raster = mesh->raster_space ? mesh->rin : nullptr;
@@ -723,15 +747,19 @@ struct BatchLaunch {
     BatchLists bl = {};
     uint32_t raster_grid = 0, tiles_grid = 0;
     size_t arena_part_bytes = 0;   // statistics: first bytes of the arena (the batch's, then every frame's)
-    bool persistent = false;       // one k_batch_frame launch (queue q), else five launches
-    BatchQueue q = {};
-    size_t part_off = 0;           // the batch's statistics, after the queue block
+    size_t part_off = 0;           // the batch's statistics, after the barrier block
+    // the persistent launch (k_batch_frame): each mesh's launch of generation 1 (frames[]
+    // holds generation 0), the lists of each generation, the barriers
+    bool persistent = false;
+    std::vector<FrameLaunch> frames1;
+    BatchLists bl1 = {};
+    BatchSync q = {};
 };
 
-// The persistent launch's queue block at the start of the arena: ticket, bin and overflow
-// counts, error flag, the phase clock, then one vertex count per mesh (all 8 bytes).
-constexpr size_t kQueueFixed = 1 + 2 + 1 + 10;
-size_t queue_bytes(int32_t n) { return (kQueueFixed + (size_t)n) * 8; }
+// The persistent launch's block at the start of the arena: 3 barriers of 8 counters (one
+// per 128-byte line), the error flag, the phase clock (4 phases).
+constexpr size_t kSyncWords = 3 * 8 * (kCountStride / 2);
+constexpr size_t kSyncBytes = (kSyncWords + 1 + 8) * 8;
 
 rz_status prepare_batch(rz_ctx* ctx, const rz_batch_item* items, int32_t n, uint32_t flags, DevBuf& arena,
                         DevBuf& tables, BatchLaunch& B) {
@@ -745,18 +773,15 @@ rz_status prepare_batch(rz_ctx* ctx, const rz_batch_item* items, int32_t n, uint
         return fail(ctx, RZ_ERR_INVALID_ARG, "RZ_FLAG_MLP_SHADE without network weights (rz_mlp_init / rz_mlp_set_weights)");
     B.ts = ctx->tile_size;
     B.mode = (flags & RZ_FLAG_MLP_SHADE) ? kShadeMlp : kShadePlain;
-    B.persistent = ctx->batch_persistent;
-    // the persistent launch's workgroups are its raster and overflow-shade slots
-    const int bblock = B.persistent ? batch_block(B.ts) : 256;
-    const uint32_t per_cu = B.persistent ? batch_frame_blocks_per_cu(B.ts, B.mode)
-                                         : raster_batch_blocks_per_cu(B.ts, B.mode);
-    uint32_t grid = (uint32_t)ctx->cus * per_cu / 8u * 8u;
+    B.persistent = false;
+    const int bblock = 256;
+    uint32_t grid = (uint32_t)ctx->cus * raster_batch_blocks_per_cu(B.ts, B.mode) / 8u * 8u;
     B.raster_grid = grid < 8 ? 8 : grid;
-    B.tiles_grid = B.persistent ? B.raster_grid : (uint32_t)ctx->persistent_blocks;
+    B.tiles_grid = (uint32_t)ctx->persistent_blocks;
     std::vector<FrameLayout> lays(n);
-    // arena: [queue][batch statistics][statistics of every frame][tile lists][the other buffers of every frame]
-    const size_t qbytes = align256(queue_bytes(n));
-    const size_t batch_stat_bytes = qbytes + align256(((size_t)B.raster_grid * 2 + B.tiles_grid) * 4);
+    // arena: [batch statistics][statistics of every frame][tile lists][the other buffers of every frame]
+    const size_t qbytes = 0;
+    const size_t batch_stat_bytes = align256(((size_t)B.raster_grid * 2 + B.tiles_grid) * 4);
     size_t stat_bytes = batch_stat_bytes, work_bytes = 0, stride = 0;
     for (int32_t i = 0; i < n; ++i) {
         const rz_batch_item& it = items[i];
@@ -779,7 +804,6 @@ rz_status prepare_batch(rz_ctx* ctx, const rz_batch_item* items, int32_t n, uint
     B.arena_part_bytes = stat_bytes;
     B.part_off = qbytes;
     char* base = (char*)arena.ptr;
-    unsigned long long* queue = (unsigned long long*)base;
     uint32_t* batch_part = (uint32_t*)(base + qbytes);
     uint32_t* lists = (uint32_t*)(base + stat_bytes);
     size_t so = batch_stat_bytes, wo = stat_bytes + list_bytes;
@@ -834,11 +858,6 @@ rz_status prepare_batch(rz_ctx* ctx, const rz_batch_item* items, int32_t n, uint
         total = align256(total + map[p].size() * sizeof(uint2));
         B.blocks[p] = (uint32_t)map[p].size();
     }
-    // per mesh: vertex items of one frame (the persistent launch's bin items wait for them)
-    const size_t off_nvb = total;
-    std::vector<uint32_t> nvb(n, 0);
-    for (const uint2& e : map[0]) ++nvb[e.x];
-    total = align256(total + (size_t)n * 4);
     // the items' lights beyond the kernel arguments, one after the other
     const size_t off_ext = total;
     std::vector<size_t> ext_at(n, 0);
@@ -866,7 +885,6 @@ rz_status prepare_batch(rz_ctx* ctx, const rz_batch_item* items, int32_t n, uint
     }
     for (int p = 0; p < 3; ++p)
         if (!map[p].empty()) std::memcpy(host.data() + off[4 + p], map[p].data(), map[p].size() * sizeof(uint2));
-    std::memcpy(host.data() + off_nvb, nvb.data(), (size_t)n * 4);
     // the table buffer may still be read by queued work of an earlier batch
     RZ_HIP(ctx, hipStreamSynchronize(ctx->stream));
     RZ_HIP(ctx, hipMemcpy(tables.ptr, host.data(), total, hipMemcpyHostToDevice));
@@ -877,31 +895,186 @@ rz_status prepare_batch(rz_ctx* ctx, const rz_batch_item* items, int32_t n, uint
     B.d_sp = (const ShadeParams*)(t + off[3]);
     B.d_orp = (const RasterParams*)(t + off_orp);
     for (int p = 0; p < 3; ++p) B.d_map[p] = (const uint2*)(t + off[4 + p]);
-    // the queue's counters start at zero (they only grow over the frames of this batch)
-    const std::vector<unsigned long long> zeros(queue_bytes(n) / 8, 0ull);
-    RZ_HIP(ctx, hipMemcpy(queue, zeros.data(), queue_bytes(n), hipMemcpyHostToDevice));
-    BatchQueue& q = B.q;
-    q.ticket = queue;
-    q.done = queue + 1;
-    q.error = (uint32_t*)(queue + 3);
-    q.clock = nullptr;   // (queue + 4 when rz_set_profiling: set per frame)
-    q.vdone = queue + kQueueFixed;
-    q.nvb = (const uint32_t*)(t + off_nvb);
-    q.vmap = B.d_map[0];
-    q.bmap = B.d_map[1];
-    q.omap = B.d_map[2];
-    q.nV = B.blocks[0];
-    q.nB = B.blocks[1];
-    q.nO = B.blocks[2];
-    q.G = B.raster_grid;
     return RZ_OK;
+}
+
+// A batch for the persistent launch (k_batch_frame): every mesh laid out as a single frame
+// (two generations of counter lines), the tile lists and raster statistics shared, the
+// parameter tables holding each mesh's bin, raster and overflow parameters for generation 0
+// and then generation 1, and the barriers at the start of the arena, zeroed.
+rz_status prepare_batch_sync(rz_ctx* ctx, const rz_batch_item* items, int32_t n, uint32_t flags, DevBuf& arena,
+                             DevBuf& tables, BatchLaunch& B) {
+    if (n < 1 || !items) return fail(ctx, RZ_ERR_INVALID_ARG, "empty batch");
+    if (n > kMaxBatchMeshes) return fail(ctx, RZ_ERR_INVALID_ARG, "batch of %d meshes: at most %d", n, kMaxBatchMeshes);
+    if (!(flags & RZ_FLAG_DEVICE_BUFFERS))
+        return fail(ctx, RZ_ERR_INVALID_ARG, "a batch needs device buffers (RZ_FLAG_DEVICE_BUFFERS)");
+    RZ_HIP(ctx, hipSetDevice(ctx->device));
+    const bool cleared = (flags & RZ_FLAG_CLEARED) != 0;
+    if ((flags & RZ_FLAG_MLP_SHADE) && !ctx->mlp_ready)
+        return fail(ctx, RZ_ERR_INVALID_ARG, "RZ_FLAG_MLP_SHADE without network weights (rz_mlp_init / rz_mlp_set_weights)");
+    B.ts = ctx->tile_size;
+    B.mode = (flags & RZ_FLAG_MLP_SHADE) ? kShadeMlp : kShadePlain;
+    B.persistent = true;
+    const int bblock = batch_block(B.ts);
+    // every workgroup of the launch resident at once: the launch's occupancy on every CU
+    uint32_t grid = (uint32_t)ctx->cus * batch_frame_blocks_per_cu(B.ts, B.mode) / 8u * 8u;
+    B.raster_grid = B.tiles_grid = grid < 8 ? 8 : grid;
+    const int cap = ctx->bin_cap > 0 ? ctx->bin_cap : kDefaultBinCap;
+    std::vector<FrameLayout> lays(n);
+    // arena: [barriers][batch statistics][statistics of every mesh][tile lists][every mesh's buffers]
+    const size_t sync_bytes = align256(kSyncBytes);
+    const size_t batch_stat_bytes = sync_bytes + align256(((size_t)B.raster_grid * 2 + B.tiles_grid) * 4);
+    size_t stat_bytes = batch_stat_bytes, work_bytes = 0, stride = 0;
+    for (int32_t i = 0; i < n; ++i) {
+        const rz_batch_item& it = items[i];
+        rz_status st = check_frame(ctx, it.mesh, it.lights, it.num_lights, &it.args);
+        if (st != RZ_OK) return st;
+        if (!it.frame || !it.depth) return fail(ctx, RZ_ERR_INVALID_ARG, "batch item %d: null frame or depth", i);
+        lays[i] = layout_batch_sync(it.mesh, it.args, B.ts, cap, bblock);
+        if ((uint32_t)lays[i].ntiles > kListTileMask)
+            return fail(ctx, RZ_ERR_INVALID_ARG, "batch item %d: %d tiles, at most %u", i, lays[i].ntiles, kListTileMask);
+        stat_bytes += align256(lays[i].bytes[kBufPart]);
+        for (int b = 0; b < kNumBufs; ++b)
+            if (b != kBufPart) work_bytes += align256(lays[i].bytes[b]);
+        stride += (size_t)lays[i].ntiles;
+    }
+    const size_t list_bytes = align256(8 * stride * 4);
+    rz_status st = ensure(ctx, arena, stat_bytes + list_bytes + work_bytes);
+    if (st != RZ_OK) return st;
+    B.frames.assign(n, FrameLaunch());
+    B.frames1.assign(n, FrameLaunch());
+    B.faces.assign(n, 0);
+    B.arena_part_bytes = stat_bytes;
+    B.part_off = sync_bytes;
+    char* base = (char*)arena.ptr;
+    uint32_t* batch_part = (uint32_t*)(base + sync_bytes);
+    uint32_t* lists = (uint32_t*)(base + stat_bytes);
+    size_t so = batch_stat_bytes, wo = stat_bytes + list_bytes;
+    std::vector<uint2> bmap, omap;
+    for (int32_t i = 0; i < n; ++i) {
+        const rz_batch_item& it = items[i];
+        void* ptrs[kNumBufs];
+        for (int b = 0; b < kNumBufs; ++b) {
+            if (b == kBufPart) continue;
+            ptrs[b] = base + wo;
+            wo += align256(lays[i].bytes[b]);
+        }
+        ptrs[kBufPart] = base + so;
+        so += align256(lays[i].bytes[kBufPart]);
+        for (int g = 0; g < 2; ++g) {
+            FrameLaunch& L = g ? B.frames1[i] : B.frames[i];
+            fill_frame(it.mesh, it.lights, it.num_lights, it.args, flags, lays[i], ptrs, L, g);
+            L.sp.mlp = (const uint16_t*)ctx->mlp_packed.ptr;
+            L.sp.mode = B.mode;
+            bind_outputs(L, cleared, it.frame, it.depth, it.ids);
+            // the shared lists; their 8 counters, and the batch's overflow count, are the
+            // first mesh's (its bin work zeroes the other generation's for the next frame)
+            const FrameLaunch& L0 = g ? B.frames1[0] : B.frames[0];
+            L.bp.lists = lists;
+            L.bp.list_count = L0.bp.list_count;
+            L.bp.ntiles = (int32_t)stride;
+            L.bp.list_tag = (uint32_t)i << kListMeshShift;
+            L.bp.batch_ovf = L0.counters + kCntBatchOverflow;
+            L.rp.lists = L.orp.lists = lists;
+            L.rp.list_count = L.orp.list_count = L0.bp.list_count;
+            L.orp.part_frag = L.part + lays[i].off_ovf;
+        }
+        B.faces[i] = (uint64_t)it.mesh->nf;
+        const FrameLaunch& L = B.frames[i];
+        const uint32_t nb = bin_blocks(L.bp.nf, bblock) + (L.bp.zero_count + bblock - 1) / bblock + L.bp.clear_blocks;
+        const uint32_t no = overflow_blocks(L.rp.nf, bblock);
+        for (uint32_t b = 0; b < nb; ++b) bmap.push_back(make_uint2((uint32_t)i, b));
+        for (uint32_t b = 0; b < no; ++b) omap.push_back(make_uint2((uint32_t)i, b));
+    }
+    for (int g = 0; g < 2; ++g) {
+        BatchLists& bl = g ? B.bl1 : B.bl;
+        const FrameLaunch& L0 = g ? B.frames1[0] : B.frames[0];
+        bl.lists = lists;
+        bl.list_count = L0.bp.list_count;
+        bl.stride = (uint32_t)stride;
+        bl.overflow = L0.counters + kCntBatchOverflow;
+        bl.part_frag = batch_part;
+        bl.part_cov = batch_part + B.raster_grid;
+    }
+    // tables: bin, raster and overflow parameters of both generations, shade parameters,
+    // the two maps, the lights beyond the kernel arguments
+    size_t total = 0;
+    const size_t off_bp = total;
+    total = align256(total + 2 * sizeof(BinParams) * (size_t)n);
+    const size_t off_rp = total;
+    total = align256(total + 2 * sizeof(RasterParams) * (size_t)n);
+    const size_t off_orp = total;
+    total = align256(total + 2 * sizeof(RasterParams) * (size_t)n);
+    const size_t off_sp = total;
+    total = align256(total + sizeof(ShadeParams) * (size_t)n);
+    const size_t off_bmap = total;
+    total = align256(total + bmap.size() * sizeof(uint2));
+    const size_t off_omap = total;
+    total = align256(total + omap.size() * sizeof(uint2));
+    std::vector<size_t> ext_at(n, 0);
+    for (int32_t i = 0; i < n; ++i) {
+        ext_at[i] = total;
+        total += B.frames[i].ext_lights.size() * sizeof(float);
+    }
+    total = align256(total);
+    if ((st = ensure(ctx, tables, total)) != RZ_OK) return st;
+    std::vector<char> host(total, 0);
+    for (int32_t i = 0; i < n; ++i) {
+        FrameLaunch& L = B.frames[i];
+        L.sp.lights_ext = L.ext_lights.empty() ? nullptr : (const float*)((char*)tables.ptr + ext_at[i]);
+        if (!L.ext_lights.empty())
+            std::memcpy(host.data() + ext_at[i], L.ext_lights.data(), L.ext_lights.size() * sizeof(float));
+        for (int g = 0; g < 2; ++g) {
+            const FrameLaunch& Lg = g ? B.frames1[i] : B.frames[i];
+            const size_t k = (size_t)g * n + i;
+            std::memcpy(host.data() + off_bp + k * sizeof(BinParams), &Lg.bp, sizeof(BinParams));
+            std::memcpy(host.data() + off_rp + k * sizeof(RasterParams), &Lg.rp, sizeof(RasterParams));
+            std::memcpy(host.data() + off_orp + k * sizeof(RasterParams), &Lg.orp, sizeof(RasterParams));
+        }
+        std::memcpy(host.data() + off_sp + i * sizeof(ShadeParams), &L.sp, sizeof(ShadeParams));
+    }
+    if (!bmap.empty()) std::memcpy(host.data() + off_bmap, bmap.data(), bmap.size() * sizeof(uint2));
+    if (!omap.empty()) std::memcpy(host.data() + off_omap, omap.data(), omap.size() * sizeof(uint2));
+    // the table buffer and the arena may still be read by queued work of an earlier batch
+    RZ_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    RZ_HIP(ctx, hipMemcpy(tables.ptr, host.data(), total, hipMemcpyHostToDevice));
+    char* t = (char*)tables.ptr;
+    B.d_bp = (const BinParams*)(t + off_bp);
+    B.d_rp = (const RasterParams*)(t + off_rp);
+    B.d_orp = (const RasterParams*)(t + off_orp);
+    B.d_sp = (const ShadeParams*)(t + off_sp);
+    // the barriers start at zero, and so does generation 0 of every mesh's counter lines
+    // (each frame's bin work zeroes the other generation for the next)
+    // (on the context stream, which the launches follow; this is never inside a capture)
+    RZ_HIP(ctx, hipMemsetAsync(arena.ptr, 0, sync_bytes, ctx->stream));
+    for (int32_t i = 0; i < n; ++i)
+        RZ_HIP(ctx, hipMemsetAsync(B.frames[i].bp.tile_count, 0, (size_t)count_lines(lays[i].ntiles) * 4 * kCountStride,
+                                   ctx->stream));
+    RZ_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    BatchSync& q = B.q;
+    q.bar = (unsigned long long*)arena.ptr;
+    q.error = (uint32_t*)((unsigned long long*)arena.ptr + kSyncWords);
+    q.clock = nullptr;   // (set per frame when profiling)
+    q.G = B.raster_grid;
+    q.nB = (uint32_t)bmap.size();
+    q.nO = (uint32_t)omap.size();
+    q.bmap = (const uint2*)(t + off_bmap);
+    q.omap = (const uint2*)(t + off_omap);
+    q.nmesh = (uint32_t)n;
+    return RZ_OK;
+}
+
+rz_status prepare_any_batch(rz_ctx* ctx, const rz_batch_item* items, int32_t n, uint32_t flags, DevBuf& arena,
+                            DevBuf& tables, BatchLaunch& B) {
+    return ctx->batch_persistent ? prepare_batch_sync(ctx, items, n, flags, arena, tables, B)
+                                 : prepare_batch(ctx, items, n, flags, arena, tables, B);
 }
 
 rz_status enqueue_batch(rz_ctx* ctx, const BatchLaunch& B, hipStream_t s, bool prof) {
     if (B.persistent) {
         if (prof) RZ_HIP(ctx, launch_hold(kHoldTicks, s));
         if (prof) RZ_HIP(ctx, hipEventRecord(ctx->ev[0], s));
-        RZ_HIP(ctx, launch_batch_frame(B.d_vp, B.d_bp, B.d_rp, B.d_orp, B.d_sp, B.bl, B.q, B.ts, B.mode, s));
+        RZ_HIP(ctx, launch_batch_frame(B.d_bp, B.d_rp, B.d_orp, B.d_sp, B.bl, B.bl1, B.q, B.ts, B.mode, s));
         if (prof) RZ_HIP(ctx, hipEventRecord(ctx->ev[5], s));
         return RZ_OK;
     }
@@ -958,28 +1131,44 @@ rz_status capture(rz_ctx* ctx, int32_t frames, F enqueue, hipGraphExec_t* out) {
 rz_status batch_stats(rz_ctx* ctx, const BatchLaunch& B, const DevBuf& arena, rz_frame_stats* stats) {
     std::vector<char> host(B.arena_part_bytes);
     RZ_HIP(ctx, hipMemcpyAsync(host.data(), arena.ptr, host.size(), hipMemcpyDeviceToHost, ctx->stream));
-    std::vector<uint32_t> list_counts(8 * kCountStride), lists((size_t)8 * B.bl.stride);
-    RZ_HIP(ctx, hipMemcpyAsync(list_counts.data(), B.bl.list_count, list_counts.size() * 4, hipMemcpyDeviceToHost,
-                               ctx->stream));
-    RZ_HIP(ctx, hipMemcpyAsync(lists.data(), B.bl.lists, lists.size() * 4, hipMemcpyDeviceToHost, ctx->stream));
     RZ_HIP(ctx, hipStreamSynchronize(ctx->stream));
     std::memset(stats, 0, sizeof *stats);
-    if (B.persistent && ((const uint32_t*)host.data())[6] != 0u)   // BatchQueue::error (queue word 3)
-        return fail(ctx, RZ_ERR_HIP, "batch frame: a work-queue wait gave up");
+    // the persistent launch: the last frame's generation of counter lines, from the
+    // arrivals at its final barrier
+    int gen = 0;
+    if (B.persistent) {
+        const unsigned long long* sync = (const unsigned long long*)host.data();
+        if (((const uint32_t*)(sync + kSyncWords))[0] != 0u)   // BatchSync::error
+            return fail(ctx, RZ_ERR_HIP, "batch frame: a barrier wait gave up");
+        unsigned long long arrived = 0;
+        for (int x = 0; x < 8; ++x) arrived += sync[(size_t)(2 * 8 + x) * (kCountStride / 2)];
+        const unsigned long long frames = arrived / B.q.G;
+        gen = frames > 0 ? (int)((frames - 1) & 1ull) : 0;
+    }
+    const std::vector<FrameLaunch>& launches = gen ? B.frames1 : B.frames;
+    const BatchLists& bl = gen ? B.bl1 : B.bl;
+    std::vector<uint32_t> list_counts(8 * kCountStride), lists((size_t)8 * bl.stride);
+    RZ_HIP(ctx, hipMemcpyAsync(list_counts.data(), bl.list_count, list_counts.size() * 4, hipMemcpyDeviceToHost,
+                               ctx->stream));
+    RZ_HIP(ctx, hipMemcpyAsync(lists.data(), bl.lists, lists.size() * 4, hipMemcpyDeviceToHost, ctx->stream));
+    // each mesh's global counters (in the statistics region, or in its counter lines)
+    std::vector<unsigned long long> counters(launches.size() * kNumCounters);
+    for (size_t i = 0; i < launches.size(); ++i)
+        RZ_HIP(ctx, hipMemcpyAsync(counters.data() + i * kNumCounters, launches[i].counters, kNumCounters * 8,
+                                   hipMemcpyDeviceToHost, ctx->stream));
+    RZ_HIP(ctx, hipStreamSynchronize(ctx->stream));
     const uint32_t* bp = (const uint32_t*)(host.data() + B.part_off);   // the batch's raster and overflow-shade partials
     for (uint32_t i = 0; i < B.raster_grid; ++i) stats->fragments += bp[i];
     for (uint32_t i = 0; i < B.raster_grid + B.tiles_grid; ++i) stats->covered_pixels += bp[B.raster_grid + i];
-    for (size_t i = 0; i < B.frames.size(); ++i) {
-        const FrameLaunch& L = B.frames[i];
-        const size_t oc = (size_t)((char*)L.counters - (char*)arena.ptr);
+    for (size_t i = 0; i < launches.size(); ++i) {
+        const FrameLaunch& L = launches[i];
         const size_t op = (size_t)((char*)L.part - (char*)arena.ptr);
-        reduce_stats(L, (const unsigned long long*)(host.data() + oc), (const uint32_t*)(host.data() + op),
-                     B.faces[i], stats);
+        reduce_stats(L, counters.data() + i * kNumCounters, (const uint32_t*)(host.data() + op), B.faces[i], stats);
     }
     std::vector<FrameLayout> lays;
-    for (const FrameLaunch& L : B.frames) lays.push_back(L.lay);
+    for (const FrameLaunch& L : launches) lays.push_back(L.lay);
     std::vector<uint64_t> tiles(lays.size(), 0), pixels(lays.size(), 0);
-    count_listed(list_counts.data(), lists.data(), B.bl.stride, lays.data(), lays.size(), true, tiles.data(),
+    count_listed(list_counts.data(), lists.data(), bl.stride, lays.data(), lays.size(), true, tiles.data(),
                  pixels.data());
     for (size_t i = 0; i < lays.size(); ++i) {
         stats->raster_tiles += tiles[i];
@@ -1846,17 +2035,17 @@ void rz_graph_destroy(rz_graph* g) {
 rz_status rz_draw_batch(rz_ctx* ctx, const rz_batch_item* items, int32_t n, uint32_t flags, rz_frame_stats* stats) {
     if (!ctx) return RZ_ERR_INVALID_ARG;
     BatchLaunch B;
-    rz_status st = prepare_batch(ctx, items, n, flags, ctx->batch_arena, ctx->batch_tables, B);
+    rz_status st = prepare_any_batch(ctx, items, n, flags, ctx->batch_arena, ctx->batch_tables, B);
     if (st != RZ_OK) return st;
     const bool phase_clock = ctx->profiling && B.persistent;
     if (phase_clock) {
         // per phase: first start (min, from all ones) and last end (max, from zero)
-        unsigned long long init[10];
-        for (int k = 0; k < 5; ++k) {
+        unsigned long long init[8];
+        for (int k = 0; k < 4; ++k) {
             init[2 * k] = ~0ull;
             init[2 * k + 1] = 0ull;
         }
-        B.q.clock = B.q.ticket + 4;
+        B.q.clock = B.q.bar + kSyncWords + 1;
         RZ_HIP(ctx, hipMemcpy(B.q.clock, init, sizeof init, hipMemcpyHostToDevice));
     }
     if ((st = enqueue_batch(ctx, B, ctx->stream, ctx->profiling)) != RZ_OK) return st;
@@ -1864,12 +2053,13 @@ rz_status rz_draw_batch(rz_ctx* ctx, const rz_batch_item* items, int32_t n, uint
         if ((st = batch_stats(ctx, B, ctx->batch_arena, stats)) != RZ_OK) return st;
         if (ctx->profiling) {
             if (phase_clock) {
-                // the phases overlap: each is reported as the span of its items
-                unsigned long long clk[10];
+                // each phase as the span of its work (first start to last end: the phases
+                // of different workgroups overlap around the barriers)
+                unsigned long long clk[8];
                 RZ_HIP(ctx, hipMemcpy(clk, B.q.clock, sizeof clk, hipMemcpyDeviceToHost));
-                float* ms[5] = {&stats->ms_vertex, &stats->ms_bin, &stats->ms_raster, &stats->ms_overflow,
-                                &stats->ms_shade};
-                for (int k = 0; k < 5; ++k)
+                float* ms[4] = {&stats->ms_bin, &stats->ms_raster, &stats->ms_overflow, &stats->ms_shade};
+                stats->ms_vertex = 0.0f;   // (the bin work transforms each face's corners)
+                for (int k = 0; k < 4; ++k)
                     *ms[k] = clk[2 * k + 1] > clk[2 * k] ? (float)(clk[2 * k + 1] - clk[2 * k]) * 1e-5f : 0.0f;
                 RZ_HIP(ctx, hipEventElapsedTime(&stats->ms_frame, ctx->ev[0], ctx->ev[5]));
             } else if ((st = event_times(ctx, stats)) != RZ_OK) {
@@ -1881,7 +2071,7 @@ rz_status rz_draw_batch(rz_ctx* ctx, const rz_batch_item* items, int32_t n, uint
         if (B.persistent) {
             uint32_t err = 0;
             RZ_HIP(ctx, hipMemcpy(&err, B.q.error, 4, hipMemcpyDeviceToHost));
-            if (err) return fail(ctx, RZ_ERR_HIP, "batch frame: a work-queue wait gave up");
+            if (err) return fail(ctx, RZ_ERR_HIP, "batch frame: a barrier wait gave up");
         }
     }
     return RZ_OK;
@@ -1905,7 +2095,7 @@ rz_status rz_batch_graph_create(rz_ctx* ctx, const rz_batch_item* items, int32_t
     gr->frames = frames;
     gr->owns_buffers = true;
     BatchLaunch B;
-    rz_status st = prepare_batch(ctx, items, n, flags, gr->arena, gr->tables, B);
+    rz_status st = prepare_any_batch(ctx, items, n, flags, gr->arena, gr->tables, B);
     if (st == RZ_OK)
         st = capture(ctx, frames, [&](int32_t, hipStream_t s) { return enqueue_batch(ctx, B, s, false); },
                      &gr->exec);
@@ -1932,7 +2122,7 @@ rz_status rz_autotune(rz_ctx* ctx, const rz_batch_item* items, int32_t n, uint32
         ctx->tile_size = ts;
         ctx->bin_cap = 64;
         BatchLaunch B;
-        if ((st = prepare_batch(ctx, items, n, flags, ctx->batch_arena, ctx->batch_tables, B)) != RZ_OK) break;
+        if ((st = prepare_any_batch(ctx, items, n, flags, ctx->batch_arena, ctx->batch_tables, B)) != RZ_OK) break;
         if ((st = enqueue_batch(ctx, B, ctx->stream, false)) != RZ_OK) break;
         uint32_t need = 1;
         size_t tiles = 0;
@@ -1949,7 +2139,7 @@ rz_status rz_autotune(rz_ctx* ctx, const rz_batch_item* items, int32_t n, uint32
         while (cap < need && cap < (1u << 20)) cap <<= 1;
         while (cap > 16 && tiles * (size_t)cap * 4 > kBinBudget) cap >>= 1;   // overflow path beyond the budget
         ctx->bin_cap = (int)cap;
-        if ((st = prepare_batch(ctx, items, n, flags, ctx->batch_arena, ctx->batch_tables, B)) != RZ_OK) break;
+        if ((st = prepare_any_batch(ctx, items, n, flags, ctx->batch_arena, ctx->batch_tables, B)) != RZ_OK) break;
         for (int w = 0; w < 3 && st == RZ_OK; ++w) st = enqueue_batch(ctx, B, ctx->stream, false);
         if (st != RZ_OK) break;
         const int reps = 10;

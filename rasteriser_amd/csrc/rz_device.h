@@ -247,32 +247,30 @@ __host__ __device__ inline uint32_t overflow_blocks(int32_t nf, int block = 256)
     return b > most ? most : b;
 }
 
-// The persistent batch launch's work queue.  Every workgroup takes tickets in order until
-// it draws an exit ticket; a frame's tickets are, in order: the vertex items, the bin items
-// (binning and clearing workgroups of every mesh), G raster slots (each runs the raster
-// pass over the shared tile lists until they are exhausted), the overflow items, G
-// overflow-shade slots, and G exit tickets (G = workgroups of the launch), so each frame
-// takes exactly `per_frame` tickets and frame f's tickets are [f, f + 1) * per_frame.  An
-// item waits only on items of lower tickets (a bin item on its mesh's vertex items and on
-// the first mesh's, which clear the shared list counters; raster slots and overflow items
-// on every bin item; overflow-shade slots on every overflow item), which the running
-// workgroups that drew them finish without waiting on later ones: no wait can deadlock,
-// whatever the number of resident workgroups.  The counters only grow (a frame waits for
-// (f + 1) times its per-frame counts) and start at zero when the batch is prepared.
-struct BatchQueue {
-    unsigned long long* ticket;          // tickets drawn
-    unsigned long long* vdone;           // per mesh: vertex items finished
-    unsigned long long* done;            // [0] bin items, [1] overflow items finished
-    uint32_t* error;                     // nonzero: a wait gave up (a bug guard; never set by a correct queue)
-    const uint32_t* nvb;                 // per mesh: vertex items of one frame
-    const uint2* vmap;                   // vertex item -> (mesh, workgroup of the mesh's pass)
-    const uint2* bmap;                   // bin item -> (mesh, workgroup)
-    const uint2* omap;                   // overflow item -> (mesh, workgroup)
-    uint32_t nV, nB, nO, G;
-    unsigned long long* clock;           // null, or per phase (vertex, bin, raster, overflow, shade) the
-                                         // first item's start (min) and the last item's end (max), 100 MHz
-};
-__host__ __device__ inline uint32_t shade_blocks_x(uint32_t W) { return (W + 63) / 64; }
+// The persistent batch launch (k_batch_frame): G workgroups, all resident at once (G is
+// the launch's occupancy times the CUs), share each phase's work by static striding and
+// meet at three barriers.  A frame: phase 1, every mesh's bin work (each face's corners
+// transformed in place, as a single frame's bin pass does), the zeroing of the other
+// generation of counter lines (each mesh keeps two, as single frames do) and the buffer
+// clears; barrier 1; phase 2, raster and shade of the tiles of all meshes from per-list
+// cursors; phase 3, the overflow loop of overflowed tiles; barrier 2; phase 4, their
+// shade; barrier 3 (arrival only: the next frame's launch reads it).  A barrier is 8
+// counters, one per XCD (workgroup b counts on counter b % 8), each on its own 128-byte
+// line; they only grow: frame f's workgroups wait for (f + 1) * G / 8 on each, and learn
+// f from the third barrier's counters at the start (every workgroup of frame f - 1 has
+// arrived there, and none of frame f can before all of them passed barrier 1).  The
+// counters start at zero when the batch is prepared.
+struct BatchSync {
+    unsigned long long* bar;     // 3 barriers x 8 counters, kCountStride / 2 words apart
+    uint32_t* error;             // nonzero: a barrier wait gave up (a guard; never set when all G are resident)
+    uint32_t G;                  // workgroups of the launch (a multiple of 8)
+    uint32_t nB, nO;             // phase 1 items (bin, zero and clear workgroups of every mesh), overflow items
+    const uint2* bmap;           // phase 1 item -> (mesh, workgroup of the mesh's bin launch)
+    const uint2* omap;           // overflow item -> (mesh, workgroup)
+    uint32_t nmesh;              // meshes (the parameter tables hold generation 0, then generation 1)
+    unsigned long long* clock;   // null, or per phase (bin, raster, overflow, shade) the first start (min)
+                                 // and the last end (max) of its work, 100 MHz
+};__host__ __device__ inline uint32_t shade_blocks_x(uint32_t W) { return (W + 63) / 64; }
 __host__ __device__ inline uint32_t shade_blocks_y(uint32_t H) { return (H + 3) / 4; }
 
 // XCD-aware placement of a 2-D grid of work units.  Workgroups are dealt to the eight

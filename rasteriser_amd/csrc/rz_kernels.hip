@@ -26,6 +26,8 @@
 // The packed key makes the winner the lexicographic minimum of (depth, face index)
 // over covered fragments with depth below the initial depth: the same fragment the
 // reference's in-order strict `<` test (drawing.cpp:119, faces in file order) leaves.
+#include <algorithm>
+
 #include "rz_device.h"
 #include "rz_mlp.h"
 
@@ -1286,46 +1288,41 @@ __global__ __launch_bounds__(BLOCK) void k_shade_tiles_batch(const RasterParams*
 }
 
 // ---------------------------------------------------------------------------------
-// A batch frame as one persistent launch (work queue: BatchQueue in rz_device.h).
+// A batch frame as one persistent launch (BatchSync in rz_device.h).
 
-// Polls (thread 0) until *c reaches target; the workgroup waits at the barrier and then
-// acquires what the counted items wrote.  Gives up after kMaxSpins polls (about a second:
-// far beyond any batch frame) and flags q.error, after which every wait of the launch
-// gives up at once (a bug guard: the queue's order makes every wait end); returns
-// whether the count was reached.
+// Spins of a barrier wait before it gives up (about a second, far beyond a batch frame):
+// reached only if the launch's workgroups are not all resident at once.
 constexpr uint32_t kMaxSpins = 1u << 20;
-__device__ __forceinline__ bool wait_count(const unsigned long long* c, unsigned long long target, uint32_t* error) {
-    __shared__ int s_ok;
-    if (threadIdx.x == 0) {
-        int ok = 1;
-        uint32_t spins = 0;
-        while (__hip_atomic_load(const_cast<unsigned long long*>(c), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) <
-               target) {
-            if (++spins > kMaxSpins ||
-                __hip_atomic_load(error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u) {
-                atomicOr(error, 1u);
-                ok = 0;
-                break;
-            }
-            __builtin_amdgcn_s_sleep(2);
-        }
-        s_ok = ok;
-    }
-    __syncthreads();
-    const bool ok = s_ok != 0;
-    __threadfence();   // acquire: the finished items' writes, from any XCD
-    __syncthreads();   // (s_ok is rewritten by the next wait)
-    return ok;
-}
+constexpr uint32_t kBarStride = kCountStride / 2;   // 64-bit words between barrier counters
 
-// Releases this workgroup's writes at device scope and counts the item finished.
-__device__ __forceinline__ void finish_item(unsigned long long* c) {
+// Arrives at barrier b (after releasing this workgroup's writes at device scope) and,
+// with wait, waits until each of the barrier's 8 counters reaches `target`, then
+// acquires what the arrivals wrote.  A wait that gives up flags q.error, after which
+// every wait of the launch gives up at once.  Every thread calls it.
+__device__ __forceinline__ void batch_barrier(const BatchSync& q, int b, unsigned long long target, bool wait) {
     __threadfence();
     __syncthreads();
-    if (threadIdx.x == 0) atomicAdd(c, 1ull);
+    unsigned long long* bar = q.bar + (size_t)b * 8 * kBarStride;
+    if (threadIdx.x == 0) atomicAdd(bar + (blockIdx.x & 7u) * kBarStride, 1ull);
+    if (!wait) return;
+    if (threadIdx.x == 0) {
+        for (uint32_t x = 0; x < 8; ++x) {
+            uint32_t spins = 0;
+            while (__hip_atomic_load(bar + x * kBarStride, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
+                if (++spins > kMaxSpins || __hip_atomic_load(q.error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+                    atomicOr(q.error, 1u);
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(2);
+            }
+        }
+    }
+    __syncthreads();
+    __threadfence();   // acquire: the other workgroups' writes, from any XCD
+    __syncthreads();
 }
 
-__device__ __forceinline__ void phase_clock(const BatchQueue& q, int phase, bool start) {
+__device__ __forceinline__ void phase_clock(const BatchSync& q, int phase, bool start) {
     if (q.clock && threadIdx.x == 0) {
         const unsigned long long now = wall_clock64();
         if (start)
@@ -1341,107 +1338,130 @@ struct BatchFrameLds {
         BinLds<BLOCK>::kBytes > RasterLds<BLOCK, MODE>::kBytes ? BinLds<BLOCK>::kBytes : RasterLds<BLOCK, MODE>::kBytes;
 };
 
+// bitems / ritems / oitems: nmesh parameter records of generation 0, then of generation 1
+// (their counter lines differ); sitems: one per mesh.
 template <int TS, int BLOCK, int MODE>
 __global__ __launch_bounds__(BLOCK, MODE == kShadeMlp ? 3 : 4) void k_batch_frame(
-    const VertexParams* __restrict__ vitems, const BinParams* __restrict__ bitems,
-    const RasterParams* __restrict__ ritems, const RasterParams* __restrict__ oitems,
-    const ShadeParams* __restrict__ sitems, BatchLists bl, BatchQueue q) {
+    const BinParams* __restrict__ bitems, const RasterParams* __restrict__ ritems,
+    const RasterParams* __restrict__ oitems, const ShadeParams* __restrict__ sitems, BatchLists bl0,
+    BatchLists bl1, BatchSync q) {
     // one LDS region for the bin phase's tile aggregation and the raster phase's face data
     __shared__ __align__(16) unsigned char s_lds[BatchFrameLds<BLOCK, MODE>::kBytes];
     __shared__ __align__(16) uint16_t s_net[MODE == kShadeMlp ? kPackedBytes / 2 : 8];
-    __shared__ unsigned long long s_ticket;
+    __shared__ unsigned long long s_frame;
     __shared__ uint32_t s_take;
     if (MODE == kShadeMlp) stage_net(sitems[0].mlp, s_net, threadIdx.x, BLOCK);   // one network per batch
-    const unsigned long long per_frame =
-        (unsigned long long)q.nV + q.nB + q.G + q.nO + q.G + q.G;
-    uint32_t frags = 0, cov = 0, cov_ovf = 0;
-    for (;;) {
-        if (threadIdx.x == 0) s_ticket = atomicAdd(q.ticket, 1ull);
-        __syncthreads();
-        const unsigned long long t = s_ticket;
-        __syncthreads();   // (s_ticket is rewritten by the next draw)
-        const unsigned long long f1 = t / per_frame + 1;   // frames up to and including this one
-        uint32_t u = (uint32_t)(t % per_frame);
-        if (u < q.nV) {
-            phase_clock(q, 0, true);
-            const uint2 m = q.vmap[u];
-            vertex_body<BLOCK>(vitems[m.x], (int)m.y);
-            finish_item(q.vdone + m.x);
-            phase_clock(q, 0, false);
-        } else if ((u -= q.nV) < q.nB) {
-            const uint2 m = q.bmap[u];
-            // this mesh's vertices and counters, and the shared list counters (first mesh)
-            bool ok = wait_count(q.vdone + m.x, f1 * q.nvb[m.x], q.error);
-            ok = wait_count(q.vdone, f1 * q.nvb[0], q.error) && ok;
-            phase_clock(q, 1, true);
-            const BinParams& p = bitems[m.x];
-            const uint32_t nb = bin_blocks(p.nf, BLOCK);
-            if (ok) {
-                if (m.y < nb)
-                    bin_body<BLOCK>(p, (int)m.y, s_lds);
-                else
-                    clear_body<BLOCK>(p, m.y - nb, p.clear_blocks);
-            }
-            finish_item(q.done);
-            phase_clock(q, 1, false);
-        } else if ((u -= q.nB) < q.G) {
-            // a raster slot: tiles of the shared lists, one at a time from per-list cursors
-            // (first list u % 8, then the others), once every mesh is binned
-            if (wait_count(q.done, f1 * q.nB, q.error)) {
-                phase_clock(q, 2, true);
-                for (uint32_t j = 0; j < 8; ++j) {
-                    const uint32_t x = (u + j) & 7u;
-                    const uint32_t n = bl.list_count[x * kCountStride];
-                    uint32_t* cursor = bl.list_count + x * kCountStride + 1;
-                    for (;;) {
-                        if (threadIdx.x == 0) s_take = atomicAdd(cursor, 1u);
-                        __syncthreads();
-                        const uint32_t k = s_take;
-                        __syncthreads();   // (s_take is rewritten by the next take)
-                        if (k >= n) break;
-                        const uint32_t e = bl.lists[(size_t)x * bl.stride + k];
-                        const uint32_t m = e >> kListMeshShift;
-                        raster_body<TS, BLOCK, true, MODE, kShrinkPixelsBatch, true>(
-                            ritems[m], &sitems[m], sitems[m].luts, nullptr, s_net, (int)(e & kListTileMask), frags,
-                            cov, s_lds);
-                    }
-                }
-                phase_clock(q, 2, false);
-            }
-        } else if ((u -= q.G) < q.nO) {
-            const uint2 m = q.omap[u];
-            const bool ok = wait_count(q.done, f1 * q.nB, q.error);
-            phase_clock(q, 3, true);
-            if (ok) {
-                const RasterParams& p = oitems[m.x];
-                overflow_body<BLOCK>(p, (int)m.y, (int)overflow_blocks(p.nf, BLOCK));
-            }
-            finish_item(q.done + 1);
-            phase_clock(q, 3, false);
-        } else if ((u -= q.nO) < q.G) {
-            // an overflow-shade slot: the overflowed tiles of list u % 8, stride G / 8
-            if (wait_count(q.done + 1, f1 * q.nO, q.error) && *bl.overflow != 0ull) {
-                phase_clock(q, 4, true);
-                const uint32_t x = u & 7u, step = q.G >> 3;
-                const uint32_t n = bl.list_count[x * kCountStride];
-                for (uint32_t k = u >> 3; k < n; k += step) {
-                    const uint32_t e = bl.lists[(size_t)x * bl.stride + k];
-                    const uint32_t m = e >> kListMeshShift, tile = e & kListTileMask;
-                    const RasterParams& p = ritems[m];
-                    if (p.tile_count[(size_t)tile * kCountStride] > p.cap)
-                        shade_tile_body<TS, BLOCK, MODE>(p, sitems[m], sitems[m].luts, nullptr, s_net, (int)tile,
-                                                         cov_ovf);
-                }
-                phase_clock(q, 4, false);
-            }
-        } else {
-            break;   // this workgroup's exit ticket
-        }
+    // this frame's number: every workgroup of the previous frames has arrived at barrier 2
+    if (threadIdx.x == 0) {
+        unsigned long long arrived = 0;
+        for (uint32_t x = 0; x < 8; ++x)
+            arrived += __hip_atomic_load(q.bar + (size_t)(2 * 8 + x) * kBarStride, __ATOMIC_RELAXED,
+                                         __HIP_MEMORY_SCOPE_AGENT);
+        s_frame = arrived / q.G;
     }
     __syncthreads();
-    store_block_sum(frags, bl.part_frag + blockIdx.x, threadIdx.x, BLOCK);
-    store_block_sum(cov, bl.part_cov + blockIdx.x, threadIdx.x, BLOCK);
-    store_block_sum(cov_ovf, bl.part_cov + q.G + blockIdx.x, threadIdx.x, BLOCK);
+    const unsigned long long f = s_frame;
+    const unsigned long long target = (f + 1) * (q.G / 8u);
+    const uint32_t gen = (uint32_t)(f & 1ull);
+    const BatchLists& bl = gen ? bl1 : bl0;
+    const uint32_t nm = q.nmesh;
+    const uint32_t w = blockIdx.x;
+
+    // phase 1: binning (with the vertex transform), the other generation's zeroing, clears
+    phase_clock(q, 0, true);
+    for (uint32_t i = w; i < q.nB; i += q.G) {
+        const uint2 m = q.bmap[i];
+        const BinParams& p = bitems[gen * nm + m.x];
+        const uint32_t nb = bin_blocks(p.nf, BLOCK);
+        const uint32_t nz = (p.zero_count + BLOCK - 1) / BLOCK;
+        if (m.y < nb)
+            bin_body<BLOCK>(p, (int)m.y, s_lds);
+        else if (m.y < nb + nz)
+            zero_lines_body<BLOCK>(p, m.y - nb);
+        else
+            clear_body<BLOCK>(p, m.y - nb - nz, p.clear_blocks);
+        __syncthreads();   // (s_lds is reused by the next item)
+    }
+    phase_clock(q, 0, false);
+    batch_barrier(q, 0, target, true);
+
+    // phase 2: tiles of the shared lists, one at a time from per-list cursors (first list
+    // w % 8, then the others)
+    uint32_t frags = 0, cov = 0, cov_ovf = 0;
+    phase_clock(q, 1, true);
+    for (uint32_t j = 0; j < 8; ++j) {
+        const uint32_t x = (w + j) & 7u;
+        const uint32_t n = bl.list_count[x * kCountStride];
+        uint32_t* cursor = bl.list_count + x * kCountStride + 1;
+        for (;;) {
+            if (threadIdx.x == 0) s_take = atomicAdd(cursor, 1u);
+            __syncthreads();
+            const uint32_t k = s_take;
+            __syncthreads();   // (s_take is rewritten by the next take)
+            if (k >= n) break;
+            const uint32_t e = bl.lists[(size_t)x * bl.stride + k];
+            const uint32_t m = e >> kListMeshShift;
+            raster_body<TS, BLOCK, true, MODE, kShrinkPixelsBatch, true>(ritems[gen * nm + m], &sitems[m],
+                                                                          sitems[m].luts, nullptr, s_net,
+                                                                          (int)(e & kListTileMask), frags, cov, s_lds);
+        }
+    }
+    phase_clock(q, 1, false);
+
+    // phases 3 and 4, when a bin overflowed (uniform: the count is final after barrier 0)
+    const bool overflow = __builtin_amdgcn_readfirstlane(
+                              (uint32_t)(__hip_atomic_load(const_cast<unsigned long long*>(bl.overflow),
+                                                           __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0ull)) != 0u;
+    if (overflow) {
+        phase_clock(q, 2, true);
+        for (uint32_t i = w; i < q.nO; i += q.G) {
+            const uint2 m = q.omap[i];
+            const RasterParams& p = oitems[gen * nm + m.x];
+            overflow_body<BLOCK>(p, (int)m.y, (int)overflow_blocks(p.nf, BLOCK));
+        }
+        phase_clock(q, 2, false);
+    }
+    batch_barrier(q, 1, target, overflow);
+    if (overflow) {
+        phase_clock(q, 3, true);
+        const uint32_t x = w & 7u, step = q.G >> 3;
+        const uint32_t n = bl.list_count[x * kCountStride];
+        for (uint32_t k = w >> 3; k < n; k += step) {
+            const uint32_t e = bl.lists[(size_t)x * bl.stride + k];
+            const uint32_t m = e >> kListMeshShift, tile = e & kListTileMask;
+            const RasterParams& p = ritems[gen * nm + m];
+            if (p.tile_count[(size_t)tile * kCountStride] > p.cap)
+                shade_tile_body<TS, BLOCK, MODE>(p, sitems[m], sitems[m].luts, nullptr, s_net, (int)tile, cov_ovf);
+        }
+        phase_clock(q, 3, false);
+    }
+    __syncthreads();
+    store_block_sum(frags, bl.part_frag + w, threadIdx.x, BLOCK);
+    store_block_sum(cov, bl.part_cov + w, threadIdx.x, BLOCK);
+    store_block_sum(cov_ovf, bl.part_cov + q.G + w, threadIdx.x, BLOCK);
+    batch_barrier(q, 2, target, false);   // the next frame's launch counts these arrivals
+}
+
+// Workgroups of `f` (BLOCK threads) that a CU holds at once, for a launch whose
+// workgroups wait on each other: the runtime's occupancy, bounded by this count of the
+// kernel's own resources with margin: waves by VGPRs (512 per lane per SIMD, granules of
+// 8, at most 8 waves per SIMD) and workgroups by LDS within 7/8 of the CU's 160 KiB.
+// (A workgroup of 7,896 LDS bytes was counted 20 to a CU, 160 KiB once rounded to
+// 512-byte granules, and fewer were resident.)  A smaller grid is only slower; a grid
+// beyond residency leaves a barrier waiting on workgroups that cannot start.
+uint32_t resident_blocks_per_cu(const void* f, int block) {
+    int n = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, f, block, 0) != hipSuccess || n < 1) n = 1;
+    hipFuncAttributes at{};
+    if (hipFuncGetAttributes(&at, f) != hipSuccess) return 1;
+    const int vgprs = ((at.numRegs > 0 ? at.numRegs : 512) + 7) / 8 * 8;
+    const int waves_per_simd = std::min(8, 512 / vgprs);
+    const int waves_per_block = (block + 63) / 64;
+    const int by_waves = 4 * waves_per_simd / waves_per_block;
+    const size_t lds = ((size_t)at.sharedSizeBytes + 511) / 512 * 512;
+    const int by_lds = lds > 0 ? (int)((size_t)163840 * 7 / 8 / lds) : by_waves;
+    n = std::min(n, std::min(by_waves, by_lds));
+    return (uint32_t)(n < 1 ? 1 : n);
 }
 
 // Resident workgroups per CU of the persistent batch launch.
@@ -1458,16 +1478,14 @@ uint32_t batch_frame_blocks_per_cu(int tile_size, int mode) {
         default: RZ_PICK(32, 256) break;
     }
 #undef RZ_PICK
-    int n = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, f, block, 0) != hipSuccess || n < 1) n = 1;
-    return (uint32_t)n;
+    return resident_blocks_per_cu(f, block);
 }
 
-hipError_t launch_batch_frame(const VertexParams* v, const BinParams* b, const RasterParams* r,
-                              const RasterParams* o, const ShadeParams* sh, const BatchLists& bl,
-                              const BatchQueue& q, int tile_size, int mode, hipStream_t s) {
+hipError_t launch_batch_frame(const BinParams* b, const RasterParams* r, const RasterParams* o,
+                              const ShadeParams* sh, const BatchLists& bl0, const BatchLists& bl1,
+                              const BatchSync& q, int tile_size, int mode, hipStream_t s) {
 #define RZ_LAUNCH(TS, B, M) \
-    hipLaunchKernelGGL((k_batch_frame<TS, B, M>), dim3(q.G), dim3(B), 0, s, v, b, r, o, sh, bl, q)
+    hipLaunchKernelGGL((k_batch_frame<TS, B, M>), dim3(q.G), dim3(B), 0, s, b, r, o, sh, bl0, bl1, q)
     const bool net = mode == kShadeMlp;
     if (mode != kShadePlain && mode != kShadeMlp) return hipErrorInvalidValue;
     switch (tile_size) {
