@@ -667,15 +667,21 @@ __device__ __forceinline__ bool fragment_key(float pxf, float pyf, float x0, flo
     return true;
 }
 
-// Candidate rectangles larger than this are shrunk first (shrink_rect): single frames,
-// and batches (whose triangle soups make large rectangles the common case).
-constexpr int kShrinkPixels = 128;
-constexpr int kShrinkPixelsBatch = 32;
+// Candidate rectangles larger than this are shrunk first (shrink_rect).  A wave runs the
+// shrink when any of its lanes needs it, so the threshold trades that cost against the
+// candidates it removes: 32 pixels for single frames rastered by 256-thread workgroups,
+// 128 for frames of 16,384+ tiles (64-thread workgroups; their faces are small), and 8
+// for batches, whose triangle soups make large rectangles the common case (the 256-mesh
+// batch: 778 M candidates for 360 M fragments before the shrink's margins were
+// tightened, 623 M after, 567 M with this threshold).
+template <int BLOCK>
+constexpr int shrink_pixels() { return BLOCK == 64 ? 128 : 32; }
+constexpr int kShrinkPixelsBatch = 8;
 
 // Range of T ∩ {lo <= v <= hi} along the other axis (u), T the triangle (u_k, v_k):
 // the vertices inside the slab and the edges' crossings of its two lines.  Returns false
 // when the slab misses T.  The crossings use the hardware reciprocal; callers widen the
-// range by a pixel for that.
+// range by a bound on that error.
 __device__ __forceinline__ bool slab_range(float u0, float v0, float u1, float v1, float u2, float v2, float lo,
                                            float hi, float& umin, float& umax) {
     umin = INFINITY; umax = -INFINITY;
@@ -703,9 +709,14 @@ __device__ __forceinline__ bool slab_range(float u0, float v0, float u1, float v
 // inradius, >= |area| / 2 / perimeter): the offset triangle is the triangle scaled by
 // 1 + m / r about its incentre, and no vertex is farther than 2 sqrt(2) D from it.
 // (Lengths are bounded by |dx| + |dy|; the products' and the subtraction's roundings
-// give 4 u (|A| |py - ay| + |B| |px - ax|) <= 5.7 u L (D + 1).)  The rectangle becomes the bounding box of the
-// triangle's part within g of the rectangle, widened by g + 1 pixel (crossing rounding).
-// Faces where g would exceed 4 pixels keep their rectangle.
+// give 4 u (|A| |py - ay| + |B| |px - ax|) <= 5.7 u L (D + 1).)  So a passing pixel p has
+// a triangle point t with |p - t| <= g: t lies in the slab of rows [Y0 - g, Y1 + g], and
+// px is within g of that slab's part of the triangle.  The slabs are widened to
+// w = g + d and the x (y) range taken as [ceil(min - w), floor(max + w)], where
+// d = 2^-19 (C + 16), C the largest coordinate magnitude, exceeds the error of a computed
+// edge crossing (pu + (lo - pv) (qu - pu) rcp(qv - pv): |error| <= 15.2 u C, the
+// hardware reciprocal within 1 ulp) plus the roundings of the slab bounds and of
+// min - w, max + w.  Faces where g would exceed 4 pixels keep their rectangle.
 __device__ __forceinline__ bool shrink_rect(const float4& a, const float4& b, const float4& c, int ox, int oy,
                                             int& rx0, int& rx1, int& ry0, int& ry1) {
     const float X0 = (float)(ox + rx0), X1 = (float)(ox + rx1), Y0 = (float)(oy + ry0), Y1 = (float)(oy + ry1);
@@ -722,12 +733,14 @@ __device__ __forceinline__ bool shrink_rect(const float4& a, const float4& b, co
     const float r = 0.5f * area / ((l0 + l1 + l2) * 1.01f);
     const float g = m * (1.0f + 3.0f * D / r) * 1.01f;
     if (!(g <= 4.0f)) return true;
+    const float C = fmaxf(fmaxf(fmaxf(fabsf(a.x), fabsf(a.y)), fmaxf(fabsf(b.x), fabsf(b.y))),
+                          fmaxf(fmaxf(fabsf(c.x), fabsf(c.y)), fmaxf(X1, Y1)));
+    const float w = g + 0x1p-19f * (C + 16.0f);
     float xmin, xmax, ymin, ymax;
-    if (!slab_range(a.x, a.y, b.x, b.y, c.x, c.y, Y0 - g, Y1 + g, xmin, xmax)) return false;
-    if (!slab_range(a.y, a.x, b.y, b.x, c.y, c.x, X0 - g, X1 + g, ymin, ymax)) return false;
-    const float w = g + 1.0f;
-    rx0 = max(rx0, (int)floorf(xmin - w) - ox); rx1 = min(rx1, (int)ceilf(xmax + w) - ox);
-    ry0 = max(ry0, (int)floorf(ymin - w) - oy); ry1 = min(ry1, (int)ceilf(ymax + w) - oy);
+    if (!slab_range(a.x, a.y, b.x, b.y, c.x, c.y, Y0 - w, Y1 + w, xmin, xmax)) return false;
+    if (!slab_range(a.y, a.x, b.y, b.x, c.y, c.x, X0 - w, X1 + w, ymin, ymax)) return false;
+    rx0 = max(rx0, (int)ceilf(xmin - w) - ox); rx1 = min(rx1, (int)floorf(xmax + w) - ox);
+    ry0 = max(ry0, (int)ceilf(ymin - w) - oy); ry1 = min(ry1, (int)floorf(ymax + w) - oy);
     return rx0 <= rx1 && ry0 <= ry1;
 }
 
@@ -753,7 +766,7 @@ struct RasterLds {
 };
 
 // EXT_RAW: the scratch LDS (RasterLds bytes) is the caller's `raw`, else the function's own.
-template <int TS, int BLOCK, bool FUSED, int MODE, int SHRINK = kShrinkPixels, bool EXT_RAW = false>
+template <int TS, int BLOCK, bool FUSED, int MODE, int SHRINK = shrink_pixels<BLOCK>(), bool EXT_RAW = false>
 __device__ __forceinline__ void raster_body(const RasterParams& p, const ShadeParams* sp, const float* s_lut,
                                             const DevMaterial* s_mats, const uint16_t* s_net, int tile,
                                             uint32_t& frags, uint32_t& cov, unsigned char* raw = nullptr) {
