@@ -317,7 +317,7 @@ FrameLayout layout_frame(const rz_mesh* mesh, const rz_args& a, int ts, int cap,
     L.bytes[kBufRec] = (size_t)mesh->nf * sizeof(FaceRecord);
     L.bytes[kBufTiles] = batch ? ((size_t)L.ntiles + 8) * 4 * kCountStride
                                : (size_t)2 * count_lines(L.ntiles) * 4 * kCountStride;
-    L.bytes[kBufLists] = batch ? 0 : (size_t)8 * L.ntiles * 4;
+    L.bytes[kBufLists] = batch ? 0 : (size_t)9 * L.ntiles * 4;   // 8 lists, then the overflowed tiles
     L.bytes[kBufBins] = (size_t)L.ntiles * L.cap * 4;
     L.bytes[kBufVis] = L.np * 8;
     L.bytes[kBufCounters] = batch ? kNumCounters * 8 : 0;
@@ -444,6 +444,7 @@ void fill_frame(const rz_mesh* mesh, rz_light* lights, int32_t nl, const rz_args
     bp.wind_cw = a.wind_clockwise != 0;
     bp.ntiles = lay.ntiles;
     bp.lists = (uint32_t*)buf[kBufLists];   // (batches: replaced by the batch's lists)
+    bp.ovf_list = bp.lists ? bp.lists + (size_t)8 * lay.ntiles : nullptr;
     bp.list_count = bp.tile_count + (size_t)lay.ntiles * kCountStride;
     bp.ovf_vis = (unsigned long long*)buf[kBufVis];
     if (lay.twogen) {
@@ -472,6 +473,7 @@ void fill_frame(const rz_mesh* mesh, rz_light* lights, int32_t nl, const rz_args
     rp.tiles_x = lay.tiles_x; rp.ntiles = lay.ntiles; rp.ts_log2 = lay.tsl; rp.cap = lay.cap;
     rp.lists = bp.lists;
     rp.list_count = bp.list_count;
+    rp.ovf_list = bp.ovf_list;
     rp.max_blocks = lay.raster_grid;
     L.orp = rp;   // batches: k_overflow_batch
     L.orp.part_frag = part + lay.off_ovf;
@@ -796,7 +798,7 @@ rz_status prepare_batch(rz_ctx* ctx, const rz_batch_item* items, int32_t n, uint
         for (int b = 0; b < kBufCounters; ++b) work_bytes += align256(lays[i].bytes[b]);
         stride += (size_t)lays[i].ntiles;
     }
-    const size_t list_bytes = align256(8 * stride * 4);
+    const size_t list_bytes = align256(9 * stride * 4);   // 8 lists, then the overflowed tiles
     rz_status st = ensure(ctx, arena, stat_bytes + list_bytes + work_bytes);
     if (st != RZ_OK) return st;
     B.frames.assign(n, FrameLaunch());
@@ -830,8 +832,10 @@ rz_status prepare_batch(rz_ctx* ctx, const rz_batch_item* items, int32_t n, uint
         L.bp.ntiles = (int32_t)stride;
         L.bp.list_tag = (uint32_t)i << kListMeshShift;
         L.bp.batch_ovf = L0.counters + kCntBatchOverflow;
+        L.bp.ovf_list = lists + (size_t)8 * stride;
         L.rp.lists = L.orp.lists = lists;
         L.rp.list_count = L.orp.list_count = L0.bp.list_count;
+        L.rp.ovf_list = L.orp.ovf_list = L.bp.ovf_list;
         B.faces[i] = (uint64_t)it.mesh->nf;
         const uint32_t nb[3] = {vertex_blocks(L.vp.nv, L.vp.nn, L.vp.ntiles, bblock),
                                 bin_blocks(L.bp.nf, bblock) + L.bp.clear_blocks, overflow_blocks(L.rp.nf, bblock)};
@@ -842,6 +846,7 @@ rz_status prepare_batch(rz_ctx* ctx, const rz_batch_item* items, int32_t n, uint
     B.bl.list_count = B.frames[0].bp.list_count;
     B.bl.stride = (uint32_t)stride;
     B.bl.overflow = B.frames[0].counters + kCntBatchOverflow;
+    B.bl.ovf_list = lists + (size_t)8 * stride;
     B.bl.part_frag = batch_part;
     B.bl.part_cov = batch_part + B.raster_grid;
     // tables: the four parameter arrays and the three block maps
@@ -938,7 +943,7 @@ rz_status prepare_batch_sync(rz_ctx* ctx, const rz_batch_item* items, int32_t n,
             if (b != kBufPart) work_bytes += align256(lays[i].bytes[b]);
         stride += (size_t)lays[i].ntiles;
     }
-    const size_t list_bytes = align256(8 * stride * 4);
+    const size_t list_bytes = align256(9 * stride * 4);   // 8 lists, then the overflowed tiles
     rz_status st = ensure(ctx, arena, stat_bytes + list_bytes + work_bytes);
     if (st != RZ_OK) return st;
     B.frames.assign(n, FrameLaunch());
@@ -975,8 +980,10 @@ rz_status prepare_batch_sync(rz_ctx* ctx, const rz_batch_item* items, int32_t n,
             L.bp.ntiles = (int32_t)stride;
             L.bp.list_tag = (uint32_t)i << kListMeshShift;
             L.bp.batch_ovf = L0.counters + kCntBatchOverflow;
+            L.bp.ovf_list = lists + (size_t)8 * stride;   // (one frame at a time: shared by both)
             L.rp.lists = L.orp.lists = lists;
             L.rp.list_count = L.orp.list_count = L0.bp.list_count;
+            L.rp.ovf_list = L.orp.ovf_list = L.bp.ovf_list;
             L.orp.part_frag = L.part + lays[i].off_ovf;
         }
         B.faces[i] = (uint64_t)it.mesh->nf;
@@ -993,6 +1000,7 @@ rz_status prepare_batch_sync(rz_ctx* ctx, const rz_batch_item* items, int32_t n,
         bl.list_count = L0.bp.list_count;
         bl.stride = (uint32_t)stride;
         bl.overflow = L0.counters + kCntBatchOverflow;
+        bl.ovf_list = lists + (size_t)8 * stride;
         bl.part_frag = batch_part;
         bl.part_cov = batch_part + B.raster_grid;
     }

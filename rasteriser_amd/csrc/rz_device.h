@@ -132,6 +132,9 @@ struct BinParams {
     // tile (the raster pass skips it and the overflow loop fills them)
     unsigned long long* ovf_vis;
     unsigned long long* batch_ovf;   // batches: also counts the overflowed tile here
+    // the overflowed tiles (list_tag | tile), each at the index its count returned (the
+    // batch's count for batches): the overflow shade visits these, not every listed tile
+    uint32_t* ovf_list;
     // extra workgroups of the bin launch that clear buffers (clear_np = 0: none): the
     // visibility words (separate-shade pipeline) and, for the fused pipeline on cleared
     // buffers, frame 0, depth 1.0, ids -1
@@ -167,6 +170,7 @@ struct RasterParams {
     int32_t pre_cleared;        // outputs cleared by the bin launch
     const uint32_t* lists;      // per-XCD lists of tiles with faces (see BinParams)
     const uint32_t* list_count;
+    const uint32_t* ovf_list;   // overflowed tiles, counters[kCntOverflowTiles] of them
 };
 
 // A batch's raster and overflow-shade launches: the shared tile lists and the launches'
@@ -176,6 +180,7 @@ struct BatchLists {
     uint32_t* list_count;       // 8 counters, kCountStride apart (second word: work cursor)
     uint32_t stride;
     const unsigned long long* overflow;   // tiles overflowed over the whole batch
+    const uint32_t* ovf_list;   // those tiles, (mesh << kListMeshShift) | tile
     uint32_t* part_frag;        // per raster workgroup: fragments passing the inside test
     uint32_t* part_cov;         // per raster workgroup, then per overflow-shade workgroup: covered pixels
 };

@@ -134,8 +134,9 @@ __device__ __forceinline__ void list_tile(const BinParams& p, uint32_t tile) {
 
 // Called once per tile, by the push that takes its bin past capacity.
 __device__ __forceinline__ void tile_overflowed(const BinParams& p, uint32_t tile) {
-    atomicAdd(&p.counters[kCntOverflowTiles], 1ull);
-    if (p.batch_ovf) atomicAdd(p.batch_ovf, 1ull);
+    const unsigned long long k = atomicAdd(&p.counters[kCntOverflowTiles], 1ull);
+    const unsigned long long kb = p.batch_ovf ? atomicAdd(p.batch_ovf, 1ull) : k;
+    if (p.ovf_list) p.ovf_list[kb] = p.list_tag | tile;   // (a tile overflows once: kb < tiles)
     // the raster kernel's overflow loop merges this tile's fragments into these words
     const uint32_t ts = 1u << p.ts_log2;
     const uint32_t ox = (tile % (uint32_t)p.tiles_x) * ts, oy = (tile / (uint32_t)p.tiles_x) * ts;
@@ -1194,13 +1195,10 @@ __global__ __launch_bounds__(BLOCK) void k_shade_tiles(RasterParams p, ShadePara
         stage_mats(sp, s_mats, threadIdx.x, BLOCK);
         if (MODE == kShadeMlp) stage_net(sp.mlp, s_net, threadIdx.x, BLOCK);
         __syncthreads();
-        const uint32_t xcd = blockIdx.x & 7u, step = gridDim.x >> 3;
-        const uint32_t n = p.list_count[xcd * kCountStride];
-        for (uint32_t k = blockIdx.x >> 3; k < n; k += step) {
-            const int tile = (int)p.lists[xcd * p.ntiles + k];
-            if (p.tile_count[(size_t)tile * kCountStride] > p.cap)   // the others were shaded by the raster pass
-                shade_tile_body<TS, BLOCK, MODE>(p, sp, s_lut, s_mats, s_net, tile, cov);
-        }
+        // (the raster pass shaded every other tile)
+        const uint32_t n = (uint32_t)p.counters[kCntOverflowTiles];
+        for (uint32_t k = blockIdx.x; k < n; k += gridDim.x)
+            shade_tile_body<TS, BLOCK, MODE>(p, sp, s_lut, s_mats, s_net, (int)(p.ovf_list[k] & kListTileMask), cov);
     }
     store_block_sum(cov, sp.part_cov + blockIdx.x, threadIdx.x, BLOCK);
 }
@@ -1287,14 +1285,12 @@ __global__ __launch_bounds__(BLOCK) void k_shade_tiles_batch(const RasterParams*
         __shared__ __align__(16) uint16_t s_net[MODE == kShadeMlp ? kPackedBytes / 2 : 8];
         if (MODE == kShadeMlp) stage_net(sitems[0].mlp, s_net, threadIdx.x, BLOCK);
         __syncthreads();
-        const uint32_t xcd = blockIdx.x & 7u, step = gridDim.x >> 3;
-        const uint32_t n = bl.list_count[xcd * kCountStride];
-        for (uint32_t k = blockIdx.x >> 3; k < n; k += step) {
-            const uint32_t e = bl.lists[(size_t)xcd * bl.stride + k];
-            const uint32_t m = e >> kListMeshShift, tile = e & kListTileMask;
-            const RasterParams& p = items[m];
-            if (p.tile_count[(size_t)tile * kCountStride] > p.cap)
-                shade_tile_body<TS, BLOCK, MODE>(p, sitems[m], sitems[m].luts, nullptr, s_net, (int)tile, cov);
+        const uint32_t n = (uint32_t)*bl.overflow;
+        for (uint32_t k = blockIdx.x; k < n; k += gridDim.x) {
+            const uint32_t e = bl.ovf_list[k];
+            const uint32_t m = e >> kListMeshShift;
+            shade_tile_body<TS, BLOCK, MODE>(items[m], sitems[m], sitems[m].luts, nullptr, s_net,
+                                             (int)(e & kListTileMask), cov);
         }
     }
     store_block_sum(cov, part_cov + blockIdx.x, threadIdx.x, BLOCK);
@@ -1437,14 +1433,13 @@ __global__ __launch_bounds__(BLOCK, MODE == kShadeMlp ? 3 : 4) void k_batch_fram
     batch_barrier(q, 1, target, overflow);
     if (overflow) {
         phase_clock(q, 3, true);
-        const uint32_t x = w & 7u, step = q.G >> 3;
-        const uint32_t n = bl.list_count[x * kCountStride];
-        for (uint32_t k = w >> 3; k < n; k += step) {
-            const uint32_t e = bl.lists[(size_t)x * bl.stride + k];
-            const uint32_t m = e >> kListMeshShift, tile = e & kListTileMask;
-            const RasterParams& p = ritems[gen * nm + m];
-            if (p.tile_count[(size_t)tile * kCountStride] > p.cap)
-                shade_tile_body<TS, BLOCK, MODE>(p, sitems[m], sitems[m].luts, nullptr, s_net, (int)tile, cov_ovf);
+        const uint32_t n = (uint32_t)__hip_atomic_load(const_cast<unsigned long long*>(bl.overflow),
+                                                       __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        for (uint32_t k = w; k < n; k += q.G) {
+            const uint32_t e = bl.ovf_list[k];
+            const uint32_t m = e >> kListMeshShift;
+            shade_tile_body<TS, BLOCK, MODE>(ritems[gen * nm + m], sitems[m], sitems[m].luts, nullptr, s_net,
+                                             (int)(e & kListTileMask), cov_ovf);
         }
         phase_clock(q, 3, false);
     }
