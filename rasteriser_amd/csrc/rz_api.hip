@@ -699,7 +699,9 @@ void reduce_stats(const FrameLaunch& L, const unsigned long long* counters, cons
     st->front_triangles += sum(y.off_front, y.off_pairs);
     st->bin_pairs += sum(y.off_pairs, y.off_frag);
     st->overflow_tiles += counters[kCntOverflowTiles];
-    st->fragments += sum(y.off_frag, y.off_cov);
+    // (batch meshes: that range holds only the overflow loop's partials, which the
+    // one-launch form leaves unwritten for a mesh without an overflowed tile)
+    if (!y.batch || counters[kCntOverflowTiles] != 0ull) st->fragments += sum(y.off_frag, y.off_cov);
     st->covered_pixels += sum(y.off_cov, y.n_part);
 }
 
@@ -955,7 +957,7 @@ rz_status prepare_batch_sync(rz_ctx* ctx, const rz_batch_item* items, int32_t n,
     uint32_t* batch_part = (uint32_t*)(base + sync_bytes);
     uint32_t* lists = (uint32_t*)(base + stat_bytes);
     size_t so = batch_stat_bytes, wo = stat_bytes + list_bytes;
-    std::vector<uint2> bmap, omap;
+    std::vector<uint2> bmap;
     for (int32_t i = 0; i < n; ++i) {
         const rz_batch_item& it = items[i];
         void* ptrs[kNumBufs];
@@ -989,9 +991,7 @@ rz_status prepare_batch_sync(rz_ctx* ctx, const rz_batch_item* items, int32_t n,
         B.faces[i] = (uint64_t)it.mesh->nf;
         const FrameLaunch& L = B.frames[i];
         const uint32_t nb = bin_blocks(L.bp.nf, bblock) + (L.bp.zero_count + bblock - 1) / bblock + L.bp.clear_blocks;
-        const uint32_t no = overflow_blocks(L.rp.nf, bblock);
         for (uint32_t b = 0; b < nb; ++b) bmap.push_back(make_uint2((uint32_t)i, b));
-        for (uint32_t b = 0; b < no; ++b) omap.push_back(make_uint2((uint32_t)i, b));
     }
     for (int g = 0; g < 2; ++g) {
         BatchLists& bl = g ? B.bl1 : B.bl;
@@ -1005,7 +1005,7 @@ rz_status prepare_batch_sync(rz_ctx* ctx, const rz_batch_item* items, int32_t n,
         bl.part_cov = batch_part + B.raster_grid;
     }
     // tables: bin, raster and overflow parameters of both generations, shade parameters,
-    // the two maps, the lights beyond the kernel arguments
+    // the bin items' map, the lights beyond the kernel arguments
     size_t total = 0;
     const size_t off_bp = total;
     total = align256(total + 2 * sizeof(BinParams) * (size_t)n);
@@ -1017,8 +1017,6 @@ rz_status prepare_batch_sync(rz_ctx* ctx, const rz_batch_item* items, int32_t n,
     total = align256(total + sizeof(ShadeParams) * (size_t)n);
     const size_t off_bmap = total;
     total = align256(total + bmap.size() * sizeof(uint2));
-    const size_t off_omap = total;
-    total = align256(total + omap.size() * sizeof(uint2));
     std::vector<size_t> ext_at(n, 0);
     for (int32_t i = 0; i < n; ++i) {
         ext_at[i] = total;
@@ -1042,7 +1040,6 @@ rz_status prepare_batch_sync(rz_ctx* ctx, const rz_batch_item* items, int32_t n,
         std::memcpy(host.data() + off_sp + i * sizeof(ShadeParams), &L.sp, sizeof(ShadeParams));
     }
     if (!bmap.empty()) std::memcpy(host.data() + off_bmap, bmap.data(), bmap.size() * sizeof(uint2));
-    if (!omap.empty()) std::memcpy(host.data() + off_omap, omap.data(), omap.size() * sizeof(uint2));
     // the table buffer and the arena may still be read by queued work of an earlier batch
     RZ_HIP(ctx, hipStreamSynchronize(ctx->stream));
     RZ_HIP(ctx, hipMemcpy(tables.ptr, host.data(), total, hipMemcpyHostToDevice));
@@ -1065,9 +1062,7 @@ rz_status prepare_batch_sync(rz_ctx* ctx, const rz_batch_item* items, int32_t n,
     q.clock = nullptr;   // (set per frame when profiling)
     q.G = B.raster_grid;
     q.nB = (uint32_t)bmap.size();
-    q.nO = (uint32_t)omap.size();
     q.bmap = (const uint2*)(t + off_bmap);
-    q.omap = (const uint2*)(t + off_omap);
     q.nmesh = (uint32_t)n;
     return RZ_OK;
 }

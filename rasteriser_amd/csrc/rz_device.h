@@ -269,9 +269,8 @@ struct BatchSync {
     unsigned long long* bar;     // 3 barriers x 8 counters, kCountStride / 2 words apart
     uint32_t* error;             // nonzero: a barrier wait gave up (a guard; never set when all G are resident)
     uint32_t G;                  // workgroups of the launch (a multiple of 8)
-    uint32_t nB, nO;             // phase 1 items (bin, zero and clear workgroups of every mesh), overflow items
+    uint32_t nB;                 // phase 1 items (bin, zero and clear workgroups of every mesh)
     const uint2* bmap;           // phase 1 item -> (mesh, workgroup of the mesh's bin launch)
-    const uint2* omap;           // overflow item -> (mesh, workgroup)
     uint32_t nmesh;              // meshes (the parameter tables hold generation 0, then generation 1)
     unsigned long long* clock;   // null, or per phase (bin, raster, overflow, shade) the first start (min)
                                  // and the last end (max) of its work, 100 MHz

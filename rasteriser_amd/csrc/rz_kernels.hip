@@ -1422,11 +1422,24 @@ __global__ __launch_bounds__(BLOCK, MODE == kShadeMlp ? 3 : 4) void k_batch_fram
                               (uint32_t)(__hip_atomic_load(const_cast<unsigned long long*>(bl.overflow),
                                                            __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0ull)) != 0u;
     if (overflow) {
+        // only the meshes with an overflowed tile (their bits in LDS, read once): the
+        // others' overflow statistics are left unwritten and not summed
         phase_clock(q, 2, true);
-        for (uint32_t i = w; i < q.nO; i += q.G) {
-            const uint2 m = q.omap[i];
-            const RasterParams& p = oitems[gen * nm + m.x];
-            overflow_body<BLOCK>(p, (int)m.y, (int)overflow_blocks(p.nf, BLOCK));
+        __shared__ uint32_t s_ovf_mesh[(kMaxBatchMeshes + 31) / 32];
+        for (uint32_t k = threadIdx.x; k < (nm + 31) / 32; k += BLOCK) s_ovf_mesh[k] = 0u;
+        __syncthreads();
+        for (uint32_t k = threadIdx.x; k < nm; k += BLOCK)
+            if (oitems[gen * nm + k].counters[kCntOverflowTiles] != 0ull) atomicOr(&s_ovf_mesh[k >> 5], 1u << (k & 31u));
+        __syncthreads();
+        for (uint32_t wd = 0; wd < (nm + 31) / 32; ++wd) {
+            uint32_t bits = s_ovf_mesh[wd];   // (uniform)
+            while (bits) {
+                const uint32_t k = wd * 32u + (uint32_t)(__ffs(bits) - 1);
+                bits &= bits - 1u;
+                const RasterParams& p = oitems[gen * nm + k];
+                const uint32_t nblk = overflow_blocks(p.nf, BLOCK);
+                for (uint32_t b = w; b < nblk; b += q.G) overflow_body<BLOCK>(p, (int)b, (int)nblk);
+            }
         }
         phase_clock(q, 2, false);
     }
