@@ -311,19 +311,22 @@ typedef struct rz_batch_item {
 } rz_batch_item;
 
 /* No reference counterpart (the reference draws one mesh per call).  Draws every
- * item as rz_draw_frame would (bit for bit), each pass of every item in one launch (see
- * rz_set_batch_persistent: all passes in one persistent launch, a work queue of every
- * item's vertex transform, binning and buffer clears, then the raster-and-shade of the
- * tiles of all items, the overflow loop of tiles whose bins overflowed and their shade,
- * each waiting only on the work it reads).  At most 1024 items.  Requires RZ_FLAG_DEVICE_BUFFERS; RZ_FLAG_CLEARED and RZ_FLAG_ASYNC apply to
+ * item as rz_draw_frame would (bit for bit).  By default a batch frame is five
+ * launches, each running one pass of every item (vertex transform; binning and buffer
+ * clears; raster-and-shade of the tiles of all items; the overflow loop of tiles whose
+ * bins overflowed; their shade); with rz_set_batch_persistent it is one launch of a
+ * resident grid that runs those passes as phases between grid barriers.  At most 1024
+ * items.  Requires RZ_FLAG_DEVICE_BUFFERS; RZ_FLAG_CLEARED and RZ_FLAG_ASYNC apply to
  * all items.  stats (may be NULL) sums over the items; with rz_set_profiling its ms_*
- * are the spans of each phase's work inside the launch (they overlap) and ms_frame the
- * launch. */
+ * are the passes' times (five launches: HIP events between them) or the phases' spans
+ * inside the one launch (first start to last end; they overlap; ms_vertex 0, the bin
+ * phase transforms the vertices), and ms_frame the whole frame. */
 rz_status rz_draw_batch(rz_ctx* ctx, const rz_batch_item* items, int32_t num_items, uint32_t flags,
                         rz_frame_stats* stats);
-/* 1: a batch frame is the one persistent launch above; 0 (default): five launches, one
- * per pass (vertex, bin, raster-and-shade, overflow, overflow shade).  Same bits.  Read
- * when a batch is prepared (rz_draw_batch, rz_batch_graph_create, rz_autotune). */
+/* 1: a batch frame is one persistent launch (every workgroup resident; a wait that
+ * would exceed about a second gives up and the call returns RZ_ERR_HIP); 0 (default):
+ * five launches, one per pass.  Same bits and statistics.  Read when a batch is
+ * prepared (rz_draw_batch, rz_batch_graph_create, rz_autotune). */
 rz_status rz_set_batch_persistent(rz_ctx* ctx, int32_t enable);
 int32_t rz_get_batch_persistent(const rz_ctx* ctx);
 /* `frames` repetitions of the batch captured in one HIP graph that owns its working

@@ -668,16 +668,13 @@ __device__ __forceinline__ bool fragment_key(float pxf, float pyf, float x0, flo
     return true;
 }
 
-// Candidate rectangles larger than this are shrunk first (shrink_rect).  A wave runs the
-// shrink when any of its lanes needs it, so the threshold trades that cost against the
-// candidates it removes: 32 pixels for single frames rastered by 256-thread workgroups,
-// 128 for frames of 16,384+ tiles (64-thread workgroups; their faces are small), and 8
-// for batches, whose triangle soups make large rectangles the common case (the 256-mesh
-// batch: 778 M candidates for 360 M fragments before the shrink's margins were
-// tightened, 623 M after, 567 M with this threshold).
-template <int BLOCK>
-constexpr int shrink_pixels() { return BLOCK == 64 ? 128 : 32; }
-constexpr int kShrinkPixelsBatch = 8;
+// Candidate rectangles larger than this are shrunk first (shrink_rect): every rectangle
+// of two or more pixels.  The shrink costs about 300 instructions a wave; the candidates
+// it removes cost about 113 each per 64 (the 4M-triangle frame: 27.3 M candidates for
+// 3.0 M fragments unshrunk, 6.1 M shrunk, raster+shade pass 212 -> 195 us; the
+// 82k-face 1080p frame 1.20 M -> 0.78 M candidates for 0.36 M fragments, 19.6 -> 18.2
+// us; the 256-mesh batch 778 M -> 567 M candidates for 360 M fragments).
+constexpr int kShrinkPixels = 1;
 
 // Range of T ∩ {lo <= v <= hi} along the other axis (u), T the triangle (u_k, v_k):
 // the vertices inside the slab and the edges' crossings of its two lines.  Returns false
@@ -700,7 +697,7 @@ __device__ __forceinline__ bool slab_range(float u0, float v0, float u1, float v
     return umin <= umax;
 }
 
-// Shrinks a large face's candidate rectangle in its tile (tile-local [rx0, rx1] x
+// Shrinks a face's candidate rectangle in its tile (tile-local [rx0, rx1] x
 // [ry0, ry1], origin ox, oy) to the pixels that can pass the inside test; returns false
 // when none can.  Bound used: the computed edge function differs from the exact one by
 // at most 5.7 u L (D + 1) (u = 2^-24, L the edge's length, D the largest coordinate
@@ -717,7 +714,8 @@ __device__ __forceinline__ bool slab_range(float u0, float v0, float u1, float v
 // d = 2^-19 (C + 16), C the largest coordinate magnitude, exceeds the error of a computed
 // edge crossing (pu + (lo - pv) (qu - pu) rcp(qv - pv): |error| <= 15.2 u C, the
 // hardware reciprocal within 1 ulp) plus the roundings of the slab bounds and of
-// min - w, max + w.  Faces where g would exceed 4 pixels keep their rectangle.
+// min - w, max + w.  Faces where g would exceed 4 pixels, or whose doubled area lies
+// outside [2^-40, 2^40], keep their rectangle.
 __device__ __forceinline__ bool shrink_rect(const float4& a, const float4& b, const float4& c, int ox, int oy,
                                             int& rx0, int& rx1, int& ry0, int& ry1) {
     const float X0 = (float)(ox + rx0), X1 = (float)(ox + rx1), Y0 = (float)(oy + ry0), Y1 = (float)(oy + ry1);
@@ -726,7 +724,7 @@ __device__ __forceinline__ bool shrink_rect(const float4& a, const float4& b, co
     const float D = fmaxf(fmaxf(vx1 - X0, X1 - vx0), fmaxf(vy1 - Y0, Y1 - vy0));
     const float area = fabsf(a.w);
     // (comparisons false for NaN: such faces keep their rectangle)
-    if (!(D <= 1048576.0f) || !(area >= 1.0f) || !(area <= 0x1p40f)) return true;
+    if (!(D <= 1048576.0f) || !(area >= 0x1p-40f) || !(area <= 0x1p40f)) return true;
     const float l0 = fabsf(b.x - a.x) + fabsf(b.y - a.y), l1 = fabsf(c.x - b.x) + fabsf(c.y - b.y);
     const float l2 = fabsf(a.x - c.x) + fabsf(a.y - c.y);
     const float m = 8.01f * 0x1p-24f * (D + 1.0f);
@@ -767,7 +765,7 @@ struct RasterLds {
 };
 
 // EXT_RAW: the scratch LDS (RasterLds bytes) is the caller's `raw`, else the function's own.
-template <int TS, int BLOCK, bool FUSED, int MODE, int SHRINK = shrink_pixels<BLOCK>(), bool EXT_RAW = false>
+template <int TS, int BLOCK, bool FUSED, int MODE, int SHRINK = kShrinkPixels, bool EXT_RAW = false>
 __device__ __forceinline__ void raster_body(const RasterParams& p, const ShadeParams* sp, const float* s_lut,
                                             const DevMaterial* s_mats, const uint16_t* s_net, int tile,
                                             uint32_t& frags, uint32_t& cov, unsigned char* raw = nullptr) {
@@ -1260,7 +1258,7 @@ __global__ __launch_bounds__(BLOCK, MODE == kShadeMlp ? 3 : 6) void k_raster_sha
             if (k >= n) break;
             const uint32_t e = bl.lists[(size_t)x * bl.stride + k];
             const uint32_t m = e >> kListMeshShift;
-            raster_body<TS, BLOCK, true, MODE, kShrinkPixelsBatch>(items[m], &sitems[m], sitems[m].luts, nullptr, s_net,
+            raster_body<TS, BLOCK, true, MODE, kShrinkPixels>(items[m], &sitems[m], sitems[m].luts, nullptr, s_net,
                                                (int)(e & kListTileMask), frags, cov);
         }
     }
@@ -1410,7 +1408,7 @@ __global__ __launch_bounds__(BLOCK, MODE == kShadeMlp ? 3 : 4) void k_batch_fram
             if (k >= n) break;
             const uint32_t e = bl.lists[(size_t)x * bl.stride + k];
             const uint32_t m = e >> kListMeshShift;
-            raster_body<TS, BLOCK, true, MODE, kShrinkPixelsBatch, true>(ritems[gen * nm + m], &sitems[m],
+            raster_body<TS, BLOCK, true, MODE, kShrinkPixels, true>(ritems[gen * nm + m], &sitems[m],
                                                                           sitems[m].luts, nullptr, s_net,
                                                                           (int)(e & kListTileMask), frags, cov, s_lds);
         }
