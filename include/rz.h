@@ -177,9 +177,9 @@ int32_t rz_abi_version(void);
 
 /* Tile edge (8, 16 or 32 pixels) and per-tile bin capacity (triangles) used by the
  * next frames.  Defaults: 16 and 0, automatic: a single frame gets room in every tile
- * for every face of its mesh when those bins fit 4 GiB (no bin can overflow), else 512;
- * batches get 512.  A tile whose faces exceed the capacity takes the overflow path
- * (same pixels, slower). */
+ * for every face of its mesh when those bins fit 4 GiB and an eighth of the device
+ * memory free at rz_create (no bin can overflow), else 512; batches get 512.  A tile
+ * whose faces exceed the capacity takes the overflow path (same pixels, slower). */
 rz_status rz_set_tuning(rz_ctx* ctx, int32_t tile_size, int32_t bin_capacity);
 rz_status rz_get_tuning(const rz_ctx* ctx, int32_t* tile_size, int32_t* bin_capacity);
 /* Single frames and frame graphs: 1 (default) shades each tile in the raster pass from

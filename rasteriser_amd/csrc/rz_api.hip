@@ -114,6 +114,7 @@ struct rz_ctx {
     bool profiling = false;
     uint32_t persistent_blocks = 0;   // workgroups of the overflow shade launch (a few per CU)
     int cus = 0;
+    size_t auto_bin_bytes = 0;        // automatic bin capacity: the most bytes of bins (rz_create)
     hipEvent_t ev[6] = {};
     std::string err;
     DevBuf raster, cnormal, rec, tile_count, bins, vis, lists, frame, depth, ids, counters, partials;
@@ -514,19 +515,20 @@ void fill_frame(const rz_mesh* mesh, rz_light* lights, int32_t nl, const rz_args
 constexpr uint32_t kNarrowTiles = 16384;
 
 // Bin capacity of the automatic setting when a bin per tile for every face would not fit
-// kAutoBinBytes (and of batches, which the autotuner sizes).
+// the context's auto_bin_bytes (and of batches, which the autotuner sizes).
 constexpr int kDefaultBinCap = 512;
-constexpr size_t kAutoBinBytes = (size_t)4 << 30;
+// auto_bin_bytes: 4 GiB, or an eighth of the device memory free at rz_create if less.
+constexpr size_t kAutoBinBytesMax = (size_t)4 << 30;
 
 // A single frame's per-tile bin capacity: the context's, or (automatic) room in every
-// tile for every face of the mesh when that fits kAutoBinBytes — bins are written only
-// where faces land, so their size costs address space, not traffic.  With a capacity of
-// at least the face count no bin can overflow and the frame needs no overflow-shade
-// launch.
+// tile for every face of the mesh when that fits auto_bin_bytes — bins are written only
+// where faces land, so their size costs device memory (hipMalloc commits it), not
+// traffic.  With a capacity of at least the face count no bin can overflow and the frame
+// needs no overflow-shade launch.
 int frame_bin_cap(const rz_ctx* ctx, const rz_mesh* mesh, uint32_t ntiles) {
     if (ctx->bin_cap > 0) return ctx->bin_cap;
     const size_t nf = mesh->nf > 0 ? (size_t)mesh->nf : 1;
-    if (nf <= ((size_t)1 << 20) && (size_t)ntiles * nf * 4 <= kAutoBinBytes) return (int)nf;
+    if (nf <= ((size_t)1 << 20) && (size_t)ntiles * nf * 4 <= ctx->auto_bin_bytes) return (int)nf;
     return kDefaultBinCap;
 }
 
@@ -1238,6 +1240,9 @@ rz_status rz_create(int32_t device, rz_ctx** out) {
         if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device);
         if (e == hipSuccess) ctx->persistent_blocks = (uint32_t)(cus * kPersistentPerCU) / 8u * 8u;
         ctx->cus = cus;
+        size_t free_bytes = 0, total_bytes = 0;
+        if (e == hipSuccess) e = hipMemGetInfo(&free_bytes, &total_bytes);
+        ctx->auto_bin_bytes = std::min(kAutoBinBytesMax, free_bytes / 8);
         if (e != hipSuccess) st = fail(ctx, RZ_ERR_HIP, "context setup: %s", hipGetErrorString(e));
         if (st == RZ_OK) st = scratch_address(ctx);
     } while (0);
