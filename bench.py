@@ -315,7 +315,27 @@ def batched(ctx, rz, scenes, np, frames):
 
 
 def learned_shading(ctx, rz, scenes, np, steps, batch, frames):
-    scene, args = scenes.learned_shading_scene()
+    """The learned shader on the 64-light normal-mapped 1080p icosphere, then the same
+    frame with 256 and 1,024 lights (intensities scaled by 64 / lights, so the frame
+    keeps its brightness): the light loop grows with the lights, the network does not."""
+    out = learned_at(ctx, rz, scenes, np, 64, steps, batch, frames)
+    sweep = []
+    for nl in (64, 256, 1024):
+        r = out if nl == 64 else learned_at(ctx, rz, scenes, np, nl, steps, batch, frames)
+        sweep.append({"lights": nl, "shade_pass_us": r["shade_pass_us"], "light_loop_us": r["light_loop_us"],
+                      "network_us": r["network_us"], "speedup_shade_pass": r["speedup_shade_pass"],
+                      "speedup_frame": r["speedup_frame"], "frame_ms": r["frame_ms"],
+                      "err_mean_255": r["err_mean_255"], "err_max_per_channel_255": r["err_max_per_channel_255"]})
+        log(f"mlp at {nl} lights: shade pass plain {r['shade_pass_us']['plain']:.1f} us, "
+            f"mlp {r['shade_pass_us']['mlp']:.1f} us, chain {r['shade_pass_us']['chain']:.1f} us; "
+            f"mean error {r['err_mean_255']:.2f}/255")
+    out["light_sweep"] = sweep
+    return out
+
+
+def learned_at(ctx, rz, scenes, np, nl, steps, batch, frames):
+    scene, args = scenes.learned_shading_scene(lights=nl)
+    scene.lights[:, 3] *= np.float32(64.0 / nl)
     W, H = args.image_width, args.image_height
     mesh = ctx.upload(scene)
     frame = ctx.empty((3, H, W), np.uint8)
@@ -373,7 +393,7 @@ def learned_shading(ctx, rz, scenes, np, steps, batch, frames):
         network_us = sep["mlp_us"] - sep["chain_us"]
         net_tflops = frags * MLP_FLOPS_PER_FRAGMENT / (network_us * 1e-6) / 1e12 if network_us > 0 else None
         return {
-            "workload": "icosphere-s6-1080p-64-lights-normal-map-1024",
+            "workload": f"icosphere-s6-1080p-{nl}-lights-normal-map-1024",
             "training_samples": int(n), "training_rotations": rotations,
             "train_steps": steps, "batch": batch, "train_s": t_train, "collect_s": t_collect,
             "loss_first": float(losses[0]), "loss_last100": float(losses[-100:].mean()),

@@ -530,7 +530,9 @@ __device__ __forceinline__ bool shade_pixel(const ShadeParams& p, const float* s
                 acc1 = acc1 + p.light_ic[l][1] * alb[1] * mm * kOneOverPi;
                 acc2 = acc2 + p.light_ic[l][2] * alb[2] * mm * kOneOverPi;
             }
-            // the lights beyond the kernel arguments, in order (uniform: scalar loads)
+            // the lights beyond the kernel arguments, in order (uniform: scalar loads; unrolled
+            // so that several lights' loads are in flight at once, the sums still in order)
+#pragma unroll 8
             for (int l = kMaxLights; l < p.nl; ++l) {
                 const float* e = p.lights_ext + 6 * (l - kMaxLights);
                 const float dd = (nx * e[3] + ny * e[4]) + nz * e[5];
