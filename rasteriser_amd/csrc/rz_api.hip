@@ -813,7 +813,7 @@ rz_status prepare_batch(rz_ctx* ctx, const rz_batch_item* items, int32_t n, uint
     uint32_t* batch_part = (uint32_t*)(base + qbytes);
     uint32_t* lists = (uint32_t*)(base + stat_bytes);
     size_t so = batch_stat_bytes, wo = stat_bytes + list_bytes;
-    std::vector<uint2> map[3];
+    std::vector<uint2> map[3], clear_items;
     for (int32_t i = 0; i < n; ++i) {
         const rz_batch_item& it = items[i];
         void* ptrs[kNumBufs];
@@ -844,8 +844,14 @@ rz_status prepare_batch(rz_ctx* ctx, const rz_batch_item* items, int32_t n, uint
         const uint32_t nb[3] = {vertex_blocks(L.vp.nv, L.vp.nn, L.vp.ntiles, bblock),
                                 bin_blocks(L.bp.nf, bblock) + L.bp.clear_blocks, overflow_blocks(L.rp.nf, bblock)};
         for (int p = 0; p < 3; ++p)
-            for (uint32_t b = 0; b < nb[p]; ++b) map[p].push_back(make_uint2((uint32_t)i, b));
+            for (uint32_t b = 0; b < nb[p]; ++b)
+                (p == 1 && b >= bin_blocks(L.bp.nf, bblock) ? clear_items : map[p]).push_back(make_uint2((uint32_t)i, b));
     }
+    // the clearing workgroups after every mesh's bin workgroups: stores that only stream
+    // run best once the bin workgroups' dependent loads and atomics are done (the
+    // 256-mesh batch's bin launch 0.88 -> 0.83 ms against each mesh's clears after its
+    // bins, and 0.88 ms with the clears dealt evenly between the bin workgroups)
+    map[1].insert(map[1].end(), clear_items.begin(), clear_items.end());
     B.bl.lists = lists;
     B.bl.list_count = B.frames[0].bp.list_count;
     B.bl.stride = (uint32_t)stride;
@@ -959,7 +965,7 @@ rz_status prepare_batch_sync(rz_ctx* ctx, const rz_batch_item* items, int32_t n,
     uint32_t* batch_part = (uint32_t*)(base + sync_bytes);
     uint32_t* lists = (uint32_t*)(base + stat_bytes);
     size_t so = batch_stat_bytes, wo = stat_bytes + list_bytes;
-    std::vector<uint2> bmap;
+    std::vector<uint2> bmap, store_items;
     for (int32_t i = 0; i < n; ++i) {
         const rz_batch_item& it = items[i];
         void* ptrs[kNumBufs];
@@ -993,8 +999,10 @@ rz_status prepare_batch_sync(rz_ctx* ctx, const rz_batch_item* items, int32_t n,
         B.faces[i] = (uint64_t)it.mesh->nf;
         const FrameLaunch& L = B.frames[i];
         const uint32_t nb = bin_blocks(L.bp.nf, bblock) + (L.bp.zero_count + bblock - 1) / bblock + L.bp.clear_blocks;
-        for (uint32_t b = 0; b < nb; ++b) bmap.push_back(make_uint2((uint32_t)i, b));
+        const uint32_t nbin = bin_blocks(L.bp.nf, bblock);
+        for (uint32_t b = 0; b < nb; ++b) (b < nbin ? bmap : store_items).push_back(make_uint2((uint32_t)i, b));
     }
+    bmap.insert(bmap.end(), store_items.begin(), store_items.end());   // (as for the bin launch)
     for (int g = 0; g < 2; ++g) {
         BatchLists& bl = g ? B.bl1 : B.bl;
         const FrameLaunch& L0 = g ? B.frames1[0] : B.frames[0];
