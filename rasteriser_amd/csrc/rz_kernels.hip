@@ -731,8 +731,10 @@ __device__ __forceinline__ bool shrink_rect(const float4& a, const float4& b, co
     const float l2 = fabsf(a.x - c.x) + fabsf(a.y - c.y);
     const float m = 8.01f * 0x1p-24f * (D + 1.0f);
     if (!(area > 2.0f * m * fmaxf(fmaxf(l0, l1), l2) * 1.01f)) return true;
-    const float r = 0.5f * area / ((l0 + l1 + l2) * 1.01f);
-    const float g = m * (1.0f + 3.0f * D / r) * 1.01f;
+    // g = m (1 + 3 D / r) with r = area / (2 P) (P = l0 + l1 + l2 bounds the perimeter):
+    // 6 D P / area through the hardware reciprocal (within 1 ulp), the factor 6.07 and the
+    // final 1.01 covering its and the products' roundings
+    const float g = m * (1.0f + 6.07f * D * (l0 + l1 + l2) * __builtin_amdgcn_rcpf(area)) * 1.01f;
     if (!(g <= 4.0f)) return true;
     const float C = fmaxf(fmaxf(fmaxf(fabsf(a.x), fabsf(a.y)), fmaxf(fabsf(b.x), fabsf(b.y))),
                           fmaxf(fmaxf(fabsf(c.x), fabsf(c.y)), fmaxf(X1, Y1)));
