@@ -677,6 +677,12 @@ __device__ __forceinline__ bool fragment_key(float pxf, float pyf, float x0, flo
 // 82k-face 1080p frame 1.20 M -> 0.78 M candidates for 0.36 M fragments, 19.6 -> 18.2
 // us; the 256-mesh batch 778 M -> 567 M candidates for 360 M fragments).
 constexpr int kShrinkPixels = 1;
+// Single frames first cut the rectangle to the triangle's vertex extent and run the slab
+// cut (below) only on rectangles still larger than kSlabPixels (4M-triangle raster+shade
+// pass 186 -> 179 us); batches, whose triangle soups make large rectangles the common
+// case, run the slab cut directly (kSlabAlways: the extra cut cost 2.39 -> 2.55 ms there).
+constexpr int kSlabPixels = 32;
+constexpr int kSlabAlways = -1;
 
 // Range of T ∩ {lo <= v <= hi} along the other axis (u), T the triangle (u_k, v_k):
 // the vertices inside the slab and the edges' crossings of its two lines.  Returns false
@@ -716,8 +722,12 @@ __device__ __forceinline__ bool slab_range(float u0, float v0, float u1, float v
 // d = 2^-19 (C + 16), C the largest coordinate magnitude, exceeds the error of a computed
 // edge crossing (pu + (lo - pv) (qu - pu) rcp(qv - pv): |error| <= 15.2 u C, the
 // hardware reciprocal within 1 ulp) plus the roundings of the slab bounds and of
-// min - w, max + w.  Faces where g would exceed 4 pixels, or whose doubled area lies
-// outside [2^-40, 2^40], keep their rectangle.
+// min - w, max + w.  With SLAB >= 0 the rectangle is first cut to the triangle's own
+// extent, [ceil(v_min - w), floor(v_max + w)] per axis (a point within g of the triangle
+// lies within g of its vertices' range), and only a rectangle still larger than SLAB
+// pixels takes the slab cut.  Faces where g would exceed 4 pixels, or whose doubled area
+// lies outside [2^-40, 2^40], keep their rectangle.
+template <int SLAB = kSlabPixels>
 __device__ __forceinline__ bool shrink_rect(const float4& a, const float4& b, const float4& c, int ox, int oy,
                                             int& rx0, int& rx1, int& ry0, int& ry1) {
     const float X0 = (float)(ox + rx0), X1 = (float)(ox + rx1), Y0 = (float)(oy + ry0), Y1 = (float)(oy + ry1);
@@ -739,6 +749,13 @@ __device__ __forceinline__ bool shrink_rect(const float4& a, const float4& b, co
     const float C = fmaxf(fmaxf(fmaxf(fabsf(a.x), fabsf(a.y)), fmaxf(fabsf(b.x), fabsf(b.y))),
                           fmaxf(fmaxf(fabsf(c.x), fabsf(c.y)), fmaxf(X1, Y1)));
     const float w = g + 0x1p-19f * (C + 16.0f);
+    if constexpr (SLAB >= 0) {
+        // first the triangle's own extent (its vertices), which is all a small face needs
+        rx0 = max(rx0, (int)ceilf(vx0 - w) - ox); rx1 = min(rx1, (int)floorf(vx1 + w) - ox);
+        ry0 = max(ry0, (int)ceilf(vy0 - w) - oy); ry1 = min(ry1, (int)floorf(vy1 + w) - oy);
+        if (rx0 > rx1 || ry0 > ry1) return false;
+        if ((rx1 - rx0 + 1) * (ry1 - ry0 + 1) <= SLAB) return true;
+    }
     float xmin, xmax, ymin, ymax;
     if (!slab_range(a.x, a.y, b.x, b.y, c.x, c.y, Y0 - w, Y1 + w, xmin, xmax)) return false;
     if (!slab_range(a.y, a.x, b.y, b.x, c.y, c.x, X0 - w, X1 + w, ymin, ymax)) return false;
@@ -769,7 +786,8 @@ struct RasterLds {
 };
 
 // EXT_RAW: the scratch LDS (RasterLds bytes) is the caller's `raw`, else the function's own.
-template <int TS, int BLOCK, bool FUSED, int MODE, int SHRINK = kShrinkPixels, bool EXT_RAW = false>
+template <int TS, int BLOCK, bool FUSED, int MODE, int SHRINK = kShrinkPixels, bool EXT_RAW = false,
+          int SLAB = kSlabPixels>
 __device__ __forceinline__ void raster_body(const RasterParams& p, const ShadeParams* sp, const float* s_lut,
                                             const DevMaterial* s_mats, const uint16_t* s_net, int tile,
                                             uint32_t& frags, uint32_t& cov, unsigned char* raw = nullptr) {
@@ -838,7 +856,7 @@ __device__ __forceinline__ void raster_body(const RasterParams& p, const ShadePa
             // large rectangles: only the part the triangle can reach
             bool keep = true;
             if (rx0 <= rx1 && ry0 <= ry1 && (rx1 - rx0 + 1) * (ry1 - ry0 + 1) > SHRINK)
-                keep = shrink_rect(a, b, c, ox, oy, rx0, rx1, ry0, ry1);
+                keep = shrink_rect<SLAB>(a, b, c, ox, oy, rx0, rx1, ry0, ry1);
             if (keep && rx0 <= rx1 && ry0 <= ry1) {
                 const uint32_t rw = rx1 - rx0 + 1;
                 cnt = rw * (ry1 - ry0 + 1);
@@ -1262,7 +1280,8 @@ __global__ __launch_bounds__(BLOCK, MODE == kShadeMlp ? 3 : 6) void k_raster_sha
             if (k >= n) break;
             const uint32_t e = bl.lists[(size_t)x * bl.stride + k];
             const uint32_t m = e >> kListMeshShift;
-            raster_body<TS, BLOCK, true, MODE, kShrinkPixels>(items[m], &sitems[m], sitems[m].luts, nullptr, s_net,
+            raster_body<TS, BLOCK, true, MODE, kShrinkPixels, false, kSlabAlways>(items[m], &sitems[m], sitems[m].luts,
+                                                                                 nullptr, s_net,
                                                (int)(e & kListTileMask), frags, cov);
         }
     }
@@ -1412,7 +1431,7 @@ __global__ __launch_bounds__(BLOCK, MODE == kShadeMlp ? 3 : 4) void k_batch_fram
             if (k >= n) break;
             const uint32_t e = bl.lists[(size_t)x * bl.stride + k];
             const uint32_t m = e >> kListMeshShift;
-            raster_body<TS, BLOCK, true, MODE, kShrinkPixels, true>(ritems[gen * nm + m], &sitems[m],
+            raster_body<TS, BLOCK, true, MODE, kShrinkPixels, true, kSlabAlways>(ritems[gen * nm + m], &sitems[m],
                                                                           sitems[m].luts, nullptr, s_net,
                                                                           (int)(e & kListTileMask), frags, cov, s_lds);
         }
