@@ -763,9 +763,11 @@ struct BatchLaunch {
 };
 
 // The persistent launch's block at the start of the arena: 3 barriers of 8 counters (one
-// per 128-byte line), the error flag, the phase clock (4 phases).
+// per 128-byte line), the error flag and the phase clock (4 phases) on the next lines,
+// then the phase-1 item cursor on a line of its own.
 constexpr size_t kSyncWords = 3 * 8 * (kCountStride / 2);
-constexpr size_t kSyncBytes = (kSyncWords + 1 + 8) * 8;
+constexpr size_t kCursorWord = kSyncWords + 16;   // (64-bit words from the block's start)
+constexpr size_t kSyncBytes = (kCursorWord + 16) * 8;
 
 rz_status prepare_batch(rz_ctx* ctx, const rz_batch_item* items, int32_t n, uint32_t flags, DevBuf& arena,
                         DevBuf& tables, BatchLaunch& B) {
@@ -1070,6 +1072,7 @@ rz_status prepare_batch_sync(rz_ctx* ctx, const rz_batch_item* items, int32_t n,
     q.bar = (unsigned long long*)arena.ptr;
     q.error = (uint32_t*)((unsigned long long*)arena.ptr + kSyncWords);
     q.clock = nullptr;   // (set per frame when profiling)
+    q.cursor = (uint32_t*)((unsigned long long*)arena.ptr + kCursorWord);
     q.G = B.raster_grid;
     q.nB = (uint32_t)bmap.size();
     q.bmap = (const uint2*)(t + off_bmap);

@@ -253,18 +253,19 @@ __host__ __device__ inline uint32_t overflow_blocks(int32_t nf, int block = 256)
 }
 
 // The persistent batch launch (k_batch_frame): G workgroups, all resident at once (G is
-// the launch's occupancy times the CUs), share each phase's work by static striding and
-// meet at three barriers.  A frame: phase 1, every mesh's bin work (each face's corners
-// transformed in place, as a single frame's bin pass does), the zeroing of the other
-// generation of counter lines (each mesh keeps two, as single frames do) and the buffer
-// clears; barrier 1; phase 2, raster and shade of the tiles of all meshes from per-list
-// cursors; phase 3, the overflow loop of overflowed tiles; barrier 2; phase 4, their
-// shade; barrier 3 (arrival only: the next frame's launch reads it).  A barrier is 8
-// counters, one per XCD (workgroup b counts on counter b % 8), each on its own 128-byte
+// the launch's occupancy times the CUs), meet at three barriers.  A frame: phase 1, every
+// mesh's bin work (each face's corners transformed in place, as a single frame's bin pass
+// does), the zeroing of the other generation of counter lines (each mesh keeps two, as
+// single frames do) and the buffer clears, taken one item at a time from a cursor;
+// barrier 1; phase 2, raster and shade of the tiles of all meshes from per-list cursors;
+// phase 3, the overflow loop of overflowed tiles (static striding); barrier 2; phase 4,
+// their shade; barrier 3 (arrival only: the next frame's launch reads it).  A barrier is
+// 8 counters, one per XCD (workgroup b counts on counter b % 8), each on its own 128-byte
 // line; they only grow: frame f's workgroups wait for (f + 1) * G / 8 on each, and learn
 // f from the third barrier's counters at the start (every workgroup of frame f - 1 has
 // arrived there, and none of frame f can before all of them passed barrier 1).  The
-// counters start at zero when the batch is prepared.
+// counters and the cursor start at zero when the batch is prepared; workgroup 0 resets
+// the cursor once barrier 1 is passed, when every take of the frame is done.
 struct BatchSync {
     unsigned long long* bar;     // 3 barriers x 8 counters, kCountStride / 2 words apart
     uint32_t* error;             // nonzero: a barrier wait gave up (a guard; never set when all G are resident)
@@ -274,7 +275,10 @@ struct BatchSync {
     uint32_t nmesh;              // meshes (the parameter tables hold generation 0, then generation 1)
     unsigned long long* clock;   // null, or per phase (bin, raster, overflow, shade) the first start (min)
                                  // and the last end (max) of its work, 100 MHz
-};__host__ __device__ inline uint32_t shade_blocks_x(uint32_t W) { return (W + 63) / 64; }
+    uint32_t* cursor;            // phase 1's item cursor (zero at a frame's start; reset after barrier 0)
+};
+
+__host__ __device__ inline uint32_t shade_blocks_x(uint32_t W) { return (W + 63) / 64; }
 __host__ __device__ inline uint32_t shade_blocks_y(uint32_t H) { return (H + 3) / 4; }
 
 // XCD-aware placement of a 2-D grid of work units.  Workgroups are dealt to the eight

@@ -1397,9 +1397,19 @@ __global__ __launch_bounds__(BLOCK, MODE == kShadeMlp ? 3 : 4) void k_batch_fram
     const uint32_t nm = q.nmesh;
     const uint32_t w = blockIdx.x;
 
-    // phase 1: binning (with the vertex transform), the other generation's zeroing, clears
+    // phase 1: binning (with the vertex transform), the other generation's zeroing, clears.
+    // Items are taken one at a time from one cursor, the next take issued as the current
+    // item starts so that its round trip overlaps the item (the 256-mesh batch: bin phase
+    // 1.33 ms with static striding, which left workgroups idle behind the slowest items,
+    // 1.07 ms this way; 1.13, 1.30 and 1.67 ms taking 2, 4 or 8 consecutive items).
     phase_clock(q, 0, true);
-    for (uint32_t i = w; i < q.nB; i += q.G) {
+    uint32_t next = 0;
+    if (threadIdx.x == 0) s_take = atomicAdd(q.cursor, 1u);
+    __syncthreads();
+    uint32_t i = s_take;
+    __syncthreads();   // (s_take is rewritten below)
+    while (i < q.nB) {
+        if (threadIdx.x == 0) next = atomicAdd(q.cursor, 1u);
         const uint2 m = q.bmap[i];
         const BinParams& p = bitems[gen * nm + m.x];
         const uint32_t nb = bin_blocks(p.nf, BLOCK);
@@ -1411,9 +1421,15 @@ __global__ __launch_bounds__(BLOCK, MODE == kShadeMlp ? 3 : 4) void k_batch_fram
         else
             clear_body<BLOCK>(p, m.y - nb - nz, p.clear_blocks);
         __syncthreads();   // (s_lds is reused by the next item)
+        if (threadIdx.x == 0) s_take = next;
+        __syncthreads();
+        i = s_take;
+        __syncthreads();
     }
     phase_clock(q, 0, false);
     batch_barrier(q, 0, target, true);
+    // every take of this frame is done: the cursor starts the next frame at zero
+    if (w == 0 && threadIdx.x == 0) __hip_atomic_store(q.cursor, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 
     // phase 2: tiles of the shared lists, one at a time from per-list cursors (first list
     // w % 8, then the others)
