@@ -1398,18 +1398,17 @@ __global__ __launch_bounds__(BLOCK, MODE == kShadeMlp ? 3 : 4) void k_batch_fram
     const uint32_t w = blockIdx.x;
 
     // phase 1: binning (with the vertex transform), the other generation's zeroing, clears.
-    // Items are taken one at a time from one cursor, the next take issued as the current
-    // item starts so that its round trip overlaps the item (the 256-mesh batch: bin phase
-    // 1.33 ms with static striding, which left workgroups idle behind the slowest items,
-    // 1.07 ms this way; 1.13, 1.30 and 1.67 ms taking 2, 4 or 8 consecutive items).
+    // Items are taken one at a time from one cursor (the 256-mesh batch's frame: 4.00 ms
+    // with static striding, which left workgroups idle behind the slowest items, 3.57-3.61
+    // ms this way; 3.67-3.69 ms with the next take issued as an item starts, since the
+    // item's first wait for its own loads then also waits for the atomic; 3.74, 4.00 and
+    // 4.29 ms taking 2, 4 or 8 consecutive items).
     phase_clock(q, 0, true);
-    uint32_t next = 0;
     if (threadIdx.x == 0) s_take = atomicAdd(q.cursor, 1u);
     __syncthreads();
     uint32_t i = s_take;
     __syncthreads();   // (s_take is rewritten below)
     while (i < q.nB) {
-        if (threadIdx.x == 0) next = atomicAdd(q.cursor, 1u);
         const uint2 m = q.bmap[i];
         const BinParams& p = bitems[gen * nm + m.x];
         const uint32_t nb = bin_blocks(p.nf, BLOCK);
@@ -1421,7 +1420,7 @@ __global__ __launch_bounds__(BLOCK, MODE == kShadeMlp ? 3 : 4) void k_batch_fram
         else
             clear_body<BLOCK>(p, m.y - nb - nz, p.clear_blocks);
         __syncthreads();   // (s_lds is reused by the next item)
-        if (threadIdx.x == 0) s_take = next;
+        if (threadIdx.x == 0) s_take = atomicAdd(q.cursor, 1u);
         __syncthreads();
         i = s_take;
         __syncthreads();
