@@ -315,7 +315,8 @@ FrameLayout layout_frame(const rz_mesh* mesh, const rz_args& a, int ts, int cap,
     // each frame's bin pass zeroes the other one for the next frame.
     L.bytes[kBufRaster] = batch && !mesh->raster_space ? (size_t)mesh->nv * 16 : 0;
     L.bytes[kBufCnormal] = batch ? (size_t)L.nfn * 16 : 0;
-    L.bytes[kBufRec] = (size_t)mesh->nf * sizeof(FaceRecord);
+    // the face records, then the bin pass's front mask (a bit per face)
+    L.bytes[kBufRec] = (size_t)mesh->nf * sizeof(FaceRecord) + ((size_t)mesh->nf + 63) / 64 * 8;
     L.bytes[kBufTiles] = batch ? ((size_t)L.ntiles + 8) * 4 * kCountStride
                                : (size_t)2 * count_lines(L.ntiles) * 4 * kCountStride;
     L.bytes[kBufLists] = batch ? 0 : (size_t)9 * L.ntiles * 4;   // 8 lists, then the overflowed tiles
@@ -435,6 +436,7 @@ void fill_frame(const rz_mesh* mesh, rz_light* lights, int32_t nl, const rz_args
     const bool uniform = mesh->uniform && !a.flat;
     bp.gather = uniform ? 0 : 1;
     bp.rec = (FaceRecord*)buf[kBufRec];
+    bp.front_mask = (unsigned long long*)(bp.rec + mesh->nf);
     bp.tile_count = vp.tile_count;
     bp.bins = (uint32_t*)buf[kBufBins];
     bp.counters = counters;
@@ -475,6 +477,8 @@ void fill_frame(const rz_mesh* mesh, rz_light* lights, int32_t nl, const rz_args
     rp.lists = bp.lists;
     rp.list_count = bp.list_count;
     rp.ovf_list = bp.ovf_list;
+    rp.front_mask = bp.front_mask;
+    rp.face_v = mesh->bin_face_v;
     rp.max_blocks = lay.raster_grid;
     L.orp = rp;   // batches: k_overflow_batch
     L.orp.part_frag = part + lay.off_ovf;

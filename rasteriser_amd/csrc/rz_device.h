@@ -135,6 +135,9 @@ struct BinParams {
     // the overflowed tiles (list_tag | tile), each at the index its count returned (the
     // batch's count for batches): the overflow shade visits these, not every listed tile
     uint32_t* ovf_list;
+    // bit i: the face at binning position i faces the camera (one word per 64 positions;
+    // a culled face's record is not written)
+    unsigned long long* front_mask;
     // extra workgroups of the bin launch that clear buffers (clear_np = 0: none): the
     // visibility words (separate-shade pipeline) and, for the fused pipeline on cleared
     // buffers, frame 0, depth 1.0, ids -1
@@ -171,6 +174,10 @@ struct RasterParams {
     const uint32_t* lists;      // per-XCD lists of tiles with faces (see BinParams)
     const uint32_t* list_count;
     const uint32_t* ovf_list;   // overflowed tiles, counters[kCntOverflowTiles] of them
+    // the overflow loop's faces: the bin pass's front mask (by binning position) and the
+    // faces' indices in binning order (the face id is the fourth word)
+    const unsigned long long* front_mask;
+    const int4* face_v;
 };
 
 // A batch's raster and overflow-shade launches: the shared tile lists and the launches'

@@ -226,10 +226,12 @@ __device__ __forceinline__ void bin_body(const BinParams& p, int blk, unsigned c
                 tx0 = tlx >> p.ts_log2; tx1 = brx >> p.ts_log2;
                 ty0 = tly >> p.ts_log2; ty1 = bry >> p.ts_log2;
             }
-        } else {
-            r->r1 = make_float4(0.f, 0.f, 0.f, __uint_as_float(0xFFFFFFFFu));
-            r->r2 = make_float4(0.f, 0.f, 0.f, __uint_as_float(0u));
         }
+    }
+    {
+        // the wave's 64 binning positions (blk * BLOCK is a multiple of 64)
+        const unsigned long long fm = __ballot(front);
+        if ((threadIdx.x & 63) == 0 && i < p.nf) p.front_mask[(uint32_t)i >> 6] = fm;
     }
     const uint32_t nt = (tx0 <= tx1 && ty0 <= ty1) ? (tx1 - tx0 + 1) * (ty1 - ty0 + 1) : 0u;
     store_block_sums(front ? 1u : 0u, p.part_front + blk, nt, p.part_pairs + blk, threadIdx.x, BLOCK);
@@ -1013,9 +1015,13 @@ __device__ __forceinline__ uint32_t overflow_faces(const RasterParams& p, uint32
     const uint32_t lane = t & 63u;
     uint32_t frags = 0;
     for (uint32_t base = t - lane; base < (uint32_t)p.nf; base += nthreads) {   // uniform over the wave
-        const uint32_t f = base + lane;
+        // binning positions base + lane: the front ones, through their face ids
+        const uint32_t i = base + lane;
+        const unsigned long long fm = p.front_mask[base >> 6];
         bool meets = false;
-        if (f < (uint32_t)p.nf) {
+        uint32_t f = 0;
+        if (i < (uint32_t)p.nf && ((fm >> lane) & 1ull)) {
+            f = (uint32_t)p.face_v[i].w;
             const FaceRecord* r = p.rec + f;
             const uint32_t lo = __float_as_uint(r->r1.w), hi = __float_as_uint(r->r2.w);
             const uint32_t bx0 = lo & 0xFFFFu, by0 = lo >> 16, bx1 = hi & 0xFFFFu, by1 = hi >> 16;
@@ -1027,7 +1033,7 @@ __device__ __forceinline__ uint32_t overflow_faces(const RasterParams& p, uint32
         }
         unsigned long long todo = __ballot(meets);
         while (todo) {
-            const uint32_t g = base + (uint32_t)(__ffsll((long long)todo) - 1);
+            const uint32_t g = __shfl(f, __ffsll((long long)todo) - 1, 64);
             todo &= todo - 1ull;
             const FaceRecord* r = p.rec + g;
             const float4 a = r->r0, b = r->r1, c = r->r2;
