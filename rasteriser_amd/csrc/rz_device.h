@@ -135,8 +135,8 @@ struct BinParams {
     // the overflowed tiles (list_tag | tile), each at the index its count returned (the
     // batch's count for batches): the overflow shade visits these, not every listed tile
     uint32_t* ovf_list;
-    // bit i: the face at binning position i faces the camera (one word per 64 positions;
-    // a culled face's record is not written)
+    // bit i: the face at binning position i faces the camera and its box holds a pixel
+    // (one word per 64 positions; only such faces have their record written)
     unsigned long long* front_mask;
     // extra workgroups of the bin launch that clear buffers (clear_np = 0: none): the
     // visibility words (separate-shade pipeline) and, for the fused pipeline on cleared

@@ -208,9 +208,11 @@ __device__ __forceinline__ void bin_body(const BinParams& p, int blk, unsigned c
         const float s = signed_area(a.x, a.y, b.x, b.y, c.x, c.y);
         front = (s > 0.0f) != (p.wind_cw != 0);
         FaceRecord* r = p.rec + f;
-        if (front) {
-            uint32_t tlx, tly, brx, bry;
-            bounding_box(a.x, a.y, b.x, b.y, c.x, c.y, p.W, p.H, tlx, tly, brx, bry);
+        uint32_t tlx = 1, tly = 1, brx = 0, bry = 0;
+        if (front) bounding_box(a.x, a.y, b.x, b.y, c.x, c.y, p.W, p.H, tlx, tly, brx, bry);
+        // a record only for a face with pixels to raster (a culled face, or one whose box
+        // holds no pixel, is never read: the mask below leaves it out)
+        if (front && tlx <= brx && tly <= bry) {
             const float area = edge(c.x, c.y, a.x, a.y, b.x, b.y);
             r->r0 = make_float4(a.x, a.y, a.z, area);
             r->r1 = make_float4(b.x, b.y, b.z, __uint_as_float(tlx | (tly << 16)));
@@ -222,15 +224,14 @@ __device__ __forceinline__ void bin_body(const BinParams& p, int blk, unsigned c
                 r->r6 = make_float4(n2.x, n2.y, n2.z, t1.x);
                 r->r7 = make_float4(t1.y, t2.x, t2.y, __int_as_float(mat));
             }
-            if (tlx <= brx && tly <= bry) {
-                tx0 = tlx >> p.ts_log2; tx1 = brx >> p.ts_log2;
-                ty0 = tly >> p.ts_log2; ty1 = bry >> p.ts_log2;
-            }
+            tx0 = tlx >> p.ts_log2; tx1 = brx >> p.ts_log2;
+            ty0 = tly >> p.ts_log2; ty1 = bry >> p.ts_log2;
         }
     }
     {
-        // the wave's 64 binning positions (blk * BLOCK is a multiple of 64)
-        const unsigned long long fm = __ballot(front);
+        // the wave's 64 binning positions (blk * BLOCK is a multiple of 64): the faces
+        // with a record
+        const unsigned long long fm = __ballot(tx0 <= tx1 && ty0 <= ty1);
         if ((threadIdx.x & 63) == 0 && i < p.nf) p.front_mask[(uint32_t)i >> 6] = fm;
     }
     const uint32_t nt = (tx0 <= tx1 && ty0 <= ty1) ? (tx1 - tx0 + 1) * (ty1 - ty0 + 1) : 0u;
@@ -1015,7 +1016,7 @@ __device__ __forceinline__ uint32_t overflow_faces(const RasterParams& p, uint32
     const uint32_t lane = t & 63u;
     uint32_t frags = 0;
     for (uint32_t base = t - lane; base < (uint32_t)p.nf; base += nthreads) {   // uniform over the wave
-        // binning positions base + lane: the front ones, through their face ids
+        // binning positions base + lane: those with a record, through their face ids
         const uint32_t i = base + lane;
         const unsigned long long fm = p.front_mask[base >> 6];
         bool meets = false;
