@@ -141,7 +141,8 @@ typedef struct rz_matrices {
 typedef struct rz_frame_stats {
     uint64_t triangles;           /* input faces                                             */
     uint64_t front_triangles;     /* faces passing the backface test (drawing.cpp:176-178)   */
-    uint64_t bin_pairs;           /* (front face, tile) pairs whose clamped bbox meets the tile */
+    uint64_t bin_pairs;           /* (front face, tile) pairs binned: the face's clamped bbox,
+                                     cut to the pixels its inside test can pass, meets the tile */
     uint64_t overflow_tiles;      /* tiles whose bin overflowed and went the overflow path   */
     uint64_t fragments;           /* (pixel, front face) pairs passing the inside test (drawing.cpp:111) */
     uint64_t covered_pixels;      /* pixels holding a winning fragment                       */

@@ -4,14 +4,15 @@
 //               z-divide, NDC->raster (geometry.cpp:44-74, 85-95, 110-116, 135-142),
 //               and of its normals (geometry.cpp:35-42, 97-108); backface test
 //               (geometry.cpp:76-83, drawing.cpp:176-178), clamped bbox
-//               (drawing.cpp:77-93), per-face setup and shading record, appends the
-//               face to the bin of every tile its bbox meets (fixed capacity per tile)
+//               (drawing.cpp:77-93) cut to the triangle's extent, per-face setup and
+//               shading record, appends the face to the bin of every tile its cut box
+//               meets (fixed capacity per tile)
 //               and each tile's first face puts the tile on a per-XCD list; more
 //               workgroups zero the next frame's generation of counter lines.
 //               (Batches run the transform as a vertex pass of their own,
 //               k_vertex_batch, which also clears the meshes' counters.)
 //   k_raster    resident grid over the tile lists, one tile per workgroup at a time:
-//               expands (face, bbox pixel) candidates with wavefront prefix scans,
+//               expands (face, box pixel) candidates with wavefront prefix scans,
 //               runs the inside and depth tests of drawing.cpp:96-121 and keeps per
 //               pixel the minimum 64-bit (ordered depth << 32 | face) key in LDS;
 //               writes the visibility words; then its share of the faces of tiles
@@ -154,6 +155,104 @@ __device__ __forceinline__ void bin_push(const BinParams& p, uint32_t face, uint
         tile_overflowed(p, tile);
 }
 
+// A face's candidate rectangle is cut twice.  The bin pass cuts the face's whole box to
+// the triangle's vertex extent (shrink_rect<kSlabNever>), once per face: the 256-mesh
+// batch bins 14.0 M -> 11.4 M (face, tile) pairs, the 4M-triangle frame 2.65 M -> 2.0 M.
+// The raster pass then cuts a rectangle (the cut box inside its tile) larger than SHRINK
+// pixels to the triangle's part of the tile's slabs (shrink_rect<kSlabAlways>): every
+// rectangle of two or more pixels in batches (kShrinkPixels), whose triangle soups make
+// large rectangles the common case (8 and 32 measured no faster), and only rectangles
+// larger than kSlabPixels in single frames (4M-triangle raster+shade pass 179 -> 166 us
+// against every rectangle).  Each raster-pass cut costs about 300 instructions a wave;
+// a candidate it removes costs about 113 per 64.
+constexpr int kShrinkPixels = 1;
+constexpr int kSlabPixels = 32;
+constexpr int kSlabAlways = -1;
+// The vertex-extent cut alone (the bin pass's cut of a face's whole box).
+constexpr int kSlabNever = -2;
+
+// Range of T ∩ {lo <= v <= hi} along the other axis (u), T the triangle (u_k, v_k):
+// the vertices inside the slab and the edges' crossings of its two lines.  Returns false
+// when the slab misses T.  The crossings use the hardware reciprocal; callers widen the
+// range by a bound on that error.
+__device__ __forceinline__ bool slab_range(float u0, float v0, float u1, float v1, float u2, float v2, float lo,
+                                           float hi, float& umin, float& umax) {
+    umin = INFINITY; umax = -INFINITY;
+    auto vert = [&](float u, float v) {
+        if (v >= lo && v <= hi) { umin = fminf(umin, u); umax = fmaxf(umax, u); }
+    };
+    auto cross = [&](float pu, float pv, float qu, float qv) {
+        if (pv == qv) return;
+        const float k = (qu - pu) * __builtin_amdgcn_rcpf(qv - pv);
+        if ((pv - lo) * (qv - lo) <= 0.0f) { const float u = pu + (lo - pv) * k; umin = fminf(umin, u); umax = fmaxf(umax, u); }
+        if ((pv - hi) * (qv - hi) <= 0.0f) { const float u = pu + (hi - pv) * k; umin = fminf(umin, u); umax = fmaxf(umax, u); }
+    };
+    vert(u0, v0); vert(u1, v1); vert(u2, v2);
+    cross(u0, v0, u1, v1); cross(u1, v1, u2, v2); cross(u2, v2, u0, v0);
+    return umin <= umax;
+}
+
+// Shrinks a face's candidate rectangle in its tile (tile-local [rx0, rx1] x
+// [ry0, ry1], origin ox, oy) to the pixels that can pass the inside test; returns false
+// when none can.  Bound used: the computed edge function differs from the exact one by
+// at most 5.7 u L (D + 1) (u = 2^-24, L the edge's length, D the largest coordinate
+// distance between a vertex and a rectangle pixel), so a passing pixel lies within
+// m = 8.01 u (D + 1) of every edge's half-plane — when the area's sign is certain
+// (|area| > 2 m L_max) — hence within g = m (1 + 3 D / r) of the triangle (r its
+// inradius, >= |area| / 2 / perimeter): the offset triangle is the triangle scaled by
+// 1 + m / r about its incentre, and no vertex is farther than 2 sqrt(2) D from it.
+// (Lengths are bounded by |dx| + |dy|; the products' and the subtraction's roundings
+// give 4 u (|A| |py - ay| + |B| |px - ax|) <= 5.7 u L (D + 1).)  So a passing pixel p has
+// a triangle point t with |p - t| <= g: t lies in the slab of rows [Y0 - g, Y1 + g], and
+// px is within g of that slab's part of the triangle.  The slabs are widened to
+// w = g + d and the x (y) range taken as [ceil(min - w), floor(max + w)], where
+// d = 2^-19 (C + 16), C the largest coordinate magnitude, exceeds the error of a computed
+// edge crossing (pu + (lo - pv) (qu - pu) rcp(qv - pv): |error| <= 15.2 u C, the
+// hardware reciprocal within 1 ulp) plus the roundings of the slab bounds and of
+// min - w, max + w.  With SLAB >= 0 the rectangle is first cut to the triangle's own
+// extent, [ceil(v_min - w), floor(v_max + w)] per axis (a point within g of the triangle
+// lies within g of its vertices' range), and only a rectangle still larger than SLAB
+// pixels takes the slab cut; kSlabNever stops after that cut, kSlabAlways skips it.
+// Faces where g would exceed 4 pixels, or whose doubled area lies outside
+// [2^-40, 2^40], keep their rectangle.  The rectangle may be a whole image (the bin
+// pass): its corners stay below 65,536.
+template <int SLAB>
+__device__ __forceinline__ bool shrink_rect(const float4& a, const float4& b, const float4& c, int ox, int oy,
+                                            int& rx0, int& rx1, int& ry0, int& ry1) {
+    const float X0 = (float)(ox + rx0), X1 = (float)(ox + rx1), Y0 = (float)(oy + ry0), Y1 = (float)(oy + ry1);
+    const float vx0 = fminf(fminf(a.x, b.x), c.x), vx1 = fmaxf(fmaxf(a.x, b.x), c.x);
+    const float vy0 = fminf(fminf(a.y, b.y), c.y), vy1 = fmaxf(fmaxf(a.y, b.y), c.y);
+    const float D = fmaxf(fmaxf(vx1 - X0, X1 - vx0), fmaxf(vy1 - Y0, Y1 - vy0));
+    const float area = fabsf(a.w);
+    // (comparisons false for NaN: such faces keep their rectangle)
+    if (!(D <= 1048576.0f) || !(area >= 0x1p-40f) || !(area <= 0x1p40f)) return true;
+    const float l0 = fabsf(b.x - a.x) + fabsf(b.y - a.y), l1 = fabsf(c.x - b.x) + fabsf(c.y - b.y);
+    const float l2 = fabsf(a.x - c.x) + fabsf(a.y - c.y);
+    const float m = 8.01f * 0x1p-24f * (D + 1.0f);
+    if (!(area > 2.0f * m * fmaxf(fmaxf(l0, l1), l2) * 1.01f)) return true;
+    // g = m (1 + 3 D / r) with r = area / (2 P) (P = l0 + l1 + l2 bounds the perimeter):
+    // 6 D P / area through the hardware reciprocal (within 1 ulp), the factor 6.07 and the
+    // final 1.01 covering its and the products' roundings
+    const float g = m * (1.0f + 6.07f * D * (l0 + l1 + l2) * __builtin_amdgcn_rcpf(area)) * 1.01f;
+    if (!(g <= 4.0f)) return true;
+    const float C = fmaxf(fmaxf(fmaxf(fabsf(a.x), fabsf(a.y)), fmaxf(fabsf(b.x), fabsf(b.y))),
+                          fmaxf(fmaxf(fabsf(c.x), fabsf(c.y)), fmaxf(X1, Y1)));
+    const float w = g + 0x1p-19f * (C + 16.0f);
+    if constexpr (SLAB >= 0 || SLAB == kSlabNever) {
+        // first the triangle's own extent (its vertices), which is all a small face needs
+        rx0 = max(rx0, (int)ceilf(vx0 - w) - ox); rx1 = min(rx1, (int)floorf(vx1 + w) - ox);
+        ry0 = max(ry0, (int)ceilf(vy0 - w) - oy); ry1 = min(ry1, (int)floorf(vy1 + w) - oy);
+        if (rx0 > rx1 || ry0 > ry1) return false;
+        if (SLAB == kSlabNever || (rx1 - rx0 + 1) * (ry1 - ry0 + 1) <= SLAB) return true;
+    }
+    float xmin, xmax, ymin, ymax;
+    if (!slab_range(a.x, a.y, b.x, b.y, c.x, c.y, Y0 - w, Y1 + w, xmin, xmax)) return false;
+    if (!slab_range(a.y, a.x, b.y, b.x, c.y, c.x, X0 - w, X1 + w, ymin, ymax)) return false;
+    rx0 = max(rx0, (int)ceilf(xmin - w) - ox); rx1 = min(rx1, (int)floorf(xmax + w) - ox);
+    ry0 = max(ry0, (int)ceilf(ymin - w) - oy); ry1 = min(ry1, (int)floorf(ymax + w) - oy);
+    return rx0 <= rx1 && ry0 <= ry1;
+}
+
 // LDS bin_body needs (carved by the caller: kernels that run other phases share it):
 // per-tile counts and bases of the union rectangle, each face's slots, the rectangle.
 template <int BLOCK>
@@ -210,10 +309,22 @@ __device__ __forceinline__ void bin_body(const BinParams& p, int blk, unsigned c
         FaceRecord* r = p.rec + f;
         uint32_t tlx = 1, tly = 1, brx = 0, bry = 0;
         if (front) bounding_box(a.x, a.y, b.x, b.y, c.x, c.y, p.W, p.H, tlx, tly, brx, bry);
+        float area = 0.0f;
+        if (front && tlx <= brx && tly <= bry) {
+            // the box cut to the triangle's own extent, widened by shrink_rect's bound on
+            // the inside test's rounding: every pixel that can pass the test stays in it,
+            // and a face that can pass none keeps no pixel
+            area = edge(c.x, c.y, a.x, a.y, b.x, b.y);
+            int x0 = (int)tlx, x1 = (int)brx, y0 = (int)tly, y1 = (int)bry;
+            if (shrink_rect<kSlabNever>(make_float4(a.x, a.y, a.z, area), b, c, 0, 0, x0, x1, y0, y1)) {
+                tlx = (uint32_t)x0; brx = (uint32_t)x1; tly = (uint32_t)y0; bry = (uint32_t)y1;
+            } else {
+                tlx = tly = 1; brx = bry = 0;
+            }
+        }
         // a record only for a face with pixels to raster (a culled face, or one whose box
         // holds no pixel, is never read: the mask below leaves it out)
         if (front && tlx <= brx && tly <= bry) {
-            const float area = edge(c.x, c.y, a.x, a.y, b.x, b.y);
             r->r0 = make_float4(a.x, a.y, a.z, area);
             r->r1 = make_float4(b.x, b.y, b.z, __uint_as_float(tlx | (tly << 16)));
             r->r2 = make_float4(c.x, c.y, c.z, __uint_as_float(brx | (bry << 16)));
@@ -673,100 +784,6 @@ __device__ __forceinline__ bool fragment_key(float pxf, float pyf, float x0, flo
     return true;
 }
 
-// Candidate rectangles larger than this are shrunk first (shrink_rect): every rectangle
-// of two or more pixels.  The shrink costs about 300 instructions a wave; the candidates
-// it removes cost about 113 each per 64 (the 4M-triangle frame: 27.3 M candidates for
-// 3.0 M fragments unshrunk, 6.1 M shrunk, raster+shade pass 212 -> 195 us; the
-// 82k-face 1080p frame 1.20 M -> 0.78 M candidates for 0.36 M fragments, 19.6 -> 18.2
-// us; the 256-mesh batch 778 M -> 567 M candidates for 360 M fragments).
-constexpr int kShrinkPixels = 1;
-// Single frames first cut the rectangle to the triangle's vertex extent and run the slab
-// cut (below) only on rectangles still larger than kSlabPixels (4M-triangle raster+shade
-// pass 186 -> 179 us); batches, whose triangle soups make large rectangles the common
-// case, run the slab cut directly (kSlabAlways: the extra cut cost 2.39 -> 2.55 ms there).
-constexpr int kSlabPixels = 32;
-constexpr int kSlabAlways = -1;
-
-// Range of T ∩ {lo <= v <= hi} along the other axis (u), T the triangle (u_k, v_k):
-// the vertices inside the slab and the edges' crossings of its two lines.  Returns false
-// when the slab misses T.  The crossings use the hardware reciprocal; callers widen the
-// range by a bound on that error.
-__device__ __forceinline__ bool slab_range(float u0, float v0, float u1, float v1, float u2, float v2, float lo,
-                                           float hi, float& umin, float& umax) {
-    umin = INFINITY; umax = -INFINITY;
-    auto vert = [&](float u, float v) {
-        if (v >= lo && v <= hi) { umin = fminf(umin, u); umax = fmaxf(umax, u); }
-    };
-    auto cross = [&](float pu, float pv, float qu, float qv) {
-        if (pv == qv) return;
-        const float k = (qu - pu) * __builtin_amdgcn_rcpf(qv - pv);
-        if ((pv - lo) * (qv - lo) <= 0.0f) { const float u = pu + (lo - pv) * k; umin = fminf(umin, u); umax = fmaxf(umax, u); }
-        if ((pv - hi) * (qv - hi) <= 0.0f) { const float u = pu + (hi - pv) * k; umin = fminf(umin, u); umax = fmaxf(umax, u); }
-    };
-    vert(u0, v0); vert(u1, v1); vert(u2, v2);
-    cross(u0, v0, u1, v1); cross(u1, v1, u2, v2); cross(u2, v2, u0, v0);
-    return umin <= umax;
-}
-
-// Shrinks a face's candidate rectangle in its tile (tile-local [rx0, rx1] x
-// [ry0, ry1], origin ox, oy) to the pixels that can pass the inside test; returns false
-// when none can.  Bound used: the computed edge function differs from the exact one by
-// at most 5.7 u L (D + 1) (u = 2^-24, L the edge's length, D the largest coordinate
-// distance between a vertex and a rectangle pixel), so a passing pixel lies within
-// m = 8.01 u (D + 1) of every edge's half-plane — when the area's sign is certain
-// (|area| > 2 m L_max) — hence within g = m (1 + 3 D / r) of the triangle (r its
-// inradius, >= |area| / 2 / perimeter): the offset triangle is the triangle scaled by
-// 1 + m / r about its incentre, and no vertex is farther than 2 sqrt(2) D from it.
-// (Lengths are bounded by |dx| + |dy|; the products' and the subtraction's roundings
-// give 4 u (|A| |py - ay| + |B| |px - ax|) <= 5.7 u L (D + 1).)  So a passing pixel p has
-// a triangle point t with |p - t| <= g: t lies in the slab of rows [Y0 - g, Y1 + g], and
-// px is within g of that slab's part of the triangle.  The slabs are widened to
-// w = g + d and the x (y) range taken as [ceil(min - w), floor(max + w)], where
-// d = 2^-19 (C + 16), C the largest coordinate magnitude, exceeds the error of a computed
-// edge crossing (pu + (lo - pv) (qu - pu) rcp(qv - pv): |error| <= 15.2 u C, the
-// hardware reciprocal within 1 ulp) plus the roundings of the slab bounds and of
-// min - w, max + w.  With SLAB >= 0 the rectangle is first cut to the triangle's own
-// extent, [ceil(v_min - w), floor(v_max + w)] per axis (a point within g of the triangle
-// lies within g of its vertices' range), and only a rectangle still larger than SLAB
-// pixels takes the slab cut.  Faces where g would exceed 4 pixels, or whose doubled area
-// lies outside [2^-40, 2^40], keep their rectangle.
-template <int SLAB = kSlabPixels>
-__device__ __forceinline__ bool shrink_rect(const float4& a, const float4& b, const float4& c, int ox, int oy,
-                                            int& rx0, int& rx1, int& ry0, int& ry1) {
-    const float X0 = (float)(ox + rx0), X1 = (float)(ox + rx1), Y0 = (float)(oy + ry0), Y1 = (float)(oy + ry1);
-    const float vx0 = fminf(fminf(a.x, b.x), c.x), vx1 = fmaxf(fmaxf(a.x, b.x), c.x);
-    const float vy0 = fminf(fminf(a.y, b.y), c.y), vy1 = fmaxf(fmaxf(a.y, b.y), c.y);
-    const float D = fmaxf(fmaxf(vx1 - X0, X1 - vx0), fmaxf(vy1 - Y0, Y1 - vy0));
-    const float area = fabsf(a.w);
-    // (comparisons false for NaN: such faces keep their rectangle)
-    if (!(D <= 1048576.0f) || !(area >= 0x1p-40f) || !(area <= 0x1p40f)) return true;
-    const float l0 = fabsf(b.x - a.x) + fabsf(b.y - a.y), l1 = fabsf(c.x - b.x) + fabsf(c.y - b.y);
-    const float l2 = fabsf(a.x - c.x) + fabsf(a.y - c.y);
-    const float m = 8.01f * 0x1p-24f * (D + 1.0f);
-    if (!(area > 2.0f * m * fmaxf(fmaxf(l0, l1), l2) * 1.01f)) return true;
-    // g = m (1 + 3 D / r) with r = area / (2 P) (P = l0 + l1 + l2 bounds the perimeter):
-    // 6 D P / area through the hardware reciprocal (within 1 ulp), the factor 6.07 and the
-    // final 1.01 covering its and the products' roundings
-    const float g = m * (1.0f + 6.07f * D * (l0 + l1 + l2) * __builtin_amdgcn_rcpf(area)) * 1.01f;
-    if (!(g <= 4.0f)) return true;
-    const float C = fmaxf(fmaxf(fmaxf(fabsf(a.x), fabsf(a.y)), fmaxf(fabsf(b.x), fabsf(b.y))),
-                          fmaxf(fmaxf(fabsf(c.x), fabsf(c.y)), fmaxf(X1, Y1)));
-    const float w = g + 0x1p-19f * (C + 16.0f);
-    if constexpr (SLAB >= 0) {
-        // first the triangle's own extent (its vertices), which is all a small face needs
-        rx0 = max(rx0, (int)ceilf(vx0 - w) - ox); rx1 = min(rx1, (int)floorf(vx1 + w) - ox);
-        ry0 = max(ry0, (int)ceilf(vy0 - w) - oy); ry1 = min(ry1, (int)floorf(vy1 + w) - oy);
-        if (rx0 > rx1 || ry0 > ry1) return false;
-        if ((rx1 - rx0 + 1) * (ry1 - ry0 + 1) <= SLAB) return true;
-    }
-    float xmin, xmax, ymin, ymax;
-    if (!slab_range(a.x, a.y, b.x, b.y, c.x, c.y, Y0 - w, Y1 + w, xmin, xmax)) return false;
-    if (!slab_range(a.y, a.x, b.y, b.x, c.y, c.x, X0 - w, X1 + w, ymin, ymax)) return false;
-    rx0 = max(rx0, (int)ceilf(xmin - w) - ox); rx1 = min(rx1, (int)floorf(xmax + w) - ox);
-    ry0 = max(ry0, (int)ceilf(ymin - w) - oy); ry1 = min(ry1, (int)floorf(ymax + w) - oy);
-    return rx0 <= rx1 && ry0 <= ry1;
-}
-
 // One tile, by one workgroup.  The tile's bin is taken BLOCK faces at a time: the faces
 // whose bbox meets the tile are compacted (ballot + popcount ranks, wave totals in LDS)
 // and their candidate pixel counts (bbox inside the tile) prefix-summed across the
@@ -789,8 +806,8 @@ struct RasterLds {
 };
 
 // EXT_RAW: the scratch LDS (RasterLds bytes) is the caller's `raw`, else the function's own.
-template <int TS, int BLOCK, bool FUSED, int MODE, int SHRINK = kShrinkPixels, bool EXT_RAW = false,
-          int SLAB = kSlabPixels>
+template <int TS, int BLOCK, bool FUSED, int MODE, int SHRINK = kSlabPixels, bool EXT_RAW = false,
+          int SLAB = kSlabAlways>
 __device__ __forceinline__ void raster_body(const RasterParams& p, const ShadeParams* sp, const float* s_lut,
                                             const DevMaterial* s_mats, const uint16_t* s_net, int tile,
                                             uint32_t& frags, uint32_t& cov, unsigned char* raw = nullptr) {
