@@ -399,6 +399,19 @@ rz_status rz_copy_to_host(rz_ctx* ctx, void* dst_host, const void* src_device, s
  * words; any unaligned remainder by hipMemcpy); waits for it. */
 rz_status rz_copy_device(rz_ctx* ctx, void* dst_device, const void* src_device, size_t bytes);
 
+/* ---- self-check ------------------------------------------------------------------- */
+
+/* No counterpart in the reference.  The raster pass computes the barycentric quotients
+ * e / area of most candidates from a per-face reciprocal: the compiler's correctly
+ * rounded division with its scaling and fix-up steps left out, which are identities for
+ * |area| in [2^-40, 2^40] and |e| in [2^-20, 2^40].  This runs that computation against
+ * the IEEE division on about `samples` random operand pairs (every sign, exponents one
+ * beyond each end of the ranges, mantissas random, all ones or zero) on the device and
+ * reports the pairs in range compared and how many differed in any bit (0 expected).
+ * Synchronous; allocates 16 bytes of device memory for the call. */
+rz_status rz_check_quotients(rz_ctx* ctx, uint64_t samples, uint32_t seed, uint64_t* compared,
+                             uint64_t* mismatches);
+
 /* ---- host helpers -------------------------------------------------------------- */
 
 /* Reference defaults of Args (arguments.cpp:20-31) for a width x height image, plus

@@ -329,6 +329,14 @@ class Context:
         _check(self.lib.rz_draw_text(self.handle, C.c_void_p(frame_ptr), int(width), int(height), int(x), int(y),
                                      text.encode(), f, b), self.handle)
 
+    def check_quotients(self, samples: int, seed: int = 0):
+        """rz_check_quotients: (pairs compared, mismatches) of the raster pass's shared-
+        reciprocal quotient against the IEEE division on random in-range operands."""
+        n, bad = C.c_uint64(0), C.c_uint64(0)
+        _check(self.lib.rz_check_quotients(self.handle, int(samples), int(seed) & 0xFFFFFFFF, C.byref(n),
+                                           C.byref(bad)), self.handle)
+        return n.value, bad.value
+
     def _batch_items(self, batch):
         """batch: list of (Mesh, lights (L,10) float32, Args, frame DeviceArray, depth DeviceArray,
         ids DeviceArray or None)."""

@@ -157,6 +157,7 @@ def declare(lib):
         "rz_copy_to_device": (C.c_int, [P, P, P, C.c_size_t]),
         "rz_copy_to_host": (C.c_int, [P, P, P, C.c_size_t]),
         "rz_copy_device": (C.c_int, [P, P, P, C.c_size_t]),
+        "rz_check_quotients": (C.c_int, [P, C.c_uint64, C.c_uint32, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
         "rz_default_args": (None, [C.POINTER(Args), C.c_uint32, C.c_uint32]),
         "rz_frame_matrices": (C.c_int, [C.POINTER(Args), C.POINTER(Matrices)]),
     }
@@ -176,4 +177,4 @@ EXPORTED = ("rz_abi_version", "rz_status_string", "rz_last_error", "rz_create", 
             "rz_time_passes", "rz_mlp_init", "rz_mlp_set_weights", "rz_mlp_get_weights", "rz_mlp_collect",
             "rz_mlp_train", "rz_mlp_infer",
             "rz_device_alloc", "rz_device_free", "rz_copy_to_device", "rz_copy_to_host", "rz_copy_device",
-            "rz_default_args", "rz_frame_matrices")
+            "rz_check_quotients", "rz_default_args", "rz_frame_matrices")

@@ -34,6 +34,8 @@ hipError_t launch_batch_tiles(const RasterParams* items, const ShadeParams* site
                               uint32_t raster_grid, uint32_t tiles_grid, int tile_size, int mode, hipStream_t s,
                               bool raster);
 hipError_t launch_hold(uint64_t ticks, hipStream_t s);
+hipError_t launch_check_quotients(uint32_t blocks, uint32_t per_thread, uint32_t seed, unsigned long long* counts,
+                                  hipStream_t s);
 hipError_t launch_text(uint8_t* frame, uint32_t W, uint32_t H, int32_t x0, int32_t y0, const char* text,
                        const uint8_t fg[3], const uint8_t bg[3], hipStream_t s);
 uint32_t raster_blocks_per_cu(int tile_size, bool fused, int mode, bool narrow);
@@ -2022,6 +2024,33 @@ rz_status rz_draw_text(rz_ctx* ctx, uint8_t* frame_device, uint32_t width, uint3
     if (!ctx || !frame_device || !text || !foreground || !background) return RZ_ERR_INVALID_ARG;
     RZ_HIP(ctx, hipSetDevice(ctx->device));
     RZ_HIP(ctx, launch_text(frame_device, width, height, x, y, text, foreground, background, ctx->stream));
+    return RZ_OK;
+}
+
+rz_status rz_check_quotients(rz_ctx* ctx, uint64_t samples, uint32_t seed, uint64_t* compared,
+                             uint64_t* mismatches) {
+    if (!ctx) return RZ_ERR_INVALID_ARG;
+    if (!compared || !mismatches || samples == 0 || samples > (1ull << 40))
+        return fail(ctx, RZ_ERR_INVALID_ARG, "rz_check_quotients: 1 <= samples <= 2^40 and two outputs required");
+    constexpr uint32_t kPerThread = 1024, kThreads = 256;
+    const uint64_t blocks = (samples + (uint64_t)kPerThread * kThreads - 1) / ((uint64_t)kPerThread * kThreads);
+    RZ_HIP(ctx, hipSetDevice(ctx->device));
+    unsigned long long* counts = nullptr;
+    RZ_HIP(ctx, hipMalloc(&counts, 2 * sizeof(unsigned long long)));
+    unsigned long long h[2] = {0, 0};
+    hipError_t e = hipMemsetAsync(counts, 0, sizeof h, ctx->stream);
+    // grids of at most 65,536 workgroups, each thread its own 1,024 pairs
+    for (uint64_t b0 = 0; e == hipSuccess && b0 < blocks; b0 += 65536) {
+        const uint32_t nb = (uint32_t)std::min<uint64_t>(65536, blocks - b0);
+        e = launch_check_quotients(nb, kPerThread, seed + (uint32_t)(b0 * kThreads * 0x632BE5ABull), counts,
+                                   ctx->stream);
+    }
+    if (e == hipSuccess) e = hipMemcpyAsync(h, counts, sizeof h, hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    (void)hipFree(counts);
+    if (e != hipSuccess) return fail(ctx, RZ_ERR_HIP, "rz_check_quotients: %s", hipGetErrorString(e));
+    *compared = h[0];
+    *mismatches = h[1];
     return RZ_OK;
 }
 

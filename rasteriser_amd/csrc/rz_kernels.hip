@@ -768,12 +768,9 @@ __device__ __forceinline__ bool surely_outside(float e, bool area_pos) {
     return fabsf(e) >= 0x1p-20f && ((e > 0.0f) != area_pos);
 }
 
-__device__ __forceinline__ bool fragment_key(float pxf, float pyf, float x0, float y0, float z0, float x1,
-                                             float y1, float z1, float x2, float y2, float z2, float area,
-                                             float& z) {
-    const float e0 = edge(pxf, pyf, x1, y1, x2, y2);
-    const float e1 = edge(pxf, pyf, x2, y2, x0, y0);
-    const float e2 = edge(pxf, pyf, x0, y0, x1, y1);
+// The general test from the three edge values.
+__device__ __forceinline__ bool key_from_edges(float e0, float e1, float e2, float z0, float z1, float z2,
+                                               float area, float& z) {
     if (area_in_range(area)) {
         const bool ap = area > 0.0f;
         if (surely_outside(e0, ap) || surely_outside(e1, ap) || surely_outside(e2, ap)) return false;
@@ -782,6 +779,72 @@ __device__ __forceinline__ bool fragment_key(float pxf, float pyf, float x0, flo
     if (!(b0 >= 0.0f && b1 >= 0.0f && b2 >= 0.0f)) return false;
     z = (z0 * b0 + z1 * b1) + z2 * b2;
     return true;
+}
+
+__device__ __forceinline__ bool fragment_key(float pxf, float pyf, float x0, float y0, float z0, float x1,
+                                             float y1, float z1, float x2, float y2, float z2, float area,
+                                             float& z) {
+    const float e0 = edge(pxf, pyf, x1, y1, x2, y2);
+    const float e1 = edge(pxf, pyf, x2, y2, x0, y0);
+    const float e2 = edge(pxf, pyf, x0, y0, x1, y1);
+    return key_from_edges(e0, e1, e2, z0, z1, z2, area, z);
+}
+
+// Quotients e / d sharing one divisor d, bit for bit.  The compiler's correctly rounded
+// f32 division is div_scale of both operands, rcp of the scaled divisor, two Newton
+// steps on the reciprocal (t = fma(-d, r, 1), r1 = fma(t, r, r)), then q = n r1,
+// q = fma(fma(-d, q, n), r1, q), div_fmas(fma(-d, q, n), r1, q), div_fixup.  div_scale
+// returns its operand unchanged and clears the flag div_fmas scales by unless an operand
+// is zero, the exponent difference is 96 or more, d or 1 / d is not normal, the
+// quotient is not normal or |n| < 2^-103; div_fixup returns the magnitude it is given
+// with the quotient's sign unless an operand is zero, infinite or NaN or the result
+// under- or overflows.  None of that happens for |d| in [2^-40, 2^40] and |n| in
+// [2^-20, 2^40] (exponent difference at most 80, quotient in [2^-60, 2^80]), where the
+// division is therefore the sequence without them: r1 depends on d alone and is computed
+// once per face (refined_rcp), and each quotient costs one multiply and four FMAs instead
+// of the ten-instruction sequence.  The fused raster's candidates, the overwhelming
+// majority of which fall in that range, use it; the others take the general test.
+__device__ __forceinline__ bool divisor_in_range(float d) {
+    const float a = fabsf(d);
+    return a >= 0x1p-40f && a <= 0x1p40f;
+}
+
+__device__ __forceinline__ bool dividend_in_range(float n) {
+    const float a = fabsf(n);
+    return (a >= 0x1p-20f) & (a <= 0x1p40f);
+}
+
+// The divisor's refined reciprocal, or 0 for a divisor outside divisor_in_range.
+__device__ __forceinline__ float refined_rcp(float d) {
+    if (!divisor_in_range(d)) return 0.0f;
+    const float r = __builtin_amdgcn_rcpf(d);
+    return __builtin_fmaf(__builtin_fmaf(-d, r, 1.0f), r, r);
+}
+
+__device__ __forceinline__ float quotient_in_range(float n, float d, float r1) {
+    float q = n * r1;
+    q = __builtin_fmaf(__builtin_fmaf(-d, q, n), r1, q);
+    return __builtin_fmaf(__builtin_fmaf(-d, q, n), r1, q);
+}
+
+// fragment_key with the face's refined_rcp(area) at hand.  In range, each quotient is a
+// normal number, so the inside test is the edge values' signs against the area's.
+__device__ __forceinline__ bool fragment_key_rcp(float pxf, float pyf, float x0, float y0, float z0, float x1,
+                                                 float y1, float z1, float x2, float y2, float z2, float area,
+                                                 float ra, float& z) {
+    const float e0 = edge(pxf, pyf, x1, y1, x2, y2);
+    const float e1 = edge(pxf, pyf, x2, y2, x0, y0);
+    const float e2 = edge(pxf, pyf, x0, y0, x1, y1);
+    // (non-short-circuit: every operand is a compare, and branches would cost more)
+    if ((ra != 0.0f) & dividend_in_range(e0) & dividend_in_range(e1) & dividend_in_range(e2)) {
+        const bool ap = area > 0.0f;
+        if (((e0 > 0.0f) != ap) | ((e1 > 0.0f) != ap) | ((e2 > 0.0f) != ap)) return false;
+        const float b0 = quotient_in_range(e0, area, ra), b1 = quotient_in_range(e1, area, ra);
+        const float b2 = quotient_in_range(e2, area, ra);
+        z = (z0 * b0 + z1 * b1) + z2 * b2;
+        return true;
+    }
+    return key_from_edges(e0, e1, e2, z0, z1, z2, area, z);
 }
 
 // One tile, by one workgroup.  The tile's bin is taken BLOCK faces at a time: the faces
@@ -800,7 +863,7 @@ __device__ __forceinline__ bool fragment_key(float pxf, float pyf, float x0, flo
 // staging after the raster loop (and, in the batch-frame kernel, by its bin phase).
 template <int BLOCK, int MODE>
 struct RasterLds {
-    static constexpr int kRaster = 13 * BLOCK * 4;
+    static constexpr int kRaster = 14 * BLOCK * 4;
     static constexpr int kMlp = MODE == kShadeMlp ? (BLOCK / 64) * (64 * 16 * 2 + 64 * 4 * 4) : 0;
     static constexpr int kBytes = kRaster > kMlp ? kRaster : kMlp;
 };
@@ -824,8 +887,9 @@ __device__ __forceinline__ void raster_body(const RasterParams& p, const ShadePa
         __shared__ __align__(16) unsigned char s_own[RasterLds<BLOCK, MODE>::kBytes];
         s_raw = s_own;
     }
-    float (*s_v)[BLOCK] = reinterpret_cast<float (*)[BLOCK]>(s_raw);   // x0 y0 z0 x1 y1 z1 x2 y2 z2 area
-    uint32_t* s_rect = reinterpret_cast<uint32_t*>(s_raw + 10 * BLOCK * 4);   // rx0 | ry0 << 8 | rw << 16
+    // x0 y0 z0 x1 y1 z1 x2 y2 z2 area refined_rcp(area)
+    float (*s_v)[BLOCK] = reinterpret_cast<float (*)[BLOCK]>(s_raw);
+    uint32_t* s_rect = reinterpret_cast<uint32_t*>(s_raw + 11 * BLOCK * 4);   // rx0 | ry0 << 8 | rw << 16
     uint32_t* s_face = s_rect + BLOCK;
     uint32_t* s_start = s_face + BLOCK;
     __shared__ uint32_t s_wsum[NW], s_wlive[NW];
@@ -909,7 +973,7 @@ __device__ __forceinline__ void raster_body(const RasterParams& p, const ShadePa
             s_v[0][rank] = a.x; s_v[1][rank] = a.y; s_v[2][rank] = a.z;
             s_v[3][rank] = b.x; s_v[4][rank] = b.y; s_v[5][rank] = b.z;
             s_v[6][rank] = c.x; s_v[7][rank] = c.y; s_v[8][rank] = c.z;
-            s_v[9][rank] = a.w;
+            s_v[9][rank] = a.w; s_v[10][rank] = refined_rcp(a.w);
             s_rect[rank] = rect;
             s_face[rank] = f;
             s_start[rank] = start;
@@ -946,8 +1010,9 @@ __device__ __forceinline__ void raster_body(const RasterParams& p, const ShadePa
                 const uint32_t dx = l - dy * rw;
                 const int lx = (int)(rc & 0xFFu) + (int)dx, ly = (int)((rc >> 8) & 0xFFu) + (int)dy;
                 float z;
-                if (fragment_key((float)(ox + lx), (float)(oy + ly), s_v[0][j], s_v[1][j], s_v[2][j], s_v[3][j],
-                                 s_v[4][j], s_v[5][j], s_v[6][j], s_v[7][j], s_v[8][j], s_v[9][j], z)) {
+                if (fragment_key_rcp((float)(ox + lx), (float)(oy + ly), s_v[0][j], s_v[1][j], s_v[2][j],
+                                     s_v[3][j], s_v[4][j], s_v[5][j], s_v[6][j], s_v[7][j], s_v[8][j], s_v[9][j],
+                                     s_v[10][j], z)) {
                     ++frags;
                     const int pl = ly * TS + lx;
                     if (z < s_d0[pl]) atomicMin(&s_key[pl], vis_key(z, s_face[j]));
@@ -1675,6 +1740,44 @@ hipError_t launch_text(uint8_t* frame, uint32_t W, uint32_t H, int32_t x0, int32
     if (n == 0) return hipSuccess;
     const int threads = 12 * n * 16;
     hipLaunchKernelGGL(k_text, dim3((threads + 255) / 256), dim3(256), 0, s, p);
+    return hipGetLastError();
+}
+
+// Self-check of quotient_in_range: random operands (every sign, exponents one beyond
+// each end of the ranges, mantissas random or all ones or zero), each pair in range
+// compared bit for bit with the IEEE division.  counts[0] += pairs compared,
+// counts[1] += mismatches.
+__device__ __forceinline__ uint32_t mix32(uint32_t x) {
+    x ^= x >> 16; x *= 0x7feb352dU; x ^= x >> 15; x *= 0x846ca68bU; x ^= x >> 16;
+    return x;
+}
+
+__global__ void k_check_quotients(uint32_t per_thread, uint32_t seed, unsigned long long* counts) {
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    unsigned long long tested = 0, bad = 0;
+    for (uint32_t k = 0; k < per_thread; ++k) {
+        const uint32_t h1 = mix32(t * 0x9E3779B9u + k * 0x85EBCA6Bu + seed);
+        const uint32_t h2 = mix32(h1 ^ 0xC2B2AE35u), h3 = mix32(h2 + 0x27D4EB2Fu);
+        const int ed = -41 + (int)(h3 % 83u), en = -21 + (int)((h3 >> 8) % 63u);
+        uint32_t md = h1 & 0x7FFFFFu, mn = h2 & 0x7FFFFFu;
+        if ((h3 >> 16) % 16u == 0) md = 0x7FFFFFu;
+        if ((h3 >> 20) % 16u == 1) mn = 0x7FFFFFu;
+        if ((h3 >> 24) % 16u == 2) md = 0;
+        if ((h3 >> 28) % 16u == 3) mn = 0;
+        const float d = __uint_as_float((h1 & 0x80000000u) | ((uint32_t)(ed + 127) << 23) | md);
+        const float n = __uint_as_float((h2 & 0x80000000u) | ((uint32_t)(en + 127) << 23) | mn);
+        const float r1 = refined_rcp(d);
+        if (r1 == 0.0f || !dividend_in_range(n)) continue;
+        ++tested;
+        bad += __float_as_uint(quotient_in_range(n, d, r1)) != __float_as_uint(n / d) ? 1u : 0u;
+    }
+    atomicAdd(&counts[0], tested);
+    if (bad) atomicAdd(&counts[1], bad);
+}
+
+hipError_t launch_check_quotients(uint32_t blocks, uint32_t per_thread, uint32_t seed, unsigned long long* counts,
+                                  hipStream_t s) {
+    hipLaunchKernelGGL(k_check_quotients, dim3(blocks), dim3(256), 0, s, per_thread, seed, counts);
     return hipGetLastError();
 }
 

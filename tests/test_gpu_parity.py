@@ -106,6 +106,57 @@ def test_screen_soup_small(ctx):
     assert gpu[3]["front_triangles"] > 80_000
 
 
+def _grid_scene(cells, step, origin, jitter, seed):
+    """Screen-space grid of cells x cells quads of `step` pixels at `origin`, two
+    triangles each (half wound backwards), vertices moved by up to `jitter` pixels,
+    depths random: with no jitter every shared edge passes through pixel centres, whose
+    edge values are exactly zero."""
+    import math
+    from rasteriser_amd.abi import make_args
+    from rasteriser_amd.scene import MaterialSpec, Scene, make_faces, make_lights
+    rng = np.random.default_rng(seed)
+    g = np.arange(cells + 1, dtype=np.float64) * step + origin
+    xs, ys = np.meshgrid(g, g, indexing="xy")
+    xs = xs + rng.uniform(-jitter, jitter, xs.shape)
+    ys = ys + rng.uniform(-jitter, jitter, ys.shape)
+    n = (cells + 1) ** 2
+    d = rng.uniform(0.5, 50.0, n)
+    z = 1.0 - 2.0 / d
+    verts = np.stack([xs.ravel(), ys.ravel(), z, 1.0 / d], 1).astype(np.float32)
+    i = np.arange(cells)
+    a = (i[:, None] * (cells + 1) + i[None, :]).ravel()
+    tri = np.concatenate([np.stack([a, a + cells + 1, a + 1], 1), np.stack([a + 1, a + cells + 1, a + cells + 2], 1)])
+    back = rng.uniform(size=len(tri)) < 0.5
+    tri[back, 1], tri[back, 2] = tri[back, 2].copy(), tri[back, 1].copy()
+    faces = make_faces(tri.astype(np.int64), n=np.zeros_like(tri, dtype=np.int64), t=None)
+    size = int(math.ceil(origin + cells * step)) + 2
+    scene = Scene(verts, np.array([[0.0, 0.0, 1.0]], np.float32), np.zeros((1, 2), np.float32), faces,
+                  [MaterialSpec((1.0, 1.0, 1.0))],
+                  make_lights([[0.0, 0.0, -1.0]], [255.0 * math.pi], [[1.0, 1.0, 1.0]]), raster_space=True)
+    return scene, make_args(size, size, fovy=60.0, z_near=0.1, z_far=100.0)
+
+
+@pytest.mark.parametrize("cells,step,origin,jitter", [(64, 8.0, 0.0, 0.0), (96, 5.0, 0.5, 0.0),
+                                                      (16, 1.0, 0.0, 2.0 ** -19), (24, 0.75, 1.0, 2.0 ** -21)])
+def test_edges_through_pixel_centres(ctx, cells, step, origin, jitter):
+    """Candidates whose edge values are zero or near 2^-20 — the edge of the range where
+    the raster pass takes its quotients from the face's reciprocal — take the general
+    test; the frame must still match the oracle bit for bit (ties between faces sharing
+    an edge included)."""
+    scene, args = _grid_scene(cells, step, origin, jitter, seed=cells)
+    for tuning in ((16, 0), (8, 64)):
+        check(ctx, scene, args, tuning=tuning, label=f"grid {cells}x{step} jitter {jitter} tiles {tuning}")
+
+
+def test_shared_reciprocal_quotients(ctx):
+    """The raster pass's quotient from a per-face reciprocal equals the IEEE division
+    bit for bit on 2^30 random operand pairs in its range (and is not used outside it)."""
+    for seed in (1, 2):
+        compared, bad = ctx.check_quotients(1 << 30, seed=seed)
+        assert bad == 0, f"{bad} of {compared} quotients differ"
+        assert compared > 0.9 * (1 << 30)
+
+
 def test_screen_soup_full_4m(ctx):
     """The small-triangle depth stress at its full size: 4M screen-space triangles at
     4096x4096, about half back-facing, bit for bit against the oracle — once with the
