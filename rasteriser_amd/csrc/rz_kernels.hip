@@ -912,14 +912,19 @@ __device__ __forceinline__ void raster_body(const RasterParams& p, const ShadePa
     // the bin pass cleared an overflowed tile's visibility words and the overflow loop
     // fills them (uniform over the workgroup)
     if (count > p.cap) return;
+    // initial depths: the caller's depth buffer, staged in LDS, or 1 everywhere for a
+    // frame without one (uniform; then nothing is staged)
+    const bool has_d0 = p.depth_in != nullptr;
 #pragma unroll
     for (int j = 0; j < PPT; ++j) {
         const int l = tid + j * BLOCK;
-        const uint32_t x = ox + (l % TS), y = oy + (l / TS);
-        float d0 = 1.0f;
-        if (p.depth_in && x < p.W && y < p.H) d0 = p.depth_in[(size_t)y * p.W + x];
         s_key[l] = kEmptyKey;
-        s_d0[l] = d0;
+        if (has_d0) {
+            const uint32_t x = ox + (l % TS), y = oy + (l / TS);
+            float d0 = 1.0f;
+            if (x < p.W && y < p.H) d0 = p.depth_in[(size_t)y * p.W + x];
+            s_d0[l] = d0;
+        }
     }
     __syncthreads();
 
@@ -1015,7 +1020,7 @@ __device__ __forceinline__ void raster_body(const RasterParams& p, const ShadePa
                                      s_v[10][j], z)) {
                     ++frags;
                     const int pl = ly * TS + lx;
-                    if (z < s_d0[pl]) atomicMin(&s_key[pl], vis_key(z, s_face[j]));
+                    if (z < (has_d0 ? s_d0[pl] : 1.0f)) atomicMin(&s_key[pl], vis_key(z, s_face[j]));
                 }
             }
             before += s_wpre[kWords - 1] + (uint32_t)__popcll(s_mask[kWords - 1]);
@@ -1254,7 +1259,13 @@ struct ListWalk {
         first = issue_now(list + (k0 < (uint32_t)p.ntiles ? k0 : 0u));
         n = issue_now(p.list_count + xcd * kCountStride);
     }
-    __device__ __forceinline__ int tile(uint32_t k) const { return (int)(k == k0 ? first : list[k]); }
+    // The entry after k, read now, one tile ahead of its use: the tile that follows then
+    // starts without waiting for it (the read goes out beside the current tile's first
+    // reads, so their wait covers it).  Past the list's end it rereads entry k.
+    __device__ __forceinline__ uint32_t prefetch(uint32_t k) const {
+        const uint32_t kn = k + step;
+        return issue_now(list + (kn < n ? kn : k));
+    }
 };
 
 template <int TS, int BLOCK>
@@ -1262,8 +1273,12 @@ __global__ __launch_bounds__(BLOCK, TS == 32 ? 6 : (BLOCK == 256 ? 7 : 8)) void 
     const ListWalk w(p);
     const bool overflow = overflowed_tiles(p);
     uint32_t frags = 0, cov = 0;
-    for (uint32_t k = w.k0; k < w.n; k += w.step)
-        raster_body<TS, BLOCK, false, kShadePlain>(p, nullptr, nullptr, nullptr, nullptr, w.tile(k), frags, cov);
+    uint32_t t = w.first;
+    for (uint32_t k = w.k0; k < w.n; k += w.step) {
+        const uint32_t t_next = w.prefetch(k);
+        raster_body<TS, BLOCK, false, kShadePlain>(p, nullptr, nullptr, nullptr, nullptr, (int)t, frags, cov);
+        t = t_next;
+    }
     if (overflow)
         frags += overflow_faces(p, blockIdx.x * BLOCK + threadIdx.x, gridDim.x * BLOCK);
     store_block_sum(frags, p.part_frag + blockIdx.x, threadIdx.x, BLOCK);
@@ -1285,8 +1300,12 @@ __global__ __launch_bounds__(BLOCK, MODE == kShadeMlp ? 3 : 6) void k_raster_sha
     mats.store(s_mats, threadIdx.x);
     // (raster_body's first barrier orders these stores before any read)
     uint32_t frags = 0, cov = 0;
-    for (uint32_t k = w.k0; k < w.n; k += w.step)
-        raster_body<TS, BLOCK, true, MODE>(p, &sp, s_lut, s_mats, s_net, w.tile(k), frags, cov);
+    uint32_t t = w.first;
+    for (uint32_t k = w.k0; k < w.n; k += w.step) {
+        const uint32_t t_next = w.prefetch(k);
+        raster_body<TS, BLOCK, true, MODE>(p, &sp, s_lut, s_mats, s_net, (int)t, frags, cov);
+        t = t_next;
+    }
     if (overflow)
         frags += overflow_faces(p, blockIdx.x * BLOCK + threadIdx.x, gridDim.x * BLOCK);
     __syncthreads();
