@@ -36,9 +36,31 @@ namespace rz {
 
 __device__ __forceinline__ int lane_id() { return threadIdx.x & 63; }
 
+// A read the compiler issues where it is written, ahead of the caller's staging loads,
+// instead of sinking it to its first use (a relaxed atomic load: no ordering, no wait).
+template <typename T>
+__device__ __forceinline__ T issue_now(const T* q) {
+    return __hip_atomic_load(const_cast<T*>(q), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 __device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
     for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, 64);
     return v;
+}
+
+// Inclusive prefix sum of v over the wave's 64 lanes, all of them active: shifts within
+// each row of 16 lanes (a lane the shift would take past its row's start adds 0), then
+// the totals of the rows below through the row broadcasts (lane 15 into rows 1 and 3,
+// lane 31 into rows 2 and 3).  Six DPP adds, no LDS round trip.
+__device__ __forceinline__ uint32_t wave_scan_incl(uint32_t v) {
+    int x = (int)v;
+    x += __builtin_amdgcn_update_dpp(0, x, 0x111, 0xf, 0xf, true);    // row_shr:1
+    x += __builtin_amdgcn_update_dpp(0, x, 0x112, 0xf, 0xf, true);    // row_shr:2
+    x += __builtin_amdgcn_update_dpp(0, x, 0x114, 0xf, 0xf, true);    // row_shr:4
+    x += __builtin_amdgcn_update_dpp(0, x, 0x118, 0xf, 0xf, true);    // row_shr:8
+    x += __builtin_amdgcn_update_dpp(0, x, 0x142, 0xa, 0xf, false);   // row_bcast:15
+    x += __builtin_amdgcn_update_dpp(0, x, 0x143, 0xc, 0xf, false);   // row_bcast:31
+    return (uint32_t)x;
 }
 
 // Sum of v over the workgroup, stored by thread 0 (no atomics).  Every thread of the
@@ -859,6 +881,22 @@ __device__ __forceinline__ bool fragment_key_rcp(float pxf, float pyf, float x0,
 // deferred shade, without the round trip of the visibility buffer through HBM).  Tiles
 // whose bin overflowed are skipped: the overflow loop fills their visibility words and
 // k_shade_tiles shades them.
+// A tile's face count and the first BLOCK entries of its bin (thread tid's one), read
+// together: what raster_body needs before anything else.  An entry past the count is
+// never used; every index below the capacity is inside the bin.
+struct TileHead {
+    uint32_t count, f_first;
+};
+// z: 0, or an opaque copy of 0 that keeps the count in a vector register until its use
+// (a uniform load result is otherwise moved to a scalar register, and waited for, where
+// it is read).
+__device__ __forceinline__ TileHead tile_head(const RasterParams& p, int tile, int tid, uint32_t z = 0u) {
+    TileHead h;
+    h.f_first = issue_now(p.bins + (size_t)tile * p.cap + ((uint32_t)tid < p.cap ? (uint32_t)tid : 0u));
+    h.count = issue_now(p.tile_count + (size_t)tile * kCountStride + z);
+    return h;
+}
+
 // Bytes of raster_body's scratch LDS: face data by rank, reused by the learned shader's
 // staging after the raster loop (and, in the batch-frame kernel, by its bin phase).
 template <int BLOCK, int MODE>
@@ -873,7 +911,8 @@ template <int TS, int BLOCK, bool FUSED, int MODE, int SHRINK = kSlabPixels, boo
           int SLAB = kSlabAlways>
 __device__ __forceinline__ void raster_body(const RasterParams& p, const ShadeParams* sp, const float* s_lut,
                                             const DevMaterial* s_mats, const uint16_t* s_net, int tile,
-                                            uint32_t& frags, uint32_t& cov, unsigned char* raw = nullptr) {
+                                            const TileHead& head, uint32_t& frags, uint32_t& cov,
+                                            unsigned char* raw = nullptr) {
     constexpr int NPX = TS * TS;
     constexpr int PPT = NPX / BLOCK;
     constexpr int NW = BLOCK / 64;
@@ -901,13 +940,13 @@ __device__ __forceinline__ void raster_body(const RasterParams& p, const ShadePa
     int tid = threadIdx.x;
     asm volatile("" : "+v"(tid));
     const int lane = tid & 63, wave = tid >> 6;
+    tile = __builtin_amdgcn_readfirstlane(tile);   // (uniform; read ahead in a vector register)
     const int tx = tile % p.tiles_x, ty = tile / p.tiles_x;
     const int ox = tx * TS, oy = ty * TS;
 
-    // the first chunk's bin entries are read beside the count, not after it (entries past
-    // the count are never used; every index below the capacity is inside the bin)
-    const uint32_t f_first = p.bins[(size_t)tile * p.cap + ((uint32_t)tid < p.cap ? (uint32_t)tid : 0u)];
-    const uint32_t count = p.tile_count[(size_t)tile * kCountStride];
+    // the first chunk's bin entries, read beside the count (tile_head)
+    const uint32_t f_first = head.f_first;
+    const uint32_t count = __builtin_amdgcn_readfirstlane(head.count);
     const uint32_t n = count;
     // the bin pass cleared an overflowed tile's visibility words and the overflow loop
     // fills them (uniform over the workgroup)
@@ -954,12 +993,7 @@ __device__ __forceinline__ void raster_body(const RasterParams& p, const ShadePa
         }
         // workgroup-wide exclusive prefix sums of the candidate counts and of the live faces
         const unsigned long long live = __ballot(cnt > 0);
-        uint32_t v = cnt;
-#pragma unroll
-        for (int d = 1; d < 64; d <<= 1) {
-            const uint32_t t = __shfl_up(v, d, 64);
-            if (lane >= d) v += t;
-        }
+        const uint32_t v = wave_scan_incl(cnt);
         if (lane == 63) {
             s_wsum[wave] = v;
             s_wlive[wave] = (uint32_t)__popcll(live);
@@ -992,13 +1026,7 @@ __device__ __forceinline__ void raster_body(const RasterParams& p, const ShadePa
             __syncthreads();
             if (wave == 0) {
                 const uint32_t pc = (uint32_t)__popcll(s_mask[lane]);
-                uint32_t e = pc;
-#pragma unroll
-                for (int d = 1; d < 64; d <<= 1) {
-                    const uint32_t t = __shfl_up(e, d, 64);
-                    if (lane >= d) e += t;
-                }
-                s_wpre[lane] = e - pc;
+                s_wpre[lane] = wave_scan_incl(pc) - pc;
             }
             __syncthreads();
             const uint32_t r1 = min(total, r0 + kWords * 64u);
@@ -1236,12 +1264,6 @@ __global__ __launch_bounds__(kFrameBinBlock) void k_bin(BinParams p) {
 // below the list's count.  The first entry is read beside the count rather than after
 // it, one dependent memory round trip less before the first tile starts (an entry past
 // the count is never used; k0 < ntiles keeps the read inside the list).
-// A read the compiler issues where it is written, ahead of the caller's staging loads,
-// instead of sinking it to its first use (a relaxed atomic load: no ordering, no wait).
-template <typename T>
-__device__ __forceinline__ T issue_now(const T* q) {
-    return __hip_atomic_load(const_cast<T*>(q), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
 // Whether the bin pass overflowed a tile (the count's low word: fewer than 2^32 tiles),
 // read early like the list, held as a uniform flag.
 __device__ __forceinline__ bool overflowed_tiles(const RasterParams& p) {
@@ -1251,6 +1273,7 @@ __device__ __forceinline__ bool overflowed_tiles(const RasterParams& p) {
 struct ListWalk {
     const uint32_t* list;
     uint32_t k0, step, n, first;
+    uint32_t z;   // an opaque copy of 0 (tile_head)
     __device__ __forceinline__ explicit ListWalk(const RasterParams& p) {
         const uint32_t xcd = blockIdx.x & 7u;
         list = p.lists + (size_t)xcd * p.ntiles;
@@ -1258,27 +1281,48 @@ struct ListWalk {
         step = gridDim.x >> 3;
         first = issue_now(list + (k0 < (uint32_t)p.ntiles ? k0 : 0u));
         n = issue_now(p.list_count + xcd * kCountStride);
+        z = 0u;
+        asm volatile("" : "+v"(z));
     }
-    // The entry after k, read now, one tile ahead of its use: the tile that follows then
-    // starts without waiting for it (the read goes out beside the current tile's first
-    // reads, so their wait covers it).  Past the list's end it rereads entry k.
-    __device__ __forceinline__ uint32_t prefetch(uint32_t k) const {
-        const uint32_t kn = k + step;
-        return issue_now(list + (kn < n ? kn : k));
-    }
+    // Entry k, read now into a vector register; past the list's end, entry k0 (read only
+    // where k0 < n).
+    __device__ __forceinline__ uint32_t entry(uint32_t k) const { return issue_now(list + (k < n ? k : k0) + z); }
 };
+
+// A workgroup's tiles in list order, their reads pipelined: while tile k is rasterised,
+// the head (count, first bin entries) of tile k + step and the list entry of tile
+// k + 2 step are in flight, so a tile starts with its head in registers and its first
+// wait is for its face records alone.  (The reads go out beside the current tile's own
+// first reads, whose wait then covers them.)
+template <typename Body>
+__device__ __forceinline__ void walk_tiles(const RasterParams& p, const ListWalk& w, Body body) {
+    if (w.k0 >= w.n) return;
+    const int tid = threadIdx.x;
+    uint32_t t = w.first;
+    TileHead h = tile_head(p, (int)t, tid);
+    uint32_t t1 = w.k0 + w.step < w.n ? w.entry(w.k0 + w.step) : t;
+    for (uint32_t k = w.k0; k < w.n; k += w.step) {
+        // (nothing is read ahead for a workgroup's last tile, the only one of most
+        // workgroups in a frame of few tiles)
+        uint32_t t2 = t1;
+        TileHead h1 = h;
+        if (k + w.step < w.n) {
+            t2 = w.entry(k + 2 * w.step);
+            h1 = tile_head(p, (int)t1, tid, w.z);
+        }
+        body((int)t, h);
+        t = t1; h = h1; t1 = t2;
+    }
+}
 
 template <int TS, int BLOCK>
 __global__ __launch_bounds__(BLOCK, TS == 32 ? 6 : (BLOCK == 256 ? 7 : 8)) void k_raster(RasterParams p) {
     const ListWalk w(p);
     const bool overflow = overflowed_tiles(p);
     uint32_t frags = 0, cov = 0;
-    uint32_t t = w.first;
-    for (uint32_t k = w.k0; k < w.n; k += w.step) {
-        const uint32_t t_next = w.prefetch(k);
-        raster_body<TS, BLOCK, false, kShadePlain>(p, nullptr, nullptr, nullptr, nullptr, (int)t, frags, cov);
-        t = t_next;
-    }
+    walk_tiles(p, w, [&](int tile, const TileHead& h) {
+        raster_body<TS, BLOCK, false, kShadePlain>(p, nullptr, nullptr, nullptr, nullptr, tile, h, frags, cov);
+    });
     if (overflow)
         frags += overflow_faces(p, blockIdx.x * BLOCK + threadIdx.x, gridDim.x * BLOCK);
     store_block_sum(frags, p.part_frag + blockIdx.x, threadIdx.x, BLOCK);
@@ -1300,12 +1344,9 @@ __global__ __launch_bounds__(BLOCK, MODE == kShadeMlp ? 3 : 6) void k_raster_sha
     mats.store(s_mats, threadIdx.x);
     // (raster_body's first barrier orders these stores before any read)
     uint32_t frags = 0, cov = 0;
-    uint32_t t = w.first;
-    for (uint32_t k = w.k0; k < w.n; k += w.step) {
-        const uint32_t t_next = w.prefetch(k);
-        raster_body<TS, BLOCK, true, MODE>(p, &sp, s_lut, s_mats, s_net, (int)t, frags, cov);
-        t = t_next;
-    }
+    walk_tiles(p, w, [&](int tile, const TileHead& h) {
+        raster_body<TS, BLOCK, true, MODE>(p, &sp, s_lut, s_mats, s_net, tile, h, frags, cov);
+    });
     if (overflow)
         frags += overflow_faces(p, blockIdx.x * BLOCK + threadIdx.x, gridDim.x * BLOCK);
     __syncthreads();
@@ -1388,9 +1429,10 @@ __global__ __launch_bounds__(BLOCK, MODE == kShadeMlp ? 3 : 6) void k_raster_sha
             if (k >= n) break;
             const uint32_t e = bl.lists[(size_t)x * bl.stride + k];
             const uint32_t m = e >> kListMeshShift;
-            raster_body<TS, BLOCK, true, MODE, kShrinkPixels, false, kSlabAlways>(items[m], &sitems[m], sitems[m].luts,
-                                                                                 nullptr, s_net,
-                                               (int)(e & kListTileMask), frags, cov);
+            const int tile = (int)(e & kListTileMask);
+            raster_body<TS, BLOCK, true, MODE, kShrinkPixels, false, kSlabAlways>(
+                items[m], &sitems[m], sitems[m].luts, nullptr, s_net, tile, tile_head(items[m], tile, threadIdx.x),
+                frags, cov);
         }
     }
     __syncthreads();
@@ -1554,9 +1596,10 @@ __global__ __launch_bounds__(BLOCK, MODE == kShadeMlp ? 3 : 4) void k_batch_fram
             if (k >= n) break;
             const uint32_t e = bl.lists[(size_t)x * bl.stride + k];
             const uint32_t m = e >> kListMeshShift;
-            raster_body<TS, BLOCK, true, MODE, kShrinkPixels, true, kSlabAlways>(ritems[gen * nm + m], &sitems[m],
-                                                                          sitems[m].luts, nullptr, s_net,
-                                                                          (int)(e & kListTileMask), frags, cov, s_lds);
+            const int tile = (int)(e & kListTileMask);
+            raster_body<TS, BLOCK, true, MODE, kShrinkPixels, true, kSlabAlways>(
+                ritems[gen * nm + m], &sitems[m], sitems[m].luts, nullptr, s_net, tile,
+                tile_head(ritems[gen * nm + m], tile, threadIdx.x), frags, cov, s_lds);
         }
     }
     phase_clock(q, 1, false);
@@ -1839,9 +1882,9 @@ uint32_t raster_blocks_per_cu(int tile_size, bool fused, int mode, bool narrow) 
         default: RZ_PICK(32, 256) break;
     }
 #undef RZ_PICK
-    int n = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, f, block, 0) != hipSuccess || n < 1) n = 1;
-    return (uint32_t)n;
+    // (the walk over the tile lists is a static stride: a workgroup beyond residency
+    // starts its tiles only when another workgroup ends, doubling the launch's tail)
+    return resident_blocks_per_cu(f, block);
 }
 
 hipError_t launch_raster(const RasterParams& p, int tile_size, hipStream_t s) {
