@@ -1657,7 +1657,11 @@ __global__ __launch_bounds__(BLOCK, MODE == kShadeMlp ? 3 : 4) void k_batch_fram
 // (A workgroup of 7,896 LDS bytes was counted 20 to a CU, 160 KiB once rounded to
 // 512-byte granules, and fewer were resident.)  A smaller grid is only slower; a grid
 // beyond residency leaves a barrier waiting on workgroups that cannot start.
-uint32_t resident_blocks_per_cu(const void* f, int block) {
+// lds_budget: the LDS bytes per CU counted on (the single-frame raster passes, whose
+// workgroups never wait on each other, count on 156 KiB: the 4M-triangle raster-only
+// pass with 7,496-byte workgroups measured 84.3, 81.9, 80.5, 79.1, 80.6 and 103.7 µs at
+// 16, 18, 19, 20, 21 and 22 workgroups per CU, the last beyond residency).
+uint32_t resident_blocks_per_cu(const void* f, int block, size_t lds_budget = (size_t)163840 * 7 / 8) {
     int n = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, f, block, 0) != hipSuccess || n < 1) n = 1;
     hipFuncAttributes at{};
@@ -1667,7 +1671,7 @@ uint32_t resident_blocks_per_cu(const void* f, int block) {
     const int waves_per_block = (block + 63) / 64;
     const int by_waves = 4 * waves_per_simd / waves_per_block;
     const size_t lds = ((size_t)at.sharedSizeBytes + 511) / 512 * 512;
-    const int by_lds = lds > 0 ? (int)((size_t)163840 * 7 / 8 / lds) : by_waves;
+    const int by_lds = lds > 0 ? (int)(lds_budget / lds) : by_waves;
     n = std::min(n, std::min(by_waves, by_lds));
     return (uint32_t)(n < 1 ? 1 : n);
 }
@@ -1884,7 +1888,7 @@ uint32_t raster_blocks_per_cu(int tile_size, bool fused, int mode, bool narrow) 
 #undef RZ_PICK
     // (the walk over the tile lists is a static stride: a workgroup beyond residency
     // starts its tiles only when another workgroup ends, doubling the launch's tail)
-    return resident_blocks_per_cu(f, block);
+    return resident_blocks_per_cu(f, block, (size_t)163840 - 4096);
 }
 
 hipError_t launch_raster(const RasterParams& p, int tile_size, hipStream_t s) {
