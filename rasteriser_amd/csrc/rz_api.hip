@@ -38,7 +38,7 @@ hipError_t launch_check_quotients(uint32_t blocks, uint32_t per_thread, uint32_t
                                   hipStream_t s);
 hipError_t launch_text(uint8_t* frame, uint32_t W, uint32_t H, int32_t x0, int32_t y0, const char* text,
                        const uint8_t fg[3], const uint8_t bg[3], hipStream_t s);
-uint32_t raster_blocks_per_cu(int tile_size, bool fused, int mode, bool narrow);
+uint32_t raster_blocks_per_cu(int tile_size, bool fused, int mode, bool narrow, bool depth_in);
 hipError_t launch_copy(const void* src, void* dst, size_t n16, hipStream_t s);
 hipError_t launch_mlp_pack(const float* W, uint16_t* packed, hipStream_t s);
 hipError_t launch_mlp_infer(const float* x, int64_t n, const uint16_t* packed, float* y, hipStream_t s);
@@ -544,11 +544,15 @@ int frame_bin_cap(const rz_ctx* ctx, const rz_mesh* mesh, uint32_t ntiles) {
 // covered tile is in flight at once (1,490 tiles, 1,536 workgroups); with many more
 // tiles than resident workgroups, 64-thread workgroups keep four times as many tiles
 // in flight (1080p: 23 against 63 us; 4M triangles at 4096x4096: 326 against 254 us)
-void frame_shape(const rz_ctx* ctx, const rz_args& a, int mode, uint32_t& ntiles, bool& narrow, uint32_t& grid) {
+// flags: the frame's; without RZ_FLAG_CLEARED the raster pass stages the input depths
+// in dynamic LDS, which the residency count includes.
+void frame_shape(const rz_ctx* ctx, const rz_args& a, int mode, uint32_t flags, uint32_t& ntiles, bool& narrow,
+                 uint32_t& grid) {
     ntiles = ((a.image_width + ctx->tile_size - 1) / ctx->tile_size) *
              ((a.image_height + ctx->tile_size - 1) / ctx->tile_size);
     narrow = ctx->tile_size == 16 && ntiles >= kNarrowTiles;
-    const uint32_t per_cu = raster_blocks_per_cu(ctx->tile_size, ctx->fused, mode, narrow);
+    const uint32_t per_cu =
+        raster_blocks_per_cu(ctx->tile_size, ctx->fused, mode, narrow, (flags & RZ_FLAG_CLEARED) == 0);
     grid = (uint32_t)ctx->cus * per_cu / 8u * 8u;
     if (grid < 8) grid = 8;
 }
@@ -577,7 +581,7 @@ rz_status prepare_frame(rz_ctx* ctx, const rz_mesh* mesh, rz_light* lights, int3
     if (mode < 0) mode = (flags & RZ_FLAG_MLP_SHADE) ? kShadeMlp : kShadePlain;
     uint32_t ntiles = 0, grid = 0;
     bool narrow = false;
-    frame_shape(ctx, *args, mode, ntiles, narrow, grid);
+    frame_shape(ctx, *args, mode, flags, ntiles, narrow, grid);
     const int cap = frame_bin_cap(ctx, mesh, ntiles);
     const bool can_overflow = (uint32_t)cap < (uint32_t)mesh->nf;
     const FrameLayout lay = layout_frame(mesh, *args, ctx->tile_size, cap, ctx->fused, grid,
@@ -2366,7 +2370,7 @@ rz_status rz_draw_frame_noalloc(rz_ctx* ctx, const rz_mesh* mesh, rz_light* ligh
     RZ_HIP(ctx, hipSetDevice(ctx->device));
     uint32_t ntiles = 0, grid = 0;
     bool narrow = false;
-    frame_shape(ctx, *args, kShadePlain, ntiles, narrow, grid);
+    frame_shape(ctx, *args, kShadePlain, flags, ntiles, narrow, grid);
     // Working set in `scratch`: the layout of rz_draw_frame with a bin capacity that fits.
     // With the fused pipeline and room for every face in every bin no tile can overflow
     // and the visibility buffer is never touched, so it takes no space.

@@ -907,8 +907,10 @@ struct RasterLds {
 };
 
 // EXT_RAW: the scratch LDS (RasterLds bytes) is the caller's `raw`, else the function's own.
+// DYN_D0: the initial depths live in the launch's dynamic LDS (TS * TS floats when the
+// frame has an input depth buffer, none otherwise), not in a static array.
 template <int TS, int BLOCK, bool FUSED, int MODE, int SHRINK = kSlabPixels, bool EXT_RAW = false,
-          int SLAB = kSlabAlways>
+          int SLAB = kSlabAlways, bool DYN_D0 = false>
 __device__ __forceinline__ void raster_body(const RasterParams& p, const ShadeParams* sp, const float* s_lut,
                                             const DevMaterial* s_mats, const uint16_t* s_net, int tile,
                                             const TileHead& head, uint32_t& frags, uint32_t& cov,
@@ -920,7 +922,14 @@ __device__ __forceinline__ void raster_body(const RasterParams& p, const ShadePa
     static_assert(PPT >= 1 && NPX % BLOCK == 0, "tile must cover the block");
     constexpr bool kNet = FUSED && MODE == kShadeMlp;
     __shared__ unsigned long long s_key[NPX];
-    __shared__ float s_d0[NPX];
+    float* s_d0;
+    if constexpr (DYN_D0) {
+        extern __shared__ float rz_dyn_d0[];
+        s_d0 = rz_dyn_d0;
+    } else {
+        __shared__ float s_d0_static[NPX];
+        s_d0 = s_d0_static;
+    }
     unsigned char* s_raw = raw;
     if constexpr (!EXT_RAW) {
         __shared__ __align__(16) unsigned char s_own[RasterLds<BLOCK, MODE>::kBytes];
@@ -1108,11 +1117,19 @@ __device__ __forceinline__ void raster_body(const RasterParams& p, const ShadePa
             cov += hit ? 1u : 0u;
         }
     } else {
+        // the tile's pixels by rows: one address, then BLOCK / TS rows further per pass;
+        // the bounds test only in a tile at the image's right or bottom edge (uniform)
+        constexpr int kRows = BLOCK / TS;
+        const uint32_t x = ox + (tid % TS), y = oy + (tid / TS);
+        unsigned long long* dst = p.vis + (size_t)y * p.W + x;
+        const size_t stride = (size_t)kRows * p.W;
+        if ((uint32_t)ox + TS <= p.W && (uint32_t)oy + TS <= p.H) {
 #pragma unroll
-        for (int j = 0; j < PPT; ++j) {
-            const int l = tid + j * BLOCK;
-            const uint32_t x = ox + (l % TS), y = oy + (l / TS);
-            if (x < p.W && y < p.H) p.vis[(size_t)y * p.W + x] = s_key[l];
+            for (int j = 0; j < PPT; ++j) dst[j * stride] = s_key[tid + j * BLOCK];
+        } else {
+#pragma unroll
+            for (int j = 0; j < PPT; ++j)
+                if (x < p.W && y + j * kRows < p.H) dst[j * stride] = s_key[tid + j * BLOCK];
         }
     }
     __syncthreads();   // LDS is reused by the workgroup's next tile
@@ -1321,7 +1338,8 @@ __global__ __launch_bounds__(BLOCK, TS == 32 ? 6 : (BLOCK == 256 ? 7 : 8)) void 
     const bool overflow = overflowed_tiles(p);
     uint32_t frags = 0, cov = 0;
     walk_tiles(p, w, [&](int tile, const TileHead& h) {
-        raster_body<TS, BLOCK, false, kShadePlain>(p, nullptr, nullptr, nullptr, nullptr, tile, h, frags, cov);
+        raster_body<TS, BLOCK, false, kShadePlain, kSlabPixels, false, kSlabAlways, true>(
+            p, nullptr, nullptr, nullptr, nullptr, tile, h, frags, cov);
     });
     if (overflow)
         frags += overflow_faces(p, blockIdx.x * BLOCK + threadIdx.x, gridDim.x * BLOCK);
@@ -1345,7 +1363,8 @@ __global__ __launch_bounds__(BLOCK, MODE == kShadeMlp ? 3 : 6) void k_raster_sha
     // (raster_body's first barrier orders these stores before any read)
     uint32_t frags = 0, cov = 0;
     walk_tiles(p, w, [&](int tile, const TileHead& h) {
-        raster_body<TS, BLOCK, true, MODE>(p, &sp, s_lut, s_mats, s_net, tile, h, frags, cov);
+        raster_body<TS, BLOCK, true, MODE, kSlabPixels, false, kSlabAlways, true>(p, &sp, s_lut, s_mats, s_net,
+                                                                                  tile, h, frags, cov);
     });
     if (overflow)
         frags += overflow_faces(p, blockIdx.x * BLOCK + threadIdx.x, gridDim.x * BLOCK);
@@ -1661,16 +1680,18 @@ __global__ __launch_bounds__(BLOCK, MODE == kShadeMlp ? 3 : 4) void k_batch_fram
 // workgroups never wait on each other, count on 156 KiB: the 4M-triangle raster-only
 // pass with 7,496-byte workgroups measured 84.3, 81.9, 80.5, 79.1, 80.6 and 103.7 µs at
 // 16, 18, 19, 20, 21 and 22 workgroups per CU, the last beyond residency).
-uint32_t resident_blocks_per_cu(const void* f, int block, size_t lds_budget = (size_t)163840 * 7 / 8) {
+// dyn_lds: the launch's dynamic LDS bytes.
+uint32_t resident_blocks_per_cu(const void* f, int block, size_t lds_budget = (size_t)163840 * 7 / 8,
+                                size_t dyn_lds = 0) {
     int n = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, f, block, 0) != hipSuccess || n < 1) n = 1;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, f, block, dyn_lds) != hipSuccess || n < 1) n = 1;
     hipFuncAttributes at{};
     if (hipFuncGetAttributes(&at, f) != hipSuccess) return 1;
     const int vgprs = ((at.numRegs > 0 ? at.numRegs : 512) + 7) / 8 * 8;
     const int waves_per_simd = std::min(8, 512 / vgprs);
     const int waves_per_block = (block + 63) / 64;
     const int by_waves = 4 * waves_per_simd / waves_per_block;
-    const size_t lds = ((size_t)at.sharedSizeBytes + 511) / 512 * 512;
+    const size_t lds = ((size_t)at.sharedSizeBytes + dyn_lds + 511) / 512 * 512;
     const int by_lds = lds > 0 ? (int)(lds_budget / lds) : by_waves;
     n = std::min(n, std::min(by_waves, by_lds));
     return (uint32_t)(n < 1 ? 1 : n);
@@ -1859,7 +1880,8 @@ hipError_t launch_bin(const BinParams& p, hipStream_t s) {
 }
 
 // Resident workgroups per CU of the list-driven raster kernel for this configuration.
-uint32_t raster_blocks_per_cu(int tile_size, bool fused, int mode, bool narrow) {
+// depth_in: the frame stages an input depth buffer (the kernels' dynamic LDS).
+uint32_t raster_blocks_per_cu(int tile_size, bool fused, int mode, bool narrow, bool depth_in) {
     const void* f = nullptr;
     int block = 256;
 #define RZ_PICK(TS, B)                                                                              \
@@ -1888,20 +1910,32 @@ uint32_t raster_blocks_per_cu(int tile_size, bool fused, int mode, bool narrow) 
 #undef RZ_PICK
     // (the walk over the tile lists is a static stride: a workgroup beyond residency
     // starts its tiles only when another workgroup ends, doubling the launch's tail)
-    return resident_blocks_per_cu(f, block, (size_t)163840 - 4096);
+    const uint32_t n = resident_blocks_per_cu(f, block, (size_t)163840 - 4096,
+                                              depth_in ? (size_t)tile_size * tile_size * sizeof(float) : 0u);
+    // One-wave workgroups: at most 20 to a CU, whatever the registers and LDS allow (the
+    // 4M-triangle raster-only pass, 6,472 LDS bytes and 71 VGPRs per workgroup, measured
+    // 81.7, 79.1, 80.1, 103.6, 102.8 and 99.8 µs at 18, 20, 21, 22, 23 and 24 per CU; the
+    // fused pass 161.5, 155.8, 155.4, 199.2 at 18, 20, 21, 22).
+    return block == 64 && n > 20u ? 20u : n;
+}
+
+// The initial depths' dynamic LDS of a frame with an input depth buffer.
+__host__ __forceinline__ size_t d0_lds_bytes(const RasterParams& p, int tile_size) {
+    return p.depth_in ? (size_t)tile_size * tile_size * sizeof(float) : 0u;
 }
 
 hipError_t launch_raster(const RasterParams& p, int tile_size, hipStream_t s) {
     const uint32_t grid = p.max_blocks;
+    const size_t dyn = d0_lds_bytes(p, tile_size);
     switch (tile_size) {
-        case 8: hipLaunchKernelGGL((k_raster<8, 64>), dim3(grid), dim3(64), 0, s, p); break;
+        case 8: hipLaunchKernelGGL((k_raster<8, 64>), dim3(grid), dim3(64), dyn, s, p); break;
         case 16:
             if (p.narrow)
-                hipLaunchKernelGGL((k_raster<16, 64>), dim3(grid), dim3(64), 0, s, p);
+                hipLaunchKernelGGL((k_raster<16, 64>), dim3(grid), dim3(64), dyn, s, p);
             else
-                hipLaunchKernelGGL((k_raster<16, 256>), dim3(grid), dim3(256), 0, s, p);
+                hipLaunchKernelGGL((k_raster<16, 256>), dim3(grid), dim3(256), dyn, s, p);
             break;
-        case 32: hipLaunchKernelGGL((k_raster<32, 256>), dim3(grid), dim3(256), 0, s, p); break;
+        case 32: hipLaunchKernelGGL((k_raster<32, 256>), dim3(grid), dim3(256), dyn, s, p); break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
@@ -1909,23 +1943,24 @@ hipError_t launch_raster(const RasterParams& p, int tile_size, hipStream_t s) {
 
 template <int MODE>
 hipError_t launch_raster_shade_mode(const RasterParams& p, const ShadeParams& sp, int tile_size, hipStream_t s) {
+    const size_t dyn = d0_lds_bytes(p, tile_size);
     switch (tile_size) {
         case 8: {
             const uint32_t grid = p.max_blocks;
-            hipLaunchKernelGGL((k_raster_shade<8, 64, MODE>), dim3(grid), dim3(64), 0, s, p, sp);
+            hipLaunchKernelGGL((k_raster_shade<8, 64, MODE>), dim3(grid), dim3(64), dyn, s, p, sp);
             break;
         }
         case 16: {
             const uint32_t grid = p.max_blocks;
             if (p.narrow)
-                hipLaunchKernelGGL((k_raster_shade<16, 64, MODE>), dim3(grid), dim3(64), 0, s, p, sp);
+                hipLaunchKernelGGL((k_raster_shade<16, 64, MODE>), dim3(grid), dim3(64), dyn, s, p, sp);
             else
-                hipLaunchKernelGGL((k_raster_shade<16, 256, MODE>), dim3(grid), dim3(256), 0, s, p, sp);
+                hipLaunchKernelGGL((k_raster_shade<16, 256, MODE>), dim3(grid), dim3(256), dyn, s, p, sp);
             break;
         }
         case 32: {
             const uint32_t grid = p.max_blocks;
-            hipLaunchKernelGGL((k_raster_shade<32, 256, MODE>), dim3(grid), dim3(256), 0, s, p, sp);
+            hipLaunchKernelGGL((k_raster_shade<32, 256, MODE>), dim3(grid), dim3(256), dyn, s, p, sp);
             break;
         }
         default: return hipErrorInvalidValue;
