@@ -112,12 +112,33 @@ def traffic_bytes(workload, kernel):
     return None
 
 
+def valu_time_us(workload, kernel):
+    """Vector-unit time per launch of `kernel` on `workload` (its vector instructions x 4
+    cycles over 1,024 SIMDs at 2.4 GHz) from the committed counter summary
+    profiles/sq_counters_<workload>.md, or None."""
+    path = os.path.join(ROOT, "profiles", f"sq_counters_{workload}.md")
+    try:
+        with open(path) as f:
+            for line in f:
+                parts = line.split()
+                if len(parts) == 3 and parts[0] == "valu_time_us_per_launch" and parts[1] == kernel:
+                    return float(parts[2])
+    except OSError:
+        pass
+    return None
+
+
 def roofline(table, workload):
+    """The dominant pass against HBM, and beside it the bound it actually meets: the share
+    of its launch the vector units would need to issue its instructions back to back."""
     dom = max(table, key=lambda p: table[p]["avg_us"])
     r = table[dom]
+    vt = valu_time_us(workload, r["kernel"])
+    issue = None if vt is None else {"valu_time_us": vt, "frac_of_launch": vt / r["avg_us"],
+                                     "source": f"profiles/sq_counters_{workload}.md"}
     return {"bound": "hbm", "achieved": r["GBps"], "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": r["frac"],
             "traffic": traffic_bytes(workload, r["kernel"]), "kernel": r["kernel"], "pass": dom,
-            "alg_bytes_per_launch": r["alg_bytes"], "avg_launch_us": r["avg_us"]}
+            "alg_bytes_per_launch": r["alg_bytes"], "avg_launch_us": r["avg_us"], "issue": issue}
 
 
 def frame_counts(ctx, np, mesh, scene, args, frame, depth, ids, flags):
