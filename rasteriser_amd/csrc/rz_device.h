@@ -57,7 +57,8 @@ constexpr int32_t kMaxBatchMeshes = 1 << (32 - kListMeshShift);
 //   r5 = (n1.x, n1.y, n1.z, v0)
 //   r6 = (n2.x, n2.y, n2.z, u1)
 //   r7 = (v1, u2, v2, bits(material))
-// A culled face has bbox_lo = 0xFFFFFFFF, bbox_hi = 0 (empty) and nothing else.
+// The box is the clamped bbox cut to the pixels the inside test can pass.  A face that
+// is culled, or whose box holds no pixel, has no record (its front-mask bit is clear).
 struct FaceRecord {
     float4 r0, r1, r2, r3, r4, r5, r6, r7;
 };
