@@ -43,9 +43,22 @@ __device__ __forceinline__ T issue_now(const T* q) {
     return __hip_atomic_load(const_cast<T*>(q), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// op over the wave's 64 lanes, all of them active, returned to every lane: the row
+// shifts and row broadcasts of wave_scan_incl (below) with `identity` where a shift
+// leaves its row, then lane 63's value.  Six DPP operations, no LDS round trip.
+template <typename Op>
+__device__ __forceinline__ int wave_reduce(int x, int identity, Op op) {
+    x = op(x, __builtin_amdgcn_update_dpp(identity, x, 0x111, 0xf, 0xf, false));   // row_shr:1
+    x = op(x, __builtin_amdgcn_update_dpp(identity, x, 0x112, 0xf, 0xf, false));   // row_shr:2
+    x = op(x, __builtin_amdgcn_update_dpp(identity, x, 0x114, 0xf, 0xf, false));   // row_shr:4
+    x = op(x, __builtin_amdgcn_update_dpp(identity, x, 0x118, 0xf, 0xf, false));   // row_shr:8
+    x = op(x, __builtin_amdgcn_update_dpp(identity, x, 0x142, 0xa, 0xf, false));   // row_bcast:15
+    x = op(x, __builtin_amdgcn_update_dpp(identity, x, 0x143, 0xc, 0xf, false));   // row_bcast:31
+    return __builtin_amdgcn_readlane(x, 63);
+}
+
 __device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
-    for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, 64);
-    return v;
+    return (uint32_t)wave_reduce((int)v, 0, [](int a, int b) { return (int)((uint32_t)a + (uint32_t)b); });
 }
 
 // Inclusive prefix sum of v over the wave's 64 lanes, all of them active: shifts within
@@ -390,13 +403,12 @@ __device__ __forceinline__ void bin_body(const BinParams& p, int blk, unsigned c
     __syncthreads();
     {
         // union of the small faces' tile rectangles: per wave, then one LDS atomic each
-        int r0 = small ? (int)tx0 : 0x7FFFFFFF, r1 = small ? (int)ty0 : 0x7FFFFFFF;
-        int r2 = small ? (int)tx1 : -1, r3 = small ? (int)ty1 : -1;
-#pragma unroll
-        for (int d = 32; d >= 1; d >>= 1) {
-            r0 = min(r0, __shfl_xor(r0, d, 64)); r1 = min(r1, __shfl_xor(r1, d, 64));
-            r2 = max(r2, __shfl_xor(r2, d, 64)); r3 = max(r3, __shfl_xor(r3, d, 64));
-        }
+        auto mn = [](int a, int b) { return min(a, b); };
+        auto mx = [](int a, int b) { return max(a, b); };
+        const int r0 = wave_reduce(small ? (int)tx0 : 0x7FFFFFFF, 0x7FFFFFFF, mn);
+        const int r1 = wave_reduce(small ? (int)ty0 : 0x7FFFFFFF, 0x7FFFFFFF, mn);
+        const int r2 = wave_reduce(small ? (int)tx1 : -1, INT_MIN, mx);
+        const int r3 = wave_reduce(small ? (int)ty1 : -1, INT_MIN, mx);
         if ((tid & 63) == 0 && r2 >= 0) {
             atomicMin(&s_rect[0], r0); atomicMin(&s_rect[1], r1);
             atomicMax(&s_rect[2], r2); atomicMax(&s_rect[3], r3);
