@@ -458,9 +458,10 @@ __device__ __forceinline__ void bin_body(const BinParams& p, int blk, unsigned c
     while (bm) {
         const int src = __ffsll((long long)bm) - 1;
         bm &= bm - 1;
-        const uint32_t bx0 = __shfl(tx0, src, 64), bx1 = __shfl(tx1, src, 64);
-        const uint32_t by0 = __shfl(ty0, src, 64), by1 = __shfl(ty1, src, 64);
-        const uint32_t bf = __shfl((uint32_t)f, src, 64);
+        // (src is uniform: lane reads, not LDS permutes)
+        const uint32_t bx0 = __builtin_amdgcn_readlane(tx0, src), bx1 = __builtin_amdgcn_readlane(tx1, src);
+        const uint32_t by0 = __builtin_amdgcn_readlane(ty0, src), by1 = __builtin_amdgcn_readlane(ty1, src);
+        const uint32_t bf = __builtin_amdgcn_readlane((uint32_t)f, src);
         const uint32_t w = bx1 - bx0 + 1, n = w * (by1 - by0 + 1);
         for (uint32_t k = lane_id(); k < n; k += 64) {
             const uint32_t ty = by0 + k / w, tx = bx0 + k % w;
@@ -1178,7 +1179,7 @@ __device__ __forceinline__ uint32_t overflow_faces(const RasterParams& p, uint32
         }
         unsigned long long todo = __ballot(meets);
         while (todo) {
-            const uint32_t g = __shfl(f, __ffsll((long long)todo) - 1, 64);
+            const uint32_t g = __builtin_amdgcn_readlane(f, __ffsll((long long)todo) - 1);   // (todo uniform)
             todo &= todo - 1ull;
             const FaceRecord* r = p.rec + g;
             const float4 a = r->r0, b = r->r1, c = r->r2;
