@@ -92,28 +92,6 @@ __device__ __forceinline__ void store_block_sum(uint32_t v, uint32_t* out, int t
 }
 
 // ---------------------------------------------------------------------------------
-// store_block_sum of two values at once (one exchange through LDS instead of two).
-__device__ __forceinline__ void store_block_sums(uint32_t v0, uint32_t* out0, uint32_t v1, uint32_t* out1, int tid,
-                                                 int nthreads) {
-    __shared__ uint32_t s_part2[2][16];
-    v0 = wave_sum(v0);
-    v1 = wave_sum(v1);
-    if ((tid & 63) == 0) {
-        s_part2[0][tid >> 6] = v0;
-        s_part2[1][tid >> 6] = v1;
-    }
-    __syncthreads();
-    if (tid == 0) {
-        uint32_t t0 = 0, t1 = 0;
-        for (int w = 0; w < (nthreads >> 6); ++w) {
-            t0 += s_part2[0][w];
-            t1 += s_part2[1][w];
-        }
-        *out0 = t0;
-        *out1 = t1;
-    }
-    __syncthreads();
-}
 
 // A model position to (x_px, y_px, ndc_z, 1/w): GLM mat4*vec4 (pairwise) with w = 1,
 // perspective divide, viewport (geometry.cpp, drawing.cpp:20-40).
@@ -293,7 +271,7 @@ __device__ __forceinline__ bool shrink_rect(const float4& a, const float4& b, co
 template <int BLOCK>
 struct BinLds {
     static constexpr int kAggTiles = 4 * BLOCK;
-    static constexpr int kBytes = 2 * kAggTiles * 4 + BLOCK * kBigTriangleTiles + 16;
+    static constexpr int kBytes = 2 * kAggTiles * 4 + BLOCK * kBigTriangleTiles + 24;
 };
 
 template <int BLOCK>
@@ -381,7 +359,6 @@ __device__ __forceinline__ void bin_body(const BinParams& p, int blk, unsigned c
         if ((threadIdx.x & 63) == 0 && i < p.nf) p.front_mask[(uint32_t)i >> 6] = fm;
     }
     const uint32_t nt = (tx0 <= tx1 && ty0 <= ty1) ? (tx1 - tx0 + 1) * (ty1 - ty0 + 1) : 0u;
-    store_block_sums(front ? 1u : 0u, p.part_front + blk, nt, p.part_pairs + blk, threadIdx.x, BLOCK);
     const bool big = nt > (uint32_t)kBigTriangleTiles;
     const bool small = nt > 0 && !big;
 
@@ -394,27 +371,40 @@ __device__ __forceinline__ void bin_body(const BinParams& p, int blk, unsigned c
     uint32_t* s_cnt = reinterpret_cast<uint32_t*>(lds);
     uint32_t* s_base = s_cnt + kAggTiles;
     uint8_t* s_slot = reinterpret_cast<uint8_t*>(s_base + kAggTiles);
-    int* s_rect = reinterpret_cast<int*>(s_slot + BLOCK * kBigTriangleTiles);   // union tile rect: x0 y0 x1 y1
+    // union tile rect x0 y0 x1 y1, then the workgroup's front faces and (face, tile) pairs
+    int* s_rect = reinterpret_cast<int*>(s_slot + BLOCK * kBigTriangleTiles);
+    uint32_t* s_stat = reinterpret_cast<uint32_t*>(s_rect + 4);
     const int tid = threadIdx.x;
     if (tid == 0) {
         s_rect[0] = s_rect[1] = 0x7FFFFFFF;
         s_rect[2] = s_rect[3] = -1;
+        s_stat[0] = s_stat[1] = 0u;
     }
     __syncthreads();
     {
-        // union of the small faces' tile rectangles: per wave, then one LDS atomic each
+        // union of the small faces' tile rectangles and the statistics' partial sums: per
+        // wave, then one LDS atomic each (one exchange for both)
         auto mn = [](int a, int b) { return min(a, b); };
         auto mx = [](int a, int b) { return max(a, b); };
         const int r0 = wave_reduce(small ? (int)tx0 : 0x7FFFFFFF, 0x7FFFFFFF, mn);
         const int r1 = wave_reduce(small ? (int)ty0 : 0x7FFFFFFF, 0x7FFFFFFF, mn);
         const int r2 = wave_reduce(small ? (int)tx1 : -1, INT_MIN, mx);
         const int r3 = wave_reduce(small ? (int)ty1 : -1, INT_MIN, mx);
-        if ((tid & 63) == 0 && r2 >= 0) {
-            atomicMin(&s_rect[0], r0); atomicMin(&s_rect[1], r1);
-            atomicMax(&s_rect[2], r2); atomicMax(&s_rect[3], r3);
+        const uint32_t nfront = wave_sum(front ? 1u : 0u), npairs = wave_sum(nt);
+        if ((tid & 63) == 0) {
+            if (r2 >= 0) {
+                atomicMin(&s_rect[0], r0); atomicMin(&s_rect[1], r1);
+                atomicMax(&s_rect[2], r2); atomicMax(&s_rect[3], r3);
+            }
+            atomicAdd(&s_stat[0], nfront);
+            atomicAdd(&s_stat[1], npairs);
         }
     }
     __syncthreads();
+    if (tid == 0) {
+        p.part_front[blk] = s_stat[0];
+        p.part_pairs[blk] = s_stat[1];
+    }
     const int ux0 = s_rect[0], uy0 = s_rect[1];
     const int uw = s_rect[2] - ux0 + 1, uh = s_rect[3] - uy0 + 1;
     const bool agg = s_rect[2] >= 0 && uw * uh <= kAggTiles;   // uniform over the workgroup
