@@ -1006,17 +1006,21 @@ __device__ __forceinline__ void raster_body(const RasterParams& p, const ShadePa
         // workgroup-wide exclusive prefix sums of the candidate counts and of the live faces
         const unsigned long long live = __ballot(cnt > 0);
         const uint32_t v = wave_scan_incl(cnt);
-        if (lane == 63) {
-            s_wsum[wave] = v;
-            s_wlive[wave] = (uint32_t)__popcll(live);
-        }
-        __syncthreads();
         uint32_t wbase = 0, lbase = 0, total = 0;
+        if constexpr (NW == 1) {
+            total = __builtin_amdgcn_readlane(v, 63);
+        } else {
+            if (lane == 63) {
+                s_wsum[wave] = v;
+                s_wlive[wave] = (uint32_t)__popcll(live);
+            }
+            __syncthreads();
 #pragma unroll
-        for (int w = 0; w < NW; ++w) {
-            const uint32_t sw = s_wsum[w], lw = s_wlive[w];
-            if (w < wave) { wbase += sw; lbase += lw; }
-            total += sw;
+            for (int w = 0; w < NW; ++w) {
+                const uint32_t sw = s_wsum[w], lw = s_wlive[w];
+                if (w < wave) { wbase += sw; lbase += lw; }
+                total += sw;
+            }
         }
         const uint32_t start = wbase + v - cnt;
         if (cnt > 0) {
@@ -1029,42 +1033,62 @@ __device__ __forceinline__ void raster_body(const RasterParams& p, const ShadePa
             s_face[rank] = f;
             s_start[rank] = start;
         }
+        // candidate k of face j (rank): its pixel's inside and depth tests
+        auto test = [&](uint32_t k, int j) {
+            const uint32_t rc = s_rect[j];
+            const uint32_t rw = rc >> 16;
+            const uint32_t l = k - s_start[j];
+            // l / rw exactly: |error| of the product < 1/(2*rw) for l < 1024
+            const uint32_t dy = (uint32_t)(((float)l + 0.5f) * __builtin_amdgcn_rcpf((float)rw));
+            const uint32_t dx = l - dy * rw;
+            const int lx = (int)(rc & 0xFFu) + (int)dx, ly = (int)((rc >> 8) & 0xFFu) + (int)dy;
+            float z;
+            if (fragment_key_rcp((float)(ox + lx), (float)(oy + ly), s_v[0][j], s_v[1][j], s_v[2][j],
+                                 s_v[3][j], s_v[4][j], s_v[5][j], s_v[6][j], s_v[7][j], s_v[8][j], s_v[9][j],
+                                 s_v[10][j], z)) {
+                ++frags;
+                const int pl = ly * TS + lx;
+                if (z < (has_d0 ? s_d0[pl] : 1.0f)) atomicMin(&s_key[pl], vis_key(z, s_face[j]));
+            }
+        };
         uint32_t before = 0;   // live faces whose first candidate precedes the round
-        for (uint32_t r0 = 0; r0 < total; r0 += kWords * 64) {
-            if (tid < kWords) s_mask[tid] = 0ull;
-            __syncthreads();
-            if (cnt > 0 && start >= r0 && start - r0 < kWords * 64u)
-                atomicOr(&s_mask[(start - r0) >> 6], 1ull << ((start - r0) & 63u));
-            __syncthreads();
-            if (wave == 0) {
-                const uint32_t pc = (uint32_t)__popcll(s_mask[lane]);
-                s_wpre[lane] = wave_scan_incl(pc) - pc;
+        if constexpr (NW == 1) {
+            // One wave: each pass of 64 candidates finds their faces from a 64-bit mask of
+            // the face starts inside it, OR-reduced across the wave, with no LDS round trip.
+            auto bor = [](int x, int y) { return x | y; };
+            __syncthreads();   // (the face table above is read by other lanes)
+            for (uint32_t r0 = 0; r0 < total; r0 += 64) {   // (uniform)
+                const bool here = cnt > 0 && start >= r0 && start - r0 < 64u;
+                const uint32_t sb = start - r0;
+                const uint32_t lo = wave_reduce(here && sb < 32u ? (int)(1u << sb) : 0, 0, bor);
+                const uint32_t hi = wave_reduce(here && sb >= 32u ? (int)(1u << (sb - 32u)) : 0, 0, bor);
+                const unsigned long long m = ((unsigned long long)hi << 32) | lo;
+                const uint32_t k = r0 + (uint32_t)lane;
+                if (k < total) test(k, (int)(before + (uint32_t)__popcll(m & upto)) - 1);
+                before += (uint32_t)__popcll(m);
             }
-            __syncthreads();
-            const uint32_t r1 = min(total, r0 + kWords * 64u);
-            for (uint32_t k = r0 + tid; k < r1; k += BLOCK) {
-                // (r0 and BLOCK are multiples of 64: candidate k's bit in its mask word is
-                // the lane, so the mask of bits up to it is the loop-invariant upto)
-                const uint32_t wi = (k - r0) >> 6;
-                const int j = (int)(before + s_wpre[wi] + (uint32_t)__popcll(s_mask[wi] & upto)) - 1;
-                const uint32_t rc = s_rect[j];
-                const uint32_t rw = rc >> 16;
-                const uint32_t l = k - s_start[j];
-                // l / rw exactly: |error| of the product < 1/(2*rw) for l < 1024
-                const uint32_t dy = (uint32_t)(((float)l + 0.5f) * __builtin_amdgcn_rcpf((float)rw));
-                const uint32_t dx = l - dy * rw;
-                const int lx = (int)(rc & 0xFFu) + (int)dx, ly = (int)((rc >> 8) & 0xFFu) + (int)dy;
-                float z;
-                if (fragment_key_rcp((float)(ox + lx), (float)(oy + ly), s_v[0][j], s_v[1][j], s_v[2][j],
-                                     s_v[3][j], s_v[4][j], s_v[5][j], s_v[6][j], s_v[7][j], s_v[8][j], s_v[9][j],
-                                     s_v[10][j], z)) {
-                    ++frags;
-                    const int pl = ly * TS + lx;
-                    if (z < (has_d0 ? s_d0[pl] : 1.0f)) atomicMin(&s_key[pl], vis_key(z, s_face[j]));
+        } else {
+            for (uint32_t r0 = 0; r0 < total; r0 += kWords * 64) {
+                if (tid < kWords) s_mask[tid] = 0ull;
+                __syncthreads();
+                if (cnt > 0 && start >= r0 && start - r0 < kWords * 64u)
+                    atomicOr(&s_mask[(start - r0) >> 6], 1ull << ((start - r0) & 63u));
+                __syncthreads();
+                if (wave == 0) {
+                    const uint32_t pc = (uint32_t)__popcll(s_mask[lane]);
+                    s_wpre[lane] = wave_scan_incl(pc) - pc;
                 }
+                __syncthreads();
+                const uint32_t r1 = min(total, r0 + kWords * 64u);
+                for (uint32_t k = r0 + tid; k < r1; k += BLOCK) {
+                    // (r0 and BLOCK are multiples of 64: candidate k's bit in its mask word
+                    // is the lane, so the mask of bits up to it is the loop-invariant upto)
+                    const uint32_t wi = (k - r0) >> 6;
+                    test(k, (int)(before + s_wpre[wi] + (uint32_t)__popcll(s_mask[wi] & upto)) - 1);
+                }
+                before += s_wpre[kWords - 1] + (uint32_t)__popcll(s_mask[kWords - 1]);
+                __syncthreads();   // the mask is cleared for the next round
             }
-            before += s_wpre[kWords - 1] + (uint32_t)__popcll(s_mask[kWords - 1]);
-            __syncthreads();   // the mask is cleared for the next round
         }
         __syncthreads();       // ranks and wave totals are rewritten by the next chunk
     }
