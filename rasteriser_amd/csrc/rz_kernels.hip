@@ -1109,16 +1109,20 @@ __device__ __forceinline__ void raster_body(const RasterParams& p, const ShadePa
             const bool hit = valid && s_key[l] != kEmptyKey;
             if (valid && !hit) clear_pixel(*sp, (size_t)y * p.W + x);
             const unsigned long long bal = __ballot(hit);
-            if (lane == 0) s_wsum[wave] = (uint32_t)__popcll(bal);
-            __syncthreads();
             uint32_t off = nh, tot = nh;
+            if constexpr (NW == 1) {
+                tot += (uint32_t)__popcll(bal);
+            } else {
+                if (lane == 0) s_wsum[wave] = (uint32_t)__popcll(bal);
+                __syncthreads();
 #pragma unroll
-            for (int w = 0; w < NW; ++w) {
-                const uint32_t v = s_wsum[w];
-                if (w < wave) off += v;
-                tot += v;
+                for (int w = 0; w < NW; ++w) {
+                    const uint32_t v = s_wsum[w];
+                    if (w < wave) off += v;
+                    tot += v;
+                }
+                __syncthreads();   // s_wsum is rewritten by the next pass
             }
-            __syncthreads();   // s_wsum is rewritten by the next pass
             if (hit) s_hits[off + (uint32_t)__popcll(bal & below)] = (uint16_t)l;
             nh = tot;
         }
