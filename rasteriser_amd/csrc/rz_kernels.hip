@@ -959,10 +959,16 @@ __device__ __forceinline__ void raster_body(const RasterParams& p, const ShadePa
     // the first chunk's bin entries, read beside the count (tile_head)
     const uint32_t f_first = head.f_first;
     const uint32_t count = __builtin_amdgcn_readfirstlane(head.count);
-    const uint32_t n = count;
     // the bin pass cleared an overflowed tile's visibility words and the overflow loop
-    // fills them (uniform over the workgroup)
-    if (count > p.cap) return;
+    // fills them: such a tile has no chunk and no stores here (uniform over the workgroup)
+    const bool overflowed = count > p.cap;
+    const uint32_t n = overflowed ? 0u : count;
+    // the first chunk's face records, read before anything waits
+    float4 a0 = make_float4(0.f, 0.f, 0.f, 0.f), b0 = a0, c0 = a0;
+    if ((uint32_t)tid < n) {
+        const FaceRecord* r = p.rec + f_first;
+        a0 = r->r0; b0 = r->r1; c0 = r->r2;
+    }
     // initial depths: the caller's depth buffer, staged in LDS, or 1 everywhere for a
     // frame without one (uniform; then nothing is staged)
     const bool has_d0 = p.depth_in != nullptr;
@@ -982,14 +988,14 @@ __device__ __forceinline__ void raster_body(const RasterParams& p, const ShadePa
     const unsigned long long below = (1ull << lane) - 1ull;
     const unsigned long long upto = (below << 1) | 1ull;   // bits 0..lane
     static_assert(BLOCK % 64 == 0, "candidates are dealt in whole waves");
-    for (uint32_t base = 0; base < n; base += BLOCK) {
-        const uint32_t i = base + tid;
-        uint32_t cnt = 0, rect = 0, f = 0;
-        float4 a = make_float4(0.f, 0.f, 0.f, 0.f), b = a, c = a;
+    // A chunk's faces (entry i of the bin: face f, record a b c) into the LDS face table
+    // by rank: their candidate rectangles, the prefix sums, the chunk's candidate total.
+    struct Staged {
+        uint32_t cnt, start, total;
+    };
+    auto stage = [&](uint32_t i, uint32_t f, const float4& a, const float4& b, const float4& c) -> Staged {
+        uint32_t cnt = 0, rect = 0;
         if (i < n) {
-            f = base == 0 ? f_first : p.bins[(size_t)tile * p.cap + i];
-            const FaceRecord* r = p.rec + f;
-            a = r->r0; b = r->r1; c = r->r2;
             const uint32_t lo = __float_as_uint(b.w), hi = __float_as_uint(c.w);
             int rx0 = max((int)(lo & 0xFFFFu) - ox, 0), rx1 = min((int)(hi & 0xFFFFu) - ox, TS - 1);
             int ry0 = max((int)(lo >> 16) - oy, 0), ry1 = min((int)(hi >> 16) - oy, TS - 1);
@@ -1033,6 +1039,25 @@ __device__ __forceinline__ void raster_body(const RasterParams& p, const ShadePa
             s_face[rank] = f;
             s_start[rank] = start;
         }
+        return Staged{cnt, start, total};
+    };
+    // the first chunk from the records read above (which are then dead: they are not
+    // carried through the candidate loop), each later one read when its turn comes
+    Staged st{0u, 0u, 0u};
+    if (n > 0) st = stage((uint32_t)tid, f_first, a0, b0, c0);
+    for (uint32_t base = 0; base < n; base += BLOCK) {
+        if (base > 0) {
+            const uint32_t i = base + tid;
+            uint32_t f = 0;
+            float4 a = make_float4(0.f, 0.f, 0.f, 0.f), b = a, c = a;
+            if (i < n) {
+                f = p.bins[(size_t)tile * p.cap + i];
+                const FaceRecord* r = p.rec + f;
+                a = r->r0; b = r->r1; c = r->r2;
+            }
+            st = stage(i, f, a, b, c);
+        }
+        const uint32_t cnt = st.cnt, start = st.start, total = st.total;
         // candidate k of face j (rank): its pixel's inside and depth tests
         auto test = [&](uint32_t k, int j) {
             const uint32_t rc = s_rect[j];
@@ -1092,6 +1117,7 @@ __device__ __forceinline__ void raster_body(const RasterParams& p, const ShadePa
         }
         __syncthreads();       // ranks and wave totals are rewritten by the next chunk
     }
+    if (overflowed) return;
 
     if (FUSED && PPT > 1) {
         // Several pixels per thread: the covered pixels are compacted first, so that a
