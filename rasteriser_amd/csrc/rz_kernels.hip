@@ -506,7 +506,10 @@ __device__ __forceinline__ void bend_normal(const ShadeParams& p, uint32_t f, co
                                             float& nz) {
     const float tx = p.ftx[f], ty = p.fty[f], tz = p.ftz[f];
     const float* M = p.normal_mat;
-    const float w = 0.0f;
+    // (an opaque 0: the products M[12..14] * 0 are formed here, not hoisted out of the
+    // tile loop into registers that then spill)
+    float w = 0.0f;
+    asm volatile("" : "+v"(w));
     const float cx = (M[0] * tx + M[4] * ty) + (M[8] * tz + M[12] * w);
     const float cy = (M[1] * tx + M[5] * ty) + (M[9] * tz + M[13] * w);
     const float cz = (M[2] * tx + M[6] * ty) + (M[10] * tz + M[14] * w);
@@ -874,11 +877,12 @@ __device__ __forceinline__ bool fragment_key_rcp(float pxf, float pyf, float x0,
 
 // One tile, by one workgroup.  The tile's bin is taken BLOCK faces at a time: the faces
 // whose bbox meets the tile are compacted (ballot + popcount ranks, wave totals in LDS)
-// and their candidate pixel counts (bbox inside the tile) prefix-summed across the
-// workgroup.  Candidates are then handed out one per thread, up to 4096 per round: a
-// bitmask in LDS marks each face's first candidate, and a candidate's face is found by
-// counting the marks at or before it (per-word prefix counts + one popcount), with no
-// search.
+// and their counts of candidate pairs (two pixels of the face's rectangle in its tile)
+// prefix-summed across the workgroup.  Pairs are then handed out one per thread, up to
+// 4096 per round: a bitmask in LDS marks each face's first pair, and a pair's face is
+// found by counting the marks at or before it (per-word prefix counts + one popcount),
+// with no search.  A pair's two pixels share the face's LDS reads and the pixel's place
+// in the rectangle (a face with an odd count leaves one lane idle in its last pair).
 //
 // FUSED: the workgroup then shades its own pixels from the LDS visibility words (the
 // deferred shade, without the round trip of the visibility buffer through HBM).  Tiles
@@ -921,7 +925,7 @@ __device__ __forceinline__ void raster_body(const RasterParams& p, const ShadePa
     constexpr int NPX = TS * TS;
     constexpr int PPT = NPX / BLOCK;
     constexpr int NW = BLOCK / 64;
-    constexpr int kWords = 64;                 // mask words per round: 4096 candidates
+    constexpr int kWords = 64;                 // mask words per round: 4096 candidate pairs
     static_assert(PPT >= 1 && NPX % BLOCK == 0, "tile must cover the block");
     constexpr bool kNet = FUSED && MODE == kShadeMlp;
     __shared__ unsigned long long s_key[NPX];
@@ -989,7 +993,7 @@ __device__ __forceinline__ void raster_body(const RasterParams& p, const ShadePa
     const unsigned long long upto = (below << 1) | 1ull;   // bits 0..lane
     static_assert(BLOCK % 64 == 0, "candidates are dealt in whole waves");
     // A chunk's faces (entry i of the bin: face f, record a b c) into the LDS face table
-    // by rank: their candidate rectangles, the prefix sums, the chunk's candidate total.
+    // by rank: their candidate rectangles, the prefix sums, the chunk's total of pairs.
     struct Staged {
         uint32_t cnt, start, total;
     };
@@ -1004,12 +1008,13 @@ __device__ __forceinline__ void raster_body(const RasterParams& p, const ShadePa
             if (rx0 <= rx1 && ry0 <= ry1 && (rx1 - rx0 + 1) * (ry1 - ry0 + 1) > SHRINK)
                 keep = shrink_rect<SLAB>(a, b, c, ox, oy, rx0, rx1, ry0, ry1);
             if (keep && rx0 <= rx1 && ry0 <= ry1) {
-                const uint32_t rw = rx1 - rx0 + 1;
-                cnt = rw * (ry1 - ry0 + 1);
-                rect = (uint32_t)rx0 | ((uint32_t)ry0 << 8) | (rw << 16);
+                const uint32_t rw = rx1 - rx0 + 1, rh = ry1 - ry0 + 1;
+                cnt = (rw * rh + 1u) >> 1;   // candidate pairs
+                rect = (uint32_t)rx0 | ((uint32_t)ry0 << 8) | (rw << 16) | (rh << 24);
             }
         }
-        // workgroup-wide exclusive prefix sums of the candidate counts and of the live faces
+        // workgroup-wide exclusive prefix sums of the candidate pair counts and of the live
+        // faces
         const unsigned long long live = __ballot(cnt > 0);
         const uint32_t v = wave_scan_incl(cnt);
         uint32_t wbase = 0, lbase = 0, total = 0;
@@ -1058,27 +1063,45 @@ __device__ __forceinline__ void raster_body(const RasterParams& p, const ShadePa
             st = stage(i, f, a, b, c);
         }
         const uint32_t cnt = st.cnt, start = st.start, total = st.total;
-        // candidate k of face j (rank): its pixel's inside and depth tests
+        // candidate pair k of face j (rank): the inside and depth tests of the pixels 2k and
+        // 2k + 1 of its rectangle (row by row; the second only if the rectangle has it),
+        // which share the face's reads and the pixel's place in the rectangle
         auto test = [&](uint32_t k, int j) {
             const uint32_t rc = s_rect[j];
-            const uint32_t rw = rc >> 16;
-            const uint32_t l = k - s_start[j];
+            const uint32_t rw = (rc >> 16) & 0xFFu, rh = rc >> 24;
+            const uint32_t l = 2u * (k - s_start[j]);
             // l / rw exactly: |error| of the product < 1/(2*rw) for l < 1024
             const uint32_t dy = (uint32_t)(((float)l + 0.5f) * __builtin_amdgcn_rcpf((float)rw));
             const uint32_t dx = l - dy * rw;
             const int lx = (int)(rc & 0xFFu) + (int)dx, ly = (int)((rc >> 8) & 0xFFu) + (int)dy;
-            float z;
-            if (fragment_key_rcp((float)(ox + lx), (float)(oy + ly), s_v[0][j], s_v[1][j], s_v[2][j],
-                                 s_v[3][j], s_v[4][j], s_v[5][j], s_v[6][j], s_v[7][j], s_v[8][j], s_v[9][j],
-                                 s_v[10][j], z)) {
-                ++frags;
-                const int pl = ly * TS + lx;
-                if (z < (has_d0 ? s_d0[pl] : 1.0f)) atomicMin(&s_key[pl], vis_key(z, s_face[j]));
+            const bool wrap = dx + 1u == rw;
+            const int lx2 = wrap ? (int)(rc & 0xFFu) : lx + 1, ly2 = wrap ? ly + 1 : ly;
+            const bool two = l + 1u < rw * rh;
+            const float x0 = s_v[0][j], y0 = s_v[1][j], z0 = s_v[2][j], x1 = s_v[3][j], y1 = s_v[4][j];
+            const float z1 = s_v[5][j], x2 = s_v[6][j], y2 = s_v[7][j], z2 = s_v[8][j], ar = s_v[9][j];
+            const float ra = s_v[10][j];
+            float z, zb;
+            const bool ina = fragment_key_rcp((float)(ox + lx), (float)(oy + ly), x0, y0, z0, x1, y1, z1, x2, y2,
+                                              z2, ar, ra, z);
+            const bool inb = two && fragment_key_rcp((float)(ox + lx2), (float)(oy + ly2), x0, y0, z0, x1, y1, z1,
+                                                     x2, y2, z2, ar, ra, zb);
+            if (ina | inb) {
+                const uint32_t fj = s_face[j];
+                if (ina) {
+                    ++frags;
+                    const int pl = ly * TS + lx;
+                    if (z < (has_d0 ? s_d0[pl] : 1.0f)) atomicMin(&s_key[pl], vis_key(z, fj));
+                }
+                if (inb) {
+                    ++frags;
+                    const int pl = ly2 * TS + lx2;
+                    if (zb < (has_d0 ? s_d0[pl] : 1.0f)) atomicMin(&s_key[pl], vis_key(zb, fj));
+                }
             }
         };
         uint32_t before = 0;   // live faces whose first candidate precedes the round
         if constexpr (NW == 1) {
-            // One wave: each pass of 64 candidates finds their faces from a 64-bit mask of
+            // One wave: each pass of 64 candidate pairs finds their faces from a 64-bit mask of
             // the face starts inside it, OR-reduced across the wave, with no LDS round trip.
             auto bor = [](int x, int y) { return x | y; };
             __syncthreads();   // (the face table above is read by other lanes)
@@ -1330,10 +1353,10 @@ __global__ __launch_bounds__(kFrameBinBlock) void k_bin(BinParams p) {
 // the overflow loop (the tile's visibility words were cleared by the bin pass and the
 // raster loop skipped it), so the frame needs no launch of its own for it.
 // Statistics are per-workgroup partial sums.
-// (minimum waves per SIMD in the launch bounds: 7 for the raster kernel on 16-pixel
-// tiles (8 for 8-pixel ones, 6 for 32-pixel tiles, whose LDS allows no more), 6 for the
-// fused one, 3 for the fused learned shader: the most each reaches without register
-// spills)
+// (minimum waves per SIMD in the launch bounds: 6 for the 256-thread raster and fused
+// kernels, 8 for the 64-thread raster kernel, 3 for the fused learned shader: the most
+// the 16-pixel-tile kernels reach without register spills; the sample-collecting and
+// timing modes and other tile sizes spill a few registers at these bounds)
 // A resident workgroup's walk over its XCD's tile list: list entries k0, k0 + step, ...
 // below the list's count.  The first entry is read beside the count rather than after
 // it, one dependent memory round trip less before the first tile starts (an entry past
@@ -1390,7 +1413,7 @@ __device__ __forceinline__ void walk_tiles(const RasterParams& p, const ListWalk
 }
 
 template <int TS, int BLOCK>
-__global__ __launch_bounds__(BLOCK, TS == 32 ? 6 : (BLOCK == 256 ? 7 : 8)) void k_raster(RasterParams p) {
+__global__ __launch_bounds__(BLOCK, BLOCK == 256 ? 6 : 8) void k_raster(RasterParams p) {
     const ListWalk w(p);
     const bool overflow = overflowed_tiles(p);
     uint32_t frags = 0, cov = 0;
